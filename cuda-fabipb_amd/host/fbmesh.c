@@ -1,0 +1,413 @@
+/*
+ * fbmesh: deterministic synthetic molecule generator.
+ *
+ * Writes the three files the fabipb input reader consumes for one molecule:
+ *   <base>.vert  MSMS vertex file: 2 comment lines, "nv natm den prob",
+ *                then "x y z nx ny nz 0 atom 2" per vertex
+ *   <base>.face  MSMS face file:   2 comment lines, "nf natm den prob",
+ *                then "i j k 1 1" per face (1-based, outward CCW)
+ *   <base>.pqr   ATOM records "... x y z charge radius"
+ * which is the format src/input.c:320-821 (loadPanel) of the reference reads
+ * when FABIPB_REUSE_MESH=1 is set.
+ *
+ * Every number is printed with %.17g so the reader recovers the exact binary
+ * value, and every random draw comes from one named generator: SplitMix64
+ * seeding xoshiro256** (Blackman & Vigna), uniform doubles from the top 53
+ * bits, normals from Box-Muller. The same seed gives bit-identical files on
+ * any x86-64 host.
+ *
+ * Shapes:
+ *   sphere   level-L icosphere of radius R (standard 12-vertex icosahedron,
+ *            4-way midpoint subdivision, vertices projected to radius R),
+ *            radial vertex normals.
+ *   globule  the same icosphere with each vertex direction u moved to
+ *            r(u) = R (1 + amp * sum_{l=1..lmax} sum_m a_lm Y_lm(u)),
+ *            a_lm ~ N(0,1)/l, real orthonormal Y_lm (no Condon-Shortley
+ *            phase); vertex normals are the analytic surface normals,
+ *            evaluated by central differences of the parametrisation.
+ * Charges:
+ *   ball     nq charges uniform in the ball |x| <= rq, q ~ U[-1,1]
+ *   center   one charge +q0 at the origin
+ *   inside   nq charges uniform inside the star-shaped surface at least
+ *            `depth` from every mesh triangle (exact point-triangle
+ *            distance), q ~ N(0, sigma), shifted so the total is the
+ *            nearest integer.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ---------------- PRNG: SplitMix64 -> xoshiro256** ---------------- */
+typedef struct { uint64_t s[4]; } rng_t;
+
+static uint64_t splitmix64(uint64_t *x) {
+  uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+static void rng_seed(rng_t *r, uint64_t seed) {
+  uint64_t x = seed;
+  for (int i = 0; i < 4; i++) r->s[i] = splitmix64(&x);
+}
+static uint64_t rng_next(rng_t *r) {
+  uint64_t *s = r->s;
+  uint64_t result = rotl(s[1] * 5, 7) * 9;
+  uint64_t t = s[1] << 17;
+  s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+  s[2] ^= t; s[3] = rotl(s[3], 45);
+  return result;
+}
+/* uniform in [0,1) from the top 53 bits */
+static double rng_uniform(rng_t *r) { return (double)(rng_next(r) >> 11) * 0x1.0p-53; }
+/* standard normal, Box-Muller (one draw per call, the sine branch is dropped) */
+static double rng_normal(rng_t *r) {
+  double u1 = rng_uniform(r), u2 = rng_uniform(r);
+  if (u1 < 1e-300) u1 = 1e-300;
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+/* ---------------- icosphere ---------------- */
+typedef struct {
+  int nv, nf, capv, capf;
+  double *v;   /* 3*nv unit directions */
+  int *f;      /* 3*nf */
+} mesh_t;
+
+static void mesh_push_v(mesh_t *m, double x, double y, double z) {
+  if (m->nv == m->capv) {
+    m->capv = m->capv ? 2 * m->capv : 64;
+    m->v = realloc(m->v, sizeof(double) * 3 * (size_t)m->capv);
+  }
+  double n = sqrt(x * x + y * y + z * z);
+  m->v[3 * m->nv] = x / n; m->v[3 * m->nv + 1] = y / n; m->v[3 * m->nv + 2] = z / n;
+  m->nv++;
+}
+static void mesh_push_f(mesh_t *m, int a, int b, int c) {
+  if (m->nf == m->capf) {
+    m->capf = m->capf ? 2 * m->capf : 64;
+    m->f = realloc(m->f, sizeof(int) * 3 * (size_t)m->capf);
+  }
+  m->f[3 * m->nf] = a; m->f[3 * m->nf + 1] = b; m->f[3 * m->nf + 2] = c;
+  m->nf++;
+}
+
+/* open-addressing map from an undirected edge (a<b) to its midpoint vertex */
+typedef struct { uint64_t *key; int *val; size_t cap; } emap_t;
+static int emap_get(emap_t *e, mesh_t *m, int a, int b) {
+  if (a > b) { int t = a; a = b; b = t; }
+  uint64_t k = ((uint64_t)a << 32) | (uint64_t)b;
+  size_t h = (size_t)((k * 0x9E3779B97F4A7C15ULL) >> 17) & (e->cap - 1);
+  while (e->key[h] != UINT64_MAX) {
+    if (e->key[h] == k) return e->val[h];
+    h = (h + 1) & (e->cap - 1);
+  }
+  const double *va = &m->v[3 * a], *vb = &m->v[3 * b];
+  mesh_push_v(m, va[0] + vb[0], va[1] + vb[1], va[2] + vb[2]);
+  e->key[h] = k;
+  e->val[h] = m->nv - 1;
+  return m->nv - 1;
+}
+
+static void icosphere(mesh_t *m, int level) {
+  const double p = (1.0 + sqrt(5.0)) / 2.0;
+  static const int F[20][3] = {
+      {0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11},
+      {1, 5, 9}, {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+      {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8}, {3, 8, 9},
+      {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+  memset(m, 0, sizeof(*m));
+  mesh_push_v(m, -1, p, 0); mesh_push_v(m, 1, p, 0);
+  mesh_push_v(m, -1, -p, 0); mesh_push_v(m, 1, -p, 0);
+  mesh_push_v(m, 0, -1, p); mesh_push_v(m, 0, 1, p);
+  mesh_push_v(m, 0, -1, -p); mesh_push_v(m, 0, 1, -p);
+  mesh_push_v(m, p, 0, -1); mesh_push_v(m, p, 0, 1);
+  mesh_push_v(m, -p, 0, -1); mesh_push_v(m, -p, 0, 1);
+  for (int i = 0; i < 20; i++) mesh_push_f(m, F[i][0], F[i][1], F[i][2]);
+  for (int l = 0; l < level; l++) {
+    emap_t e;
+    e.cap = 1;
+    while (e.cap < (size_t)m->nf * 4) e.cap <<= 1;
+    e.key = malloc(sizeof(uint64_t) * e.cap);
+    e.val = malloc(sizeof(int) * e.cap);
+    for (size_t i = 0; i < e.cap; i++) e.key[i] = UINT64_MAX;
+    int nf0 = m->nf;
+    int *f0 = malloc(sizeof(int) * 3 * (size_t)nf0);
+    memcpy(f0, m->f, sizeof(int) * 3 * (size_t)nf0);
+    m->nf = 0;
+    for (int i = 0; i < nf0; i++) {
+      int a = f0[3 * i], b = f0[3 * i + 1], c = f0[3 * i + 2];
+      int ab = emap_get(&e, m, a, b), bc = emap_get(&e, m, b, c), ca = emap_get(&e, m, c, a);
+      mesh_push_f(m, a, ab, ca);
+      mesh_push_f(m, b, bc, ab);
+      mesh_push_f(m, c, ca, bc);
+      mesh_push_f(m, ab, bc, ca);
+    }
+    free(f0); free(e.key); free(e.val);
+  }
+}
+
+/* ---------------- real spherical harmonics ---------------- */
+typedef struct { int lmax; double amp, R; double *a; } shape_t; /* a[l*l + l + m] */
+
+/* associated Legendre P_l^m(x) without Condon-Shortley phase, m >= 0 */
+static double plm(int l, int m, double x) {
+  double pmm = 1.0, s = sqrt(fmax(0.0, 1.0 - x * x)), fact = 1.0;
+  for (int i = 1; i <= m; i++) { pmm *= fact * s; fact += 2.0; }
+  if (l == m) return pmm;
+  double pmmp1 = x * (2 * m + 1) * pmm;
+  if (l == m + 1) return pmmp1;
+  double pll = 0.0;
+  for (int ll = m + 2; ll <= l; ll++) {
+    pll = (x * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+    pmm = pmmp1; pmmp1 = pll;
+  }
+  return pll;
+}
+static double ylm(int l, int m, const double *u) {
+  int am = m < 0 ? -m : m;
+  double lf = 1.0;  /* (l-|m|)!/(l+|m|)! */
+  for (int i = l - am + 1; i <= l + am; i++) lf /= i;
+  double N = sqrt((2 * l + 1) / (4.0 * M_PI) * lf);
+  double ph = atan2(u[1], u[0]);
+  double P = plm(l, am, u[2]);
+  if (m == 0) return N * P;
+  if (m > 0) return sqrt(2.0) * N * P * cos(m * ph);
+  return sqrt(2.0) * N * P * sin(am * ph);
+}
+static double shape_r(const shape_t *s, const double *u) {
+  if (s->lmax <= 0) return s->R;
+  double sum = 0.0;
+  for (int l = 1; l <= s->lmax; l++)
+    for (int m = -l; m <= l; m++) sum += s->a[l * l + l + m] * ylm(l, m, u);
+  return s->R * (1.0 + s->amp * sum);
+}
+static void normalize3(double *x) {
+  double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  x[0] /= n; x[1] /= n; x[2] /= n;
+}
+static void shape_point(const shape_t *s, const double *u, double *p) {
+  double r = shape_r(s, u);
+  p[0] = r * u[0]; p[1] = r * u[1]; p[2] = r * u[2];
+}
+/* outward surface normal at direction u, by central differences of p(u) */
+static void shape_normal(const shape_t *s, const double *u, double *n) {
+  if (s->lmax <= 0) { n[0] = u[0]; n[1] = u[1]; n[2] = u[2]; return; }
+  double t1[3], t2[3], ax[3] = {1, 0, 0};
+  if (fabs(u[0]) > 0.9) { ax[0] = 0; ax[1] = 1; }
+  t1[0] = u[1] * ax[2] - u[2] * ax[1]; t1[1] = u[2] * ax[0] - u[0] * ax[2]; t1[2] = u[0] * ax[1] - u[1] * ax[0];
+  normalize3(t1);
+  t2[0] = u[1] * t1[2] - u[2] * t1[1]; t2[1] = u[2] * t1[0] - u[0] * t1[2]; t2[2] = u[0] * t1[1] - u[1] * t1[0];
+  const double h = 1e-5;
+  double d1[3], d2[3], q[3], pp[3], pm[3];
+  for (int k = 0; k < 3; k++) q[k] = u[k] + h * t1[k];
+  normalize3(q); shape_point(s, q, pp);
+  for (int k = 0; k < 3; k++) q[k] = u[k] - h * t1[k];
+  normalize3(q); shape_point(s, q, pm);
+  for (int k = 0; k < 3; k++) d1[k] = pp[k] - pm[k];
+  for (int k = 0; k < 3; k++) q[k] = u[k] + h * t2[k];
+  normalize3(q); shape_point(s, q, pp);
+  for (int k = 0; k < 3; k++) q[k] = u[k] - h * t2[k];
+  normalize3(q); shape_point(s, q, pm);
+  for (int k = 0; k < 3; k++) d2[k] = pp[k] - pm[k];
+  n[0] = d1[1] * d2[2] - d1[2] * d2[1];
+  n[1] = d1[2] * d2[0] - d1[0] * d2[2];
+  n[2] = d1[0] * d2[1] - d1[1] * d2[0];
+  normalize3(n);
+  if (n[0] * u[0] + n[1] * u[1] + n[2] * u[2] < 0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+}
+
+/* ---------------- point-triangle distance (Ericson, RTCD 5.1.5) ---------------- */
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double pt_tri_dist2(const double *p, const double *a, const double *b, const double *c) {
+  double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+  for (int k = 0; k < 3; k++) { ab[k] = b[k] - a[k]; ac[k] = c[k] - a[k]; ap[k] = p[k] - a[k]; }
+  double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0 && d2 <= 0) { for (int k = 0; k < 3; k++) q[k] = a[k]; goto done; }
+  for (int k = 0; k < 3; k++) bp[k] = p[k] - b[k];
+  double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0 && d4 <= d3) { for (int k = 0; k < 3; k++) q[k] = b[k]; goto done; }
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    double v = d1 / (d1 - d3);
+    for (int k = 0; k < 3; k++) q[k] = a[k] + v * ab[k];
+    goto done;
+  }
+  for (int k = 0; k < 3; k++) cp[k] = p[k] - c[k];
+  double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0 && d5 <= d6) { for (int k = 0; k < 3; k++) q[k] = c[k]; goto done; }
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    double w = d2 / (d2 - d6);
+    for (int k = 0; k < 3; k++) q[k] = a[k] + w * ac[k];
+    goto done;
+  }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for (int k = 0; k < 3; k++) q[k] = b[k] + w * (c[k] - b[k]);
+    goto done;
+  }
+  {
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom, w = vc * denom;
+    for (int k = 0; k < 3; k++) q[k] = a[k] + ab[k] * v + ac[k] * w;
+  }
+done:
+  return (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
+}
+
+/* ---------------- output ---------------- */
+static FILE *open_out(const char *base, const char *ext) {
+  char path[4096];
+  snprintf(path, sizeof(path), "%s%s", base, ext);
+  FILE *fp = fopen(path, "w");
+  if (!fp) { fprintf(stderr, "fbmesh: cannot write %s\n", path); exit(1); }
+  return fp;
+}
+
+static void usage(void) {
+  fprintf(stderr,
+          "usage: fbmesh <sphere|globule> <out-base> [--level L] [--radius R]\n"
+          "              [--amp A] [--lmax L] [--charges ball|center|inside]\n"
+          "              [--nq N] [--rq r] [--q0 q] [--sigma s] [--depth d] [--seed S]\n");
+  exit(2);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 3) usage();
+  const char *kind = argv[1], *base = argv[2];
+  int level = 3, lmax = 6, nq = 1;
+  double R = 10.0, amp = 0.08, rq = 8.0, q0 = 1.0, sigma = 0.4, depth = 1.5;
+  const char *charges = "center";
+  uint64_t seed = 1;
+  for (int i = 3; i < argc; i++) {
+    const char *a = argv[i];
+    if (i + 1 >= argc) usage();
+    const char *v = argv[++i];
+    if (!strcmp(a, "--level")) level = atoi(v);
+    else if (!strcmp(a, "--radius")) R = atof(v);
+    else if (!strcmp(a, "--amp")) amp = atof(v);
+    else if (!strcmp(a, "--lmax")) lmax = atoi(v);
+    else if (!strcmp(a, "--charges")) charges = v;
+    else if (!strcmp(a, "--nq")) nq = atoi(v);
+    else if (!strcmp(a, "--rq")) rq = atof(v);
+    else if (!strcmp(a, "--q0")) q0 = atof(v);
+    else if (!strcmp(a, "--sigma")) sigma = atof(v);
+    else if (!strcmp(a, "--depth")) depth = atof(v);
+    else if (!strcmp(a, "--seed")) seed = strtoull(v, NULL, 10);
+    else usage();
+  }
+  if (level < 0 || level > 9 || nq < 1) usage();
+
+  rng_t rng;
+  rng_seed(&rng, seed);
+  shape_t shape;
+  shape.R = R; shape.amp = amp; shape.lmax = 0; shape.a = NULL;
+  if (!strcmp(kind, "globule")) {
+    shape.lmax = lmax;
+    shape.a = calloc((size_t)(lmax + 1) * (lmax + 1), sizeof(double));
+    for (int l = 1; l <= lmax; l++)
+      for (int m = -l; m <= l; m++) shape.a[l * l + l + m] = rng_normal(&rng) / l;
+  } else if (strcmp(kind, "sphere") != 0) {
+    usage();
+  }
+
+  mesh_t m;
+  icosphere(&m, level);
+  double *pos = malloc(sizeof(double) * 3 * (size_t)m.nv);
+  double *nrm = malloc(sizeof(double) * 3 * (size_t)m.nv);
+  for (int i = 0; i < m.nv; i++) {
+    shape_point(&shape, &m.v[3 * i], &pos[3 * i]);
+    shape_normal(&shape, &m.v[3 * i], &nrm[3 * i]);
+  }
+
+  /* charges */
+  double *qx = malloc(sizeof(double) * 3 * (size_t)nq), *qq = malloc(sizeof(double) * (size_t)nq);
+  if (!strcmp(charges, "center")) {
+    nq = 1; qx[0] = qx[1] = qx[2] = 0.0; qq[0] = q0;
+  } else if (!strcmp(charges, "ball")) {
+    for (int j = 0; j < nq; j++) {
+      double d[3];
+      for (int k = 0; k < 3; k++) d[k] = rng_normal(&rng);
+      normalize3(d);
+      double r = rq * cbrt(rng_uniform(&rng));
+      for (int k = 0; k < 3; k++) qx[3 * j + k] = r * d[k];
+      qq[j] = 2.0 * rng_uniform(&rng) - 1.0;
+    }
+  } else if (!strcmp(charges, "inside")) {
+    double rmax = 0.0, lmaxEdge = 0.0;
+    for (int i = 0; i < m.nv; i++) rmax = fmax(rmax, sqrt(dot3(&pos[3 * i], &pos[3 * i])));
+    double *cen = malloc(sizeof(double) * 3 * (size_t)m.nf);
+    for (int f = 0; f < m.nf; f++) {
+      const double *a = &pos[3 * m.f[3 * f]], *b = &pos[3 * m.f[3 * f + 1]], *c = &pos[3 * m.f[3 * f + 2]];
+      for (int k = 0; k < 3; k++) cen[3 * f + k] = (a[k] + b[k] + c[k]) / 3.0;
+      for (int k = 0; k < 3; k++) {
+        const double *v = (k == 0) ? a : (k == 1) ? b : c;
+        double d2 = (v[0] - cen[3 * f]) * (v[0] - cen[3 * f]) + (v[1] - cen[3 * f + 1]) * (v[1] - cen[3 * f + 1]) +
+                    (v[2] - cen[3 * f + 2]) * (v[2] - cen[3 * f + 2]);
+        lmaxEdge = fmax(lmaxEdge, sqrt(d2));
+      }
+    }
+    double reach = depth + lmaxEdge, reach2 = reach * reach, depth2 = depth * depth;
+    long tries = 0;
+    for (int j = 0; j < nq;) {
+      if (++tries > 1000000000L) { fprintf(stderr, "fbmesh: charge placement failed\n"); return 1; }
+      double p[3], d[3];
+      for (int k = 0; k < 3; k++) p[k] = rmax * (2.0 * rng_uniform(&rng) - 1.0);
+      double rho = sqrt(dot3(p, p));
+      if (rho < 1e-12) continue;
+      for (int k = 0; k < 3; k++) d[k] = p[k] / rho;
+      if (rho > shape_r(&shape, d) - depth * 0.5) continue;  /* outside, or too close radially */
+      int ok = 1;
+      for (int f = 0; f < m.nf && ok; f++) {
+        double dx = p[0] - cen[3 * f], dy = p[1] - cen[3 * f + 1], dz = p[2] - cen[3 * f + 2];
+        if (dx * dx + dy * dy + dz * dz > reach2) continue;
+        if (pt_tri_dist2(p, &pos[3 * m.f[3 * f]], &pos[3 * m.f[3 * f + 1]], &pos[3 * m.f[3 * f + 2]]) < depth2) ok = 0;
+      }
+      if (!ok) continue;
+      for (int k = 0; k < 3; k++) qx[3 * j + k] = p[k];
+      qq[j] = sigma * rng_normal(&rng);
+      j++;
+    }
+    double s = 0.0;
+    for (int j = 0; j < nq; j++) s += qq[j];
+    double shift = (nearbyint(s) - s) / nq;
+    for (int j = 0; j < nq; j++) qq[j] += shift;
+    free(cen);
+  } else {
+    usage();
+  }
+
+  FILE *fp = open_out(base, ".vert");
+  fprintf(fp, "# fbmesh %s level=%d R=%.17g amp=%.17g lmax=%d seed=%llu\n", kind, level, R, amp,
+          shape.lmax, (unsigned long long)seed);
+  fprintf(fp, "#faces  #sphere density probe_r\n");
+  fprintf(fp, "%d %d %.2f %.2f\n", m.nv, nq, 1.0, 1.4);
+  for (int i = 0; i < m.nv; i++)
+    fprintf(fp, "%.17g %.17g %.17g %.17g %.17g %.17g 0 %d 2\n", pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
+            nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 1);
+  fclose(fp);
+  fp = open_out(base, ".face");
+  fprintf(fp, "# fbmesh %s level=%d\n", kind, level);
+  fprintf(fp, "#faces  #sphere density probe_r\n");
+  fprintf(fp, "%d %d %.2f %.2f\n", m.nf, nq, 1.0, 1.4);
+  for (int f = 0; f < m.nf; f++) fprintf(fp, "%d %d %d 1 1\n", m.f[3 * f] + 1, m.f[3 * f + 1] + 1, m.f[3 * f + 2] + 1);
+  fclose(fp);
+  fp = open_out(base, ".pqr");
+  for (int j = 0; j < nq; j++)
+    fprintf(fp, "ATOM %6d  C   CHG A%4d    %.17g %.17g %.17g %.17g 1.5000\n", j + 1, j + 1, qx[3 * j],
+            qx[3 * j + 1], qx[3 * j + 2], qq[j]);
+  fprintf(fp, "END\n");
+  fclose(fp);
+  printf("fbmesh: %s vertices=%d faces=%d charges=%d\n", base, m.nv, m.nf, nq);
+  return 0;
+}
