@@ -1,0 +1,372 @@
+"""fabipb_amd - Python host mirror of the fabipb MI355X build.
+
+The product is C and HIP: lib/libfabipb_host.so (mesh input, octree,
+interaction lists, charge tree) and lib/libfabipb_hip.so (the device operator
+behind the flat C ABI of include/fabipb_hip.h).  This module binds both with
+ctypes and strings them together the way the reference driver
+src/fabipb.c:804-1229 does: load panels -> tree -> operator -> RHS -> GMRES ->
+panel-tree energy.  PyTorch only supplies device memory and the stream.
+
+The HIP library must be built (make -C cuda-fabipb_amd); nothing here falls
+back to a CPU path.
+"""
+import ctypes as C
+import math
+import os
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+BINDIR = os.path.join(HERE, "bin")
+ENERGY_SCALE = 2.0 * math.pi * 332.0716   # src/fabipb.c:810, 1207
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class FabipbError(RuntimeError):
+    pass
+
+
+_host = None
+_hip = None
+
+
+def _load(name):
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise FabipbError(f"{path} is not built; run `make -C cuda-fabipb_amd`")
+    return C.CDLL(path)
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        L = _load("libfabipb_host.so")
+        L.fb_mesh_load.restype = _vp
+        L.fb_mesh_load.argtypes = [C.c_char_p, C.c_int]
+        L.fb_mesh_free.argtypes = [_vp]
+        L.fb_host_error.restype = C.c_char_p
+        L.fb_tree_build.restype = _vp
+        L.fb_tree_build.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.fb_tree_free.argtypes = [_vp]
+        L.fb_ctree_build.restype = _vp
+        L.fb_ctree_build.argtypes = [_vp, C.c_int]
+        L.fb_ctree_free.argtypes = [_vp]
+        for f in ("fb_tree_array", "fb_ctree_array", "fb_mesh_array"):
+            getattr(L, f).restype = _vp
+            getattr(L, f).argtypes = [_vp, C.c_char_p, _ip, _ip]
+        L.fb_charge_tree_order.restype = C.c_int
+        L.fb_charge_tree_order.argtypes = [_vp, C.c_int]
+        L.fb_should_use_tree_rhs.restype = C.c_int
+        L.fb_should_use_tree_rhs.argtypes = [C.c_int, C.c_int, C.c_int]
+        _host = L
+    return _host
+
+
+def hip_lib():
+    global _hip
+    if _hip is None:
+        L = _load("libfabipb_hip.so")
+        L.fbh_available.restype = C.c_int
+        L.fbh_last_error.restype = C.c_char_p
+        L.fbh_create.restype = _vp
+        L.fbh_create.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int,
+                                 C.c_int, _dp, _dp, _dp,
+                                 C.c_int, C.c_int, _ip, _ip,
+                                 C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _ip,
+                                 C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _vp]
+        L.fbh_destroy.argtypes = [_vp]
+        L.fbh_matvec_dev.restype = C.c_int
+        L.fbh_matvec_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp]
+        L.fbh_matvec.restype = C.c_int
+        L.fbh_matvec.argtypes = [_vp, C.c_double, _dp, C.c_double, _dp]
+        L.fbh_near_apply.restype = C.c_int
+        L.fbh_near_apply.argtypes = [_vp, C.c_double, _dp, _dp]
+        L.fbh_far_apply.restype = C.c_int
+        L.fbh_far_apply.argtypes = [_vp, C.c_double, _dp, C.c_double, _dp]
+        L.fbh_expansions.restype = C.c_int
+        L.fbh_expansions.argtypes = [_vp, _dp, _dp]
+        L.fbh_q2m_matrices.restype = C.c_int
+        L.fbh_q2m_matrices.argtypes = [_vp, _dp, _dp]
+        L.fbh_near_coeff.restype = C.c_int
+        L.fbh_near_coeff.argtypes = [_vp, C.c_int, C.c_int, _dp]
+        L.fbh_psolve_diag_dev.restype = C.c_int
+        L.fbh_psolve_diag_dev.argtypes = [_vp, _vp, _vp]
+        L.fbh_gmres_dev.restype = C.c_int
+        L.fbh_gmres_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
+                                    _ip, _dp, _dp, _ip]
+        L.fbh_rhs.restype = C.c_int
+        L.fbh_rhs.argtypes = [_vp, C.c_int, _dp, _dp, _dp]
+        L.fbh_rhs_dev.restype = C.c_int
+        L.fbh_rhs_dev.argtypes = [_vp, C.c_int, _dp, _dp, _vp]
+        L.fbh_energy_dev.restype = C.c_int
+        L.fbh_energy_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip,
+                                     _ip, _ip, _ip, _ip, C.c_double, _vp, _dp]
+        L.fbh_profile.restype = C.c_int
+        L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
+        L.fbh_stats.restype = C.c_int
+        L.fbh_stats.argtypes = [_vp, C.POINTER(C.c_longlong)]
+        L.fbh_stream.restype = _vp
+        L.fbh_stream.argtypes = [_vp]
+        _hip = L
+    return _hip
+
+
+def _hip_err():
+    return hip_lib().fbh_last_error().decode()
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _arr(fn, h, name):
+    cnt, isd = C.c_int(0), C.c_int(0)
+    p = fn(h, name.encode(), C.byref(cnt), C.byref(isd))
+    if not p or cnt.value == 0:
+        return np.zeros(0, np.float64 if isd.value else np.int32)
+    ct = C.c_double if isd.value else C.c_int
+    return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(cnt.value,)).copy()
+
+
+class Mesh:
+    """Panels and charges from <base>.vert/.face/.pqr (src/input.c:320-821)."""
+
+    def __init__(self, base, geometric_normals=False):
+        L = host_lib()
+        self.h = L.fb_mesh_load(base.encode(), int(geometric_normals))
+        if not self.h:
+            raise FabipbError(L.fb_host_error().decode())
+        self.pos = _arr(L.fb_mesh_array, self.h, "pos").reshape(-1, 3)
+        self.chr = _arr(L.fb_mesh_array, self.h, "chr")
+        self.n = len(_arr(L.fb_mesh_array, self.h, "area"))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            host_lib().fb_mesh_free(self.h)
+            self.h = None
+
+
+TREE_KEYS = ["ordMom", "ordM2L", "perm", "vtx", "nrm", "area", "face", "levStart", "levCount",
+             "cubeLevel", "kidStart", "nKids", "parent", "cubeX", "eRad", "leafStart", "leafCount",
+             "nearOff", "nearSrc", "m2lSrc", "m2lDst", "m2lOff"]
+
+
+class Tree:
+    """Octree, interaction lists and flattened layouts (src/gkSetup.c, src/fmm.c:234-524)."""
+
+    def __init__(self, mesh, depth, order=6, height=2, order_mom=0, sep=0.8):
+        L = host_lib()
+        self.h = L.fb_tree_build(mesh.h, depth, height, order, order_mom, sep)
+        if not self.h:
+            raise FabipbError(L.fb_host_error().decode())
+        self.depth, self.height = depth, height
+        self.a = {k: _arr(L.fb_tree_array, self.h, k) for k in TREE_KEYS}
+        self.n = len(self.a["area"])
+        self.ncube = len(self.a["cubeLevel"])
+        self.nleaf = len(self.a["leafStart"])
+        self.max_order = int(max(self.a["ordMom"][height:].max(), self.a["ordM2L"][height:].max()))
+        self.mesh = mesh
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            host_lib().fb_tree_free(self.h)
+            self.h = None
+
+    def oracle_layout(self, kappa, eps1, eps2, qorder):
+        a = self.a
+        return dict(kappa=kappa, eps=eps2 / eps1, eps1=eps1, qorder=qorder,
+                    vtx=a["vtx"].reshape(-1, 9), nrm=a["nrm"].reshape(-1, 3), area=a["area"],
+                    depth=self.depth, height=self.height, maxOrder=self.max_order,
+                    ordMom=a["ordMom"], ordM2L=a["ordM2L"], levStart=a["levStart"],
+                    levCount=a["levCount"], cubeLevel=a["cubeLevel"],
+                    cubeX=a["cubeX"].reshape(-1, 3), kidStart=a["kidStart"], nKids=a["nKids"],
+                    leafStart=a["leafStart"], leafCount=a["leafCount"], nearOff=a["nearOff"],
+                    nearSrc=a["nearSrc"], m2lSrc=a["m2lSrc"], m2lDst=a["m2lDst"])
+
+    def charge_tree(self):
+        """Charge octree over the mesh's charges (src/chargeTree.c:228-307)."""
+        L = host_lib()
+        h = L.fb_ctree_build(self.mesh.h, self.depth)
+        try:
+            keys = ["levStart", "levCount", "level", "kidStart", "nKids", "x", "eRad", "chgStart",
+                    "nChgs", "chgIdx"]
+            ct = {k: _arr(L.fb_ctree_array, h, k) for k in keys}
+        finally:
+            L.fb_ctree_free(h)
+        ct["depth"] = self.depth
+        ct["x"] = ct["x"].reshape(-1, 3)
+        ct["order"] = np.array([L.fb_charge_tree_order(self.h, lev) for lev in range(self.depth + 1)],
+                               np.int32)
+        return ct
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise FabipbError("no GPU visible to torch")
+    return torch
+
+
+class Operator:
+    """The device-resident BIE operator (fbh_create ... fbh_destroy)."""
+
+    def __init__(self, tree, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, stream=None):
+        torch = _torch()
+        L = hip_lib()
+        if not L.fbh_available():
+            raise FabipbError(_hip_err())
+        a = tree.a
+        self.tree, self.kappa, self.eps1, self.eps2, self.qorder = tree, kappa, eps1, eps2, qorder
+        self.n = tree.n
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._keep = a
+        t0 = time.perf_counter()
+        self.h = L.fbh_create(kappa, eps1, eps2, qorder, tree.n, _d(a["vtx"]), _d(a["nrm"]),
+                              _d(a["area"]), tree.depth, tree.height, _i(a["ordMom"]),
+                              _i(a["ordM2L"]), tree.ncube, _i(a["levStart"]), _i(a["levCount"]),
+                              _i(a["cubeLevel"]), _d(a["cubeX"]), _i(a["kidStart"]), _i(a["nKids"]),
+                              _i(a["parent"]), tree.nleaf, _i(a["leafStart"]), _i(a["leafCount"]),
+                              _i(a["nearOff"]), _i(a["nearSrc"]), _i(a["m2lOff"]), _i(a["m2lSrc"]),
+                              C.c_void_p(stream))
+        if not self.h:
+            raise FabipbError(_hip_err())
+        self.setup_s = time.perf_counter() - t0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            hip_lib().fbh_destroy(self.h)
+            self.h = None
+
+    def _ck(self, ok):
+        if not ok:
+            raise FabipbError(_hip_err())
+
+    def stats(self):
+        out = (C.c_longlong * 8)()
+        self._ck(hip_lib().fbh_stats(self.h, out))
+        keys = ["n", "near_interactions", "m2l_pairs", "nmom_max", "nmom_leaf", "device_bytes",
+                "links", "cubes"]
+        return dict(zip(keys, list(out)))
+
+    # host-array calls
+    def matvec(self, x, alpha=1.0, beta=0.0, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(2 * self.n) if y is None else np.array(y, dtype=np.float64)
+        self._ck(hip_lib().fbh_matvec(self.h, alpha, _d(x), beta, _d(y)))
+        return y
+
+    def near_apply(self, x, alpha=1.0, pot=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        pot = np.zeros(2 * self.n) if pot is None else np.array(pot, dtype=np.float64)
+        self._ck(hip_lib().fbh_near_apply(self.h, alpha, _d(x), _d(pot)))
+        return pot
+
+    def far_apply(self, x, alpha=1.0, beta=0.0, pot=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        pot = np.zeros(2 * self.n) if pot is None else np.array(pot, dtype=np.float64)
+        self._ck(hip_lib().fbh_far_apply(self.h, alpha, _d(x), beta, _d(pot)))
+        return pot
+
+    def expansions(self):
+        st = self.stats()
+        nm = st["nmom_max"]
+        mom = np.zeros(self.tree.ncube * 2 * nm)
+        lec = np.zeros(self.tree.ncube * 4 * nm)
+        self._ck(hip_lib().fbh_expansions(self.h, _d(mom), _d(lec)))
+        return mom, lec
+
+    def q2m_matrices(self):
+        nm = self.stats()["nmom_leaf"]
+        q0 = np.zeros(self.n * nm)
+        q1 = np.zeros(self.n * nm)
+        self._ck(hip_lib().fbh_q2m_matrices(self.h, _d(q0), _d(q1)))
+        return q0, q1
+
+    def near_coeff(self, x, y):
+        k = np.zeros(4)
+        self._ck(hip_lib().fbh_near_coeff(self.h, int(x), int(y), _d(k)))
+        return k
+
+    def rhs(self, pos, chr_):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        b = np.zeros(2 * self.n)
+        self._ck(hip_lib().fbh_rhs(self.h, len(chr_), _d(pos), _d(chr_), _d(b)))
+        return b
+
+    # device-tensor calls (torch float64 cuda tensors of length 2n)
+    def matvec_dev(self, x, y, alpha=1.0, beta=0.0):
+        self._ck(hip_lib().fbh_matvec_dev(self.h, alpha, C.c_void_p(x.data_ptr()), beta,
+                                          C.c_void_p(y.data_ptr())))
+
+    def rhs_dev(self, pos, chr_, b):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        self._ck(hip_lib().fbh_rhs_dev(self.h, len(chr_), _d(pos), _d(chr_),
+                                       C.c_void_p(b.data_ptr())))
+
+    def gmres_dev(self, b, x, restart=30, tol=1e-6, maxit=100):
+        it, res, info = C.c_int(0), C.c_double(0.0), C.c_int(0)
+        hist = np.zeros(2 * maxit + 2)
+        self._ck(hip_lib().fbh_gmres_dev(self.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+                                         restart, tol, maxit, C.byref(it), C.byref(res), _d(hist),
+                                         C.byref(info)))
+        return dict(iters=it.value, resid=res.value, info=info.value, hist=hist)
+
+    def energy_dev(self, ct, pos, chr_, sigma, theta=0.8):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        keep = {k: np.ascontiguousarray(v) for k, v in ct.items() if isinstance(v, np.ndarray)}
+        e = C.c_double(0.0)
+        self._ck(hip_lib().fbh_energy_dev(
+            self.h, len(chr_), _d(pos), _d(chr_), len(keep["level"]), _i(keep["level"]),
+            _d(keep["x"]), _d(keep["eRad"]), _i(keep["kidStart"]), _i(keep["nKids"]),
+            _i(keep["chgStart"]), _i(keep["nChgs"]), _i(keep["chgIdx"]), _i(keep["order"]),
+            theta, C.c_void_p(sigma.data_ptr()), C.byref(e)))
+        return e.value
+
+    def profile(self, x, y, iters=10):
+        ms = np.zeros(8)
+        self._ck(hip_lib().fbh_profile(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                                       iters, _d(ms)))
+        return dict(zip(["near", "q2m", "m2m", "m2l", "l2l", "l2p", "combine", "total"], ms))
+
+
+def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,
+          restart=30, maxit=100, height=2, x0="rhs"):
+    """One plain-FP64 solve (src/fabipb.c:1067-1210): returns energies, iterations and times."""
+    torch = _torch()
+    t0 = time.perf_counter()
+    mesh = Mesh(base)
+    tree = Tree(mesh, depth, order=order, height=height, sep=sep)
+    t1 = time.perf_counter()
+    op = Operator(tree, kappa, eps1, eps2, qorder)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    n2 = 2 * tree.n
+    b = torch.empty(n2, dtype=torch.float64, device="cuda")
+    op.rhs_dev(mesh.pos, mesh.chr, b)
+    x = b.clone() if x0 == "rhs" else torch.zeros_like(b)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    g = op.gmres_dev(b, x, restart=restart, tol=tol, maxit=maxit)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    ct = tree.charge_tree()
+    raw = op.energy_dev(ct, mesh.pos, mesh.chr, x)
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    return dict(energy=raw * ENERGY_SCALE, energy_raw=raw, iters=g["iters"], resid=g["resid"],
+                info=g["info"], hist=g["hist"][:g["iters"] + 2], n=tree.n,
+                t_tree=t1 - t0, t_setup=t2 - t1, t_rhs=t3 - t2, t_gmres=t4 - t3, t_energy=t5 - t4,
+                t_solve=t5 - t2, x=x.cpu().numpy(), op=op, tree=tree, mesh=mesh)

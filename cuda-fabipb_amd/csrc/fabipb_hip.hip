@@ -1,0 +1,1788 @@
+// fabipb_hip.hip - MI355X (gfx950) implementation of FABIPB's Galerkin BIE
+// matvec, right-hand side, solvation energy and GMRES, behind the flat C ABI
+// of include/fabipb_hip.h.
+//
+// Device layout (HBM, structure of arrays, panels in leaf depth-first order):
+//   pnl[n]          22 doubles per panel: vertices, edges a0=v2-v1 a1=v0-v2
+//                   a2=v1-v0 (src/input.c:743-747), normal, area
+//   Q0,Q1[n][nmL]   leaf Q2M matrices, one contiguous column per panel
+//                   (Q2M0 moments, Q2M1 noments; L2P uses the same, fmm.c:457)
+//   mom[nCube][2][nm], lec[nCube][4][nm]   multipole / local expansions
+//   coef            nearfield Galerkin coefficients KER[4] per (dst, src)
+//                   panel pair, row-major per destination leaf: for leaf D the
+//                   rows are its panels, the columns the concatenated panels
+//                   of D's near leaves (srcList), so a row is one contiguous
+//                   stream of 32-byte records
+//
+// Kernels (FP64 throughout):
+//   k_q2m_mats   setup: leaf moment/noment columns, one wave per panel
+//   k_near_build setup: KER4 Galerkin quadrature per pair (disjoint Duffy rule
+//                and the Sauter-Schwab vertex/edge/self rules), one block per
+//                destination panel
+//   k_near_apply one block per 16-row tile of a destination leaf; the
+//                sources' densities are staged in LDS once per tile, each wave
+//                streams four rows of coefficients (HBM-bound)
+//   k_q2m, k_m2m, k_m2l, k_l2l, k_l2p   the FMM passes, thread per multi-index
+//   k_combine    y = alpha*(D x - far - near) + beta*y
+//   k_rhs        direct right-hand side, thread per panel, charges in LDS
+//   k_energy     panel-tree energy, one wave per panel walking the charge tree
+//   GMRES helpers: fixed-grid deterministic dot products, MGS update,
+//                normalisation, diagonal preconditioner, x += V y
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/fabipb_hip.h"
+
+#define NMOM(p) (((p) + 1) * ((p) + 2) * ((p) + 3) / 6)
+#define MAXP 12           // largest expansion order supported (nMom(13) = 560)
+#define NEAR_ROWS 16      // rows per nearfield apply tile
+#define NEAR_CHUNK 2048   // source columns staged in LDS per pass (32 KiB)
+#define RED_BLOCKS 256    // fixed grid of the deterministic reductions
+
+static char gErr[1024];
+extern "C" const char *fbh_last_error(void) { return gErr; }
+static void setErr(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(gErr, sizeof(gErr), fmt, ap);
+  va_end(ap);
+}
+#define HCK(call)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      setErr("%s failed at %s:%d: %s", #call, __FILE__, __LINE__, hipGetErrorString(e_)); \
+      return 0;                                                                    \
+    }                                                                              \
+  } while (0)
+
+static const double kFourPiI = 1.0 / (4 * M_PI);
+
+// Gauss-Legendre table on [0,1], orders 1..10 (the reference's 15-digit table,
+// src/numQuad.c:221-294).
+static const double hT[11][10] = {
+    {0},
+    {0.500000000000000},
+    {0.211324865405187, 0.788675134594813},
+    {0.112701665379258, 0.500000000000000, 0.887298334620742},
+    {0.069431844202974, 0.330009478207572, 0.669990521792428, 0.930568155797026},
+    {0.046910077030668, 0.230765344947158, 0.500000000000000, 0.769234655052841, 0.953089922969332},
+    {0.033765242898424, 0.169395306766868, 0.380690406958402, 0.619309593041598, 0.830604693233132,
+     0.966234757101576},
+    {0.025446043828621, 0.129234407200303, 0.297077424311301, 0.500000000000000, 0.702922575688699,
+     0.870765592799697, 0.974553956171379},
+    {0.019855071751232, 0.101666761293187, 0.237233795041836, 0.408282678752175, 0.591717321247825,
+     0.762766204958164, 0.898333238706813, 0.980144928248768},
+    {0.015919880246187, 0.081984446336682, 0.193314283649705, 0.337873288298095, 0.500000000000000,
+     0.662126711701904, 0.806685716350295, 0.918015553663318, 0.984080119753813},
+    {0.013046735741414, 0.067468316655508, 0.160295215850488, 0.283302302935376, 0.425562830509184,
+     0.574437169490816, 0.716697697064624, 0.839704784149512, 0.932531683344492, 0.986953264258586}};
+static const double hW[11][10] = {
+    {0},
+    {1.000000000000000},
+    {0.500000000000000, 0.500000000000000},
+    {0.277777777777778, 0.444444444444444, 0.277777777777778},
+    {0.173927422568727, 0.326072577431273, 0.326072577431273, 0.173927422568727},
+    {0.118463442528095, 0.239314335249683, 0.284444444444444, 0.239314335249683, 0.118463442528095},
+    {0.085662246189585, 0.180380786524070, 0.233956967286346, 0.233956967286346, 0.180380786524069,
+     0.085662246189585},
+    {0.064742483084435, 0.139852695744638, 0.190915025252560, 0.208979591836735, 0.190915025252559,
+     0.139852695744638, 0.064742483084435},
+    {0.050614268145188, 0.111190517226687, 0.156853322938944, 0.181341891689181, 0.181341891689181,
+     0.156853322938944, 0.111190517226687, 0.050614268145188},
+    {0.040637194180787, 0.090324080347429, 0.130305348201467, 0.156173538520002, 0.165119677500630,
+     0.156173538520002, 0.130305348201468, 0.090324080347429, 0.040637194180787},
+    {0.033335672154344, 0.074725674575290, 0.109543181257991, 0.134633359654998, 0.147762112357376,
+     0.147762112357376, 0.134633359654998, 0.109543181257991, 0.074725674575290, 0.033335672154344}};
+
+struct Pnl {
+  double v[9];   // v0 v1 v2
+  double a[9];   // a0 a1 a2
+  double nr[3];
+  double area;
+};
+
+struct QTab {     // quadrature table for one order, passed by value
+  double t[10], w[10];
+};
+
+// ------------------------------------------------------------------ kernels
+
+// src/kernel.c:109-147 (kernelKER4): nX target normal, nY source normal
+__device__ __forceinline__ void ker4(double kappa, double eps, const double *x, const double *nX,
+                                     const double *nY, double *y) {
+  double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  double r = sqrt(r2);
+  double expKa = exp(-kappa * r);
+  double ip0 = nX[0] * nY[0] + nX[1] * nY[1] + nX[2] * nY[2];
+  double ipX = nX[0] * x[0] + nX[1] * x[1] + nX[2] * x[2];
+  double ipY = nY[0] * x[0] + nY[1] * x[1] + nY[2] * x[2];
+  double G0 = 1.0 / r * kFourPiI;
+  double Gk = expKa * G0;
+  double coef = (kappa * r + 1.0) * expKa;
+  double dG0dy = ipY * G0 / r2;
+  double dGkdy = coef * dG0dy;
+  double dG0dx = ipX * G0 / r2;
+  double dGkdx = coef * dG0dx;
+  double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) / r2;
+  double ddGk = coef * ddG0 - kappa * kappa * expKa * ipX * dG0dy;
+  y[0] = G0 - Gk;
+  y[1] = eps * dGkdy - dG0dy;
+  y[2] = dGkdx / eps - dG0dx;
+  y[3] = ddGk - ddG0;
+}
+
+struct KC {
+  double kappa, eps;
+  const double *nX, *nY;
+};
+
+// src/numQuad.c:586-618 (pnlNil0): disjoint panels, Duffy squares
+__device__ void dNil0(const KC &K, const QTab &Q, int q, const double *r0, const double *ax2, const double *ax0,
+                      const double *ay1, const double *ay2, double *in) {
+  double r[3], r1[3], f[4], iy[4];
+  for (int i = 0; i < 4; i++) in[i] = 0.0;
+  for (int ix = 0; ix < q; ix++)
+    for (int jx = 0; jx < q; jx++) {
+      for (int k = 0; k < 3; k++) r1[k] = r0[k] + Q.t[ix] * (ax2[k] + Q.t[jx] * ax0[k]);
+      for (int i = 0; i < 4; i++) iy[i] = 0.0;
+      for (int a = 0; a < q; a++)
+        for (int b = 0; b < q; b++) {
+          for (int k = 0; k < 3; k++) r[k] = r1[k] - Q.t[a] * (ay1[k] + Q.t[b] * ay2[k]);
+          ker4(K.kappa, K.eps, r, K.nX, K.nY, f);
+          for (int i = 0; i < 4; i++) iy[i] += f[i] * Q.t[a] * Q.w[a] * Q.w[b];
+        }
+      for (int i = 0; i < 4; i++) in[i] += iy[i] * Q.t[ix] * Q.w[ix] * Q.w[jx];
+    }
+}
+
+// src/numQuad.c:535-571 (pnlOne0): one common vertex
+__device__ void dOne0(const KC &K, const QTab &Q, int q, const double *a2, const double *a0, const double *b2,
+                      const double *b0, double *in) {
+  double r1[3], r2[3], f1[4], f2[4], iy[4];
+  for (int i = 0; i < 4; i++) in[i] = 0.0;
+  for (int ix = 0; ix < q; ix++) {
+    double omg = Q.t[ix];
+    for (int i = 0; i < 4; i++) iy[i] = 0.0;
+    for (int a = 0; a < q; a++) {
+      double eta1 = Q.t[a];
+      for (int b = 0; b < q; b++) {
+        double eta2 = Q.t[b], wij = Q.w[a] * Q.w[b] * eta1;
+        for (int c = 0; c < q; c++) {
+          double eta3 = Q.t[c];
+          for (int k = 0; k < 3; k++) {
+            r1[k] = omg * (a2[k] + eta2 * a0[k] - eta1 * (b2[k] + eta3 * b0[k]));
+            r2[k] = omg * (eta1 * (a2[k] + eta3 * a0[k]) - (b2[k] + eta2 * b0[k]));
+          }
+          ker4(K.kappa, K.eps, r1, K.nX, K.nY, f1);
+          ker4(K.kappa, K.eps, r2, K.nX, K.nY, f2);
+          for (int i = 0; i < 4; i++) iy[i] += (f1[i] + f2[i]) * Q.w[c] * wij;
+        }
+      }
+    }
+    for (int i = 0; i < 4; i++) in[i] += iy[i] * (omg * omg) * omg * Q.w[ix];
+  }
+}
+
+// src/numQuad.c:470-518 (pnlTwo0): one common edge
+__device__ void dTwo0(const KC &K, const QTab &Q, int q, const double *a, const double *b, const double *c,
+                      double *in) {
+  double r[6][3], f[4], iy[4];
+  for (int i = 0; i < 4; i++) in[i] = 0.0;
+  for (int ix = 0; ix < q; ix++)
+    for (int jx = 0; jx < q; jx++) {
+      double omg1 = Q.t[ix] * Q.t[jx];
+      for (int i = 0; i < 4; i++) iy[i] = 0.0;
+      for (int ia = 0; ia < q; ia++) {
+        double eta1 = Q.t[ia], fta1 = 1.0 - eta1;
+        for (int jy = 0; jy < q; jy++) {
+          double eta2 = Q.t[jy], eta3 = eta1 * eta2, fta2 = 1.0 - eta2, fta3 = 1.0 - eta3;
+          for (int k = 0; k < 3; k++) {
+            r[0][k] = omg1 * (eta1 * (a[k] + eta2 * b[k]) + fta1 * c[k]);
+            r[1][k] = omg1 * (eta1 * (b[k] + eta2 * a[k]) + fta3 * c[k]);
+            r[2][k] = omg1 * (fta1 * a[k] + b[k] + eta3 * c[k]);
+            r[3][k] = omg1 * (eta1 * (eta2 * b[k] - fta2 * a[k]) + c[k]);
+            r[4][k] = omg1 * (eta3 * b[k] - fta3 * a[k] + eta1 * c[k]);
+            r[5][k] = omg1 * (eta1 * b[k] - fta1 * a[k] + eta3 * c[k]);
+          }
+          // fcn1+fcn3+fcn4+fcn2+fcn5+fcn6, the reference's summation order
+          double s[4] = {0, 0, 0, 0};
+          const int ord[6] = {0, 2, 3, 1, 4, 5};
+          for (int m = 0; m < 6; m++) {
+            ker4(K.kappa, K.eps, r[ord[m]], K.nX, K.nY, f);
+            for (int i = 0; i < 4; i++) s[i] = (m == 0) ? f[i] : s[i] + f[i];
+          }
+          for (int i = 0; i < 4; i++) iy[i] += (s[i] * eta1) * Q.w[ia] * Q.w[jy];
+        }
+      }
+      for (int i = 0; i < 4; i++) in[i] += iy[i] * Q.t[jx] * (omg1 * omg1) * Q.w[jx] * Q.w[ix];
+    }
+}
+
+// src/numQuad.c:401-447 (pnlThr0): the panel with itself
+__device__ void dThr0(const KC &K, const QTab &Q, int q, const double *a2, const double *a0, double *in) {
+  double r[6][3], f[4], iy[4];
+  for (int i = 0; i < 4; i++) in[i] = 0.0;
+  for (int a = 0; a < q; a++)
+    for (int b = 0; b < q; b++) {
+      double omg1 = Q.t[a] * (1.0 - Q.t[b]);
+      for (int c = 0; c < q; c++) {
+        for (int i = 0; i < 4; i++) iy[i] = 0.0;
+        for (int d = 0; d < q; d++) {
+          double eta1 = Q.t[d], fta1 = 1.0 - eta1;
+          for (int k = 0; k < 3; k++) {
+            r[0][k] = omg1 * (eta1 * a2[k] + a0[k]);
+            r[1][k] = omg1 * (fta1 * a0[k] - eta1 * a2[k]);
+            r[2][k] = omg1 * (a2[k] + eta1 * a0[k]);
+            r[3][k] = -omg1 * (eta1 * a2[k] + a0[k]);
+            r[4][k] = omg1 * (eta1 * a2[k] - fta1 * a0[k]);
+            r[5][k] = -omg1 * (a2[k] + eta1 * a0[k]);
+          }
+          double s[4] = {0, 0, 0, 0};
+          const int ord[6] = {0, 2, 3, 1, 4, 5};
+          for (int m = 0; m < 6; m++) {
+            ker4(K.kappa, K.eps, r[ord[m]], K.nX, K.nY, f);
+            for (int i = 0; i < 4; i++) s[i] = (m == 0) ? f[i] : s[i] + f[i];
+          }
+          for (int i = 0; i < 4; i++) iy[i] += s[i] * Q.w[d];
+        }
+        for (int i = 0; i < 4; i++) in[i] += iy[i] * omg1 * (Q.t[a] * Q.t[a]) * Q.t[b] * Q.w[a] * Q.w[b] * Q.w[c];
+      }
+    }
+}
+
+__device__ __forceinline__ bool veq(const double *a, const double *b) {
+  return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+// src/numQuad.c:312-353 (nrCommonVtx): exact coordinate equality
+__device__ int commonVtx(const double *p, const double *q, int *ix, int *iy) {
+  int n1, n2 = 0, cnt = 0;
+  bool flag = false;
+  ix[0] = iy[0] = 0; ix[1] = iy[1] = 1; ix[2] = iy[2] = 2;
+  for (n1 = 0; n1 < 3; n1++) {
+    flag = false;
+    for (n2 = 0; n2 < 3; n2++)
+      if (veq(&p[3 * n1], &q[3 * n2])) { flag = true; cnt++; break; }
+    if (flag) break;
+  }
+  if (!flag) return 0;
+  ix[0] = n1 % 3; ix[1] = (n1 + 1) % 3; ix[2] = (n1 + 2) % 3;
+  iy[0] = n2 % 3; iy[1] = (n2 + 1) % 3; iy[2] = (n2 + 2) % 3;
+  if (veq(&p[3 * ix[1]], &q[3 * iy[2]])) cnt = 2;
+  if (veq(&p[3 * ix[2]], &q[3 * iy[1]])) cnt = -2;
+  return cnt;
+}
+
+// src/numQuad.c:629-697 (panelIA0): KER = 4 A_x A_y * (rule for the case)
+__device__ void panelIA0(const Pnl &X, const Pnl &Y, bool same, double kappa, double eps, const QTab &Q, int q,
+                         double *out) {
+  KC K = {kappa, eps, X.nr, Y.nr};
+  int ix[3], iy[3];
+  int nv = same ? 3 : commonVtx(X.v, Y.v, ix, iy);
+  double in[4];
+  if (nv == 0) {
+    double r0[3] = {X.v[0] - Y.v[0], X.v[1] - Y.v[1], X.v[2] - Y.v[2]};
+    dNil0(K, Q, q, r0, &X.a[6], &X.a[0], &Y.a[6], &Y.a[0], in);
+  } else if (nv == 1) {
+    dOne0(K, Q, q, &X.a[3 * ix[2]], &X.a[3 * ix[0]], &Y.a[3 * iy[2]], &Y.a[3 * iy[0]], in);
+  } else if (nv == 2) {
+    dTwo0(K, Q, q, &X.a[3 * ix[2]], &X.a[3 * ix[0]], &Y.a[3 * iy[0]], in);
+  } else if (nv == -2) {
+    double b0[3], a0[3];
+    for (int k = 0; k < 3; k++) { b0[k] = -Y.a[3 * iy[0] + k]; a0[k] = -X.a[3 * ix[0] + k]; }
+    dTwo0(K, Q, q, &X.a[3 * ix[1]], b0, a0, in);
+  } else {
+    dThr0(K, Q, q, &X.a[6], &X.a[0], in);
+  }
+  for (int i = 0; i < 4; i++) out[i] = 4.0 * X.area * Y.area * in[i];
+}
+
+// ----------------------------------------------------------- nearfield
+
+// one block per destination panel; threads over its source columns
+__global__ __launch_bounds__(256) void k_near_build(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
+                                                    const int *__restrict__ leafStart, const int *__restrict__ srcOff,
+                                                    const int *__restrict__ ncolArr,
+                                                    const long long *__restrict__ coefOff,
+                                                    const int *__restrict__ srcList, double4 *__restrict__ coef,
+                                                    double kappa, double eps, QTab Q, int q) {
+  int x = blockIdx.x;
+  int D = leafOf[x];
+  int row = x - leafStart[D];
+  int ncol = ncolArr[D];
+  const Pnl X = pnl[x];
+  double4 *out = coef + coefOff[D] + (long long)row * ncol;
+  const int *src = srcList + srcOff[D];
+  for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
+    int y = src[j];
+    const Pnl Y = pnl[y];
+    double k[4];
+    panelIA0(X, Y, x == y, kappa, eps, Q, q, k);
+    out[j] = make_double4(k[0], k[1], k[2], k[3]);
+  }
+}
+
+__device__ __forceinline__ double waveSum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// tile = (destination leaf, first row); 4 waves x 4 rows; out[x] = sum over
+// sources of K0*sgm_n + K1*sgm, out[x+n] = K2*sgm_n + K3*sgm
+__global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
+                                                    const int *__restrict__ leafCount,
+                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                                    const long long *__restrict__ coefOff,
+                                                    const int *__restrict__ srcList,
+                                                    const double4 *__restrict__ coef, const double *__restrict__ x,
+                                                    double *__restrict__ out, int n) {
+  __shared__ double sS[NEAR_CHUNK], sN[NEAR_CHUNK];
+  const int2 t = tiles[blockIdx.x];
+  const int D = t.x, r0 = t.y;
+  const int rows = min(NEAR_ROWS, leafCount[D] - r0);
+  const int ncol = ncolArr[D];
+  const int *src = srcList + srcOff[D];
+  const double4 *cbase = coef + coefOff[D];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
+  for (int c0 = 0; c0 < ncol; c0 += NEAR_CHUNK) {
+    const int cn = min(NEAR_CHUNK, ncol - c0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < cn; j += 256) {
+      int y = src[c0 + j];
+      sS[j] = x[y];
+      sN[j] = x[y + n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = w + 4 * r;
+      if (row < rows) {
+        const double4 *cr = cbase + (long long)(r0 + row) * ncol + c0;
+        double s0 = 0.0, s1 = 0.0;
+        int j = lane;
+        for (; j + 64 < cn; j += 128) {
+          double4 c = cr[j], d = cr[j + 64];
+          double p = sS[j], m = sN[j], p2 = sS[j + 64], m2 = sN[j + 64];
+          s0 += c.x * m + c.y * p;
+          s1 += c.z * m + c.w * p;
+          s0 += d.x * m2 + d.y * p2;
+          s1 += d.z * m2 + d.w * p2;
+        }
+        if (j < cn) {
+          double4 c = cr[j];
+          double p = sS[j], m = sN[j];
+          s0 += c.x * m + c.y * p;
+          s1 += c.z * m + c.w * p;
+        }
+        a0[r] += s0;
+        a1[r] += s1;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int row = w + 4 * r;
+    double s0 = waveSum(a0[r]), s1 = waveSum(a1[r]);
+    if (row < rows && lane == 0) {
+      int p = leafStart[D] + r0 + row;
+      out[p] = s0;
+      out[p + n] = s1;
+    }
+  }
+}
+
+// -------------------------------------------------- multi-index helpers
+
+// mi: .x = i1 | i2<<8 | i3<<16 | deg<<24, .y = axis | fac<<8, .z = j1, .w = j2 (-1: none);
+// j1/j2 are local indices of alpha-e_axis / alpha-2e_axis inside their degree
+struct DevTabs {
+  const int4 *mi;      // nMom(P)
+  const int4 *ep;      // nMom(maxOrder): idx(a+e1), idx(a+e2), idx(a+e3), sgn
+  const double *ifact3;
+  const int *convStart, *conv;   // (beta | (alpha+beta) << 16) per alpha
+  const int *m2mStart, *m2m;     // (alpha-k | k << 16) per alpha
+};
+
+__device__ __forceinline__ int deg_of(int4 m) { return (m.x >> 24) & 0xff; }
+
+// src/kernel.c:84-104 (kernelDG0DGk)
+__device__ void radial(double kappa, double r, int p, double *G0, int s0, double *Gk, int sk) {
+  double kappa2 = kappa * kappa, rexp = exp(-kappa * r), r1 = 1.0 / r, r2 = r1 * r1, nr2 = -r2;
+  double g0m, g0 = r1 * kFourPiI, gkm, gk = rexp * g0;
+  G0[0] = g0; Gk[0] = gk;
+  if (p < 1) return;
+  g0m = g0; gkm = gk;
+  g0 = nr2 * g0m;
+  gk = -gkm * (r2 + kappa * r1);
+  G0[s0] = g0; Gk[sk] = gk;
+  for (int k = 1; k < p; k++) {
+    double g0n = (2 * k + 1) * nr2 * g0;
+    double gkn = r2 * (kappa2 * gkm - (2 * k + 1) * gk);
+    gkm = gk; g0 = g0n; gk = gkn;
+    G0[(k + 1) * s0] = g0; Gk[(k + 1) * sk] = gk;
+  }
+}
+
+// Derivative tensors of G0 and Gk at r up to total degree p (src/expan.c:138-209),
+// computed by the whole block degree by degree: block d holds entries
+// (q, j), q = 0..p-d, j < (d+1)(d+2)/2, at offset off(d) + q*len(d) + j.
+// Leaves dG[0][alpha] in o0/ok.  Ends with a barrier.
+__device__ void derivs_block(double kappa, int p, const double *r, double *s0, double *sk, double *o0, double *ok,
+                             const int4 *mi, int tid, int nt) {
+  if (tid == 0) {
+    double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    radial(kappa, rn, p, s0, 1, sk, 1);
+    o0[0] = s0[0];
+    ok[0] = sk[0];
+  }
+  __syncthreads();
+  int offPrev2 = 0, offPrev = 0, off = p + 1;   // block offsets of d-2, d-1, d
+  int lenPrev2 = 0, lenPrev = 1;
+  for (int d = 1; d <= p; d++) {
+    int len = (d + 1) * (d + 2) / 2, cnt = (p - d + 1) * len, base = NMOM(d - 1);
+    for (int e = tid; e < cnt; e += nt) {
+      int q = e / len, j = e - q * len;
+      int4 m = mi[base + j];
+      int axis = m.y & 0xff, fac = m.y >> 8;
+      double v0 = s0[offPrev + (q + 1) * lenPrev + m.z] * r[axis];
+      double vk = sk[offPrev + (q + 1) * lenPrev + m.z] * r[axis];
+      if (m.w >= 0) {
+        v0 = v0 + s0[offPrev2 + (q + 1) * lenPrev2 + m.w] * fac;
+        vk = vk + sk[offPrev2 + (q + 1) * lenPrev2 + m.w] * fac;
+      }
+      s0[off + q * len + j] = v0;
+      sk[off + q * len + j] = vk;
+      if (q == 0) { o0[base + j] = v0; ok[base + j] = vk; }
+    }
+    __syncthreads();
+    offPrev2 = offPrev; lenPrev2 = lenPrev;
+    offPrev = off; lenPrev = len;
+    off += cnt;
+  }
+}
+__host__ __device__ constexpr int derivScratch(int p) {   // sum_d (p-d+1)(d+1)(d+2)/2
+  return (p + 1) * (p + 2) * (p + 3) * (p + 4) / 24;
+}
+
+// src/moments.c:83-103 (setupConVect) for one multi-index: t^a / a!
+__device__ __forceinline__ double conVect1(const double *tr, int4 m, double ifact3) {
+  int i1 = m.x & 0xff, i2 = (m.x >> 8) & 0xff, i3 = (m.x >> 16) & 0xff;
+  double f1 = 1.0, f2 = 1.0, f3 = 1.0;
+  for (int i = 0; i < i1; i++) f1 = f1 * tr[0];
+  for (int i = 0; i < i2; i++) f2 = f2 * tr[1];
+  for (int i = 0; i < i3; i++) f3 = f3 * tr[2];
+  double v = f1 * f2 * f3;
+  return v * ifact3;
+}
+
+// ------------------------------------------------------------- Q2M matrices
+
+// src/moments.c:148-236 (calcOneMoment0, calcOneNoment, calcMoments0): one
+// wave per panel, lanes over multi-indices; columns translated to the leaf centre
+__global__ __launch_bounds__(64) void k_q2m_mats(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
+                                                 const double *__restrict__ cubeX, DevTabs T, int order,
+                                                 const double *__restrict__ fact, double *__restrict__ Q0,
+                                                 double *__restrict__ Q1) {
+  extern __shared__ double sm[];
+  const int nm = NMOM(order);
+  double *c1 = sm, *c2 = sm + nm, *M = sm + 2 * nm, *cv = sm + 3 * nm, *loc = sm + 4 * nm;
+  const int p = blockIdx.x, lane = threadIdx.x;
+  const Pnl P = pnl[p];
+  const double *ctr = &cubeX[3 * leafOf[p]];
+  double a1[3], a2[3], tr[3];
+  for (int i = 0; i < 3; i++) { a1[i] = P.v[3 + i] - P.v[i]; a2[i] = P.v[6 + i] - P.v[i]; }
+  for (int i = 0; i < 3; i++) tr[i] = P.v[i] - ctr[i];
+  const double area2 = 2.0 * P.area;
+  for (int job = 0; job < 2; job++) {
+    const int order1 = job == 0 ? order : order - 1, nm1 = NMOM(order1);
+    for (int a = lane; a < nm1; a += 64) {
+      int4 m = T.mi[a];
+      int d = deg_of(m);
+      c1[a] = conVect1(a1, m, T.ifact3[a]) * fact[d];
+      c2[a] = conVect1(a2, m, T.ifact3[a]) * fact[d];
+      cv[a] = conVect1(tr, m, T.ifact3[a]);
+    }
+    for (int a = nm1 + lane; a < nm; a += 64) cv[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
+    __syncthreads();
+    // convolution (moments.c:56-78) of c1, c2 up to order1, times 2A/(n+2)!
+    for (int a = lane; a < nm1; a += 64) {
+      double tmp = 0.0;
+      for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
+        int pr = T.m2m[e];
+        tmp += c1[pr & 0xffff] * c2[pr >> 16];
+      }
+      M[a] = tmp * (area2 * fact[MAXP + 3 + deg_of(T.mi[a]) + 2]);   // fact[MAXP+3+k] holds 1/k!
+    }
+    __syncthreads();
+    if (job == 0) {
+      for (int a = lane; a < nm; a += 64) loc[a] = M[a];
+    } else {
+      // noments (moments.c:115-133) as a gather: N[b] = n0 M[b-e1] + n1 M[b-e2] + n2 M[b-e3]
+      for (int b = lane; b < nm; b += 64) {
+        int4 m = T.mi[b];
+        int i1 = m.x & 0xff, i2 = (m.x >> 8) & 0xff, i3 = (m.x >> 16) & 0xff;
+        double v = 0.0;
+        bool any = false;
+        if (i1 > 0) { v = P.nr[0] * M[T.ep[b].x]; any = true; }   // ep[b].x holds idx(b - e1) here
+        if (i2 > 0) { double t = P.nr[1] * M[T.ep[b].y]; v = any ? v + t : t; any = true; }
+        if (i3 > 0) { double t = P.nr[2] * M[T.ep[b].z]; v = any ? v + t : t; any = true; }
+        loc[b] = v;
+      }
+    }
+    __syncthreads();
+    // translation to the leaf centre: column = conv(cv, loc) (moments.c:228-233)
+    double *Qo = job == 0 ? Q0 : Q1;
+    for (int a = lane; a < nm; a += 64) {
+      double tmp = 0.0;
+      for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
+        int pr = T.m2m[e];
+        tmp += cv[pr & 0xffff] * loc[pr >> 16];
+      }
+      Qo[(long long)p * nm + a] = tmp;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- FMM passes
+
+// mom_pot = Q2M1 sgm, mom_dpdn = Q2M0 sgm_n (src/fmm.c:745-753)
+__global__ void k_q2m(const double *__restrict__ Q0, const double *__restrict__ Q1, const int *__restrict__ leafStart,
+                      const int *__restrict__ leafCount, const double *__restrict__ x, double *__restrict__ mom,
+                      int n, int nm, int nmMax) {
+  const int L = blockIdx.x, a = threadIdx.x;
+  if (a >= nm) return;
+  const int s = leafStart[L], c = leafCount[L];
+  double m0 = 0.0, m1 = 0.0;
+  for (int j = 0; j < c; j++) {
+    long long col = (long long)(s + j) * nm + a;
+    m0 += Q1[col] * x[s + j];
+    m1 += Q0[col] * x[s + j + n];
+  }
+  mom[(long long)L * 2 * nmMax + a] = m0;
+  mom[(long long)L * 2 * nmMax + nmMax + a] = m1;
+}
+
+// src/expan.c:342-382 (transM2MWs) + convM2M (:223-247): one block per parent
+__global__ void k_m2m(int levStart, const int *__restrict__ kidStart, const int *__restrict__ nKids,
+                      const int *__restrict__ cubeLevel, const int *__restrict__ ordMom,
+                      const double *__restrict__ cubeX, double *__restrict__ mom, int nmMax, DevTabs T) {
+  extern __shared__ double sm[];
+  const int par = levStart + blockIdx.x;
+  const int ordOut = ordMom[cubeLevel[par]], nOut = NMOM(ordOut);
+  double *R = sm, *b1 = sm + nOut, *b2 = sm + 2 * nOut;
+  double acc1[2] = {0, 0}, acc2[2] = {0, 0};
+  for (int k = 0; k < nKids[par]; k++) {
+    const int kid = kidStart[par] + k;
+    const int nIn = NMOM(ordMom[cubeLevel[kid]]);
+    double tr[3];
+    for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
+    __syncthreads();
+    for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
+      R[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
+      b1[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + a] : 0.0;
+      b2[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + nmMax + a] : 0.0;
+    }
+    __syncthreads();
+    int s = 0;
+    for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) {
+      double t1 = 0.0, t2 = 0.0;
+      for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
+        int pr = T.m2m[e];
+        double rv = R[pr & 0xffff];
+        t1 += rv * b1[pr >> 16];
+        t2 += rv * b2[pr >> 16];
+      }
+      acc1[s] += t1;
+      acc2[s] += t2;
+    }
+  }
+  int s = 0;
+  for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) {
+    mom[(long long)par * 2 * nmMax + a] = acc1[s];
+    mom[(long long)par * 2 * nmMax + nmMax + a] = acc2[s];
+  }
+}
+
+// src/fmm.c:774-787, src/expan.c:471-494 (transM2L) + convM2L (:258-290):
+// one wave per destination cube, sources of its interaction list in order
+__global__ __launch_bounds__(64) void k_m2l(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
+                                            const int *__restrict__ cubeLevel, const int *__restrict__ ordM2L,
+                                            const int *__restrict__ ordMom, const double *__restrict__ cubeX,
+                                            const double *__restrict__ mom, double *__restrict__ lec, int nmMax,
+                                            double kappa, double eps, DevTabs T) {
+  extern __shared__ double sm[];
+  const int c = blockIdx.x, tid = threadIdx.x;
+  const int p = ordM2L[cubeLevel[c]], nm = NMOM(p), dsz = derivScratch(p);
+  double *s0 = sm, *sk = s0 + dsz, *G0 = sk + dsz, *Gk = G0 + nm, *c1 = Gk + nm, *c2 = c1 + nm;
+  double *v1 = c2 + nm, *v2 = v1 + nm, *v3 = v2 + nm;
+  double acc[4][4] = {};   // up to 4 multi-indices per lane (nm <= 256)
+  for (int pr = m2lOff[c]; pr < m2lOff[c + 1]; pr++) {
+    const int src = m2lSrc[pr];
+    double r[3];
+    for (int d = 0; d < 3; d++) r[d] = cubeX[3 * c + d] - cubeX[3 * src + d];
+    derivs_block(kappa, p, r, s0, sk, G0, Gk, T.mi, tid, 64);
+    const double *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
+    for (int i = tid; i < nm; i += 64) {
+      int sg = T.ep[i].w;
+      c1[i] = sg * mp[i];
+      c2[i] = sg * md[i];
+      v1[i] = G0[i] - Gk[i];
+      v2[i] = eps * Gk[i] - G0[i];
+      v3[i] = G0[i] - Gk[i] / eps;
+    }
+    __syncthreads();
+    int s = 0;
+    for (int a = tid; a < nm; a += 64, s++) {
+      const int cnt = NMOM(p - deg_of(T.mi[a]));
+      const int *cv = T.conv + T.convStart[a];
+      double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+      for (int e = 0; e < cnt; e++) {
+        int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
+        t1 += c2[j] * v1[kk];
+        t2 += c1[j] * v2[kk];
+        t3 += c2[j] * v3[kk];
+        t4 += c1[j] * (-v1[kk]);
+      }
+      acc[s][0] += t1; acc[s][1] += t2; acc[s][2] += t3; acc[s][3] += t4;
+    }
+    __syncthreads();
+  }
+  const int nmc = NMOM(ordMom[cubeLevel[c]]);
+  int s = 0;
+  for (int a = tid; a < nmMax; a += 64, s++)
+    for (int k = 0; k < 4; k++)
+      lec[((long long)c * 4 + k) * nmMax + a] = (a < nm && a < nmc) ? acc[s][k] : 0.0;
+}
+
+// src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child
+__global__ void k_l2l(int levStart, const int *__restrict__ parent, const int *__restrict__ cubeLevel,
+                      const int *__restrict__ ordMom, const double *__restrict__ cubeX, double *__restrict__ lec,
+                      int nmMax, DevTabs T) {
+  extern __shared__ double sm[];
+  const int kid = levStart + blockIdx.x, par = parent[kid];
+  const int ordIn = ordMom[cubeLevel[par]], ordOut = ordMom[cubeLevel[kid]];
+  const int nIn = NMOM(ordIn), nOut = NMOM(ordOut);
+  double *R = sm, *In = sm + nIn;   // In[4][nIn]
+  double tr[3];
+  for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
+  for (int a = threadIdx.x; a < nIn; a += blockDim.x) {
+    R[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
+    for (int k = 0; k < 4; k++) In[k * nIn + a] = lec[((long long)par * 4 + k) * nmMax + a];
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
+    const int cnt = NMOM(ordIn - deg_of(T.mi[a]));
+    const int *cv = T.conv + T.convStart[a];
+    double t[4] = {0, 0, 0, 0};
+    for (int e = 0; e < cnt; e++) {
+      int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
+      double rv = R[j];
+      for (int k = 0; k < 4; k++) t[k] += rv * In[k * nIn + kk];
+    }
+    for (int k = 0; k < 4; k++) lec[((long long)kid * 4 + k) * nmMax + a] += t[k];
+  }
+}
+
+// far[p] = Q2M0^T (l1 + l2), far[p+n] = Q2M1^T (l3 + l4) (src/fmm.c:815-828):
+// one block per leaf, one wave per panel, lanes over multi-indices
+__global__ __launch_bounds__(256) void k_l2p(const double *__restrict__ Q0, const double *__restrict__ Q1,
+                                             const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                             const double *__restrict__ lec, double *__restrict__ far, int n,
+                                             int nm, int nmMax) {
+  extern __shared__ double sm[];
+  double *L12 = sm, *L34 = sm + nm;
+  const int L = blockIdx.x;
+  const double *l = lec + (long long)L * 4 * nmMax;
+  for (int a = threadIdx.x; a < nm; a += blockDim.x) {
+    L12[a] = l[a] + l[nmMax + a];
+    L34[a] = l[2 * nmMax + a] + l[3 * nmMax + a];
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, s = leafStart[L], c = leafCount[L];
+  for (int j = w; j < c; j += 4) {
+    const double *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
+    double t0 = 0.0, t1 = 0.0;
+    for (int a = lane; a < nm; a += 64) {
+      t0 += q0[a] * L12[a];
+      t1 += q1[a] * L34[a];
+    }
+    t0 = waveSum(t0);
+    t1 = waveSum(t1);
+    if (lane == 0) {
+      far[s + j] = t0;
+      far[s + j + n] = t1;
+    }
+  }
+}
+
+// y = alpha*(D x - far - near) + beta*y (src/fabipb.c:1271-1279)
+__global__ void k_combine(const double *__restrict__ area, const double *__restrict__ x,
+                          const double *__restrict__ far, const double *__restrict__ nearv, double *__restrict__ y,
+                          int n, double alpha, double beta, double s1, double s2, int useFar, int useNear) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * n) return;
+  int p = i < n ? i : i - n;
+  double s = i < n ? s1 : s2;
+  double k = (useFar ? far[i] : 0.0) + (useNear ? nearv[i] : 0.0);
+  double v = alpha * (s * area[p] * x[i] - k);
+  y[i] = beta == 0.0 ? v : v + beta * y[i];
+}
+
+// pot = beta*pot + alpha*(far + near) on the selected parts (stage API)
+__global__ void k_accum(const double *__restrict__ far, const double *__restrict__ nearv, double *__restrict__ y,
+                        int n2, double alpha, double beta, int useFar, int useNear) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  double k = (useFar ? far[i] : 0.0) + (useNear ? nearv[i] : 0.0);
+  y[i] = (beta == 0.0 ? 0.0 : beta * y[i]) + alpha * k;
+}
+
+// --------------------------------------------------------------- RHS
+
+// src/fabipb.c:789-798 with src/numQuad.c:786-819 (panelRHS), kernelRHS (kernel.c:152-167)
+__global__ __launch_bounds__(256) void k_rhs(const Pnl *__restrict__ pnl, int n, int nChar,
+                                             const double *__restrict__ pos, const double *__restrict__ chr,
+                                             double *__restrict__ b, double fac, QTab Q, int q) {
+  __shared__ double sp[256][4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  Pnl P;
+  if (i < n) P = pnl[i];
+  double b0 = 0.0, b1 = 0.0;
+  for (int j0 = 0; j0 < nChar; j0 += 256) {
+    __syncthreads();
+    int j = j0 + threadIdx.x;
+    if (j < nChar) { sp[threadIdx.x][0] = pos[3 * j]; sp[threadIdx.x][1] = pos[3 * j + 1]; sp[threadIdx.x][2] = pos[3 * j + 2]; sp[threadIdx.x][3] = chr[j]; }
+    __syncthreads();
+    if (i < n) {
+      int cnt = min(256, nChar - j0);
+      for (int jj = 0; jj < cnt; jj++) {
+        double r0[3] = {P.v[0] - sp[jj][0], P.v[1] - sp[jj][1], P.v[2] - sp[jj][2]};
+        double I0 = 0.0, I1 = 0.0;
+        for (int ix = 0; ix < q; ix++)
+          for (int jx = 0; jx < q; jx++) {
+            double r[3];
+            for (int k = 0; k < 3; k++) r[k] = r0[k] + Q.t[ix] * (P.a[6 + k] + Q.t[jx] * P.a[k]);
+            double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2], ri = 1.0 / sqrt(r2), r3i = ri / r2;
+            double ip = P.nr[0] * r[0] + P.nr[1] * r[1] + P.nr[2] * r[2];
+            I0 += ri * Q.t[ix] * Q.w[ix] * Q.w[jx];
+            I1 += (-ip * r3i) * Q.t[ix] * Q.w[ix] * Q.w[jx];
+          }
+        I0 *= 2.0 * P.area;
+        I1 *= 2.0 * P.area;
+        b0 += sp[jj][3] * I0;
+        b1 += sp[jj][3] * I1;
+      }
+    }
+  }
+  if (i < n) {
+    b[i] = b0 * fac;
+    b[i + n] = b1 * fac;
+  }
+}
+
+// ------------------------------------------------------------ charge tree
+
+// leaf moments of the charges (src/chargeTree.c:313-337): block per leaf
+__global__ void k_ct_leaf(int levStart, const int *__restrict__ chgStart, const int *__restrict__ nChgs,
+                          const int *__restrict__ chgIdx, const double *__restrict__ ctx_,
+                          const double *__restrict__ pos, const double *__restrict__ chr, int order,
+                          double *__restrict__ cmom, int nmMax, DevTabs T) {
+  const int c = levStart + blockIdx.x, nm = NMOM(order);
+  for (int a = threadIdx.x; a < nm; a += blockDim.x) {
+    double m = 0.0;
+    int4 mi = T.mi[a];
+    for (int k = 0; k < nChgs[c]; k++) {
+      int j = chgIdx[chgStart[c] + k];
+      double tr[3] = {pos[3 * j] - ctx_[3 * c], pos[3 * j + 1] - ctx_[3 * c + 1], pos[3 * j + 2] - ctx_[3 * c + 2]};
+      m += chr[j] * conVect1(tr, mi, T.ifact3[a]);
+    }
+    cmom[(long long)c * nmMax + a] = m;
+  }
+}
+
+// upward pass of the charge moments (src/chargeTree.c:344-395, transM2Mchr)
+__global__ void k_ct_m2m(int levStart, const int *__restrict__ kidStart, const int *__restrict__ nKids,
+                         const double *__restrict__ ctx_, const int *__restrict__ ordOf, int lev,
+                         double *__restrict__ cmom, int nmMax, DevTabs T) {
+  extern __shared__ double sm[];
+  const int par = levStart + blockIdx.x;
+  const int ordOut = ordOf[lev], nOut = NMOM(ordOut), nIn = NMOM(ordOf[lev + 1]);
+  double *R = sm, *b = sm + nOut;
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = 0; k < nKids[par]; k++) {
+    const int kid = kidStart[par] + k;
+    double tr[3];
+    for (int d = 0; d < 3; d++) tr[d] = ctx_[3 * kid + d] - ctx_[3 * par + d];
+    __syncthreads();
+    for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
+      R[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
+      b[a] = a < nIn ? cmom[(long long)kid * nmMax + a] : 0.0;
+    }
+    __syncthreads();
+    int s = 0;
+    for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) {
+      double t = 0.0;
+      for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
+        int pr = T.m2m[e];
+        if ((pr >> 16) < nIn) t += R[pr & 0xffff] * b[pr >> 16];
+      }
+      acc[s] += t;
+    }
+  }
+  int s = 0;
+  for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) cmom[(long long)par * nmMax + a] = acc[s];
+}
+
+#define EN_STACK 512
+
+// Panel-tree energy (src/treecode.c:617-695 energyTreeWalk/panelEnergyTree,
+// :581-615 chgClusterEnergyEval): one wave per panel walks the charge tree
+// depth first, kids in order, accepting clusters with eRad < theta*dist at
+// levels >= height.  part[p] = I0*sgm_n[p] + I1*sgm[p].
+__global__ __launch_bounds__(64) void k_energy(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
+                                               const int *__restrict__ level, const double *__restrict__ cx,
+                                               const double *__restrict__ eRad, const int *__restrict__ kidStart,
+                                               const int *__restrict__ nKids, const int *__restrict__ chgStart,
+                                               const int *__restrict__ nChgs, const int *__restrict__ chgIdx,
+                                               const double *__restrict__ pos, const double *__restrict__ chr,
+                                               const int *__restrict__ ordOf, const double *__restrict__ cmom,
+                                               int nmMax, int depth, int height, double theta, double kappa,
+                                               double eps, QTab Q, int q, DevTabs T, double *__restrict__ part) {
+  extern __shared__ double sm[];
+  __shared__ int stack[EN_STACK];
+  const int p = blockIdx.x, lane = threadIdx.x;
+  const int P1 = ordOf[depth + 1];   // largest derivative order used
+  const int dsz = derivScratch(P1), nmP = NMOM(P1);
+  double *s0 = sm, *sk = s0 + dsz, *g0 = sk + dsz, *gk = g0 + nmP;
+  const Pnl X = pnl[p];
+  double o0 = 0.0, o1 = 0.0;
+  for (int ix = 0; ix < q; ix++)
+    for (int jx = 0; jx < q; jx++) {
+      double qp[3];
+      for (int k = 0; k < 3; k++) qp[k] = X.v[k] + Q.t[ix] * (X.a[6 + k] + Q.t[jx] * X.a[k]);
+      double y0 = 0.0, y1 = 0.0;
+      int sp = 0;
+      stack[0] = 0;
+      sp = 1;
+      __syncthreads();
+      while (sp > 0) {
+        const int cb = stack[sp - 1];
+        sp--;
+        double r[3] = {qp[0] - cx[3 * cb], qp[1] - cx[3 * cb + 1], qp[2] - cx[3 * cb + 2]};
+        double dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const int lev = level[cb];
+        if (eRad[cb] < theta * dist && lev >= height) {
+          const int order = ordOf[lev];
+          derivs_block(kappa, order + 1, r, s0, sk, g0, gk, T.mi, lane, 64);
+          const double *mom = cmom + (long long)cb * nmMax;
+          double t0 = 0.0, t1 = 0.0;
+          for (int a = lane; a < NMOM(order); a += 64) {
+            int4 e = T.ep[a];
+            double sm_ = e.w * mom[a];
+            t0 += sm_ * (g0[a] - gk[a]);
+            double dnG0 = X.nr[0] * g0[e.x] + X.nr[1] * g0[e.y] + X.nr[2] * g0[e.z];
+            double dnGk = X.nr[0] * gk[e.x] + X.nr[1] * gk[e.y] + X.nr[2] * gk[e.z];
+            t1 += sm_ * (eps * dnGk - dnG0);
+          }
+          y0 += waveSum(t0);
+          y1 += waveSum(t1);
+          __syncthreads();
+        } else if (lev == depth) {
+          double t0 = 0.0, t1 = 0.0;
+          for (int i = lane; i < nChgs[cb]; i += 64) {
+            int j = chgIdx[chgStart[cb] + i];
+            double x[3] = {qp[0] - pos[3 * j], qp[1] - pos[3 * j + 1], qp[2] - pos[3 * j + 2]};
+            double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2);
+            double G0 = kFourPiI / rn, ek = exp(-kappa * rn), Gk = ek * G0;
+            double coef = (kappa * rn + 1.0) * ek;
+            double ip = X.nr[0] * x[0] + X.nr[1] * x[1] + X.nr[2] * x[2];
+            double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+            t0 += chr[j] * (G0 - Gk);
+            t1 += chr[j] * (eps * dGkdn - dG0dn);
+          }
+          y0 += waveSum(t0);
+          y1 += waveSum(t1);
+        } else {
+          const int nk = nKids[cb], k0 = kidStart[cb];
+          if (sp + nk > EN_STACK) { sp = 0; y0 = y1 = __builtin_nan(""); break; }   // stack bound: poison, never fault
+          for (int k = nk - 1; k >= 0; k--) stack[sp++] = k0 + k;   // pushed in reverse: kids visited in order
+        }
+      }
+      o0 += y0 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+      o1 += y1 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+    }
+  o0 *= 2.0 * X.area;
+  o1 *= 2.0 * X.area;
+  if (lane == 0) part[p] = o0 * sgm[p + n] + o1 * sgm[p];
+}
+
+// ------------------------------------------------- reductions and GMRES
+
+// block partial of sum a[i]*b[i] over a fixed grid (deterministic)
+__global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ a, const double *__restrict__ b, int n,
+                                                  double *__restrict__ part) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v += a[i] * b[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__device__ double reduceParts(const double *part, int np) {   // same order in every block
+  __shared__ double s[256];
+  s[threadIdx.x] = threadIdx.x < np ? part[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  double v = s[0];
+  __syncthreads();
+  return v;
+}
+
+// MGS step: h = <w, v>; w -= h v (src/gmres.c:606-609)
+__global__ __launch_bounds__(256) void k_mgs(double *__restrict__ w, const double *__restrict__ v, int n,
+                                             const double *__restrict__ part, double *__restrict__ h) {
+  double hk = reduceParts(part, RED_BLOCKS);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *h = hk;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) w[i] += -hk * v[i];
+}
+
+// h = ||w||; v = w / h if h > 0 (src/gmres.c:612-620)
+__global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ w, double *__restrict__ v, int n,
+                                                   const double *__restrict__ part, double *__restrict__ h) {
+  double hh = sqrt(reduceParts(part, RED_BLOCKS));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *h = hh;
+  double sc = hh > 0.0 ? 1.0 / hh : 1.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v[i] = hh > 0.0 ? w[i] * sc : w[i];
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, int np, double *__restrict__ out) {
+  if (blockIdx.x == 0) {
+    double v = reduceParts(part, np);
+    if (threadIdx.x == 0) *out = v;
+  }
+}
+
+// x += sum_k y_k V_k (src/gmres.c:364-373, dgemv 'N')
+__global__ void k_update(double *__restrict__ x, const double *__restrict__ V, const double *__restrict__ y, int m,
+                         int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < m; k++) s += y[k] * V[(long long)k * n + i];
+  x[i] += s;
+}
+
+__global__ void k_psolve(const double *__restrict__ in, double *__restrict__ out, const double *__restrict__ area,
+                         int n, double s1, double s2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * n) return;
+  int p = i < n ? i : i - n;
+  out[i] = in[i] / ((i < n ? s1 : s2) * area[p]);
+}
+
+__global__ void k_scale(const double *__restrict__ in, double *__restrict__ out, int n, double s) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] * s;
+}
+
+// serial sum of energy partials in panel order (one thread; n is panels)
+__global__ void k_sum_serial(const double *__restrict__ part, int n, double *__restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += part[i];
+    *out = s;
+  }
+}
+
+// ================================================================ host side
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct fbh_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+  double kappa, eps1, eps2, eps;
+  int qorder;
+  QTab Q;
+  int n, depth, height, maxOrder, P, nmMax, nmLeaf;
+  int nCube, nLeaf;
+  std::vector<int> levStart, levCount, ordMom, ordM2L, cubeLevel;
+  std::vector<DevBuf> bufs;
+  size_t bytes = 0;
+  // device arrays
+  Pnl *pnl;
+  double *area, *cubeX, *Q0, *Q1, *mom, *lec, *far, *nearv, *xs, *ys, *fact;
+  int *leafOf, *leafStart, *leafCount, *cubeLevelD, *kidStart, *nKids, *parent, *m2lOff, *m2lSrc, *ordMomD,
+      *ordM2LD;
+  long long nInter;
+  int *srcOff, *ncol, *srcList;
+  long long *coefOff;
+  double4 *coef;
+  int2 *tiles;
+  int nTiles;
+  long long nLinks, nM2L;
+  DevTabs T;    // contraction table: ep = (a+e1, a+e2, a+e3, sign)
+  DevTabs Tm;   // moment table: ep = (a-e1, a-e2, a-e3)
+  // GMRES workspace
+  int gmRestart = 0;
+  double *V = nullptr, *w = nullptr, *r = nullptr, *part = nullptr, *hcol = nullptr, *ydev = nullptr, *scal = nullptr;
+};
+
+template <class T>
+static T *dalloc(fbh_ctx *c, size_t count) {
+  DevBuf b;
+  b.bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (hipMalloc(&b.p, b.bytes) != hipSuccess) {
+    setErr("hipMalloc of %zu bytes failed", b.bytes);
+    return nullptr;
+  }
+  c->bufs.push_back(b);
+  c->bytes += b.bytes;
+  return (T *)b.p;
+}
+
+template <class T>
+static T *dupload(fbh_ctx *c, const T *h, size_t count) {
+  T *d = dalloc<T>(c, count);
+  if (!d) return nullptr;
+  if (count && hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+    setErr("upload of %zu bytes failed", count * sizeof(T));
+    return nullptr;
+  }
+  return d;
+}
+
+extern "C" int fbh_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    setErr("no HIP device");
+    return 0;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    setErr("device 0 is %s, this build targets gfx950", prop.gcnArchName);
+    return 0;
+  }
+  return 1;
+}
+
+extern "C" void fbh_destroy(fbh_ctx *c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto &b : c->bufs) hipFree(b.p);
+  if (c->V) hipFree(c->V);
+  if (c->w) hipFree(c->w);
+  if (c->r) hipFree(c->r);
+  if (c->part) hipFree(c->part);
+  if (c->hcol) hipFree(c->hcol);
+  if (c->ydev) hipFree(c->ydev);
+  if (c->scal) hipFree(c->scal);
+  if (c->ownStream && c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int idx3of(int i1, int i2, int i3) {   // graded index: degree, then i1, then i2
+  int d = i1 + i2 + i3, base = d > 0 ? NMOM(d - 1) : 0, off = 0;
+  for (int a = 0; a < i1; a++) off += d - a + 1;
+  return base + off + i2;
+}
+
+static bool buildTables(fbh_ctx *c) {
+  const int P = c->P, nmP = NMOM(P), nmM = NMOM(c->maxOrder);
+  std::vector<int4> mi(nmP), ep(nmP);
+  std::vector<double> ifact3(nmP), fact(2 * (MAXP + 3) + 8);
+  double f[2 * MAXP + 8];
+  f[0] = 1.0;
+  for (int k = 1; k < 2 * MAXP + 8; k++) f[k] = f[k - 1] * k;
+  for (int k = 0; k < MAXP + 3; k++) fact[k] = f[k];
+  for (int k = 0; k < MAXP + 3 + 5 && MAXP + 3 + k < (int)fact.size(); k++) fact[MAXP + 3 + k] = 1.0 / f[k];
+  int g = 0, sgn = 1;
+  for (int d = 0; d <= P; d++, sgn = -sgn)
+    for (int i1 = 0; i1 <= d; i1++)
+      for (int i2 = 0; i2 <= d - i1; i2++, g++) {
+        int i3 = d - i1 - i2, axis = 0, fac = 0, j1 = -1, j2 = -1;
+        auto loc = [&](int a, int b, int cc) { return idx3of(a, b, cc) - ((a + b + cc) > 0 ? NMOM(a + b + cc - 1) : 0); };
+        if (d >= 1) {
+          if (i1 == 0 && i2 == 0) { axis = 2; j1 = loc(0, 0, d - 1); if (d >= 2) { j2 = loc(0, 0, d - 2); fac = d - 1; } }
+          else if (i1 == 0 && i2 == 1) { axis = 1; j1 = loc(0, 0, d - 1); }
+          else if (i1 == 0) { axis = 1; j1 = loc(0, i2 - 1, i3); j2 = loc(0, i2 - 2, i3); fac = i2 - 1; }
+          else if (i1 == 1) { axis = 0; j1 = loc(0, i2, i3); }
+          else { axis = 0; j1 = loc(i1 - 1, i2, i3); j2 = loc(i1 - 2, i2, i3); fac = i1 - 1; }
+        }
+        mi[g] = make_int4(i1 | (i2 << 8) | (i3 << 16) | (d << 24), axis | (fac << 8), j1, j2);
+        ifact3[g] = 1.0 / (f[i1] * f[i2] * f[i3]);
+        if (d < P) ep[g] = make_int4(idx3of(i1 + 1, i2, i3), idx3of(i1, i2 + 1, i3), idx3of(i1, i2, i3 + 1), sgn);
+        else ep[g] = make_int4(0, 0, 0, sgn);
+      }
+  // for the noment gather the moment kernel reads b-e_k from ep of the target
+  // index; keep that as a separate table appended after the contraction one
+  std::vector<int4> em(nmP);
+  g = 0;
+  for (int d = 0; d <= P; d++)
+    for (int i1 = 0; i1 <= d; i1++)
+      for (int i2 = 0; i2 <= d - i1; i2++, g++) {
+        int i3 = d - i1 - i2;
+        em[g] = make_int4(i1 > 0 ? idx3of(i1 - 1, i2, i3) : 0, i2 > 0 ? idx3of(i1, i2 - 1, i3) : 0,
+                          i3 > 0 ? idx3of(i1, i2, i3 - 1) : 0, 0);
+      }
+  // M2L/L2L pairs (beta, alpha+beta), beta graded, |alpha|+|beta| <= maxOrder
+  std::vector<int> cs(nmM + 1), cl, ms(nmP + 1), ml;
+  g = 0;
+  for (int d = 0; d <= c->maxOrder; d++)
+    for (int i1 = 0; i1 <= d; i1++)
+      for (int i2 = 0; i2 <= d - i1; i2++, g++) {
+        int i3 = d - i1 - i2;
+        cs[g] = (int)cl.size();
+        for (int m = 0; m <= c->maxOrder - d; m++)
+          for (int j1 = 0; j1 <= m; j1++)
+            for (int j2 = 0; j2 <= m - j1; j2++) {
+              int j3 = m - j1 - j2;
+              cl.push_back(idx3of(j1, j2, j3) | (idx3of(i1 + j1, i2 + j2, i3 + j3) << 16));
+            }
+      }
+  cs[nmM] = (int)cl.size();
+  // convolution pairs (alpha-k, k), k <= alpha in (k1,k2,k3) loop order
+  g = 0;
+  for (int d = 0; d <= P; d++)
+    for (int i1 = 0; i1 <= d; i1++)
+      for (int i2 = 0; i2 <= d - i1; i2++, g++) {
+        int i3 = d - i1 - i2;
+        ms[g] = (int)ml.size();
+        for (int k1 = 0; k1 <= i1; k1++)
+          for (int k2 = 0; k2 <= i2; k2++)
+            for (int k3 = 0; k3 <= i3; k3++)
+              ml.push_back(idx3of(i1 - k1, i2 - k2, i3 - k3) | (idx3of(k1, k2, k3) << 16));
+      }
+  ms[nmP] = (int)ml.size();
+  int4 *dmi = dupload(c, mi.data(), mi.size());
+  int4 *dep = dupload(c, ep.data(), ep.size());
+  int4 *dem = dupload(c, em.data(), em.size());
+  double *dif = dupload(c, ifact3.data(), ifact3.size());
+  int *dcs = dupload(c, cs.data(), cs.size()), *dcl = dupload(c, cl.data(), cl.size());
+  int *dms = dupload(c, ms.data(), ms.size()), *dml = dupload(c, ml.data(), ml.size());
+  c->fact = dupload(c, fact.data(), fact.size());
+  if (!dmi || !dep || !dem || !dif || !dcs || !dcl || !dms || !dml || !c->fact) return false;
+  c->T = DevTabs{dmi, dep, dif, dcs, dcl, dms, dml};
+  c->Tm = DevTabs{dmi, dem, dif, dcs, dcl, dms, dml};
+  return true;
+}
+
+static inline int blk64(int n, int cap) {
+  int b = ((n + 63) / 64) * 64;
+  return std::max(64, std::min(b, cap));
+}
+
+extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorder, int n, const double *vtx,
+                               const double *nrm, const double *area, int depth, int height, const int *ordMom,
+                               const int *ordM2L, int nCube, const int *levStart, const int *levCount,
+                               const int *cubeLevel, const double *cubeX, const int *kidStart, const int *nKids,
+                               const int *parent, int nLeaf, const int *leafStart, const int *leafCount,
+                               const int *nearOff, const int *nearSrc, const int *m2lOff, const int *m2lSrc,
+                               void *stream) {
+  if (!fbh_available()) return nullptr;
+  if (n <= 0 || nLeaf <= 0 || nCube < nLeaf || depth < height || qorder < 1 || qorder > 10 || !(eps1 > 0) ||
+      !(eps2 > 0)) {
+    setErr("fbh_create: bad arguments (n=%d nLeaf=%d nCube=%d depth=%d height=%d q=%d)", n, nLeaf, nCube, depth,
+           height, qorder);
+    return nullptr;
+  }
+  int maxOrder = 0;
+  for (int l = height; l <= depth; l++) maxOrder = std::max(maxOrder, std::max(ordMom[l], ordM2L[l]));
+  if (maxOrder < 1 || maxOrder > 9) {
+    setErr("fbh_create: expansion order %d outside 1..9", maxOrder);
+    return nullptr;
+  }
+  // host-side validation of every index the kernels dereference
+  for (int l = 0; l < nLeaf; l++)
+    if (leafStart[l] < 0 || leafCount[l] <= 0 || leafStart[l] + leafCount[l] > n) {
+      setErr("fbh_create: leaf %d range [%d,+%d) outside %d panels", l, leafStart[l], leafCount[l], n);
+      return nullptr;
+    }
+  for (int c = 0; c < nCube; c++) {
+    if (cubeLevel[c] < height || cubeLevel[c] > depth || (c < nLeaf) != (cubeLevel[c] == depth) ||
+        nKids[c] < 0 || nKids[c] > 8 || (nKids[c] > 0 && (kidStart[c] < 0 || kidStart[c] + nKids[c] > nCube)) ||
+        (cubeLevel[c] > height && (parent[c] < 0 || parent[c] >= nCube))) {
+      setErr("fbh_create: cube %d has an invalid level/kid/parent record", c);
+      return nullptr;
+    }
+    if (m2lOff[c + 1] < m2lOff[c]) { setErr("fbh_create: m2lOff not monotone at %d", c); return nullptr; }
+  }
+  for (int k = 0; k < m2lOff[nCube]; k++)
+    if (m2lSrc[k] < 0 || m2lSrc[k] >= nCube) { setErr("fbh_create: m2l source %d out of range", k); return nullptr; }
+  for (int l = 0; l < nLeaf; l++) {
+    if (nearOff[l + 1] < nearOff[l]) { setErr("fbh_create: nearOff not monotone at %d", l); return nullptr; }
+    for (int k = nearOff[l]; k < nearOff[l + 1]; k++)
+      if (nearSrc[k] < 0 || nearSrc[k] >= nLeaf) { setErr("fbh_create: near source %d out of range", k); return nullptr; }
+  }
+
+  fbh_ctx *c = new fbh_ctx();
+  c->kappa = kappa; c->eps1 = eps1; c->eps2 = eps2; c->eps = eps2 / eps1; c->qorder = qorder;
+  for (int i = 0; i < 10; i++) { c->Q.t[i] = hT[qorder][i]; c->Q.w[i] = hW[qorder][i]; }
+  c->n = n; c->depth = depth; c->height = height; c->nCube = nCube; c->nLeaf = nLeaf;
+  c->maxOrder = maxOrder; c->P = maxOrder + 1; c->nmMax = NMOM(maxOrder); c->nmLeaf = NMOM(ordMom[depth]);
+  c->levStart.assign(levStart, levStart + depth + 1);
+  c->levCount.assign(levCount, levCount + depth + 1);
+  c->ordMom.assign(ordMom, ordMom + depth + 1);
+  c->ordM2L.assign(ordM2L, ordM2L + depth + 1);
+  c->cubeLevel.assign(cubeLevel, cubeLevel + nCube);
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreate(&c->stream) != hipSuccess) { setErr("hipStreamCreate failed"); delete c; return nullptr; }
+    c->ownStream = true;
+  }
+#define FAIL() do { fbh_destroy(c); return nullptr; } while (0)
+  if (!buildTables(c)) FAIL();
+
+  // panels
+  std::vector<Pnl> hp(n);
+  for (int i = 0; i < n; i++) {
+    const double *v = &vtx[9 * i];
+    for (int k = 0; k < 9; k++) hp[i].v[k] = v[k];
+    for (int j = 0; j < 3; j++) {   // src/input.c:743-747
+      hp[i].a[j] = v[6 + j] - v[3 + j];
+      hp[i].a[3 + j] = v[j] - v[6 + j];
+      hp[i].a[6 + j] = v[3 + j] - v[j];
+    }
+    for (int k = 0; k < 3; k++) hp[i].nr[k] = nrm[3 * i + k];
+    hp[i].area = area[i];
+  }
+  std::vector<int> leafOf(n);
+  for (int l = 0; l < nLeaf; l++)
+    for (int j = 0; j < leafCount[l]; j++) leafOf[leafStart[l] + j] = l;
+  if (!(c->pnl = dupload(c, hp.data(), n)) || !(c->area = dupload(c, area, n)) ||
+      !(c->leafOf = dupload(c, leafOf.data(), n)) || !(c->leafStart = dupload(c, leafStart, nLeaf)) ||
+      !(c->leafCount = dupload(c, leafCount, nLeaf)) || !(c->cubeX = dupload(c, cubeX, 3 * (size_t)nCube)) ||
+      !(c->cubeLevelD = dupload(c, cubeLevel, nCube)) || !(c->kidStart = dupload(c, kidStart, nCube)) ||
+      !(c->nKids = dupload(c, nKids, nCube)) || !(c->parent = dupload(c, parent, nCube)) ||
+      !(c->m2lOff = dupload(c, m2lOff, nCube + 1)) || !(c->m2lSrc = dupload(c, m2lSrc, std::max(1, m2lOff[nCube]))) ||
+      !(c->ordMomD = dupload(c, ordMom, depth + 1)) || !(c->ordM2LD = dupload(c, ordM2L, depth + 1)))
+    FAIL();
+  c->nM2L = m2lOff[nCube];
+  c->nLinks = 0;
+  for (int l = height; l < depth; l++)
+    for (int cb = levStart[l]; cb < levStart[l] + levCount[l]; cb++) c->nLinks += nKids[cb];
+
+  // expansions and work vectors
+  if (!(c->Q0 = dalloc<double>(c, (size_t)n * c->nmLeaf)) || !(c->Q1 = dalloc<double>(c, (size_t)n * c->nmLeaf)) ||
+      !(c->mom = dalloc<double>(c, (size_t)nCube * 2 * c->nmMax)) ||
+      !(c->lec = dalloc<double>(c, (size_t)nCube * 4 * c->nmMax)) || !(c->far = dalloc<double>(c, 2 * (size_t)n)) ||
+      !(c->nearv = dalloc<double>(c, 2 * (size_t)n)) || !(c->xs = dalloc<double>(c, 2 * (size_t)n)) ||
+      !(c->ys = dalloc<double>(c, 2 * (size_t)n)))
+    FAIL();
+
+  // expansion entries beyond a level's order are never written by the passes
+  // and never read; zero them once so the buffers hold no stale values
+  if (hipMemsetAsync(c->mom, 0, (size_t)nCube * 2 * c->nmMax * sizeof(double), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->lec, 0, (size_t)nCube * 4 * c->nmMax * sizeof(double), c->stream) != hipSuccess) {
+    setErr("clearing expansions failed");
+    FAIL();
+  }
+
+  // leaf Q2M/L2P matrices on the device
+  {
+    size_t lds = 5 * (size_t)c->nmLeaf * sizeof(double);
+    hipLaunchKernelGGL(k_q2m_mats, dim3(n), dim3(64), lds, c->stream, c->pnl, c->leafOf, c->cubeX, c->Tm,
+                       ordMom[depth], c->fact, c->Q0, c->Q1);
+    if (hipGetLastError() != hipSuccess) { setErr("k_q2m_mats launch failed"); FAIL(); }
+  }
+
+  // nearfield layout: per destination leaf, its source columns and coefficient rows
+  std::vector<int> srcOff(nLeaf + 1), ncol(nLeaf);
+  std::vector<long long> coefOff(nLeaf + 1);
+  std::vector<int> srcList;
+  std::vector<int2> tiles;
+  srcOff[0] = 0; coefOff[0] = 0;
+  for (int l = 0; l < nLeaf; l++) {
+    int nc = 0;
+    for (int k = nearOff[l]; k < nearOff[l + 1]; k++) {
+      int s = nearSrc[k];
+      for (int j = 0; j < leafCount[s]; j++) srcList.push_back(leafStart[s] + j);
+      nc += leafCount[s];
+    }
+    ncol[l] = nc;
+    srcOff[l + 1] = srcOff[l] + nc;
+    coefOff[l + 1] = coefOff[l] + (long long)leafCount[l] * nc;
+    for (int r = 0; r < leafCount[l]; r += NEAR_ROWS) tiles.push_back(make_int2(l, r));
+  }
+  c->nInter = coefOff[nLeaf];
+  c->nTiles = (int)tiles.size();
+  if (!(c->srcOff = dupload(c, srcOff.data(), nLeaf + 1)) || !(c->ncol = dupload(c, ncol.data(), nLeaf)) ||
+      !(c->coefOff = dupload(c, coefOff.data(), nLeaf + 1)) ||
+      !(c->srcList = dupload(c, srcList.data(), srcList.size())) ||
+      !(c->tiles = dupload(c, tiles.data(), tiles.size())) || !(c->coef = dalloc<double4>(c, (size_t)c->nInter)))
+    FAIL();
+  hipLaunchKernelGGL(k_near_build, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart, c->srcOff,
+                     c->ncol, c->coefOff, c->srcList, c->coef, kappa, c->eps, c->Q, qorder);
+  if (hipGetLastError() != hipSuccess) { setErr("k_near_build launch failed"); FAIL(); }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { setErr("setup kernels failed: %s", hipGetErrorString(hipGetLastError())); FAIL(); }
+#undef FAIL
+  return c;
+}
+
+// far field into c->far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
+static int farPass(fbh_ctx *c, const double *x) {
+  const int nmL = c->nmLeaf;
+  hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(blk64(nmL, 512)), 0, c->stream, c->Q0, c->Q1, c->leafStart,
+                     c->leafCount, x, c->mom, c->n, nmL, c->nmMax);
+  for (int lev = c->depth - 1; lev >= c->height; lev--) {
+    int nOut = NMOM(c->ordMom[lev]);
+    if (c->levCount[lev] <= 0) continue;
+    hipLaunchKernelGGL(k_m2m, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), 3 * nOut * sizeof(double),
+                       c->stream, c->levStart[lev], c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, c->mom,
+                       c->nmMax, c->T);
+  }
+  {
+    int pm = c->maxOrder, nm = NMOM(pm);
+    size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
+    hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
+                       c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
+  }
+  for (int lev = c->height; lev < c->depth; lev++) {
+    int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
+    if (c->levCount[lev + 1] <= 0) continue;
+    hipLaunchKernelGGL(k_l2l, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), 5 * nIn * sizeof(double),
+                       c->stream, c->levStart[lev + 1], c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->lec,
+                       c->nmMax, c->T);
+  }
+  hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * nmL * sizeof(double), c->stream, c->Q0, c->Q1,
+                     c->leafStart, c->leafCount, c->lec, c->far, c->n, nmL, c->nmMax);
+  return 1;
+}
+
+static void nearPass(fbh_ctx *c, const double *x) {
+  hipLaunchKernelGGL(k_near_apply, dim3(c->nTiles), dim3(256), 0, c->stream, c->tiles, c->leafStart, c->leafCount,
+                     c->srcOff, c->ncol, c->coefOff, c->srcList, c->coef, x, c->nearv, c->n);
+}
+
+extern "C" int fbh_matvec_dev(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
+  if (!c || !x || !y) { setErr("fbh_matvec_dev: null argument"); return 0; }
+  farPass(c, x);
+  nearPass(c, x);
+  double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_combine, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
+                     c->n, alpha, beta, s1, s2, 1, 1);
+  HCK(hipGetLastError());
+  return 1;
+}
+
+extern "C" int fbh_matvec(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
+  if (!c || !x || !y) { setErr("fbh_matvec: null argument"); return 0; }
+  size_t b = 2 * (size_t)c->n * sizeof(double);
+  HCK(hipMemcpyAsync(c->xs, x, b, hipMemcpyHostToDevice, c->stream));
+  if (beta != 0.0) HCK(hipMemcpyAsync(c->ys, y, b, hipMemcpyHostToDevice, c->stream));
+  if (!fbh_matvec_dev(c, alpha, c->xs, beta, c->ys)) return 0;
+  HCK(hipMemcpyAsync(y, c->ys, b, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+static int stageApply(fbh_ctx *c, double alpha, const double *sgm, double beta, double *pot, int useFar,
+                      int useNear) {
+  if (!c || !sgm || !pot) { setErr("stage apply: null argument"); return 0; }
+  size_t b = 2 * (size_t)c->n * sizeof(double);
+  HCK(hipMemcpyAsync(c->xs, sgm, b, hipMemcpyHostToDevice, c->stream));
+  HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
+  if (useFar) farPass(c, c->xs);
+  if (useNear) nearPass(c, c->xs);
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
+                     beta, useFar, useNear);
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(pot, c->ys, b, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+extern "C" int fbh_near_apply(fbh_ctx *c, double alpha, const double *sgm, double *pot) {
+  return stageApply(c, alpha, sgm, 1.0, pot, 0, 1);
+}
+extern "C" int fbh_far_apply(fbh_ctx *c, double alpha, const double *sgm, double beta, double *pot) {
+  return stageApply(c, alpha, sgm, beta, pot, 1, 0);
+}
+
+extern "C" int fbh_expansions(fbh_ctx *c, double *mom, double *lec) {
+  if (!c) return 0;
+  HCK(hipStreamSynchronize(c->stream));
+  if (mom) HCK(hipMemcpy(mom, c->mom, (size_t)c->nCube * 2 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost));
+  if (lec) HCK(hipMemcpy(lec, c->lec, (size_t)c->nCube * 4 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost));
+  return 1;
+}
+
+extern "C" int fbh_q2m_matrices(fbh_ctx *c, double *q0, double *q1) {
+  if (!c) return 0;
+  HCK(hipStreamSynchronize(c->stream));
+  size_t b = (size_t)c->n * c->nmLeaf * sizeof(double);
+  if (q0) HCK(hipMemcpy(q0, c->Q0, b, hipMemcpyDeviceToHost));
+  if (q1) HCK(hipMemcpy(q1, c->Q1, b, hipMemcpyDeviceToHost));
+  return 1;
+}
+
+extern "C" int fbh_near_coeff(fbh_ctx *c, int x, int y, double *ker4) {
+  if (!c || x < 0 || x >= c->n || y < 0 || y >= c->n) { setErr("fbh_near_coeff: bad index"); return 0; }
+  HCK(hipStreamSynchronize(c->stream));
+  int D = 0;
+  std::vector<int> ls(c->nLeaf), lc(c->nLeaf), so(c->nLeaf + 1), nc(c->nLeaf);
+  std::vector<long long> co(c->nLeaf + 1);
+  HCK(hipMemcpy(ls.data(), c->leafStart, c->nLeaf * sizeof(int), hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(lc.data(), c->leafCount, c->nLeaf * sizeof(int), hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(so.data(), c->srcOff, (c->nLeaf + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(nc.data(), c->ncol, c->nLeaf * sizeof(int), hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(co.data(), c->coefOff, (c->nLeaf + 1) * sizeof(long long), hipMemcpyDeviceToHost));
+  while (D < c->nLeaf && !(x >= ls[D] && x < ls[D] + lc[D])) D++;
+  std::vector<int> sl(nc[D]);
+  HCK(hipMemcpy(sl.data(), c->srcList + so[D], nc[D] * sizeof(int), hipMemcpyDeviceToHost));
+  for (int j = 0; j < nc[D]; j++)
+    if (sl[j] == y) {
+      HCK(hipMemcpy(ker4, c->coef + co[D] + (long long)(x - ls[D]) * nc[D] + j, 4 * sizeof(double),
+                    hipMemcpyDeviceToHost));
+      return 1;
+    }
+  setErr("fbh_near_coeff: panels %d and %d are not near neighbours", x, y);
+  return 0;
+}
+
+extern "C" int fbh_psolve_diag_dev(fbh_ctx *c, double *out, const double *in) {
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_psolve, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, in, out, c->area, c->n,
+                     (1.0 + c->eps) / 2.0, (1.0 + 1.0 / c->eps) / 2.0);
+  HCK(hipGetLastError());
+  return 1;
+}
+
+static int ensureGmres(fbh_ctx *c, int restart) {
+  if (c->V && c->gmRestart >= restart) return 1;
+  if (c->V) { hipFree(c->V); c->V = nullptr; }
+  size_t n2 = 2 * (size_t)c->n;
+  HCK(hipMalloc(&c->V, n2 * (restart + 1) * sizeof(double)));
+  if (!c->w) HCK(hipMalloc(&c->w, n2 * sizeof(double)));
+  if (!c->r) HCK(hipMalloc(&c->r, n2 * sizeof(double)));
+  if (!c->part) HCK(hipMalloc(&c->part, RED_BLOCKS * sizeof(double)));
+  if (c->hcol) hipFree(c->hcol);
+  if (c->ydev) hipFree(c->ydev);
+  HCK(hipMalloc(&c->hcol, (restart + 2) * sizeof(double)));
+  HCK(hipMalloc(&c->ydev, (restart + 2) * sizeof(double)));
+  if (!c->scal) HCK(hipMalloc(&c->scal, 4 * sizeof(double)));
+  c->gmRestart = restart;
+  return 1;
+}
+
+static int devNorm(fbh_ctx *c, const double *v, double *out) {
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, v, v, n2, c->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->part, RED_BLOCKS, c->scal);
+  HCK(hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  *out = sqrt(*out);
+  return 1;
+}
+
+// BLAS drotg / drot as used by src/gmres.c:280-293
+static void rotg(double a, double b, double *cs, double *sn) {
+  double roe = fabs(a) > fabs(b) ? a : b, scale = fabs(a) + fabs(b);
+  if (scale == 0.0) { *cs = 1.0; *sn = 0.0; return; }
+  double r = scale * sqrt((a / scale) * (a / scale) + (b / scale) * (b / scale));
+  if (roe < 0) r = -r;
+  *cs = a / r; *sn = b / r;
+}
+static void rot(double *x, double *y, double c, double s) {
+  double t = c * *x + s * *y;
+  *y = c * *y - s * *x;
+  *x = t;
+}
+
+extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit, int *iters,
+                             double *resid, double *hist, int *info) {
+  if (!c || !b || !x || restrt < 1 || maxit < 1) { setErr("fbh_gmres_dev: bad arguments"); return 0; }
+  if (!ensureGmres(c, restrt)) return 0;
+  const int n2 = 2 * c->n, ldh = restrt + 1, g = (n2 + 255) / 256;
+  const size_t vb = (size_t)n2 * sizeof(double);
+  std::vector<double> H((size_t)ldh * (restrt + 1), 0.0), s(restrt + 2), cs(restrt + 1), sn(restrt + 1),
+      y(restrt + 1), hc(restrt + 2);
+  int nh = 0;
+  *info = 0;
+  *iters = 0;
+  double xn, bnrm2, rn;
+  HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
+  if (!devNorm(c, x, &xn)) return 0;
+  if (xn != 0.0 && !fbh_matvec_dev(c, -1.0, x, 1.0, c->w)) return 0;
+  if (!fbh_psolve_diag_dev(c, c->r, c->w)) return 0;
+  if (!devNorm(c, b, &bnrm2)) return 0;
+  if (bnrm2 == 0.0) bnrm2 = 1.0;
+  if (!devNorm(c, c->r, &rn)) return 0;
+  *resid = rn / bnrm2;
+  if (*resid < tol) return 1;
+  for (;;) {
+    int i = 0;
+    if (!devNorm(c, c->r, &rn)) return 0;
+    if (!(rn > 0.0) || rn != rn) {
+      *resid = rn != rn ? rn : 0.0;
+      *info = rn != rn ? 2 : 0;
+      return 1;
+    }
+    hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, c->stream, c->r, c->V, n2, 1.0 / rn);
+    std::fill(s.begin(), s.end(), 0.0);
+    s[0] = rn;
+    for (;;) {
+      ++i;
+      ++*iters;
+      double *vi = c->V + (size_t)(i - 1) * n2, *vn = c->V + (size_t)i * n2;
+      if (!fbh_matvec_dev(c, 1.0, vi, 0.0, c->w)) return 0;
+      if (!fbh_psolve_diag_dev(c, c->w, c->w)) return 0;
+      for (int k = 0; k < i; k++) {   // modified Gram-Schmidt
+        const double *vk = c->V + (size_t)k * n2;
+        hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, vk, n2, c->part);
+        hipLaunchKernelGGL(k_mgs, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, vk, n2, c->part, c->hcol + k);
+      }
+      hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, c->w, n2, c->part);
+      hipLaunchKernelGGL(k_normalize, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, vn, n2, c->part, c->hcol + i);
+      HCK(hipGetLastError());
+      HCK(hipMemcpyAsync(hc.data(), c->hcol, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HCK(hipStreamSynchronize(c->stream));
+      double *h = &H[(size_t)(i - 1) * ldh];
+      for (int k = 0; k <= i; k++) h[k] = hc[k];
+      for (int k = 0; k < i - 1; k++) rot(&h[k], &h[k + 1], cs[k], sn[k]);
+      rotg(h[i - 1], h[i], &cs[i - 1], &sn[i - 1]);
+      rot(&h[i - 1], &h[i], cs[i - 1], sn[i - 1]);
+      rot(&s[i - 1], &s[i], cs[i - 1], sn[i - 1]);
+      *resid = fabs(s[i]) / bnrm2;
+      if (hist) hist[nh++] = *resid;
+      if (*resid <= tol || *iters == maxit || i >= restrt) break;
+    }
+    for (int k = i - 1; k >= 0; k--) {   // y = H \ s (dtrsv upper)
+      double t = s[k];
+      for (int q = k + 1; q < i; q++) t -= H[(size_t)q * ldh + k] * y[q];
+      y[k] = t / H[(size_t)k * ldh + k];
+    }
+    HCK(hipMemcpyAsync(c->ydev, y.data(), i * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_update, dim3(g), dim3(256), 0, c->stream, x, c->V, c->ydev, i, n2);
+    HCK(hipStreamSynchronize(c->stream));
+    if (*resid <= tol) return 1;
+    HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
+    if (!fbh_matvec_dev(c, -1.0, x, 1.0, c->w)) return 0;
+    if (!fbh_psolve_diag_dev(c, c->r, c->w)) return 0;
+    if (!devNorm(c, c->r, &rn)) return 0;
+    *resid = rn / bnrm2;
+    if (hist) hist[nh++] = *resid;
+    if (*resid <= tol) return 1;
+    if (*iters == maxit) { *info = 1; return 1; }
+  }
+}
+
+extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b_dev) {
+  if (!c || nChar <= 0 || !pos || !chr || !b_dev) { setErr("fbh_rhs: bad arguments"); return 0; }
+  double *dp = nullptr, *dc = nullptr;
+  HCK(hipMalloc(&dp, 3 * (size_t)nChar * sizeof(double)));
+  HCK(hipMalloc(&dc, (size_t)nChar * sizeof(double)));
+  HCK(hipMemcpy(dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
+  HCK(hipMemcpy(dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
+  double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
+  hipLaunchKernelGGL(k_rhs, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc, b_dev,
+                     fac, c->Q, c->qorder);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(dp);
+  hipFree(dc);
+  if (e != hipSuccess) { setErr("k_rhs failed: %s", hipGetErrorString(e)); return 0; }
+  return 1;
+}
+
+extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b) {
+  if (!fbh_rhs_dev(c, nChar, pos, chr, c->ys)) return 0;
+  HCK(hipMemcpy(b, c->ys, 2 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost));
+  return 1;
+}
+
+extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                              const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                              const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                              const int *ctOrder, double theta, const double *sgm_dev, double *energy_raw) {
+  if (!c || nChar <= 0 || nc <= 0 || !energy_raw) { setErr("fbh_energy: bad arguments"); return 0; }
+  const int depth = c->depth;
+  // validate the tree on the host: every index the walk dereferences
+  std::vector<int> lvStart(depth + 2, -1), lvCount(depth + 1, 0);
+  int nLeafChg = 0, maxOrd = 0;
+  for (int i = 0; i < nc; i++) {
+    int l = ctLevel[i];
+    if (l < 0 || l > depth || (i > 0 && l < ctLevel[i - 1])) { setErr("fbh_energy: bad charge-tree levels"); return 0; }
+    if (lvStart[l] < 0) lvStart[l] = i;
+    lvCount[l]++;
+    if (l < depth && ctNKids[i] > 0 && (ctKidStart[i] <= i || ctKidStart[i] + ctNKids[i] > nc || ctNKids[i] > 8)) {
+      setErr("fbh_energy: bad kid range at cube %d", i);
+      return 0;
+    }
+    if (l == depth) {
+      if (ctChgStart[i] < 0 || ctNChgs[i] < 0 || ctChgStart[i] + ctNChgs[i] > nChar) { setErr("fbh_energy: bad charge range"); return 0; }
+      nLeafChg += ctNChgs[i];
+    }
+  }
+  if (nLeafChg != nChar) { setErr("fbh_energy: leaves hold %d of %d charges", nLeafChg, nChar); return 0; }
+  for (int k = 0; k < nChar; k++)
+    if (ctChgIdx[k] < 0 || ctChgIdx[k] >= nChar) { setErr("fbh_energy: bad charge index"); return 0; }
+  std::vector<int> ord(depth + 2, 0);
+  for (int l = 0; l <= depth; l++) {
+    ord[l] = ctOrder[l];
+    if (l >= c->height) maxOrd = std::max(maxOrd, ord[l]);
+    if (ord[l] < 0 || ord[l] > c->maxOrder) { setErr("fbh_energy: charge-tree order %d beyond %d", ord[l], c->maxOrder); return 0; }
+  }
+  ord[depth + 1] = maxOrd + 1;
+  const int nmMax = c->nmMax;
+  int *dLevel, *dKid, *dNK, *dCS, *dNC, *dCI, *dOrd;
+  double *dX, *dR, *dPos, *dChr, *dMom, *dPart, *dOut;
+  std::vector<void *> tmp;
+  auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
+    if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    tmp.push_back(*dst);
+    return !src || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  bool ok = up((void **)&dLevel, ctLevel, nc * sizeof(int)) && up((void **)&dKid, ctKidStart, nc * sizeof(int)) &&
+            up((void **)&dNK, ctNKids, nc * sizeof(int)) && up((void **)&dCS, ctChgStart, nc * sizeof(int)) &&
+            up((void **)&dNC, ctNChgs, nc * sizeof(int)) && up((void **)&dCI, ctChgIdx, nChar * sizeof(int)) &&
+            up((void **)&dOrd, ord.data(), ord.size() * sizeof(int)) &&
+            up((void **)&dX, ctX, 3 * (size_t)nc * sizeof(double)) && up((void **)&dR, ctERad, nc * sizeof(double)) &&
+            up((void **)&dPos, pos, 3 * (size_t)nChar * sizeof(double)) &&
+            up((void **)&dChr, chr, nChar * sizeof(double)) &&
+            up((void **)&dMom, nullptr, (size_t)nc * nmMax * sizeof(double)) &&
+            up((void **)&dPart, nullptr, (size_t)c->n * sizeof(double)) && up((void **)&dOut, nullptr, sizeof(double));
+  int rc = 0;
+  if (ok) {
+    // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416)
+    if (lvCount[depth] > 0)
+      hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
+                         lvStart[depth], dCS, dNC, dCI, dX, dPos, dChr, ord[depth], dMom, nmMax, c->T);
+    for (int l = depth - 1; l >= c->height; l--) {
+      if (lvCount[l] <= 0) continue;
+      int nOut = NMOM(ord[l]);
+      hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), 2 * nOut * sizeof(double),
+                         c->stream, lvStart[l], dKid, dNK, dX, dOrd, l, dMom, nmMax, c->T);
+    }
+    int P1 = ord[depth + 1];
+    size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
+    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, dR, dKid,
+                       dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
+                       c->Q, c->qorder, c->T, dPart);
+    hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, c->stream, dPart, c->n, dOut);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) setErr("energy kernels failed: %s", hipGetErrorString(e));
+    else rc = 1;
+  } else {
+    setErr("fbh_energy: device allocation/upload failed");
+  }
+  hipStreamSynchronize(c->stream);
+  for (void *p : tmp) hipFree(p);
+  return rc;
+}
+
+extern "C" int fbh_profile(fbh_ctx *c, const double *x, double *y, int iters, double *ms) {
+  if (!c || iters < 1) return 0;
+  hipEvent_t ev[9];
+  for (int i = 0; i < 9; i++) HCK(hipEventCreate(&ev[i]));
+  for (int i = 0; i < 8; i++) ms[i] = 0.0;
+  double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
+  int n2 = 2 * c->n;
+  for (int it = 0; it < iters; it++) {
+    hipEventRecord(ev[0], c->stream);
+    nearPass(c, x);
+    hipEventRecord(ev[1], c->stream);
+    hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(blk64(c->nmLeaf, 512)), 0, c->stream, c->Q0, c->Q1, c->leafStart,
+                       c->leafCount, x, c->mom, c->n, c->nmLeaf, c->nmMax);
+    hipEventRecord(ev[2], c->stream);
+    for (int lev = c->depth - 1; lev >= c->height; lev--) {
+      int nOut = NMOM(c->ordMom[lev]);
+      if (c->levCount[lev] <= 0) continue;
+      hipLaunchKernelGGL(k_m2m, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), 3 * nOut * sizeof(double),
+                         c->stream, c->levStart[lev], c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX,
+                         c->mom, c->nmMax, c->T);
+    }
+    hipEventRecord(ev[3], c->stream);
+    {
+      int pm = c->maxOrder, nm = NMOM(pm);
+      size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
+      hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
+                         c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
+    }
+    hipEventRecord(ev[4], c->stream);
+    for (int lev = c->height; lev < c->depth; lev++) {
+      int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
+      if (c->levCount[lev + 1] <= 0) continue;
+      hipLaunchKernelGGL(k_l2l, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), 5 * nIn * sizeof(double),
+                         c->stream, c->levStart[lev + 1], c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->lec,
+                         c->nmMax, c->T);
+    }
+    hipEventRecord(ev[5], c->stream);
+    hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * c->nmLeaf * sizeof(double), c->stream, c->Q0, c->Q1,
+                       c->leafStart, c->leafCount, c->lec, c->far, c->n, c->nmLeaf, c->nmMax);
+    hipEventRecord(ev[6], c->stream);
+    hipLaunchKernelGGL(k_combine, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
+                       c->n, 1.0, 0.0, s1, s2, 1, 1);
+    hipEventRecord(ev[7], c->stream);
+    HCK(hipEventSynchronize(ev[7]));
+    for (int i = 0; i < 7; i++) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+      ms[i] += t;
+    }
+    float t = 0.f;
+    hipEventElapsedTime(&t, ev[0], ev[7]);
+    ms[7] += t;
+  }
+  for (int i = 0; i < 8; i++) ms[i] /= iters;
+  for (int i = 0; i < 9; i++) hipEventDestroy(ev[i]);
+  return 1;
+}
+
+extern "C" int fbh_stats(fbh_ctx *c, long long *out) {
+  if (!c) return 0;
+  out[0] = c->n; out[1] = c->nInter; out[2] = c->nM2L; out[3] = c->nmMax; out[4] = c->nmLeaf;
+  out[5] = (long long)c->bytes; out[6] = c->nLinks; out[7] = c->nCube;
+  return 1;
+}
+
+extern "C" void *fbh_stream(fbh_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" double *fbh_dev_alloc(fbh_ctx *c, long long count) {
+  double *p = nullptr;
+  if (!c || count <= 0 || hipMalloc(&p, (size_t)count * sizeof(double)) != hipSuccess) {
+    setErr("fbh_dev_alloc of %lld doubles failed", count);
+    return nullptr;
+  }
+  return p;
+}
+extern "C" void fbh_dev_free(fbh_ctx *c, double *p) {
+  if (c && c->stream) hipStreamSynchronize(c->stream);
+  if (p) hipFree(p);
+}
+static int copyOn(fbh_ctx *c, double *dst, const double *src, long long count, hipMemcpyKind k) {
+  if (!c || count < 0) { setErr("copy: bad arguments"); return 0; }
+  HCK(hipMemcpyAsync(dst, src, (size_t)count * sizeof(double), k, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+extern "C" int fbh_copy_h2d(fbh_ctx *c, double *d, const double *s, long long n) { return copyOn(c, d, s, n, hipMemcpyHostToDevice); }
+extern "C" int fbh_copy_d2h(fbh_ctx *c, double *d, const double *s, long long n) { return copyOn(c, d, s, n, hipMemcpyDeviceToHost); }
+extern "C" int fbh_copy_d2d(fbh_ctx *c, double *d, const double *s, long long n) { return copyOn(c, d, s, n, hipMemcpyDeviceToDevice); }

@@ -1,0 +1,147 @@
+/*
+ * fabipb_hip.h - the flat C ABI of libfabipb_hip.so, the MI355X (gfx950)
+ * implementation of FABIPB's Galerkin BIE matvec and its neighbours.
+ *
+ * Plain pointers and sizes only.  Host arrays are read during the call and
+ * may be freed afterwards; device state lives in HBM inside an opaque
+ * context until fbh_destroy.  Every entry point returns 1 on success and 0 on
+ * failure with a message in fbh_last_error(); nothing falls back to a CPU
+ * path.  Calls on one context are not re-entrant (like the reference's
+ * file-scope caches); different contexts are independent.
+ *
+ * Each entry cites the reference interface it replaces
+ * (include/gpu_backend.h of Jiahuic/CUDA-FABIPB and the host routines
+ * behind it).
+ */
+#ifndef FABIPB_HIP_H
+#define FABIPB_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fbh_ctx fbh_ctx;
+
+/* replaces gpuBackendAvailable (include/gpu_backend.h:11): 1 if a gfx950 device is usable */
+int fbh_available(void);
+
+/* replaces gpuNearfieldLastError / gpuM2LLastError (include/gpu_backend.h:12-13) */
+const char *fbh_last_error(void);
+
+/*
+ * Build the device-resident operator for one mesh and tree.  Replaces the
+ * per-sys-pointer caches built lazily by gpuNearfieldApply / gpuM2LApply /
+ * gpuQ2MApply (src/gpu_backend_cuda.cu:328-798): geometry, layouts, leaf
+ * Q2M/L2P matrices and nearfield coefficients are built on the device here.
+ *   kappa, eps1, eps2   Debye parameter and dielectrics (eps = eps2/eps1)
+ *   qorder              Gauss-Legendre order 1..10 (the -q option)
+ *   n, vtx[9n], nrm[3n], area[n]  panels in leaf depth-first order
+ *   depth, height, ordMom[depth+1], ordM2L[depth+1]
+ *   nCube, levStart/levCount[depth+1], cubeLevel/cubeX[3]/kidStart/nKids/parent[nCube]
+ *                       FMM cubes, finest level first (leaves are cubes 0..nLeaf-1)
+ *   nLeaf, leafStart/leafCount[nLeaf], nearOff[nLeaf+1], nearSrc[nearOff[nLeaf]]
+ *   m2lOff[nCube+1], m2lSrc[m2lOff[nCube]]   interaction lists by destination cube
+ * stream: hipStream_t the context launches on (NULL: a stream of its own,
+ * default priority).
+ */
+fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorder,
+                    int n, const double *vtx, const double *nrm, const double *area,
+                    int depth, int height, const int *ordMom, const int *ordM2L,
+                    int nCube, const int *levStart, const int *levCount,
+                    const int *cubeLevel, const double *cubeX, const int *kidStart,
+                    const int *nKids, const int *parent,
+                    int nLeaf, const int *leafStart, const int *leafCount,
+                    const int *nearOff, const int *nearSrc,
+                    const int *m2lOff, const int *m2lSrc, void *stream);
+
+/* replaces gpuReleaseMatvecCaches (include/gpu_backend.h:19) */
+void fbh_destroy(fbh_ctx *ctx);
+
+/*
+ * y = alpha*[(D - K) x] + beta*y on device pointers (2n doubles each,
+ * [potential | normal derivative]), D = diag((1+eps)/2 area, (1+1/eps)/2 area).
+ * Replaces MtVmain (src/fabipb.c:1237-1287) -> applyFMM (src/fmm.c:710-838)
+ * with all six stages on the device.  Asynchronous on the context stream.
+ */
+int fbh_matvec_dev(fbh_ctx *ctx, double alpha, const double *x, double beta, double *y);
+
+/* the same on host arrays (copies in and out, synchronous) */
+int fbh_matvec(fbh_ctx *ctx, double alpha, const double *x, double beta, double *y);
+
+/*
+ * Stage calls on host arrays, for callers that sequence the stages
+ * themselves as src/fmm.c:710-838 does:
+ *   fbh_near_apply  pot += alpha*K_near*sgm      (gpuNearfieldApply, gpu_backend.h:14)
+ *   fbh_far_apply   pot = beta*pot + alpha*K_far*sgm, Q2M+M2M+M2L+L2L+L2P
+ *                   (gpuQ2MApply + gpuM2LApply + gpuL2PApply, gpu_backend.h:16-18,
+ *                   with M2M/L2L moved onto the device as well)
+ */
+int fbh_near_apply(fbh_ctx *ctx, double alpha, const double *sgm, double *pot);
+int fbh_far_apply(fbh_ctx *ctx, double alpha, const double *sgm, double beta, double *pot);
+
+/* device expansions after the last far-field pass: nCube*2*nMom(maxOrder)
+ * moments and nCube*4*nMom(maxOrder) locals (the reference's cube->mom_*, lec_k*) */
+int fbh_expansions(fbh_ctx *ctx, double *mom, double *lec);
+/* leaf Q2M matrices, panel-major: n*nMom(ordMom[depth]) each (Q2M0 moments, Q2M1 noments) */
+int fbh_q2m_matrices(fbh_ctx *ctx, double *q0, double *q1);
+/* nearfield coefficient of destination panel x and source panel y (KER[4], numQuad.c:629-697) */
+int fbh_near_coeff(fbh_ctx *ctx, int x, int y, double *ker4);
+
+/* diagonal (Jacobi) preconditioner on device pointers, in place allowed
+ * (PtVfmmDiagonal, src/precond_fmm.c:664-728) */
+int fbh_psolve_diag_dev(fbh_ctx *ctx, double *out, const double *in);
+
+/*
+ * Restarted left-preconditioned GMRES with the diagonal preconditioner, all
+ * vectors in HBM (src/gmres.c:153-373, modified Gram-Schmidt :527-621;
+ * convergence on ||M^-1 r|| / ||b||).  b, x are device pointers; x holds the
+ * initial guess on entry.  hist (optional, host, maxit+restarts entries)
+ * receives the residual history.  Returns info as the reference does
+ * (0 converged, 1 iteration limit, 2 NaN).
+ */
+int fbh_gmres_dev(fbh_ctx *ctx, const double *b, double *x, int restart, double tol,
+                  int maxit, int *iters, double *resid, double *hist, int *info);
+
+/* direct right-hand side on host arrays: b[2n] = fac * sum_q ... (setupRHS direct loop,
+ * src/fabipb.c:789-798; gpuSetupRHS, gpu_backend.h:21), fac = 1/(4 pi eps1) */
+int fbh_rhs(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b);
+int fbh_rhs_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b_dev);
+
+/*
+ * Panel-tree solvation energy (raw, before the 2*pi*332.0716 scale) of the
+ * density sgm (device pointer, 2n) with the charge octree of the charges
+ * (applyPanelChargeTreeEnergy, src/treecode.c:927-1017; gpuPanelChargeTreeEnergy,
+ * gpu_backend.h:27).  Charge tree arrays as built by buildChargeTree
+ * (src/chargeTree.c:228-307): nc cubes over levels 0..depth in per-level DFS
+ * order; theta is the acceptance ratio (0.8).
+ */
+int fbh_energy_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                   int nc, const int *ctLevel, const double *ctX, const double *ctERad,
+                   const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                   const int *ctNChgs, const int *ctChgIdx, const int *ctOrder,
+                   double theta, const double *sgm_dev, double *energy_raw);
+
+/* per-stage device timing over `iters` matvecs (HIP events on the context stream):
+ * ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole matvec */
+int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
+
+/* sizes: out[0]=n, [1]=nearfield interactions, [2]=M2L pairs, [3]=nMom(maxOrder),
+ * [4]=nMom(ordMom[depth]), [5]=device bytes held, [6]=M2M/L2L links, [7]=nCube */
+int fbh_stats(fbh_ctx *ctx, long long *out);
+
+/* the context's stream (hipStream_t), for callers that order their own work on it */
+void *fbh_stream(fbh_ctx *ctx);
+
+/* device vectors for C callers without a HIP toolchain (the fabipb driver):
+ * allocate/free `count` doubles and copy between host and device, ordered on
+ * the context stream; the copies complete before they return */
+double *fbh_dev_alloc(fbh_ctx *ctx, long long count);
+void fbh_dev_free(fbh_ctx *ctx, double *p);
+int fbh_copy_h2d(fbh_ctx *ctx, double *dst, const double *src, long long count);
+int fbh_copy_d2h(fbh_ctx *ctx, double *dst, const double *src, long long count);
+int fbh_copy_d2d(fbh_ctx *ctx, double *dst, const double *src, long long count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

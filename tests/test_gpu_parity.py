@@ -1,0 +1,185 @@
+"""GPU parity: the HIP path (through the C ABI of libfabipb_hip.so) against the
+reference fixtures recorded from the reference CPU path and against the oracle.
+
+Bars (plain FP64): per-matvec relative L2 difference <= 1e-13; RHS and
+nearfield coefficients <= 1e-13; solvation energy relative error <= 1e-9 and
+GMRES iterations within +-1 of the reference solve; Kirkwood analytic energies
+at the discretisation level.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import CASES, cli_cases, fixture, harness_cases, rel, splitmix_vec
+import golden_io
+import kirkwood
+
+pytestmark = pytest.mark.gpu
+FL = CASES["flags"]
+
+_ops = {}
+
+
+def case_op(fa, meshdir, name):
+    if name not in _ops:
+        print(f"\n[gpu] building operator for {name}", flush=True)
+        mesh = fa.Mesh(str(meshdir / name))
+        tree = fa.Tree(mesh, CASES["cases"][name]["t"], order=FL["p"], height=FL["H"], sep=FL["S"])
+        op = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+        _ops[name] = (mesh, tree, op)
+    return _ops[name]
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_tree_matches_reference(fa, meshdir, name):
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    for k, fk in [("leafStart", "leaf_start"), ("leafCount", "leaf_count"), ("nearOff", "near_off"),
+                  ("nearSrc", "near_src"), ("m2lSrc", "m2l_src"), ("m2lDst", "m2l_dst")]:
+        assert np.array_equal(tree.a[k], F[fk]), k
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_q2m_matrices(fa, meshdir, name):
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    q0, q1 = op.q2m_matrices()
+    m = len(F["q2m0"])
+    for got, ref in ((q0[:m], F["q2m0"]), (q1[:m], F["q2m1"])):
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_near_coefficients_every_case(fa, meshdir, name):
+    """KER4 of panelIA0 for disjoint, vertex, edge (+-2) and self pairs."""
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    idx = F["ia0_idx"].reshape(-1, 3)
+    val = F["ia0_val"].reshape(-1, 4)
+    seen = set()
+    for (x, y, c), v in zip(idx, val):
+        k = op.near_coeff(x, y)
+        assert np.max(np.abs(k - v)) <= 1e-13 * np.max(np.abs(v)), (x, y, c, k, v)
+        seen.add(int(c))
+    assert {0, 1, 2, -2, 3} <= seen
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_near_apply(fa, meshdir, name):
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    assert rel(op.near_apply(F["mv_x"]), F["mv_near"]) <= 1e-13
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_fmm_apply(fa, meshdir, name):
+    """Far field + nearfield = the reference's applyFMM (alpha=1, beta=0)."""
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    far = op.far_apply(F["mv_x"])
+    assert rel(far + op.near_apply(F["mv_x"]), F["mv_fmm"]) <= 1e-13
+
+
+def test_expansions_match_reference(fa, meshdir):
+    """Moments after Q2M+M2M and locals after M2L+L2L, cube by cube."""
+    mesh, tree, op = case_op(fa, meshdir, "sph3_born")
+    F = fixture("sph3_born")
+    op.far_apply(F["mv_x"])
+    mom, lec = op.expansions()
+    assert np.max(np.abs(mom - F["mv_mom"])) <= 1e-13 * np.max(np.abs(F["mv_mom"]))
+    assert np.max(np.abs(lec - F["mv_lec"])) <= 1e-13 * np.max(np.abs(F["mv_lec"]))
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_matvec(fa, meshdir, name):
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    assert rel(op.matvec(F["mv_x"]), F["mv_y"]) <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_matvec_alpha_beta_against_oracle(fa, meshdir, name):
+    """y = alpha (D-K) x + beta y at a fresh seed, oracle on the product's own tree."""
+    import oracle_binding as O
+    mesh, tree, op = case_op(fa, meshdir, name)
+    lay = tree.oracle_layout(FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    ora = O.Oracle(lay)
+    x = splitmix_vec(777, 2 * tree.n)
+    y0 = splitmix_vec(778, 2 * tree.n)
+    got = op.matvec(x, alpha=-0.7, beta=1.3, y=y0)
+    ref = ora.matvec(x, alpha=-0.7, beta=1.3, y=y0)
+    assert rel(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_rhs(fa, meshdir, name):
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    assert rel(op.rhs(mesh.pos, mesh.chr), F["rhs"]) <= 1e-13
+
+
+@pytest.mark.parametrize("name", harness_cases())
+def test_energy_walk(fa, meshdir, name):
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    ct = tree.charge_tree()
+    s = torch.tensor(F["en_sigma"], dtype=torch.float64, device="cuda")
+    e = op.energy_dev(ct, mesh.pos, mesh.chr, s)
+    assert abs(e - F["en_raw"][0]) <= 1e-12 * abs(F["en_raw"][0])
+
+
+@pytest.mark.parametrize("name", [c for c in cli_cases() if c != "cfg2_globule"])
+def test_solve_matches_reference(fa, meshdir, name):
+    """Full plain-FP64 solve: energy rel. error <= 1e-9, GMRES iterations within +-1."""
+    F = fixture(name)
+    cli = F["cli"]
+    r = fa.solve(str(meshdir / name), CASES["cases"][name]["t"], order=FL["p"], sep=FL["S"], kappa=FL["k"],
+                 eps1=FL["eps1"], eps2=FL["eps2"], qorder=FL["q"], tol=FL["o"], restart=FL["a"],
+                 maxit=FL["i"], height=FL["H"])
+    print(f"\n[{name}] its={r['iters']} (ref {cli['iterations']}) raw={r['energy_raw']!r} "
+          f"(ref {cli['panel_tree_raw']!r})", flush=True)
+    assert abs(r["iters"] - cli["iterations"]) <= 1
+    assert abs(r["energy_raw"] - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+    if "solution" in F and r["iters"] == cli["iterations"]:
+        assert rel(r["x"], F["solution"]) <= 1e-9
+
+
+def test_solve_cfg2_globule(fa, meshdir):
+    """The bench configuration (81,920 panels, 2,000 charges, t=4) against the reference solve."""
+    name = "cfg2_globule"
+    F = fixture(name)
+    cli = F["cli"]
+    r = fa.solve(str(meshdir / name), CASES["cases"][name]["t"], order=FL["p"], sep=FL["S"], kappa=FL["k"],
+                 eps1=FL["eps1"], eps2=FL["eps2"], qorder=FL["q"], tol=FL["o"], restart=FL["a"],
+                 maxit=FL["i"], height=FL["H"])
+    print(f"\n[{name}] its={r['iters']} (ref {cli['iterations']}) raw={r['energy_raw']!r} "
+          f"(ref {cli['panel_tree_raw']!r}) gmres={r['t_gmres']:.3f}s", flush=True)
+    assert abs(r["iters"] - cli["iterations"]) <= 1
+    assert abs(r["energy_raw"] - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+
+
+def test_kirkwood_sphere(fa, meshdir):
+    """cfg1-style sphere with 50 charges: energy against the analytic Kirkwood series."""
+    name = "cfg1_sphere"
+    r = fa.solve(str(meshdir / name), CASES["cases"][name]["t"], order=FL["p"], sep=FL["S"], kappa=FL["k"],
+                 eps1=FL["eps1"], eps2=FL["eps2"], qorder=FL["q"], tol=FL["o"], height=FL["H"])
+    mesh = r["mesh"]
+    ek = kirkwood.energy(mesh.pos, mesh.chr, 10.0, FL["eps1"], FL["eps2"], FL["k"])
+    print(f"\n[kirkwood] fabipb={r['energy']:.6f} analytic={ek:.6f}", flush=True)
+    assert abs(r["energy"] - ek) <= 2e-3 * abs(ek)
+
+
+def test_matvec_linearity_full_size(fa, meshdir):
+    """Size-independent property at the bench size: A(a x + b z) = a A x + b A z."""
+    mesh, tree, op = case_op(fa, meshdir, "cfg2_globule")
+    n2 = 2 * tree.n
+    rng = np.random.default_rng(5)
+    x, z = rng.standard_normal(n2), rng.standard_normal(n2)
+    lhs = op.matvec(0.3 * x - 1.7 * z)
+    rhs = 0.3 * op.matvec(x) - 1.7 * op.matvec(z)
+    assert rel(lhs, rhs) <= 1e-13
+    # repeatability: the apply has no atomics, results are bit-identical run to run
+    assert np.array_equal(op.matvec(x), op.matvec(x))
