@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: FABIPB Galerkin BIE matvec on one MI355X.
+
+Workload: the synthetic star-shaped globule of 81,920 flat panels (level-6
+icosphere, r = 22 A (1 + 0.08 sum_{l<=6} a_lm Y_lm)), 2,000 charges at least
+1.5 A inside, eps_in=1, eps_out=80, kappa=0.1257, Taylor order 6, MAC 0.8,
+tree depth 4 (mean 74 panels per leaf), q=1, GMRES tol 1e-6 (restart 30),
+diagonal preconditioner.  Inputs come from the seeded generator
+(cuda-fabipb_amd/bin/fbmesh, seed 2), so the reference and this build see
+the same mesh and charges.
+
+A step is one matvec y = (D - K) x of the plain FP64 operator with x, y
+resident in HBM.  `value` is matvecs per second over the timed steps.  The
+JSON line also reports the full solve (RHS + GMRES + energy) wall time, its
+iteration count and energy against the reference's recorded solve, per-stage
+device times (HIP events on the launch stream), the nearfield kernel's
+roofline position, and the reference CPU path timed on the host cores.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calib-copy]
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "PB solve wall time & matvec/s at 1e-6 energy err, FP32-refined vs FP64 plain"
+MESH = ["globule", "--level", "6", "--radius", "22", "--charges", "inside", "--nq", "2000", "--sigma", "0.4",
+        "--depth", "1.5", "--seed", "2"]
+DEPTH, ORDER, MAC, KAPPA, EPS1, EPS2, Q, TOL, RESTART, MAXIT = 4, 6, 0.8, 0.1257, 1.0, 80.0, 1, 1e-6, 30, 100
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+CPU_CORES = 16             # the GPU box's host share for one GPU
+COEF_BYTES = 32            # nearfield: KER[4] doubles streamed per (dst, src) panel pair
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("fabipb_amd", os.path.join(ROOT, "cuda-fabipb_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(base, deadline_s=40.0):
+    """The reference CPU path (oracle/_ref/fabipb_ref, -g=0) on CPU_CORES host
+    threads, stopped after 2 GMRES iterations (3 matvecs): matvecs per second
+    from its own -B=1 stage accounting."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "fabipb_ref")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               VECLIB_MAXIMUM_THREADS="1", BLIS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
+               FABIPB_NEARFIELD_APPLY_THREADS=str(CPU_CORES), FABIPB_SETUP_THREADS=str(CPU_CORES),
+               FABIPB_GMRES_STOP_AFTER_ITER="2", FABIPB_STOP_AFTER_GMRES="1")
+    cmd = [exe, "-g=0", "-B=1", f"-t={DEPTH}", f"-p={ORDER}", f"-S={MAC}", f"-q={Q}", f"-k={KAPPA}",
+           f"-eps1={EPS1}", f"-eps2={EPS2}", "-H=2", f"-o={TOL}", "-P=3", f"-a={RESTART}", f"-i={MAXIT}", base]
+    print(f"[bench] cpu baseline: reference CPU path, {CPU_CORES} threads, 3 matvecs", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=deadline_s * 4)
+    wall = time.perf_counter() - t0
+    m = re.search(r"FMM matvec stats: calls=(\d+) total=([\d.]+) s", r.stdout)
+    if not m:
+        print(r.stdout[-2000:], file=sys.stderr)
+        return None
+    calls, total = int(m[1]), float(m[2])
+    return {"value": calls / total, "unit": "matvec/s", "cores": CPU_CORES, "kind": "reference",
+            "sample": f"{calls} matvecs of the same 81,920-panel workload (GMRES stopped after 2 iterations), "
+                      f"{total:.1f} s of matvec time, {wall:.1f} s wall"}
+
+
+def calib_copy(torch, mib=1024):
+    """A plain device copy of known size, for scaling the FETCH_SIZE/WRITE_SIZE counters."""
+    a = torch.ones(mib * 131072, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written per launch", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU (one MI355X per box); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    fa = load_pkg()
+    tmp = tempfile.mkdtemp(prefix="fbbench_")
+    base = os.path.join(tmp, "glob6")
+    subprocess.check_call([os.path.join(ROOT, "cuda-fabipb_amd", "bin", "fbmesh"), MESH[0], base] + MESH[1:],
+                          stdout=subprocess.DEVNULL)
+    print("[bench] mesh written", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    mesh = fa.Mesh(base)
+    tree = fa.Tree(mesh, DEPTH, order=ORDER, sep=MAC)
+    t_tree = time.perf_counter() - t0
+    op = fa.Operator(tree, KAPPA, EPS1, EPS2, Q)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0 - t_tree
+    st = op.stats()
+    n2 = 2 * tree.n
+    print(f"[bench] operator ready: n={tree.n} near={st['near_interactions']} m2l={st['m2l_pairs']} "
+          f"setup={t_setup:.3f}s", file=sys.stderr, flush=True)
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.rand(n2, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        op.matvec_dev(x, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.matvec_dev(x, y)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    print(f"[bench] timed {args.steps} matvecs: {ms_step:.3f} ms/matvec", file=sys.stderr, flush=True)
+
+    # per-stage device times with HIP events on the launch stream
+    stages = op.profile(x, y, iters=max(3, min(args.steps, 10)))
+    near_ms = stages["near"]
+    near_bytes = st["near_interactions"] * COEF_BYTES + 2 * n2 * 8
+    achieved = near_bytes / (near_ms * 1e-3)
+
+    solve = None
+    if not args.no_solve:
+        b = torch.empty(n2, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        op.rhs_dev(mesh.pos, mesh.chr, b)
+        xs = b.clone()
+        torch.cuda.synchronize()
+        tr = time.perf_counter()
+        gm = op.gmres_dev(b, xs, restart=RESTART, tol=TOL, maxit=MAXIT)
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        ct = tree.charge_tree()
+        raw = op.energy_dev(ct, mesh.pos, mesh.chr, xs)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        solve = {"wall_s": te - ts, "rhs_s": tr - ts, "gmres_s": tg - tr, "energy_s": te - tg,
+                 "iterations": gm["iters"], "matvecs": gm["iters"] + 1, "final_residual": gm["resid"],
+                 "energy_kcal_per_mol": raw * fa.ENERGY_SCALE}
+        ref = os.path.join(ROOT, "tests", "golden", "cfg2_globule", "cli.json")
+        if os.path.exists(ref):
+            R = json.load(open(ref))
+            solve["reference_iterations"] = R["iterations"]
+            solve["energy_rel_err_vs_reference"] = abs(raw - R["panel_tree_raw"]) / abs(R["panel_tree_raw"])
+            solve["reference_cpu_gmres_s_8threads"] = R["gmres_s"]
+        print(f"[bench] solve: {solve['wall_s']:.3f}s its={gm['iters']}", file=sys.stderr, flush=True)
+
+    if args.calib_copy:
+        calib_copy(torch)
+
+    cpu = None if args.no_cpu else cpu_baseline(base)
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / dt,
+        "unit": "matvec/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, identical input to the reference run)",
+        "config": {"workload": "FP64 plain matvec, star-shaped globule level-6 icosphere R=22 A, 81920 panels, "
+                               "2000 charges, eps 1/80, kappa 0.1257, Taylor order 6, MAC 0.8, depth 4, q=1",
+                   "panels": tree.n, "charges": int(len(mesh.chr)), "depth": DEPTH, "order": ORDER, "mac": MAC,
+                   "near_interactions": st["near_interactions"], "m2l_pairs": st["m2l_pairs"]},
+        "roofline": {"bound": "hbm", "kernel": "k_near_apply", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes": near_bytes, "kernel_ms": near_ms},
+        "cpu_baseline": cpu,
+        "stages_ms": stages,
+        "setup_s": {"tree": t_tree, "device_operator": t_setup},
+        "solve": solve,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

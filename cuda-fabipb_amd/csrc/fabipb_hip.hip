@@ -115,6 +115,13 @@ struct QTab {     // quadrature table for one order, passed by value
 
 // ------------------------------------------------------------------ kernels
 
+// The Galerkin quadrature, the moment columns, the right-hand side and the
+// energy walk keep the reference's operation order: no fused multiply-add
+// contraction there (FMA rounding shows up amplified in the cancellations of
+// the self-panel and near-singular terms).  The streaming kernels below
+// (nearfield apply, FMM passes) contract freely.
+#pragma clang fp contract(off)
+
 // src/kernel.c:109-147 (kernelKER4): nX target normal, nY source normal
 __device__ __forceinline__ void ker4(double kappa, double eps, const double *x, const double *nX,
                                      const double *nY, double *y) {
@@ -330,6 +337,7 @@ __global__ __launch_bounds__(256) void k_near_build(const Pnl *__restrict__ pnl,
   }
 }
 
+#pragma clang fp contract(fast)
 __device__ __forceinline__ double waveSum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -401,6 +409,7 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
 }
 
 // -------------------------------------------------- multi-index helpers
+#pragma clang fp contract(off)
 
 // mi: .x = i1 | i2<<8 | i3<<16 | deg<<24, .y = axis | fac<<8, .z = j1, .w = j2 (-1: none);
 // j1/j2 are local indices of alpha-e_axis / alpha-2e_axis inside their degree
@@ -553,6 +562,7 @@ __global__ __launch_bounds__(64) void k_q2m_mats(const Pnl *__restrict__ pnl, co
   }
 }
 
+#pragma clang fp contract(fast)
 // ------------------------------------------------------------- FMM passes
 
 // mom_pot = Q2M1 sgm, mom_dpdn = Q2M0 sgm_n (src/fmm.c:745-753)
@@ -747,6 +757,7 @@ __global__ void k_accum(const double *__restrict__ far, const double *__restrict
   y[i] = (beta == 0.0 ? 0.0 : beta * y[i]) + alpha * k;
 }
 
+#pragma clang fp contract(off)
 // --------------------------------------------------------------- RHS
 
 // src/fabipb.c:789-798 with src/numQuad.c:786-819 (panelRHS), kernelRHS (kernel.c:152-167)
@@ -926,6 +937,7 @@ __global__ __launch_bounds__(64) void k_energy(const Pnl *__restrict__ pnl, int 
   if (lane == 0) part[p] = o0 * sgm[p + n] + o1 * sgm[p];
 }
 
+#pragma clang fp contract(fast)
 // ------------------------------------------------- reductions and GMRES
 
 // block partial of sum a[i]*b[i] over a fixed grid (deterministic)
