@@ -45,6 +45,7 @@
 #define NEAR_ROWS 16      // rows per nearfield apply tile
 #define NEAR_CHUNK 2048   // source columns staged in LDS per pass (32 KiB)
 #define RED_BLOCKS 256    // fixed grid of the deterministic reductions
+#define M2L_CACHE_BYTES (16.0 * 1024 * 1024 * 1024)   // cache M2L derivative tensors up to 16 GiB
 
 static char gErr[1024];
 extern "C" const char *fbh_last_error(void) { return gErr; }
@@ -674,6 +675,98 @@ __global__ __launch_bounds__(64) void k_m2l(const int *__restrict__ m2lOff, cons
       lec[((long long)c * 4 + k) * nmMax + a] = (a < nm && a < nmc) ? acc[s][k] : 0.0;
 }
 
+// Derivative tensors D^a G0, D^a Gk at r = x_dst - x_src for every M2L pair,
+// once per operator (src/expan.c:138-209): one wave per pair, stride 2*nm.
+__global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairDst, const int *__restrict__ m2lSrc,
+                                                   const int *__restrict__ cubeLevel, const int *__restrict__ ordM2L,
+                                                   const double *__restrict__ cubeX, double *__restrict__ G,
+                                                   int nmMax, double kappa, DevTabs T) {
+  extern __shared__ double sm[];
+  const long long pr = blockIdx.x;
+  const int c = pairDst[pr], src = m2lSrc[pr];
+  const int p = ordM2L[cubeLevel[c]], dsz = derivScratch(p);
+  double r[3];
+  for (int d = 0; d < 3; d++) r[d] = cubeX[3 * c + d] - cubeX[3 * src + d];
+  double *g0 = G + pr * 2 * nmMax, *gk = g0 + nmMax;
+  derivs_block(kappa, p, r, sm, sm + dsz, g0, gk, T.mi, threadIdx.x, 64);
+}
+
+// M2L with cached derivative tensors (src/fmm.c:774-787, src/expan.c:258-290,
+// 471-494): one block of four waves per destination cube; each wave takes
+// every fourth pair of the cube's interaction list, lanes own multi-indices
+// alpha and sum over beta with the (beta, alpha+beta) table staged in LDS;
+// the four waves' partial locals are added in wave order at the end.
+#define M2L_WAVES 4
+__global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
+                                                    const int *__restrict__ cubeLevel,
+                                                    const int *__restrict__ ordM2L, const int *__restrict__ ordMom,
+                                                    const double *__restrict__ G, const double *__restrict__ mom,
+                                                    double *__restrict__ lec, int nmMax, double eps, DevTabs T) {
+  extern __shared__ double sm[];
+  const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
+  const int nTab = T.convStart[nm];   // the table of order maxOrder, entries of alpha < nm
+  int *tab = (int *)(sm + M2L_WAVES * 5 * nm);
+  for (int i = tid; i < nTab; i += 256) tab[i] = T.conv[i];
+  double *slot = sm + w * 5 * nm;
+  double *c1 = slot, *c2 = slot + nm, *v1 = slot + 2 * nm, *v2 = slot + 3 * nm, *v3 = slot + 4 * nm;
+  double acc[4][4] = {};
+  const int b0 = m2lOff[c], b1 = m2lOff[c + 1];
+  for (int base = b0; base < b1; base += M2L_WAVES) {
+    const int pr = base + w;
+    const bool act = pr < b1;
+    __syncthreads();
+    if (act) {
+      const int src = m2lSrc[pr];
+      const double *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
+      const double *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
+      for (int i = lane; i < nm; i += 64) {
+        int sg = T.ep[i].w;
+        double a0 = g0[i], ak = gk[i];
+        c1[i] = sg * mp[i];
+        c2[i] = sg * md[i];
+        v1[i] = a0 - ak;
+        v2[i] = eps * ak - a0;
+        v3[i] = a0 - ak / eps;
+      }
+    }
+    __syncthreads();
+    if (act) {
+      int s = 0;
+      for (int a = lane; a < nm; a += 64, s++) {
+        const int cnt = NMOM(p - deg_of(T.mi[a]));
+        const int *cv = tab + T.convStart[a];
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+        for (int e = 0; e < cnt; e++) {
+          int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
+          double x1 = c1[j], x2 = c2[j], y1 = v1[kk];
+          t1 += x2 * y1;
+          t2 += x1 * v2[kk];
+          t3 += x2 * v3[kk];
+          t4 += x1 * (-y1);
+        }
+        acc[s][0] += t1; acc[s][1] += t2; acc[s][2] += t3; acc[s][3] += t4;
+      }
+    }
+  }
+  __syncthreads();
+  double *red = sm;   // [wave][k][nm], reuses the slots
+  {
+    int s = 0;
+    for (int a = lane; a < nm; a += 64, s++)
+      for (int k = 0; k < 4; k++) red[(w * 4 + k) * nm + a] = acc[s][k];
+  }
+  __syncthreads();
+  const int nmc = NMOM(ordMom[cubeLevel[c]]);
+  for (int i = tid; i < 4 * nmMax; i += 256) {
+    int k = i / nmMax, a = i - k * nmMax;
+    double v = 0.0;
+    if (a < nm && a < nmc)
+      for (int ww = 0; ww < M2L_WAVES; ww++) v += red[(ww * 4 + k) * nm + a];
+    lec[((long long)c * 4 + k) * nmMax + a] = v;
+  }
+}
+
 // src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child
 __global__ void k_l2l(int levStart, const int *__restrict__ parent, const int *__restrict__ cubeLevel,
                       const int *__restrict__ ordMom, const double *__restrict__ cubeX, double *__restrict__ lec,
@@ -1055,8 +1148,11 @@ struct fbh_ctx {
   int2 *tiles;
   int nTiles;
   long long nLinks, nM2L;
+  double *m2lG;   // per pair D^a G0, D^a Gk (2*nmMax)
+  int *m2lDst;
   DevTabs T;    // contraction table: ep = (a+e1, a+e2, a+e3, sign)
   DevTabs Tm;   // moment table: ep = (a-e1, a-e2, a-e3)
+  int nConv = 0;   // entries of the (beta, alpha+beta) table
   // GMRES workspace
   int gmRestart = 0;
   double *V = nullptr, *w = nullptr, *r = nullptr, *part = nullptr, *hcol = nullptr, *ydev = nullptr, *scal = nullptr;
@@ -1176,6 +1272,7 @@ static bool buildTables(fbh_ctx *c) {
             }
       }
   cs[nmM] = (int)cl.size();
+  c->nConv = (int)cl.size();
   // convolution pairs (alpha-k, k), k <= alpha in (k1,k2,k3) loop order
   g = 0;
   for (int d = 0; d <= P; d++)
@@ -1350,9 +1447,36 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   hipLaunchKernelGGL(k_near_build, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart, c->srcOff,
                      c->ncol, c->coefOff, c->srcList, c->coef, kappa, c->eps, c->Q, qorder);
   if (hipGetLastError() != hipSuccess) { setErr("k_near_build launch failed"); FAIL(); }
+  // M2L derivative tensors per pair, cached when they fit (1344 B per pair at p=6)
+  c->m2lG = nullptr;
+  if (c->nM2L > 0 && (double)c->nM2L * 2 * c->nmMax * sizeof(double) <= M2L_CACHE_BYTES) {
+    std::vector<int> pd((size_t)c->nM2L);
+    for (int cb = 0; cb < nCube; cb++)
+      for (int k = m2lOff[cb]; k < m2lOff[cb + 1]; k++) pd[k] = cb;
+    if (!(c->m2lDst = dupload(c, pd.data(), pd.size())) ||
+        !(c->m2lG = dalloc<double>(c, (size_t)c->nM2L * 2 * c->nmMax)))
+      FAIL();
+    hipLaunchKernelGGL(k_m2l_derivs, dim3((unsigned)c->nM2L), dim3(64),
+                       2 * (size_t)derivScratch(c->maxOrder) * sizeof(double), c->stream, c->m2lDst, c->m2lSrc,
+                       c->cubeLevelD, c->ordM2LD, c->cubeX, c->m2lG, c->nmMax, kappa, c->T);
+    if (hipGetLastError() != hipSuccess) { setErr("k_m2l_derivs launch failed"); FAIL(); }
+  }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { setErr("setup kernels failed: %s", hipGetErrorString(hipGetLastError())); FAIL(); }
 #undef FAIL
   return c;
+}
+
+static void m2lPass(fbh_ctx *c) {
+  int pm = c->maxOrder, nm = NMOM(pm);
+  if (c->m2lG) {
+    size_t lds = (size_t)M2L_WAVES * 5 * nm * sizeof(double) + (size_t)c->nConv * sizeof(int);
+    hipLaunchKernelGGL(k_m2l_cached, dim3(c->nCube), dim3(256), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
+                       c->ordM2LD, c->ordMomD, c->m2lG, c->mom, c->lec, c->nmMax, c->eps, c->T);
+  } else {
+    size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
+    hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
+                       c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
+  }
 }
 
 // far field into c->far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
@@ -1367,12 +1491,7 @@ static int farPass(fbh_ctx *c, const double *x) {
                        c->stream, c->levStart[lev], c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, c->mom,
                        c->nmMax, c->T);
   }
-  {
-    int pm = c->maxOrder, nm = NMOM(pm);
-    size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
-    hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
-                       c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
-  }
+  m2lPass(c);
   for (int lev = c->height; lev < c->depth; lev++) {
     int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
     if (c->levCount[lev + 1] <= 0) continue;
@@ -1732,12 +1851,7 @@ extern "C" int fbh_profile(fbh_ctx *c, const double *x, double *y, int iters, do
                          c->mom, c->nmMax, c->T);
     }
     hipEventRecord(ev[3], c->stream);
-    {
-      int pm = c->maxOrder, nm = NMOM(pm);
-      size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
-      hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
-                         c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
-    }
+    m2lPass(c);
     hipEventRecord(ev[4], c->stream);
     for (int lev = c->height; lev < c->depth; lev++) {
       int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
@@ -1798,3 +1912,40 @@ static int copyOn(fbh_ctx *c, double *dst, const double *src, long long count, h
 extern "C" int fbh_copy_h2d(fbh_ctx *c, double *d, const double *s, long long n) { return copyOn(c, d, s, n, hipMemcpyHostToDevice); }
 extern "C" int fbh_copy_d2h(fbh_ctx *c, double *d, const double *s, long long n) { return copyOn(c, d, s, n, hipMemcpyDeviceToHost); }
 extern "C" int fbh_copy_d2d(fbh_ctx *c, double *d, const double *s, long long n) { return copyOn(c, d, s, n, hipMemcpyDeviceToDevice); }
+
+extern "C" int fbh_q2m_host(fbh_ctx *c, const double *sgm, double *mom) {
+  if (!c || !sgm || !mom) { setErr("fbh_q2m_host: null argument"); return 0; }
+  HCK(hipMemcpyAsync(c->xs, sgm, 2 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(blk64(c->nmLeaf, 512)), 0, c->stream, c->Q0, c->Q1, c->leafStart,
+                     c->leafCount, c->xs, c->mom, c->n, c->nmLeaf, c->nmMax);
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(mom, c->mom, (size_t)c->nLeaf * 2 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+extern "C" int fbh_m2l_host(fbh_ctx *c, const double *mom, double *lec) {
+  if (!c || !mom || !lec) { setErr("fbh_m2l_host: null argument"); return 0; }
+  HCK(hipMemcpyAsync(c->mom, mom, (size_t)c->nCube * 2 * c->nmMax * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  m2lPass(c);
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(lec, c->lec, (size_t)c->nCube * 4 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+extern "C" int fbh_l2p_host(fbh_ctx *c, const double *lec, double alpha, double beta, double *pot) {
+  if (!c || !lec || !pot) { setErr("fbh_l2p_host: null argument"); return 0; }
+  size_t b = 2 * (size_t)c->n * sizeof(double);
+  HCK(hipMemcpyAsync(c->lec, lec, (size_t)c->nLeaf * 4 * c->nmMax * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * c->nmLeaf * sizeof(double), c->stream, c->Q0, c->Q1,
+                     c->leafStart, c->leafCount, c->lec, c->far, c->n, c->nmLeaf, c->nmMax);
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha, beta,
+                     1, 0);
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(pot, c->ys, b, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}

@@ -1,0 +1,248 @@
+/*
+ * gpu_backend_hip.c - the reference's GPU backend ABI (the 17 functions of
+ * include/gpu_backend.h in Jiahuic/CUDA-FABIPB) implemented on
+ * libfabipb_hip.so.
+ *
+ * This is the binding a maintainer adds to the reference's own build in
+ * place of src/gpu_backend_cuda.cu: it is compiled against the reference's
+ * gk.h (struct ssystem / cube / panel) and gpu_backend.h, reads the host
+ * layout the reference already flattened (src/fmm.c:234-524), and forwards
+ * each stage call to the flat C ABI.  Return conventions are the reference's:
+ * 1 = done, 0 = not done (the caller then runs its CPU path and prints the
+ * message from gpuNearfieldLastError / gpuM2LLastError).
+ *
+ * One operator is cached per (sys, kappa, eps1, eps2, quadrature order), as
+ * the reference caches per sys pointer (src/gpu_backend_cuda.cu:342-351).
+ * Not re-entrant, like the reference's backend.
+ *
+ * Entry points without a device implementation here (dense direct operator,
+ * rhs-reuse energy, charge-tree RHS, preconditioner builders) return 0, which
+ * the reference host treats as "use the CPU path".
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gk.h"
+#include "gpu_backend.h"
+#include "fabipb_hip.h"
+
+extern double kappa, epsilon, epsilon1, epsilon2;   /* src/fabipb.c:34 */
+int rhsChargeExpansionOrder(ssystem *sys, int level);  /* src/chargeTree.c:117 */
+double energyTreeTheta(void);                          /* src/treecode.c:133 */
+
+static fbh_ctx *gCtx;
+static ssystem *gSys;
+static double gKappa, gEps1, gEps2;
+static int gQ, gNm;
+static double *gMom, *gLec;
+static char gMsg[600];
+
+static int nmom(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+
+static int fail(const char *what) {
+  snprintf(gMsg, sizeof(gMsg), "%s: %s", what, fbh_last_error());
+  return 0;
+}
+
+int gpuBackendAvailable(void) { return fbh_available(); }
+const char *gpuNearfieldLastError(void) { return gMsg[0] ? gMsg : fbh_last_error(); }
+const char *gpuM2LLastError(void) { return gMsg[0] ? gMsg : fbh_last_error(); }
+
+void gpuReleaseMatvecCaches(void) {
+  fbh_destroy(gCtx);
+  gCtx = NULL;
+  gSys = NULL;
+  free(gMom); free(gLec);
+  gMom = gLec = NULL;
+}
+void gpuReleaseChargeTreeCache(void) {}
+
+/* build the device operator from the reference's flattened layout */
+static int ensure(ssystem *sys) {
+  if (gCtx && gSys == sys && gKappa == kappa && gEps1 == epsilon1 && gEps2 == epsilon2 && gQ == sys->maxQuadOrder)
+    return 1;
+  gpuReleaseMatvecCaches();
+  if (!fbh_available()) return fail("no gfx950 device");
+  int n = sys->nPnls, depth = sys->depth, height = sys->height, nCube = sys->nFmmCubesFlat, i, k;
+  double *vtx = malloc(sizeof(double) * 9 * (size_t)n), *nrm = malloc(sizeof(double) * 3 * (size_t)n);
+  double *area = malloc(sizeof(double) * (size_t)n), *cx = malloc(sizeof(double) * 3 * (size_t)nCube);
+  int *level = malloc(sizeof(int) * (size_t)nCube), *kidStart = malloc(sizeof(int) * (size_t)nCube);
+  int *nKids = malloc(sizeof(int) * (size_t)nCube), *parent = malloc(sizeof(int) * (size_t)nCube);
+  int *m2lOff = calloc((size_t)nCube + 1, sizeof(int));
+  int ok = vtx && nrm && area && cx && level && kidStart && nKids && parent && m2lOff;
+  for (i = 0; ok && i < n; i++) {
+    panel *p = sys->panelByIdx[i];
+    for (k = 0; k < 9; k++) vtx[9 * i + k] = p->vtx[k / 3][k % 3];
+    for (k = 0; k < 3; k++) nrm[3 * i + k] = p->normal[k];
+    area[i] = p->area;
+  }
+  for (i = 0; ok && i < nCube; i++) {
+    cube *cb = sys->fmmCubeByIdx[i];
+    level[i] = cb->level;
+    for (k = 0; k < 3; k++) cx[3 * i + k] = cb->x[k];
+    nKids[i] = cb->level < depth ? cb->nKids : 0;
+    kidStart[i] = nKids[i] ? cb->kids[0]->flatIdx : -1;
+    for (k = 1; k < nKids[i]; k++)
+      if (cb->kids[k]->flatIdx != kidStart[i] + k) ok = 0;   /* kids are contiguous in the DFS layout */
+    parent[i] = cb->level > height ? cb->parent->flatIdx : -1;
+  }
+  for (i = 0; ok && i < sys->nM2LPairsFlat; i++) m2lOff[sys->m2lPairDst[i] + 1]++;
+  for (i = 0; ok && i < nCube; i++) m2lOff[i + 1] += m2lOff[i];
+  for (i = 1; ok && i < sys->nM2LPairsFlat; i++)
+    if (sys->m2lPairDst[i] < sys->m2lPairDst[i - 1]) ok = 0;   /* grouped by destination, fmm.c:356-372 */
+  if (ok) {
+    gCtx = fbh_create(kappa, epsilon1, epsilon2, sys->maxQuadOrder, n, vtx, nrm, area, depth, height, sys->ordMom,
+                      sys->ordM2L, nCube, sys->fmmLevelStart, sys->fmmLevelCount, level, cx, kidStart, nKids, parent,
+                      sys->nLeafCubesFlat, sys->leafPanelStart, sys->leafPanelCount, sys->nearLeafPairOffset,
+                      sys->nearPairSrc, m2lOff, sys->m2lPairSrc, NULL);
+  } else {
+    snprintf(gMsg, sizeof(gMsg), "layout not in the flattened form this backend reads");
+  }
+  free(vtx); free(nrm); free(area); free(cx); free(level); free(kidStart); free(nKids); free(parent); free(m2lOff);
+  if (!gCtx) return ok ? fail("operator setup") : 0;
+  long long st[8];
+  fbh_stats(gCtx, st);
+  gNm = (int)st[3];
+  gMom = calloc((size_t)nCube * 2 * gNm, sizeof(double));
+  gLec = calloc((size_t)nCube * 4 * gNm, sizeof(double));
+  gSys = sys; gKappa = kappa; gEps1 = epsilon1; gEps2 = epsilon2; gQ = sys->maxQuadOrder;
+  gMsg[0] = 0;
+  return gMom && gLec;
+}
+
+int gpuNearfieldApply(ssystem *sys, double alpha, const double *sgm, double *pot) {
+  if (!ensure(sys)) return 0;
+  return fbh_near_apply(gCtx, alpha, sgm, pot) ? 1 : fail("nearfield");
+}
+
+int gpuQ2MApply(ssystem *sys, const double *sgm) {
+  if (!ensure(sys)) return 0;
+  if (!fbh_q2m_host(gCtx, sgm, gMom)) return fail("Q2M");
+  int m = nmom(sys->ordMom[sys->depth]);
+  for (int l = 0; l < sys->nLeafCubesFlat; l++) {
+    cube *cb = sys->fmmCubeByIdx[l];
+    memcpy(cb->mom_pot, &gMom[(size_t)l * 2 * gNm], sizeof(double) * m);
+    memcpy(cb->mom_dpdn, &gMom[(size_t)l * 2 * gNm + gNm], sizeof(double) * m);
+  }
+  return 1;
+}
+
+int gpuM2LApply(ssystem *sys) {
+  if (!ensure(sys)) return 0;
+  for (int c = 0; c < sys->nFmmCubesFlat; c++) {
+    cube *cb = sys->fmmCubeByIdx[c];
+    int m = nmom(sys->ordMom[cb->level]);
+    memset(&gMom[(size_t)c * 2 * gNm], 0, sizeof(double) * 2 * gNm);
+    memcpy(&gMom[(size_t)c * 2 * gNm], cb->mom_pot, sizeof(double) * m);
+    memcpy(&gMom[(size_t)c * 2 * gNm + gNm], cb->mom_dpdn, sizeof(double) * m);
+  }
+  if (!fbh_m2l_host(gCtx, gMom, gLec)) return fail("M2L");
+  for (int c = 0; c < sys->nFmmCubesFlat; c++) {   /* overwrite, as src/gpu_backend_cuda.cu:622-681 */
+    cube *cb = sys->fmmCubeByIdx[c];
+    int m = nmom(sys->ordMom[cb->level]);
+    double *dst[4] = {cb->lec_k1, cb->lec_k2, cb->lec_k3, cb->lec_k4};
+    for (int k = 0; k < 4; k++) memcpy(dst[k], &gLec[((size_t)c * 4 + k) * gNm], sizeof(double) * m);
+  }
+  return 1;
+}
+
+int gpuL2PApply(ssystem *sys, double alpha, double beta, double *pot) {
+  if (!ensure(sys)) return 0;
+  int m = nmom(sys->ordMom[sys->depth]);
+  for (int l = 0; l < sys->nLeafCubesFlat; l++) {
+    cube *cb = sys->fmmCubeByIdx[l];
+    double *src[4] = {cb->lec_k1, cb->lec_k2, cb->lec_k3, cb->lec_k4};
+    for (int k = 0; k < 4; k++) {
+      memset(&gLec[((size_t)l * 4 + k) * gNm], 0, sizeof(double) * gNm);
+      memcpy(&gLec[((size_t)l * 4 + k) * gNm], src[k], sizeof(double) * m);
+    }
+  }
+  return fbh_l2p_host(gCtx, gLec, alpha, beta, pot) ? 1 : fail("L2P");
+}
+
+int gpuSetupRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
+  (void)fac;   /* fbh_rhs applies 1/(4 pi eps1), the fac of src/fabipb.c:755 */
+  if (qOrder != sys->maxQuadOrder || !ensure(sys)) return 0;
+  return fbh_rhs(gCtx, sys->nChar, sys->pos, sys->chr, sgm) ? 1 : fail("setupRHS");
+}
+
+int gpuPanelChargeTreeEnergy(ssystem *sys, const double *sgm, double *pot) {
+  if (!sys->chgCubeList || !ensure(sys)) return 0;
+  int depth = sys->chgDepth, nc = 0, lev, i;
+  for (lev = 0; lev <= depth; lev++)
+    for (cube *c = sys->chgCubeList[lev]; c; c = c->next) nc++;
+  int *level = malloc(sizeof(int) * (size_t)nc), *kidStart = malloc(sizeof(int) * (size_t)nc);
+  int *nKids = malloc(sizeof(int) * (size_t)nc), *chgStart = calloc((size_t)nc, sizeof(int));
+  int *nChgs = malloc(sizeof(int) * (size_t)nc), *chgIdx = malloc(sizeof(int) * (size_t)sys->nChar);
+  int *ord = malloc(sizeof(int) * (size_t)(depth + 1));
+  double *x = malloc(sizeof(double) * 3 * (size_t)nc), *eRad = malloc(sizeof(double) * (size_t)nc);
+  int j = 0, kidNext = 0, cur = 0, rc = 0;
+  for (lev = 0; lev <= depth; lev++) {
+    ord[lev] = rhsChargeExpansionOrder(sys, lev);
+    int first = j;
+    for (cube *c = sys->chgCubeList[lev]; c; c = c->next, j++) {
+      level[j] = lev;
+      for (i = 0; i < 3; i++) x[3 * j + i] = c->x[i];
+      eRad[j] = c->eRad;
+      nKids[j] = lev < depth ? c->nKids : 0;
+      nChgs[j] = c->nChgs;
+      if (lev == depth) {
+        chgStart[j] = cur;
+        for (i = 0; i < c->nChgs; i++) chgIdx[cur++] = c->chgIdx[i];
+      }
+    }
+    (void)first;
+  }
+  /* kids of consecutive parents are consecutive in the next level's list */
+  j = 0;
+  for (lev = 0; lev <= depth; lev++) {
+    int nextStart = j;
+    for (cube *c = sys->chgCubeList[lev]; c; c = c->next) nextStart++;
+    kidNext = nextStart;
+    for (cube *c = sys->chgCubeList[lev]; c; c = c->next, j++) {
+      kidStart[j] = nKids[j] ? kidNext : -1;
+      kidNext += nKids[j];
+    }
+  }
+  int n2 = 2 * sys->nPnls;
+  double *d = fbh_dev_alloc(gCtx, n2);
+  if (d && fbh_copy_h2d(gCtx, d, sgm, n2) &&
+      fbh_energy_dev(gCtx, sys->nChar, sys->pos, sys->chr, nc, level, x, eRad, kidStart, nKids, chgStart, nChgs,
+                     chgIdx, ord, energyTreeTheta(), d, pot))
+    rc = 1;
+  else
+    fail("panel-tree energy");
+  fbh_dev_free(gCtx, d);
+  free(level); free(kidStart); free(nKids); free(chgStart); free(nChgs); free(chgIdx); free(ord); free(x); free(eRad);
+  return rc;
+}
+
+/* not implemented on the device in this build: the reference runs its CPU path */
+int gpuDirectApply(ssystem *sys, double alpha, double beta, const double *sgm, double *pot) {
+  (void)sys; (void)alpha; (void)beta; (void)sgm; (void)pot;
+  snprintf(gMsg, sizeof(gMsg), "dense direct operator not in this build");
+  return 0;
+}
+int gpuRhsReuseCoulombDot(ssystem *sys, const double *rhs, const double *sgm, double *dot) {
+  (void)sys; (void)rhs; (void)sgm; (void)dot;
+  return 0;
+}
+int gpuRhsReusePanelChargeTreeEnergy(ssystem *sys, const double *rhs, const double *sgm, double *pot) {
+  (void)sys; (void)rhs; (void)sgm; (void)pot;
+  return 0;
+}
+int gpuChargeTreeRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
+  (void)sys; (void)qOrder; (void)fac; (void)sgm;
+  return 0;
+}
+int gpuBuildPrecondPairsBatched(ssystem *sys, const int *pairSrc, const int *pairDst, long long nPairs, double *k0,
+                                double *k1, double *k2, double *k3) {
+  (void)sys; (void)pairSrc; (void)pairDst; (void)nPairs; (void)k0; (void)k1; (void)k2; (void)k3;
+  return 0;
+}
+int gpuBuildPrecondDisjointBlock(panel **panels, int nPanels, const int *dstLocal, const int *srcLocal, int nPairs,
+                                 double *block) {
+  (void)panels; (void)nPanels; (void)dstLocal; (void)srcLocal; (void)nPairs; (void)block;
+  return 0;
+}

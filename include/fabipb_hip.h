@@ -79,6 +79,21 @@ int fbh_matvec(fbh_ctx *ctx, double alpha, const double *x, double beta, double 
 int fbh_near_apply(fbh_ctx *ctx, double alpha, const double *sgm, double *pot);
 int fbh_far_apply(fbh_ctx *ctx, double alpha, const double *sgm, double beta, double *pot);
 
+/*
+ * The far field one stage at a time on host arrays, for a host that runs
+ * M2M and L2L itself between them, as src/fmm.c:737-831 does:
+ *   fbh_q2m_host  leaf moments of sgm -> mom[nLeaf][2][nMom(maxOrder)]
+ *                 (gpuQ2MApply, gpu_backend.h:17: fills cube->mom_pot/mom_dpdn)
+ *   fbh_m2l_host  all FMM cubes' moments mom[nCube][2][nm] -> locals
+ *                 lec[nCube][4][nm], overwritten (gpuM2LApply, gpu_backend.h:16)
+ *   fbh_l2p_host  leaf locals lec[nLeaf][4][nm] -> pot = beta*pot + alpha*L2P(lec)
+ *                 (gpuL2PApply, gpu_backend.h:18)
+ * nm = nMom(maxOrder) is the stride; entries beyond a cube's own order are ignored.
+ */
+int fbh_q2m_host(fbh_ctx *ctx, const double *sgm, double *mom);
+int fbh_m2l_host(fbh_ctx *ctx, const double *mom, double *lec);
+int fbh_l2p_host(fbh_ctx *ctx, const double *lec, double alpha, double beta, double *pot);
+
 /* device expansions after the last far-field pass: nCube*2*nMom(maxOrder)
  * moments and nCube*4*nMom(maxOrder) locals (the reference's cube->mom_*, lec_k*) */
 int fbh_expansions(fbh_ctx *ctx, double *mom, double *lec);

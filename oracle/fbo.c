@@ -88,6 +88,10 @@ static void tables(void) { pthread_once(&gOnce, initTables); }
 
 int fbo_nmom(int order) { return NMOM(order); }
 
+void fbo_gl_table(int q, double *t, double *w) {
+  for (int i = 0; i < q; i++) { t[i] = kT[q][i]; w[i] = kW[q][i]; }
+}
+
 /* --------------------------------------------------------------- kernels */
 /* src/kernel.c:109-147 (kernelKER4); nrmX = target normal, nrmY = source */
 void fbo_ker4(double kappa, double eps, const double *x, const double *nrmX, const double *nrmY, double *y) {

@@ -66,6 +66,7 @@ int fbo_common_vtx(const double *vx, const double *vy, int *idxX, int *idxY);
 void fbo_panel_ia0(const fbo_layout *L, int x, int y, double *out);
 void fbo_derivs(double kappa, int order, const double *r, double *g0, double *gk);
 int fbo_nmom(int order);
+void fbo_gl_table(int q, double *t, double *w);   /* the Gauss-Legendre table in use */
 
 /* context: owns the leaf Q2M matrices and per-cube expansions */
 fbo_ctx *fbo_create(const fbo_layout *L);

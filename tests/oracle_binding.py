@@ -76,6 +76,7 @@ def lib():
         L.fbo_derivs.argtypes = [C.c_double, C.c_int, _dp, _dp, _dp]
         L.fbo_common_vtx.restype = C.c_int
         L.fbo_common_vtx.argtypes = [_dp, _dp, _ip, _ip]
+        L.fbo_gl_table.argtypes = [C.c_int, _dp, _dp]
         L.fbo_nmom.restype = C.c_int
         L.fbo_nmom.argtypes = [C.c_int]
         _lib = L
@@ -235,3 +236,10 @@ def derivs(kappa, order, r):
     r = np.ascontiguousarray(r, dtype=np.float64)
     lib().fbo_derivs(kappa, order, dptr(r), dptr(g0), dptr(gk))
     return g0, gk
+
+
+def gl_table(q):
+    t = np.zeros(q)
+    w = np.zeros(q)
+    lib().fbo_gl_table(q, dptr(t), dptr(w))
+    return t, w

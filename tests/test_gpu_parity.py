@@ -188,3 +188,51 @@ def test_matvec_linearity_full_size(fa, meshdir):
     assert rel(lhs, rhs) <= 1e-13
     # repeatability: the apply has no atomics, results are bit-identical run to run
     assert np.array_equal(op.matvec(x), op.matvec(x))
+
+
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_reference_host_with_hip_backend(meshdir, name):
+    """Drop-in: the reference's own host code (oracle/_ref/fabipb_ref_hip, built with
+    cuda-fabipb_amd/integration/gpu_backend_hip.c in place of its CUDA backend) runs
+    -g=1 on the GPU.  Its built-in -c=1 check compares one CPU applyFMM with the
+    GPU stages; its solve must reproduce the reference's CPU solve."""
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle/_ref/fabipb_ref_hip")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in binary not built (needs the reference sources)")
+    F = fixture(name)
+    cli = F["cli"]
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               VECLIB_MAXIMUM_THREADS="1", BLIS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
+               FABIPB_ENERGY_MODE="compare", FABIPB_NEARFIELD_APPLY_THREADS="8", FABIPB_SETUP_THREADS="8")
+    r = subprocess.run([exe, "-g=1", "-c=1", "-B=1"] + cli["flags"] + [str(meshdir / name)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0
+    assert "unavailable" not in r.stdout and "fallback" not in r.stdout.lower()
+    m = re.search(r"applyFMM debug compare: max_abs=(\S+) rel_l2=(\S+)", r.stdout)
+    assert m and float(m[2]) <= 1e-13
+    m = re.search(r"GMRES status: info=(\d+) iterations=(\d+)", r.stdout)
+    assert abs(int(m[2]) - cli["iterations"]) <= 1
+    raw = float(re.search(r"panel-tree raw=(\S+)", r.stdout)[1])
+    assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+
+
+def test_fabipb_cli_solve(meshdir):
+    """bin/fabipb (the C host driver) on the GPU: iterations and 17-digit energy."""
+    import re
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    name = "glob4"
+    cli = fixture(name)["cli"]
+    r = subprocess.run([os.path.join(root, "cuda-fabipb_amd/bin/fabipb"), "-g=1", "-B=1"] + cli["flags"] +
+                       [str(meshdir / name)], env=dict(os.environ, FABIPB_ENERGY_MODE="compare"),
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    print(r.stdout[-1200:], flush=True)
+    assert r.returncode == 0
+    m = re.search(r"GMRES status: info=(\d+) iterations=(\d+)", r.stdout)
+    assert int(m[1]) == 0 and abs(int(m[2]) - cli["iterations"]) <= 1
+    raw = float(re.search(r"panel-tree raw=(\S+)", r.stdout)[1])
+    assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+    assert f"leaf-cubes={cli['leaf_cubes']} near-pairs={cli['near_pairs']}" in r.stdout
