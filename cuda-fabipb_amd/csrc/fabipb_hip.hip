@@ -22,7 +22,13 @@
 //   k_near_apply one block per 16-row tile of a destination leaf; the
 //                sources' densities are staged in LDS once per tile, each wave
 //                streams four rows of coefficients (HBM-bound)
-//   k_q2m, k_m2m, k_m2l, k_l2l, k_l2p   the FMM passes, thread per multi-index
+//   k_q2m        four waves split a leaf's panels, lanes over multi-indices
+//   k_m2m, k_l2l thread per multi-index, translation tables in LDS
+//   k_m2l_cached per destination cube, (alpha, beta) terms balanced over the
+//                lanes by a host plan, derivative tensors cached per pair
+//                (k_m2l recomputes them when the cache does not fit)
+//   k_l2p        eight lanes per panel
+//   the nearfield runs on a second stream beside the far-field chain
 //   k_combine    y = alpha*(D x - far - near) + beta*y
 //   k_rhs        direct right-hand side, thread per panel, charges in LDS
 //   k_energy     panel-tree energy, one wave per panel walking the charge tree
@@ -566,31 +572,63 @@ __global__ __launch_bounds__(64) void k_q2m_mats(const Pnl *__restrict__ pnl, co
 #pragma clang fp contract(fast)
 // ------------------------------------------------------------- FMM passes
 
-// mom_pot = Q2M1 sgm, mom_dpdn = Q2M0 sgm_n (src/fmm.c:745-753)
-__global__ void k_q2m(const double *__restrict__ Q0, const double *__restrict__ Q1, const int *__restrict__ leafStart,
-                      const int *__restrict__ leafCount, const double *__restrict__ x, double *__restrict__ mom,
-                      int n, int nm, int nmMax) {
-  const int L = blockIdx.x, a = threadIdx.x;
-  if (a >= nm) return;
+// mom_pot = Q2M1 sgm, mom_dpdn = Q2M0 sgm_n (src/fmm.c:745-753): one block per
+// leaf, four waves split the leaf's panels (a contiguous column each), lanes own
+// multi-indices; the wave partials are added in wave order
+__global__ __launch_bounds__(256) void k_q2m(const double *__restrict__ Q0, const double *__restrict__ Q1,
+                                             const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                             const double *__restrict__ x, double *__restrict__ mom, int n, int nm,
+                                             int nmMax) {
+  extern __shared__ double sm[];   // [4 waves][2][nm]
+  const int L = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = leafStart[L], c = leafCount[L];
-  double m0 = 0.0, m1 = 0.0;
-  for (int j = 0; j < c; j++) {
-    long long col = (long long)(s + j) * nm + a;
-    m0 += Q1[col] * x[s + j];
-    m1 += Q0[col] * x[s + j + n];
+  double m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
+  for (int j = w; j < c; j += 4) {
+    const double sp = x[s + j], sn = x[s + j + n];
+    const double *q1 = Q1 + (long long)(s + j) * nm, *q0 = Q0 + (long long)(s + j) * nm;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int a = lane + 64 * k;
+      if (a < nm) {
+        m0[k] += q1[a] * sp;
+        m1[k] += q0[a] * sn;
+      }
+    }
   }
-  mom[(long long)L * 2 * nmMax + a] = m0;
-  mom[(long long)L * 2 * nmMax + nmMax + a] = m1;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int a = lane + 64 * k;
+    if (a < nm) {
+      sm[(2 * w) * nm + a] = m0[k];
+      sm[(2 * w + 1) * nm + a] = m1[k];
+    }
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < nm; a += 256) {
+    double v0 = 0.0, v1 = 0.0;
+    for (int ww = 0; ww < 4; ww++) {
+      v0 += sm[(2 * ww) * nm + a];
+      v1 += sm[(2 * ww + 1) * nm + a];
+    }
+    mom[(long long)L * 2 * nmMax + a] = v0;
+    mom[(long long)L * 2 * nmMax + nmMax + a] = v1;
+  }
 }
 
-// src/expan.c:342-382 (transM2MWs) + convM2M (:223-247): one block per parent
-__global__ void k_m2m(int levStart, const int *__restrict__ kidStart, const int *__restrict__ nKids,
-                      const int *__restrict__ cubeLevel, const int *__restrict__ ordMom,
-                      const double *__restrict__ cubeX, double *__restrict__ mom, int nmMax, DevTabs T) {
+// src/expan.c:342-382 (transM2MWs) + convM2M (:223-247): one block per parent,
+// threads own multi-indices (two each at most); the (alpha-k, k) table in LDS
+__global__ __launch_bounds__(256) void k_m2m(int levStart, const int *__restrict__ kidStart,
+                                             const int *__restrict__ nKids, const int *__restrict__ cubeLevel,
+                                             const int *__restrict__ ordMom, const double *__restrict__ cubeX,
+                                             double *__restrict__ mom, int nmMax, DevTabs T) {
   extern __shared__ double sm[];
-  const int par = levStart + blockIdx.x;
+  const int par = levStart + blockIdx.x, bd = blockDim.x, tid = threadIdx.x;
   const int ordOut = ordMom[cubeLevel[par]], nOut = NMOM(ordOut);
   double *R = sm, *b1 = sm + nOut, *b2 = sm + 2 * nOut;
+  int *ts = (int *)(sm + 3 * nOut), *tab = ts + nOut + 1;
+  const int nT = T.m2mStart[nOut];
+  for (int i = tid; i <= nOut; i += bd) ts[i] = T.m2mStart[i];
+  for (int i = tid; i < nT; i += bd) tab[i] = T.m2m[i];
   double acc1[2] = {0, 0}, acc2[2] = {0, 0};
   for (int k = 0; k < nKids[par]; k++) {
     const int kid = kidStart[par] + k;
@@ -598,29 +636,35 @@ __global__ void k_m2m(int levStart, const int *__restrict__ kidStart, const int 
     double tr[3];
     for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
     __syncthreads();
-    for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
+    for (int a = tid; a < nOut; a += bd) {
       R[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
       b1[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + a] : 0.0;
       b2[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + nmMax + a] : 0.0;
     }
     __syncthreads();
-    int s = 0;
-    for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) {
-      double t1 = 0.0, t2 = 0.0;
-      for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
-        int pr = T.m2m[e];
-        double rv = R[pr & 0xffff];
-        t1 += rv * b1[pr >> 16];
-        t2 += rv * b2[pr >> 16];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int a = tid + q * bd;
+      if (a < nOut) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int e = ts[a]; e < ts[a + 1]; e++) {
+          const int pr = tab[e];
+          const double rv = R[pr & 0xffff];
+          t1 += rv * b1[pr >> 16];
+          t2 += rv * b2[pr >> 16];
+        }
+        acc1[q] += t1;
+        acc2[q] += t2;
       }
-      acc1[s] += t1;
-      acc2[s] += t2;
     }
   }
-  int s = 0;
-  for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) {
-    mom[(long long)par * 2 * nmMax + a] = acc1[s];
-    mom[(long long)par * 2 * nmMax + nmMax + a] = acc2[s];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int a = tid + q * bd;
+    if (a < nOut) {
+      mom[(long long)par * 2 * nmMax + a] = acc1[q];
+      mom[(long long)par * 2 * nmMax + nmMax + a] = acc2[q];
+    }
   }
 }
 
@@ -652,27 +696,32 @@ __global__ __launch_bounds__(64) void k_m2l(const int *__restrict__ m2lOff, cons
       v3[i] = G0[i] - Gk[i] / eps;
     }
     __syncthreads();
-    int s = 0;
-    for (int a = tid; a < nm; a += 64, s++) {
-      const int cnt = NMOM(p - deg_of(T.mi[a]));
-      const int *cv = T.conv + T.convStart[a];
-      double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
-      for (int e = 0; e < cnt; e++) {
-        int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
-        t1 += c2[j] * v1[kk];
-        t2 += c1[j] * v2[kk];
-        t3 += c2[j] * v3[kk];
-        t4 += c1[j] * (-v1[kk]);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int a = tid + 64 * s;
+      if (a < nm) {
+        const int cnt = NMOM(p - deg_of(T.mi[a]));
+        const int *cv = T.conv + T.convStart[a];
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+        for (int e = 0; e < cnt; e++) {
+          int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
+          t1 += c2[j] * v1[kk];
+          t2 += c1[j] * v2[kk];
+          t3 += c2[j] * v3[kk];
+          t4 += c1[j] * (-v1[kk]);
+        }
+        acc[s][0] += t1; acc[s][1] += t2; acc[s][2] += t3; acc[s][3] += t4;
       }
-      acc[s][0] += t1; acc[s][1] += t2; acc[s][2] += t3; acc[s][3] += t4;
     }
     __syncthreads();
   }
   const int nmc = NMOM(ordMom[cubeLevel[c]]);
-  int s = 0;
-  for (int a = tid; a < nmMax; a += 64, s++)
-    for (int k = 0; k < 4; k++)
-      lec[((long long)c * 4 + k) * nmMax + a] = (a < nm && a < nmc) ? acc[s][k] : 0.0;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int a = tid + 64 * s;
+    if (a < nmMax)
+      for (int k = 0; k < 4; k++) lec[((long long)c * 4 + k) * nmMax + a] = (a < nm && a < nmc) ? acc[s][k] : 0.0;
+  }
 }
 
 // Derivative tensors D^a G0, D^a Gk at r = x_dst - x_src for every M2L pair,
@@ -692,27 +741,40 @@ __global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairD
 }
 
 // M2L with cached derivative tensors (src/fmm.c:774-787, src/expan.c:258-290,
-// 471-494): one block of four waves per destination cube; each wave takes
-// every fourth pair of the cube's interaction list, lanes own multi-indices
-// alpha and sum over beta with the (beta, alpha+beta) table staged in LDS;
-// the four waves' partial locals are added in wave order at the end.
-#define M2L_WAVES 4
+// 471-494): one block of nw waves (four unless LDS is short) per destination
+// cube; wave w takes pairs w, w + nw, ... of the cube's interaction list.  The
+// (alpha, beta) terms of the convolution are cut into items (short runs of one alpha)
+// that the host spreads over the 64 lanes by term count, so every lane walks a
+// list of about 1/64 of the terms instead of one lane carrying alpha = 0's long
+// beta list.  The last term of an item carries the item number: there the lane
+// adds its running sums into the item's slot in LDS (its own, per wave).  The
+// slots are summed per alpha at the end, in item then wave order.
+#define M2L_FLUSH 0x80000000u
 __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
                                                     const int *__restrict__ cubeLevel,
                                                     const int *__restrict__ ordM2L, const int *__restrict__ ordMom,
                                                     const double *__restrict__ G, const double *__restrict__ mom,
-                                                    double *__restrict__ lec, int nmMax, double eps, DevTabs T) {
+                                                    double *__restrict__ lec, int nmMax, double eps, DevTabs T,
+                                                    const unsigned *__restrict__ planTerms,
+                                                    const int *__restrict__ planIdx, const int4 *__restrict__ planOff) {
   extern __shared__ double sm[];
-  const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
   const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
-  const int nTab = T.convStart[nm];   // the table of order maxOrder, entries of alpha < nm
-  int *tab = (int *)(sm + M2L_WAVES * 5 * nm);
-  for (int i = tid; i < nTab; i += 256) tab[i] = T.conv[i];
-  double *slot = sm + w * 5 * nm;
-  double *c1 = slot, *c2 = slot + nm, *v1 = slot + 2 * nm, *v2 = slot + 3 * nm, *v3 = slot + 4 * nm;
-  double acc[4][4] = {};
+  // plan of order p: x = first term, y = term count, z = first index entry, w = item count;
+  // index entries: 65 lane starts into the terms, then nm + 1 item starts per alpha
+  const int4 po = planOff[p];
+  const int nTerms = po.y, nItems = po.w;
+  double *slots = sm;                                  // [wave][5][nm]
+  double *red = sm + nw * 5 * nm;                      // [wave][item][4]
+  unsigned *tab = (unsigned *)(red + nw * 4 * nItems);
+  int *idx = (int *)(tab + nTerms);
+  for (int i = tid; i < nTerms; i += blockDim.x) tab[i] = planTerms[po.x + i];
+  for (int i = tid; i < 65 + nm + 1; i += blockDim.x) idx[i] = planIdx[po.z + i];
+  for (int i = tid; i < nw * 4 * nItems; i += blockDim.x) red[i] = 0.0;
+  double *c1 = slots + w * 5 * nm, *c2 = c1 + nm, *v1 = c1 + 2 * nm, *v2 = c1 + 3 * nm, *v3 = c1 + 4 * nm;
+  double *myRed = red + w * 4 * nItems;
   const int b0 = m2lOff[c], b1 = m2lOff[c + 1];
-  for (int base = b0; base < b1; base += M2L_WAVES) {
+  for (int base = b0; base < b1; base += nw) {
     const int pr = base + w;
     const bool act = pr < b1;
     __syncthreads();
@@ -732,50 +794,52 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
     }
     __syncthreads();
     if (act) {
-      int s = 0;
-      for (int a = lane; a < nm; a += 64, s++) {
-        const int cnt = NMOM(p - deg_of(T.mi[a]));
-        const int *cv = tab + T.convStart[a];
-        double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
-        for (int e = 0; e < cnt; e++) {
-          int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
-          double x1 = c1[j], x2 = c2[j], y1 = v1[kk];
-          t1 += x2 * y1;
-          t2 += x1 * v2[kk];
-          t3 += x2 * v3[kk];
-          t4 += x1 * (-y1);
+      double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+      const int e1 = idx[lane + 1];
+      for (int e = idx[lane]; e < e1; e++) {
+        const unsigned pe = tab[e];
+        const int j = pe & 0xff, kk = (pe >> 8) & 0xff;
+        const double x1 = c1[j], x2 = c2[j], y1 = v1[kk];
+        t1 += x2 * y1;
+        t2 += x1 * v2[kk];
+        t3 += x2 * v3[kk];
+        t4 += x1 * (-y1);
+        if (pe & M2L_FLUSH) {
+          double *r = myRed + 4 * ((pe >> 16) & 0x7fff);
+          r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
+          t1 = t2 = t3 = t4 = 0.0;
         }
-        acc[s][0] += t1; acc[s][1] += t2; acc[s][2] += t3; acc[s][3] += t4;
       }
     }
   }
   __syncthreads();
-  double *red = sm;   // [wave][k][nm], reuses the slots
-  {
-    int s = 0;
-    for (int a = lane; a < nm; a += 64, s++)
-      for (int k = 0; k < 4; k++) red[(w * 4 + k) * nm + a] = acc[s][k];
-  }
-  __syncthreads();
   const int nmc = NMOM(ordMom[cubeLevel[c]]);
-  for (int i = tid; i < 4 * nmMax; i += 256) {
-    int k = i / nmMax, a = i - k * nmMax;
+  const int *itemStart = idx + 65;
+  for (int i = tid; i < 4 * nmMax; i += blockDim.x) {
+    const int k = i / nmMax, a = i - k * nmMax;
     double v = 0.0;
     if (a < nm && a < nmc)
-      for (int ww = 0; ww < M2L_WAVES; ww++) v += red[(ww * 4 + k) * nm + a];
+      for (int it = itemStart[a]; it < itemStart[a + 1]; it++)
+        for (int ww = 0; ww < nw; ww++) v += red[(ww * nItems + it) * 4 + k];
     lec[((long long)c * 4 + k) * nmMax + a] = v;
   }
 }
 
-// src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child
-__global__ void k_l2l(int levStart, const int *__restrict__ parent, const int *__restrict__ cubeLevel,
-                      const int *__restrict__ ordMom, const double *__restrict__ cubeX, double *__restrict__ lec,
-                      int nmMax, DevTabs T) {
+// src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child,
+// thread per output multi-index, (beta, alpha+beta) table in LDS
+__global__ __launch_bounds__(512) void k_l2l(int levStart, const int *__restrict__ parent,
+                                             const int *__restrict__ cubeLevel, const int *__restrict__ ordMom,
+                                             const double *__restrict__ cubeX, double *__restrict__ lec, int nmMax,
+                                             DevTabs T) {
   extern __shared__ double sm[];
   const int kid = levStart + blockIdx.x, par = parent[kid];
   const int ordIn = ordMom[cubeLevel[par]], ordOut = ordMom[cubeLevel[kid]];
   const int nIn = NMOM(ordIn), nOut = NMOM(ordOut);
   double *R = sm, *In = sm + nIn;   // In[4][nIn]
+  int *cst = (int *)(sm + 5 * nIn), *tab = cst + nOut + 1;
+  const int nT = T.convStart[nOut];
+  for (int i = threadIdx.x; i <= nOut; i += blockDim.x) cst[i] = T.convStart[i];
+  for (int i = threadIdx.x; i < nT; i += blockDim.x) tab[i] = T.conv[i];
   double tr[3];
   for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
   for (int a = threadIdx.x; a < nIn; a += blockDim.x) {
@@ -785,7 +849,7 @@ __global__ void k_l2l(int levStart, const int *__restrict__ parent, const int *_
   __syncthreads();
   for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
     const int cnt = NMOM(ordIn - deg_of(T.mi[a]));
-    const int *cv = T.conv + T.convStart[a];
+    const int *cv = tab + cst[a];
     double t[4] = {0, 0, 0, 0};
     for (int e = 0; e < cnt; e++) {
       int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
@@ -797,7 +861,7 @@ __global__ void k_l2l(int levStart, const int *__restrict__ parent, const int *_
 }
 
 // far[p] = Q2M0^T (l1 + l2), far[p+n] = Q2M1^T (l3 + l4) (src/fmm.c:815-828):
-// one block per leaf, one wave per panel, lanes over multi-indices
+// one block per leaf, eight lanes per panel read its column in 64-byte pieces
 __global__ __launch_bounds__(256) void k_l2p(const double *__restrict__ Q0, const double *__restrict__ Q1,
                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
                                              const double *__restrict__ lec, double *__restrict__ far, int n,
@@ -811,17 +875,22 @@ __global__ __launch_bounds__(256) void k_l2p(const double *__restrict__ Q0, cons
     L34[a] = l[2 * nmMax + a] + l[3 * nmMax + a];
   }
   __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, s = leafStart[L], c = leafCount[L];
-  for (int j = w; j < c; j += 4) {
-    const double *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
+  const int g = threadIdx.x >> 3, l8 = threadIdx.x & 7, s = leafStart[L], c = leafCount[L];
+  for (int j0 = 0; j0 < c; j0 += 32) {   // uniform trip count: every lane takes part in the shuffles
+    const int j = j0 + g;
     double t0 = 0.0, t1 = 0.0;
-    for (int a = lane; a < nm; a += 64) {
-      t0 += q0[a] * L12[a];
-      t1 += q1[a] * L34[a];
+    if (j < c) {
+      const double *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
+      for (int a = l8; a < nm; a += 8) {
+        t0 += q0[a] * L12[a];
+        t1 += q1[a] * L34[a];
+      }
     }
-    t0 = waveSum(t0);
-    t1 = waveSum(t1);
-    if (lane == 0) {
+    for (int o = 4; o > 0; o >>= 1) {
+      t0 += __shfl_xor(t0, o, 64);
+      t1 += __shfl_xor(t1, o, 64);
+    }
+    if (j < c && l8 == 0) {
       far[s + j] = t0;
       far[s + j + n] = t1;
     }
@@ -933,18 +1002,24 @@ __global__ void k_ct_m2m(int levStart, const int *__restrict__ kidStart, const i
       b[a] = a < nIn ? cmom[(long long)kid * nmMax + a] : 0.0;
     }
     __syncthreads();
-    int s = 0;
-    for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) {
-      double t = 0.0;
-      for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
-        int pr = T.m2m[e];
-        if ((pr >> 16) < nIn) t += R[pr & 0xffff] * b[pr >> 16];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int a = threadIdx.x + s * blockDim.x;
+      if (a < nOut) {
+        double t = 0.0;
+        for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
+          int pr = T.m2m[e];
+          if ((pr >> 16) < nIn) t += R[pr & 0xffff] * b[pr >> 16];
+        }
+        acc[s] += t;
       }
-      acc[s] += t;
     }
   }
-  int s = 0;
-  for (int a = threadIdx.x; a < nOut; a += blockDim.x, s++) cmom[(long long)par * nmMax + a] = acc[s];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int a = threadIdx.x + s * blockDim.x;
+    if (a < nOut) cmom[(long long)par * nmMax + a] = acc[s];
+  }
 }
 
 #define EN_STACK 512
@@ -1153,6 +1228,18 @@ struct fbh_ctx {
   DevTabs T;    // contraction table: ep = (a+e1, a+e2, a+e3, sign)
   DevTabs Tm;   // moment table: ep = (a-e1, a-e2, a-e3)
   int nConv = 0;   // entries of the (beta, alpha+beta) table
+  std::vector<int> hConvStart, hM2mStart;   // host copies of the table offsets (LDS sizing)
+  std::vector<int> hConv;
+  // balanced M2L plan per order (cached-tensor path)
+  unsigned *planTerms = nullptr;
+  int *planIdx = nullptr;
+  int4 *planOff = nullptr;
+  size_t m2lLds = 0;
+  int m2lWaves = 0;
+  int ldsMax = 65536;
+  // nearfield on a second stream, overlapped with the far-field chain
+  hipStream_t side = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   // GMRES workspace
   int gmRestart = 0;
   double *V = nullptr, *w = nullptr, *r = nullptr, *part = nullptr, *hcol = nullptr, *ydev = nullptr, *scal = nullptr;
@@ -1208,6 +1295,12 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   if (c->hcol) hipFree(c->hcol);
   if (c->ydev) hipFree(c->ydev);
   if (c->scal) hipFree(c->scal);
+  if (c->side) {
+    hipStreamSynchronize(c->side);
+    hipStreamDestroy(c->side);
+  }
+  if (c->evFork) hipEventDestroy(c->evFork);
+  if (c->evJoin) hipEventDestroy(c->evJoin);
   if (c->ownStream && c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1294,8 +1387,97 @@ static bool buildTables(fbh_ctx *c) {
   int *dms = dupload(c, ms.data(), ms.size()), *dml = dupload(c, ml.data(), ml.size());
   c->fact = dupload(c, fact.data(), fact.size());
   if (!dmi || !dep || !dem || !dif || !dcs || !dcl || !dms || !dml || !c->fact) return false;
+  c->hConvStart = cs;
+  c->hConv = cl;
+  c->hM2mStart = ms;
   c->T = DevTabs{dmi, dep, dif, dcs, dcl, dms, dml};
   c->Tm = DevTabs{dmi, dem, dif, dcs, dcl, dms, dml};
+  return true;
+}
+
+// Balanced M2L plan (k_m2l_cached) for every order 0..maxOrder: alpha's beta
+// list is cut into items of `chunk` terms, items go to the least loaded of the
+// 64 lanes, longest first; the last term of an item is flagged with its number.
+// Returns the dynamic LDS the kernel needs with `waves` waves, at the largest
+// order used by a level.
+static size_t m2lPlanLds(const fbh_ctx *c, const std::vector<int4> &off, int waves) {
+  size_t best = 0;
+  for (int l = c->height; l <= c->depth; l++) {
+    const int p = c->ordM2L[l], nm = NMOM(p);
+    const int4 po = off[p];
+    size_t b = ((size_t)waves * 5 * nm + (size_t)waves * 4 * po.w) * sizeof(double) + (size_t)po.y * sizeof(unsigned) +
+               (size_t)(65 + nm + 1) * sizeof(int);
+    best = std::max(best, b);
+  }
+  return best;
+}
+
+static bool buildM2LPlan(fbh_ctx *c) {
+  std::vector<unsigned> terms;
+  std::vector<int> idx;
+  std::vector<int4> off(c->maxOrder + 1);
+  std::vector<int> deg(NMOM(c->maxOrder));
+  for (int d = 0, g = 0; d <= c->maxOrder; d++)
+    for (int k = 0; k < (d + 1) * (d + 2) / 2; k++) deg[g++] = d;
+  for (int p = 0; p <= c->maxOrder; p++) {
+    const int nm = NMOM(p);
+    long long total = 0;
+    for (int a = 0; a < nm; a++) total += NMOM(p - deg[a]);
+    const int chunk = std::max(8, (int)((total + 127) / 128));
+    struct Item { int alpha, first, len; };
+    std::vector<Item> items;
+    std::vector<int> itemStart(nm + 1);
+    for (int a = 0; a < nm; a++) {
+      itemStart[a] = (int)items.size();
+      const int cnt = NMOM(p - deg[a]);
+      for (int f = 0; f < cnt; f += chunk) items.push_back({a, f, std::min(chunk, cnt - f)});
+    }
+    itemStart[nm] = (int)items.size();
+    if (items.size() > 0x7fff) { setErr("M2L plan: %zu items at order %d", items.size(), p); return false; }
+    std::vector<int> order(items.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return items[x].len > items[y].len; });
+    std::vector<std::vector<int>> lane(64);
+    std::vector<long long> load(64, 0);
+    for (int it : order) {
+      int best = 0;
+      for (int l = 1; l < 64; l++)
+        if (load[l] < load[best]) best = l;
+      lane[best].push_back(it);
+      load[best] += items[it].len;
+    }
+    int4 po;
+    po.x = (int)terms.size();
+    po.z = (int)idx.size();
+    po.w = (int)items.size();
+    std::vector<int> laneStart(65);
+    for (int l = 0; l < 64; l++) {
+      laneStart[l] = (int)terms.size() - po.x;
+      for (int it : lane[l]) {
+        const Item &I = items[it];
+        const int base = c->hConvStart[I.alpha] + I.first;
+        for (int e = 0; e < I.len; e++) {
+          const int pe = c->hConv[base + e];
+          unsigned t = (unsigned)(pe & 0xffff) | ((unsigned)(pe >> 16) << 8);
+          if (e == I.len - 1) t |= M2L_FLUSH | ((unsigned)it << 16);
+          terms.push_back(t);
+        }
+      }
+    }
+    laneStart[64] = (int)terms.size() - po.x;
+    po.y = laneStart[64];
+    idx.insert(idx.end(), laneStart.begin(), laneStart.end());
+    idx.insert(idx.end(), itemStart.begin(), itemStart.end());
+    off[p] = po;
+  }
+  c->m2lWaves = 0;
+  for (int wv = 4; wv >= 1; wv /= 2)
+    if (m2lPlanLds(c, off, wv) <= (size_t)c->ldsMax) { c->m2lWaves = wv; break; }
+  if (!c->m2lWaves) return true;   // no plan fits: the recompute kernel is used
+  c->m2lLds = m2lPlanLds(c, off, c->m2lWaves);
+  if (!(c->planTerms = dupload(c, terms.data(), terms.size())) || !(c->planIdx = dupload(c, idx.data(), idx.size())) ||
+      !(c->planOff = dupload(c, off.data(), off.size())))
+    return false;
   return true;
 }
 
@@ -1319,7 +1501,13 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
     return nullptr;
   }
   int maxOrder = 0;
-  for (int l = height; l <= depth; l++) maxOrder = std::max(maxOrder, std::max(ordMom[l], ordM2L[l]));
+  for (int l = height; l <= depth; l++) {
+    if (ordMom[l] < 0 || ordM2L[l] < 0) {
+      setErr("fbh_create: negative expansion order at level %d", l);
+      return nullptr;
+    }
+    maxOrder = std::max(maxOrder, std::max(ordMom[l], ordM2L[l]));
+  }
   if (maxOrder < 1 || maxOrder > 9) {
     setErr("fbh_create: expansion order %d outside 1..9", maxOrder);
     return nullptr;
@@ -1364,6 +1552,18 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
     c->ownStream = true;
   }
 #define FAIL() do { fbh_destroy(c); return nullptr; } while (0)
+  if (hipStreamCreate(&c->side) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess) {
+    setErr("stream/event creation failed");
+    FAIL();
+  }
+  {
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+      c->ldsMax = std::min(lds, 65536);
+  }
   if (!buildTables(c)) FAIL();
 
   // panels
@@ -1449,7 +1649,8 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   if (hipGetLastError() != hipSuccess) { setErr("k_near_build launch failed"); FAIL(); }
   // M2L derivative tensors per pair, cached when they fit (1344 B per pair at p=6)
   c->m2lG = nullptr;
-  if (c->nM2L > 0 && (double)c->nM2L * 2 * c->nmMax * sizeof(double) <= M2L_CACHE_BYTES) {
+  if (!buildM2LPlan(c)) FAIL();
+  if (c->m2lWaves > 0 && c->nM2L > 0 && (double)c->nM2L * 2 * c->nmMax * sizeof(double) <= M2L_CACHE_BYTES) {
     std::vector<int> pd((size_t)c->nM2L);
     for (int cb = 0; cb < nCube; cb++)
       for (int k = m2lOff[cb]; k < m2lOff[cb + 1]; k++) pd[k] = cb;
@@ -1466,53 +1667,69 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   return c;
 }
 
-static void m2lPass(fbh_ctx *c) {
+static void m2lPass(fbh_ctx *c, hipStream_t st) {
   int pm = c->maxOrder, nm = NMOM(pm);
   if (c->m2lG) {
-    size_t lds = (size_t)M2L_WAVES * 5 * nm * sizeof(double) + (size_t)c->nConv * sizeof(int);
-    hipLaunchKernelGGL(k_m2l_cached, dim3(c->nCube), dim3(256), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
-                       c->ordM2LD, c->ordMomD, c->m2lG, c->mom, c->lec, c->nmMax, c->eps, c->T);
+    hipLaunchKernelGGL(k_m2l_cached, dim3(c->nCube), dim3(64 * c->m2lWaves), c->m2lLds, st, c->m2lOff, c->m2lSrc,
+                       c->cubeLevelD, c->ordM2LD, c->ordMomD, c->m2lG, c->mom, c->lec, c->nmMax, c->eps, c->T,
+                       c->planTerms, c->planIdx, c->planOff);
   } else {
     size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
-    hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
-                       c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
+    hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, st, c->m2lOff, c->m2lSrc, c->cubeLevelD, c->ordM2LD,
+                       c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
   }
+}
+
+static void q2mPass(fbh_ctx *c, const double *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(256), 8 * (size_t)c->nmLeaf * sizeof(double), st, c->Q0, c->Q1,
+                     c->leafStart, c->leafCount, x, c->mom, c->n, c->nmLeaf, c->nmMax);
+}
+
+static void m2mPass(fbh_ctx *c, int lev, hipStream_t st) {   // children at lev + 1 into parents at lev
+  const int nOut = NMOM(c->ordMom[lev]);
+  if (c->levCount[lev] <= 0) return;
+  size_t lds = 3 * (size_t)nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut]) * sizeof(int);
+  hipLaunchKernelGGL(k_m2m, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), lds, st, c->levStart[lev],
+                     c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, c->mom, c->nmMax, c->T);
+}
+
+static void l2lPass(fbh_ctx *c, int lev, hipStream_t st) {   // parents at lev into children at lev + 1
+  const int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
+  if (c->levCount[lev + 1] <= 0) return;
+  size_t lds = 5 * (size_t)nIn * sizeof(double) + (size_t)(nOut + 1 + c->hConvStart[nOut]) * sizeof(int);
+  hipLaunchKernelGGL(k_l2l, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), lds, st, c->levStart[lev + 1],
+                     c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->lec, c->nmMax, c->T);
+}
+
+static void l2pPass(fbh_ctx *c, hipStream_t st) {
+  hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * (size_t)c->nmLeaf * sizeof(double), st, c->Q0, c->Q1,
+                     c->leafStart, c->leafCount, c->lec, c->far, c->n, c->nmLeaf, c->nmMax);
 }
 
 // far field into c->far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
-static int farPass(fbh_ctx *c, const double *x) {
-  const int nmL = c->nmLeaf;
-  hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(blk64(nmL, 512)), 0, c->stream, c->Q0, c->Q1, c->leafStart,
-                     c->leafCount, x, c->mom, c->n, nmL, c->nmMax);
-  for (int lev = c->depth - 1; lev >= c->height; lev--) {
-    int nOut = NMOM(c->ordMom[lev]);
-    if (c->levCount[lev] <= 0) continue;
-    hipLaunchKernelGGL(k_m2m, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), 3 * nOut * sizeof(double),
-                       c->stream, c->levStart[lev], c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, c->mom,
-                       c->nmMax, c->T);
-  }
-  m2lPass(c);
-  for (int lev = c->height; lev < c->depth; lev++) {
-    int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
-    if (c->levCount[lev + 1] <= 0) continue;
-    hipLaunchKernelGGL(k_l2l, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), 5 * nIn * sizeof(double),
-                       c->stream, c->levStart[lev + 1], c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->lec,
-                       c->nmMax, c->T);
-  }
-  hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * nmL * sizeof(double), c->stream, c->Q0, c->Q1,
-                     c->leafStart, c->leafCount, c->lec, c->far, c->n, nmL, c->nmMax);
-  return 1;
+static void farPass(fbh_ctx *c, const double *x, hipStream_t st) {
+  q2mPass(c, x, st);
+  for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, lev, st);
+  m2lPass(c, st);
+  for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, lev, st);
+  l2pPass(c, st);
 }
 
-static void nearPass(fbh_ctx *c, const double *x) {
-  hipLaunchKernelGGL(k_near_apply, dim3(c->nTiles), dim3(256), 0, c->stream, c->tiles, c->leafStart, c->leafCount,
+static void nearPass(fbh_ctx *c, const double *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_near_apply, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
                      c->srcOff, c->ncol, c->coefOff, c->srcList, c->coef, x, c->nearv, c->n);
 }
 
 extern "C" int fbh_matvec_dev(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
   if (!c || !x || !y) { setErr("fbh_matvec_dev: null argument"); return 0; }
-  farPass(c, x);
-  nearPass(c, x);
+  // the nearfield runs on the side stream beside the far-field chain; both
+  // read x only, and the combine waits for both
+  HCK(hipEventRecord(c->evFork, c->stream));
+  HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
+  nearPass(c, x, c->side);
+  HCK(hipEventRecord(c->evJoin, c->side));
+  farPass(c, x, c->stream);
+  HCK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
   double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_combine, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
@@ -1538,8 +1755,8 @@ static int stageApply(fbh_ctx *c, double alpha, const double *sgm, double beta, 
   size_t b = 2 * (size_t)c->n * sizeof(double);
   HCK(hipMemcpyAsync(c->xs, sgm, b, hipMemcpyHostToDevice, c->stream));
   HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
-  if (useFar) farPass(c, c->xs);
-  if (useNear) nearPass(c, c->xs);
+  if (useFar) farPass(c, c->xs, c->stream);
+  if (useNear) nearPass(c, c->xs, c->stream);
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
                      beta, useFar, useNear);
@@ -1838,31 +2055,17 @@ extern "C" int fbh_profile(fbh_ctx *c, const double *x, double *y, int iters, do
   int n2 = 2 * c->n;
   for (int it = 0; it < iters; it++) {
     hipEventRecord(ev[0], c->stream);
-    nearPass(c, x);
+    nearPass(c, x, c->stream);
     hipEventRecord(ev[1], c->stream);
-    hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(blk64(c->nmLeaf, 512)), 0, c->stream, c->Q0, c->Q1, c->leafStart,
-                       c->leafCount, x, c->mom, c->n, c->nmLeaf, c->nmMax);
+    q2mPass(c, x, c->stream);
     hipEventRecord(ev[2], c->stream);
-    for (int lev = c->depth - 1; lev >= c->height; lev--) {
-      int nOut = NMOM(c->ordMom[lev]);
-      if (c->levCount[lev] <= 0) continue;
-      hipLaunchKernelGGL(k_m2m, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), 3 * nOut * sizeof(double),
-                         c->stream, c->levStart[lev], c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX,
-                         c->mom, c->nmMax, c->T);
-    }
+    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, lev, c->stream);
     hipEventRecord(ev[3], c->stream);
-    m2lPass(c);
+    m2lPass(c, c->stream);
     hipEventRecord(ev[4], c->stream);
-    for (int lev = c->height; lev < c->depth; lev++) {
-      int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
-      if (c->levCount[lev + 1] <= 0) continue;
-      hipLaunchKernelGGL(k_l2l, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), 5 * nIn * sizeof(double),
-                         c->stream, c->levStart[lev + 1], c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->lec,
-                         c->nmMax, c->T);
-    }
+    for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, lev, c->stream);
     hipEventRecord(ev[5], c->stream);
-    hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * c->nmLeaf * sizeof(double), c->stream, c->Q0, c->Q1,
-                       c->leafStart, c->leafCount, c->lec, c->far, c->n, c->nmLeaf, c->nmMax);
+    l2pPass(c, c->stream);
     hipEventRecord(ev[6], c->stream);
     hipLaunchKernelGGL(k_combine, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
                        c->n, 1.0, 0.0, s1, s2, 1, 1);
@@ -1916,8 +2119,7 @@ extern "C" int fbh_copy_d2d(fbh_ctx *c, double *d, const double *s, long long n)
 extern "C" int fbh_q2m_host(fbh_ctx *c, const double *sgm, double *mom) {
   if (!c || !sgm || !mom) { setErr("fbh_q2m_host: null argument"); return 0; }
   HCK(hipMemcpyAsync(c->xs, sgm, 2 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(blk64(c->nmLeaf, 512)), 0, c->stream, c->Q0, c->Q1, c->leafStart,
-                     c->leafCount, c->xs, c->mom, c->n, c->nmLeaf, c->nmMax);
+  q2mPass(c, c->xs, c->stream);
   HCK(hipGetLastError());
   HCK(hipMemcpyAsync(mom, c->mom, (size_t)c->nLeaf * 2 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
@@ -1927,7 +2129,7 @@ extern "C" int fbh_q2m_host(fbh_ctx *c, const double *sgm, double *mom) {
 extern "C" int fbh_m2l_host(fbh_ctx *c, const double *mom, double *lec) {
   if (!c || !mom || !lec) { setErr("fbh_m2l_host: null argument"); return 0; }
   HCK(hipMemcpyAsync(c->mom, mom, (size_t)c->nCube * 2 * c->nmMax * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  m2lPass(c);
+  m2lPass(c, c->stream);
   HCK(hipGetLastError());
   HCK(hipMemcpyAsync(lec, c->lec, (size_t)c->nCube * 4 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
@@ -1939,8 +2141,7 @@ extern "C" int fbh_l2p_host(fbh_ctx *c, const double *lec, double alpha, double 
   size_t b = 2 * (size_t)c->n * sizeof(double);
   HCK(hipMemcpyAsync(c->lec, lec, (size_t)c->nLeaf * 4 * c->nmMax * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * c->nmLeaf * sizeof(double), c->stream, c->Q0, c->Q1,
-                     c->leafStart, c->leafCount, c->lec, c->far, c->n, c->nmLeaf, c->nmMax);
+  l2pPass(c, c->stream);
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha, beta,
                      1, 0);
