@@ -1105,6 +1105,260 @@ __global__ __launch_bounds__(64) void k_energy(const Pnl *__restrict__ pnl, int 
   if (lane == 0) part[p] = o0 * sgm[p + n] + o1 * sgm[p];
 }
 
+// one charge of a leaf the walk did not accept (src/treecode.c:581-615): the
+// terms of the panel-tree energy for the point qp with panel normal nr
+__device__ inline void enDirect(const double *qp, const double *nr, const double *pj, double qj, double kappa,
+                                double eps, double &y0, double &y1) {
+  double x[3] = {qp[0] - pj[0], qp[1] - pj[1], qp[2] - pj[2]};
+  double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2);
+  double G0 = kFourPiI / rn, ek = exp(-kappa * rn), Gk = ek * G0;
+  double coef = (kappa * rn + 1.0) * ek;
+  double ip = nr[0] * x[0] + nr[1] * x[1] + nr[2] * x[2];
+  double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+  y0 += qj * (G0 - Gk);
+  y1 += qj * (eps * dGkdn - dG0dn);
+}
+
+#pragma clang fp contract(fast)
+// ------------------------------------- panel-tree energy, wave-coherent walk
+//
+// The far term of a charge cluster with moments M (order p) at r = x - x_c is
+// sum_a s_a M_a d^a G(r) (and its normal derivative), s_a = (-1)^|a|.  With the
+// closed form of the Cartesian derivatives of a radial function,
+//   d^a f(|r|) = sum_{2b <= a} a! / ((a-2b)! b! 2^|b|) r^(a-2b) f_(|a|-|b|),
+//   f_k = ((1/r) d/dr)^k f   (the values radial() returns),
+// the far term is a polynomial in r whose coefficients combine the f_k:
+//   sum_g r^g sum_j W[g][j] (G0 - Gk)_(|g|+j)
+//   sum_g r^g sum_j (n . V[g][j]) (eps Gk - G0)_(|g|+j)
+// W and V depend on the cluster only, so they are formed once per cluster
+// (k_ct_poly) and every point evaluates a fixed, unrolled polynomial.
+// Entry order (k_ct_poly and enFar agree on it): W then V; g1 descending,
+// g2 descending, g3 descending (the Horner order), then j; a V entry holds
+// three values.
+
+__host__ __device__ constexpr int enPolyW(int p) {
+  int s = 0;
+  for (int d = 0; d <= p; d++) s += (d + 1) * (d + 2) / 2 * ((p - d) / 2 + 1);
+  return s;
+}
+__host__ __device__ constexpr int enPolyV(int p) {
+  int s = 0;
+  for (int d = 0; d <= p + 1; d++) s += (d + 1) * (d + 2) / 2 * ((p + 1 - d) / 2 + 1);
+  return s;
+}
+__host__ __device__ constexpr int enPolySize(int p) { return enPolyW(p) + 3 * enPolyV(p); }
+
+__device__ inline int gidx3(int i1, int i2, int i3) {   // graded index: degree, then i1, then i2 ascending
+  const int d = i1 + i2 + i3;
+  return (d > 0 ? NMOM(d - 1) : 0) + i1 * (d + 1) - i1 * (i1 - 1) / 2 + i2;
+}
+
+__device__ inline double factd(int k) {
+  double f = 1.0;
+  for (int i = 2; i <= k; i++) f *= i;
+  return f;
+}
+
+// per cluster at levels >= height: the W and V coefficients from the moments.
+// desc[e] = (g1 | g2 << 8 | g3 << 16, j, kind 0 = W / 1 = V, output offset)
+__global__ __launch_bounds__(256) void k_ct_poly(int c0, const double *__restrict__ cmom, int nmMax,
+                                                 const int4 *__restrict__ desc, int nDesc, int size,
+                                                 double *__restrict__ poly) {
+  const int c = c0 + blockIdx.x;
+  const double *M = cmom + (long long)c * nmMax;
+  double *out = poly + (long long)blockIdx.x * size;
+  for (int e = threadIdx.x; e < nDesc; e += blockDim.x) {
+    const int4 ds = desc[e];
+    const int g[3] = {ds.x & 0xff, (ds.x >> 8) & 0xff, (ds.x >> 16) & 0xff};
+    const int j = ds.y, kind = ds.z, o = ds.w;
+    const double gsgn = ((g[0] + g[1] + g[2]) & 1) ? -1.0 : 1.0;
+    const double igf = 1.0 / (factd(g[0]) * factd(g[1]) * factd(g[2]) * (double)(1 << j));
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int b1 = 0; b1 <= j; b1++)
+      for (int b2 = 0; b2 <= j - b1; b2++) {
+        const int b[3] = {b1, b2, j - b1 - b2};
+        const int ap[3] = {g[0] + 2 * b[0], g[1] + 2 * b[1], g[2] + 2 * b[2]};
+        const double C = factd(ap[0]) * factd(ap[1]) * factd(ap[2]) * igf / (factd(b[0]) * factd(b[1]) * factd(b[2]));
+        if (kind == 0) {
+          v[0] += gsgn * M[gidx3(ap[0], ap[1], ap[2])] * C;
+        } else {
+          for (int i = 0; i < 3; i++) {
+            if (ap[i] < 1) continue;
+            int a[3] = {ap[0], ap[1], ap[2]};
+            a[i] -= 1;
+            v[i] += -gsgn * M[gidx3(a[0], a[1], a[2])] * C;
+          }
+        }
+      }
+    out[o] = v[0];
+    if (kind) {
+      out[o + 1] = v[1];
+      out[o + 2] = v[2];
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, double ry, double rz, double nx,
+                                      double ny, double nz, double kappa, double eps, double &y0, double &y1) {
+  const double rho = sqrt(rx * rx + ry * ry + rz * rz);
+  double g0[P + 2], gk[P + 2];
+  radial(kappa, rho, P + 1, g0, 1, gk, 1);
+  double h[P + 2], u[P + 2];
+#pragma unroll
+  for (int k = 0; k <= P + 1; k++) {
+    h[k] = g0[k] - gk[k];
+    u[k] = eps * gk[k] - g0[k];
+  }
+  // nested Horner in x (outer), y, z (inner); a coefficient of r^g is the
+  // j-sum over the radial values of degree |g| + j
+  int o = 0;
+  double s0 = 0.0;
+#pragma unroll
+  for (int a = P; a >= 0; a--) {
+    double qa = 0.0;
+#pragma unroll
+    for (int b = P - a; b >= 0; b--) {
+      double rb = 0.0;
+#pragma unroll
+      for (int c = P - a - b; c >= 0; c--) {
+        const int d = a + b + c;
+        double cw = 0.0;
+#pragma unroll
+        for (int j = 0; j <= (P - d) / 2; j++) cw += h[d + j] * co[o + j];
+        o += (P - d) / 2 + 1;
+        rb = rb * rz + cw;
+      }
+      qa = qa * ry + rb;
+    }
+    s0 = s0 * rx + qa;
+  }
+  double s1 = 0.0;
+#pragma unroll
+  for (int a = P + 1; a >= 0; a--) {
+    double qa = 0.0;
+#pragma unroll
+    for (int b = P + 1 - a; b >= 0; b--) {
+      double rb = 0.0;
+#pragma unroll
+      for (int c = P + 1 - a - b; c >= 0; c--) {
+        const int d = a + b + c;
+        double cv = 0.0;
+#pragma unroll
+        for (int j = 0; j <= (P + 1 - d) / 2; j++)
+          cv += u[d + j] * (nx * co[o + 3 * j] + ny * co[o + 3 * j + 1] + nz * co[o + 3 * j + 2]);
+        o += 3 * ((P + 1 - d) / 2 + 1);
+        rb = rb * rz + cv;
+      }
+      qa = qa * ry + rb;
+    }
+    s1 = s1 * rx + qa;
+  }
+  y0 += s0;
+  y1 += s1;
+}
+
+// Panel-tree energy (src/treecode.c:617-695, 581-615) with one lane per panel:
+// the 64 panels of a wave (consecutive in leaf order, so close together) walk
+// the charge tree together, depth first with kids in order, carrying the mask
+// of lanes still descending.  A lane's contributions arrive in the order of its
+// own walk.  Clusters at levels >= height with eRad < theta*dist are accepted;
+// unaccepted leaves are summed directly.  Cluster data are wave-uniform, so the
+// coefficient tables come in through the scalar cache.
+#define EN_WSTACK 256
+template <int P>
+__global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
+                                                     const int *__restrict__ level, const double *__restrict__ cx,
+                                                     const double *__restrict__ eRad,
+                                                     const int *__restrict__ kidStart, const int *__restrict__ nKids,
+                                                     const int *__restrict__ chgStart, const int *__restrict__ nChgs,
+                                                     const int *__restrict__ chgIdx, const double *__restrict__ pos,
+                                                     const double *__restrict__ chr, const double *__restrict__ poly,
+                                                     int c0, int depth, int height, double theta, double kappa,
+                                                     double eps, QTab Q, int q, double *__restrict__ part) {
+  __shared__ int stC[4][EN_WSTACK];
+  __shared__ unsigned long long stM[4][EN_WSTACK];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p = (blockIdx.x * 4 + w) * 64 + lane;
+  const bool live = p < n;
+  const unsigned long long all = __ballot(live);
+  if (all == 0ull) return;   // whole wave past the end (uniform)
+  const int pp = live ? p : n - 1;
+  double v0[3], e2[3], e0[3], nr[3];
+  for (int k = 0; k < 3; k++) {
+    v0[k] = pnl[pp].v[k];
+    e2[k] = pnl[pp].a[6 + k];
+    e0[k] = pnl[pp].a[k];
+    nr[k] = pnl[pp].nr[k];
+  }
+  const double area = pnl[pp].area;
+  constexpr int SZ = enPolySize(P);
+  double o0 = 0.0, o1 = 0.0;
+  for (int ix = 0; ix < q; ix++)
+    for (int jx = 0; jx < q; jx++) {
+      double qp[3];
+      for (int k = 0; k < 3; k++) qp[k] = v0[k] + Q.t[ix] * (e2[k] + Q.t[jx] * e0[k]);
+      double y0 = 0.0, y1 = 0.0;
+      stC[w][0] = 0;
+      stM[w][0] = all;
+      int sp = 1;
+      while (sp > 0) {
+        sp--;
+        const int cb = __builtin_amdgcn_readfirstlane(stC[w][sp]);
+        const unsigned long long mraw = stM[w][sp];
+        const unsigned long long m =
+            ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(mraw >> 32)) << 32) |
+            (unsigned)__builtin_amdgcn_readfirstlane((int)(mraw & 0xffffffffu));
+        const bool on = (m >> lane) & 1ull;
+        const double rx = qp[0] - cx[3 * cb], ry = qp[1] - cx[3 * cb + 1], rz = qp[2] - cx[3 * cb + 2];
+        const double dist = sqrt(rx * rx + ry * ry + rz * rz);
+        const int lev = level[cb];
+        const bool acc = on && eRad[cb] < theta * dist && lev >= height;
+        const bool dir = on && !acc && lev == depth;
+        const unsigned long long down = __ballot(on && !acc && lev < depth);
+        if (acc) enFar<P>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps, y0, y1);
+        if (__ballot(dir) != 0ull) {
+          const int s = chgStart[cb], nq = nChgs[cb];
+          for (int k = 0; k < nq; k++) {
+            const int j = chgIdx[s + k];
+            if (dir) enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
+          }
+        }
+        if (down != 0ull) {
+          const int nk = nKids[cb], k0 = kidStart[cb];
+          if (sp + nk > EN_WSTACK) {   // stack bound: poison, never fault
+            y0 = y1 = __builtin_nan("");
+            break;
+          }
+          for (int k = nk - 1; k >= 0; k--) {   // pushed in reverse: kids visited in order
+            stC[w][sp] = k0 + k;
+            stM[w][sp] = down;
+            sp++;
+          }
+        }
+      }
+      o0 += y0 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+      o1 += y1 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+    }
+  o0 *= 2.0 * area;
+  o1 *= 2.0 * area;
+  if (live) part[p] = o0 * sgm[p + n] + o1 * sgm[p];
+}
+
+// block partials of sum a[i] over a fixed grid (deterministic order)
+__global__ __launch_bounds__(256) void k_sum_part(const double *__restrict__ a, int n, double *__restrict__ part) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v += a[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
 #pragma clang fp contract(fast)
 // ------------------------------------------------- reductions and GMRES
 
@@ -1184,13 +1438,6 @@ __global__ void k_scale(const double *__restrict__ in, double *__restrict__ out,
 }
 
 // serial sum of energy partials in panel order (one thread; n is panels)
-__global__ void k_sum_serial(const double *__restrict__ part, int n, double *__restrict__ out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; i++) s += part[i];
-    *out = s;
-  }
-}
 
 // ================================================================ host side
 
@@ -1999,7 +2246,7 @@ extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const do
   ord[depth + 1] = maxOrd + 1;
   const int nmMax = c->nmMax;
   int *dLevel, *dKid, *dNK, *dCS, *dNC, *dCI, *dOrd;
-  double *dX, *dR, *dPos, *dChr, *dMom, *dPart, *dOut;
+  double *dX, *dR, *dPos, *dChr, *dMom, *dPart, *dOut, *dRed;
   std::vector<void *> tmp;
   auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
     if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
@@ -2014,8 +2261,10 @@ extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const do
             up((void **)&dPos, pos, 3 * (size_t)nChar * sizeof(double)) &&
             up((void **)&dChr, chr, nChar * sizeof(double)) &&
             up((void **)&dMom, nullptr, (size_t)nc * nmMax * sizeof(double)) &&
-            up((void **)&dPart, nullptr, (size_t)c->n * sizeof(double)) && up((void **)&dOut, nullptr, sizeof(double));
+            up((void **)&dPart, nullptr, (size_t)c->n * sizeof(double)) && up((void **)&dOut, nullptr, sizeof(double)) &&
+            up((void **)&dRed, nullptr, RED_BLOCKS * sizeof(double));
   int rc = 0;
+  bool tablesFailed = false;
   if (ok) {
     // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416)
     if (lvCount[depth] > 0)
@@ -2027,18 +2276,61 @@ extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const do
       hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), 2 * nOut * sizeof(double),
                          c->stream, lvStart[l], dKid, dNK, dX, dOrd, l, dMom, nmMax, c->T);
     }
-    int P1 = ord[depth + 1];
-    size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
-    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, dR, dKid,
-                       dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
-                       c->Q, c->qorder, c->T, dPart);
-    hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, c->stream, dPart, c->n, dOut);
+    // one expansion order over the levels the walk can accept: the wave walk
+    // with per-cluster polynomial tables; otherwise the wave-per-panel walk
+    const int c0 = lvStart[c->height], P = ord[c->height];
+    bool uni = c0 >= 0 && P >= 1 && P <= 9;
+    for (int l = c->height; l <= depth && uni; l++) uni = ord[l] == P;
+    if (uni) {
+      std::vector<int4> dsc;
+      int o = 0;
+      for (int a = P; a >= 0; a--)   // the order enFar reads them in
+        for (int b = P - a; b >= 0; b--)
+          for (int g3 = P - a - b; g3 >= 0; g3--)
+            for (int j = 0; j <= (P - a - b - g3) / 2; j++) dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 0, o++));
+      for (int a = P + 1; a >= 0; a--)
+        for (int b = P + 1 - a; b >= 0; b--)
+          for (int g3 = P + 1 - a - b; g3 >= 0; g3--)
+            for (int j = 0; j <= (P + 1 - a - b - g3) / 2; j++, o += 3)
+              dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 1, o));
+      const int SZ = enPolySize(P), nClus = nc - c0;
+      int4 *dDesc;
+      double *dPoly;
+      if (o != SZ || !up((void **)&dDesc, dsc.data(), dsc.size() * sizeof(int4)) ||
+          !up((void **)&dPoly, nullptr, (size_t)nClus * SZ * sizeof(double))) {
+        setErr("fbh_energy: coefficient tables (%d of %d entries) could not be set up", o, SZ);
+        ok = false;
+        tablesFailed = true;
+      } else {
+        hipLaunchKernelGGL(k_ct_poly, dim3(nClus), dim3(256), 0, c->stream, c0, dMom, nmMax, dDesc, (int)dsc.size(), SZ,
+                           dPoly);
+        const int blocks = ((c->n + 63) / 64 + 3) / 4;
+#define EN_WAVE(PP)                                                                                               \
+  case PP:                                                                                                        \
+    hipLaunchKernelGGL(k_energy_wave<PP>, dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, \
+                       dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta, c->kappa,   \
+                       c->eps, c->Q, c->qorder, dPart);                                                           \
+    break;
+        switch (P) { EN_WAVE(1) EN_WAVE(2) EN_WAVE(3) EN_WAVE(4) EN_WAVE(5) EN_WAVE(6) EN_WAVE(7) EN_WAVE(8) EN_WAVE(9) }
+#undef EN_WAVE
+      }
+    } else {
+      int P1 = ord[depth + 1];
+      size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
+      hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, dR, dKid,
+                         dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
+                         c->Q, c->qorder, c->T, dPart);
+    }
+  }
+  if (ok) {
+    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, c->n, dRed);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) setErr("energy kernels failed: %s", hipGetErrorString(e));
     else rc = 1;
-  } else {
+  } else if (!tablesFailed) {
     setErr("fbh_energy: device allocation/upload failed");
   }
   hipStreamSynchronize(c->stream);

@@ -177,6 +177,31 @@ def test_kirkwood_sphere(fa, meshdir):
     assert abs(r["energy"] - ek) <= 2e-3 * abs(ek)
 
 
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+@pytest.mark.parametrize("order", [6, -4])
+def test_orders_against_oracle(fa, meshdir, name, order):
+    """Fixed order 6 and the variable-order mode (finest level 4, one more per level up,
+    src/gkSetup.c:42-81): matvec and panel-tree energy against the oracle on the product's
+    own tree.  Order -4 gives M2L orders that differ by level and charge-tree orders 4/5,
+    so the energy takes the wave-per-panel walk; order 6 takes the wave walk."""
+    import torch
+    import oracle_binding as O
+    mesh = fa.Mesh(str(meshdir / name))
+    tree = fa.Tree(mesh, CASES["cases"][name]["t"], order=order, height=FL["H"], sep=FL["S"])
+    op = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    ora = O.Oracle(tree.oracle_layout(FL["k"], FL["eps1"], FL["eps2"], FL["q"]))
+    x = splitmix_vec(4242, 2 * tree.n)
+    assert rel(op.matvec(x), ora.matvec(x)) <= 1e-13
+    ct = tree.charge_tree()
+    if order < 0:
+        assert len(set(ct["order"][tree.height:].tolist())) > 1
+    sg = splitmix_vec(4243, 2 * tree.n)
+    e = op.energy_dev(ct, mesh.pos, mesh.chr, torch.tensor(sg, dtype=torch.float64, device="cuda"))
+    ref = ora.energy(ct, mesh.pos, mesh.chr, sg)
+    print(f"\n[{name} p={order}] energy {e!r} oracle {ref!r}", flush=True)
+    assert abs(e - ref) <= 1e-12 * abs(ref)
+
+
 def test_matvec_linearity_full_size(fa, meshdir):
     """Size-independent property at the bench size: A(a x + b z) = a A x + b A z."""
     mesh, tree, op = case_op(fa, meshdir, "cfg2_globule")
