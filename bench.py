@@ -12,9 +12,12 @@ the same mesh and charges.
 A step is one matvec y = (D - K) x of the plain FP64 operator with x, y
 resident in HBM.  `value` is matvecs per second over the timed steps.  The
 JSON line also reports the full solve (RHS + GMRES + energy) wall time, its
-iteration count and energy against the reference's recorded solve, per-stage
-device times (HIP events on the launch stream), the nearfield kernel's
-roofline position, and the reference CPU path timed on the host cores.
+iteration count and energy against the reference's recorded solve, the
+precision form on the same input (FP32 operator inside FP64 iterative
+refinement: its solve wall time, speedup over the plain solve and energy
+error against it), per-stage device times (HIP events on the launch stream),
+the nearfield kernel's roofline position, and the reference CPU path timed on
+the host cores.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calib-copy]
 """
@@ -139,8 +142,7 @@ def main():
     near_bytes = st["near_interactions"] * COEF_BYTES + 2 * n2 * 8
     achieved = near_bytes / (near_ms * 1e-3)
 
-    solve = None
-    if not args.no_solve:
+    def run_solve(precision):
         b = torch.empty(n2, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         ts = time.perf_counter()
@@ -148,23 +150,63 @@ def main():
         xs = b.clone()
         torch.cuda.synchronize()
         tr = time.perf_counter()
-        gm = op.gmres_dev(b, xs, restart=RESTART, tol=TOL, maxit=MAXIT)
+        if precision == "fp32":
+            gm = op.gmres_refined_dev(b, xs, restart=RESTART, tol=TOL, maxit=MAXIT)
+        else:
+            gm = op.gmres_dev(b, xs, restart=RESTART, tol=TOL, maxit=MAXIT)
         torch.cuda.synchronize()
         tg = time.perf_counter()
-        ct = tree.charge_tree()
         raw = op.energy_dev(ct, mesh.pos, mesh.chr, xs)
         torch.cuda.synchronize()
         te = time.perf_counter()
-        solve = {"wall_s": te - ts, "rhs_s": tr - ts, "gmres_s": tg - tr, "energy_s": te - tg,
-                 "iterations": gm["iters"], "matvecs": gm["iters"] + 1, "final_residual": gm["resid"],
-                 "energy_kcal_per_mol": raw * fa.ENERGY_SCALE}
+        r = {"wall_s": te - ts, "rhs_s": tr - ts, "gmres_s": tg - tr, "energy_s": te - tg,
+             "iterations": gm["iters"], "final_residual": gm["resid"], "energy_raw": raw,
+             "energy_kcal_per_mol": raw * fa.ENERGY_SCALE}
+        if precision == "fp32":
+            r.update(outer=gm["outer"], fp64_matvecs=gm["mv64"], fp32_matvecs=gm["iters"] + 1)
+        else:
+            r["matvecs"] = gm["iters"] + 1
+        return r
+
+    solve = precision = None
+    if not args.no_solve:
+        ct = tree.charge_tree()
+        run_solve("fp64")   # first call allocates the GMRES workspace; time the second
+        solve = run_solve("fp64")
+        raw = solve["energy_raw"]
         ref = os.path.join(ROOT, "tests", "golden", "cfg2_globule", "cli.json")
         if os.path.exists(ref):
             R = json.load(open(ref))
             solve["reference_iterations"] = R["iterations"]
             solve["energy_rel_err_vs_reference"] = abs(raw - R["panel_tree_raw"]) / abs(R["panel_tree_raw"])
             solve["reference_cpu_gmres_s_8threads"] = R["gmres_s"]
-        print(f"[bench] solve: {solve['wall_s']:.3f}s its={gm['iters']}", file=sys.stderr, flush=True)
+        print(f"[bench] fp64 solve: {solve['wall_s'] * 1e3:.2f} ms its={solve['iterations']}", file=sys.stderr,
+              flush=True)
+
+        # precision form: FP32 operator, FP64 refinement
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.enable_fp32()
+        torch.cuda.synchronize()
+        t_fp32 = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            op.matvec32_dev(x, y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            op.matvec32_dev(x, y)
+        torch.cuda.synchronize()
+        ms32 = 1e3 * (time.perf_counter() - t0) / args.steps
+        run_solve("fp32")
+        precision = run_solve("fp32")
+        precision.update(
+            energy_rel_err_vs_fp64=abs(precision["energy_raw"] - raw) / abs(raw),
+            wall_speedup_vs_fp64=solve["wall_s"] / precision["wall_s"],
+            gmres_speedup_vs_fp64=solve["gmres_s"] / precision["gmres_s"],
+            fp32_matvec_ms=ms32, fp32_matvec_per_s=1e3 / ms32, fp32_setup_s=t_fp32)
+        print(f"[bench] fp32-refined solve: {precision['wall_s'] * 1e3:.2f} ms its={precision['iterations']} "
+              f"outer={precision['outer']} err={precision['energy_rel_err_vs_fp64']:.2e} "
+              f"matvec32={ms32:.3f} ms", file=sys.stderr, flush=True)
 
     if args.calib_copy:
         calib_copy(torch)
@@ -195,6 +237,7 @@ def main():
         "stages_ms": stages,
         "setup_s": {"tree": t_tree, "device_operator": t_setup},
         "solve": solve,
+        "precision_form": precision,
     }
     print(json.dumps(out), flush=True)
 

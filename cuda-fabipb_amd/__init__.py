@@ -99,6 +99,13 @@ def hip_lib():
         L.fbh_gmres_dev.restype = C.c_int
         L.fbh_gmres_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
                                     _ip, _dp, _dp, _ip]
+        L.fbh_enable_fp32.restype = C.c_int
+        L.fbh_enable_fp32.argtypes = [_vp]
+        L.fbh_matvec32_dev.restype = C.c_int
+        L.fbh_matvec32_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp]
+        L.fbh_gmres_refined_dev.restype = C.c_int
+        L.fbh_gmres_refined_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                            _ip, _dp, _dp, _ip, _ip, _ip]
         L.fbh_rhs.restype = C.c_int
         L.fbh_rhs.argtypes = [_vp, C.c_int, _dp, _dp, _dp]
         L.fbh_rhs_dev.restype = C.c_int
@@ -309,6 +316,32 @@ class Operator:
         self._ck(hip_lib().fbh_matvec_dev(self.h, alpha, C.c_void_p(x.data_ptr()), beta,
                                           C.c_void_p(y.data_ptr())))
 
+    def enable_fp32(self):
+        """FP32 copies of the operator for the precision form (fbh_enable_fp32)."""
+        self._ck(hip_lib().fbh_enable_fp32(self.h))
+
+    def matvec32_dev(self, x, y, alpha=1.0, beta=0.0):
+        self._ck(hip_lib().fbh_matvec32_dev(self.h, alpha, C.c_void_p(x.data_ptr()), beta,
+                                            C.c_void_p(y.data_ptr())))
+
+    def matvec32(self, x, alpha=1.0, beta=0.0, y=None):
+        """Host-array convenience: y = alpha (D - K32) x + beta y on the FP32 operator."""
+        torch = _torch()
+        xd = torch.tensor(np.ascontiguousarray(x, dtype=np.float64), device="cuda")
+        yd = (torch.tensor(np.ascontiguousarray(y, dtype=np.float64), device="cuda") if y is not None
+              else torch.zeros_like(xd))
+        self.matvec32_dev(xd, yd, alpha, beta if y is not None else 0.0)
+        return yd.cpu().numpy()
+
+    def gmres_refined_dev(self, b, x, restart=30, tol=1e-6, maxit=100, max_outer=10):
+        it, res, info, outer, mv64 = C.c_int(0), C.c_double(0.0), C.c_int(0), C.c_int(0), C.c_int(0)
+        hist = np.zeros(max_outer)
+        self._ck(hip_lib().fbh_gmres_refined_dev(
+            self.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), restart, tol, maxit, max_outer,
+            C.byref(it), C.byref(res), _d(hist), C.byref(info), C.byref(outer), C.byref(mv64)))
+        return dict(iters=it.value, resid=res.value, info=info.value, outer=outer.value, mv64=mv64.value,
+                    hist=hist[:outer.value])
+
     def rhs_dev(self, pos, chr_, b):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
@@ -343,14 +376,21 @@ class Operator:
 
 
 def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,
-          restart=30, maxit=100, height=2, x0="rhs"):
-    """One plain-FP64 solve (src/fabipb.c:1067-1210): returns energies, iterations and times."""
+          restart=30, maxit=100, height=2, x0="rhs", precision="fp64", op=None, tree=None, mesh=None):
+    """One solve (src/fabipb.c:1067-1210): returns energies, iterations and times.
+    precision "fp64" is the plain form (GMRES on the FP64 operator); "fp32" is the
+    precision form (refinement around GMRES on the FP32 operator, FP64 residuals).
+    An existing mesh/tree/operator may be passed in to time the solve alone."""
     torch = _torch()
     t0 = time.perf_counter()
-    mesh = Mesh(base)
-    tree = Tree(mesh, depth, order=order, height=height, sep=sep)
+    mesh = mesh if mesh is not None else Mesh(base)
+    tree = tree if tree is not None else Tree(mesh, depth, order=order, height=height, sep=sep)
     t1 = time.perf_counter()
-    op = Operator(tree, kappa, eps1, eps2, qorder)
+    op = op if op is not None else Operator(tree, kappa, eps1, eps2, qorder)
+    if precision == "fp32":
+        op.enable_fp32()
+    elif precision != "fp64":
+        raise ValueError(f"precision must be fp64 or fp32, not {precision!r}")
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     n2 = 2 * tree.n
@@ -359,7 +399,10 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
     x = b.clone() if x0 == "rhs" else torch.zeros_like(b)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    g = op.gmres_dev(b, x, restart=restart, tol=tol, maxit=maxit)
+    if precision == "fp32":
+        g = op.gmres_refined_dev(b, x, restart=restart, tol=tol, maxit=maxit)
+    else:
+        g = op.gmres_dev(b, x, restart=restart, tol=tol, maxit=maxit)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
     ct = tree.charge_tree()
@@ -367,6 +410,7 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
     torch.cuda.synchronize()
     t5 = time.perf_counter()
     return dict(energy=raw * ENERGY_SCALE, energy_raw=raw, iters=g["iters"], resid=g["resid"],
-                info=g["info"], hist=g["hist"][:g["iters"] + 2], n=tree.n,
+                info=g["info"], hist=g["hist"][:g["iters"] + 2], n=tree.n, precision=precision,
+                outer=g.get("outer", 0), mv64=g.get("mv64", g["iters"] + 1),
                 t_tree=t1 - t0, t_setup=t2 - t1, t_rhs=t3 - t2, t_gmres=t4 - t3, t_energy=t5 - t4,
                 t_solve=t5 - t2, x=x.cpu().numpy(), op=op, tree=tree, mesh=mesh)

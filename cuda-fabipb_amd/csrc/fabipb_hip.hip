@@ -344,57 +344,72 @@ __global__ __launch_bounds__(256) void k_near_build(const Pnl *__restrict__ pnl,
   }
 }
 
+extern __shared__ __align__(16) unsigned char dynLds[];   // dynamic LDS of the templated kernels
+
 #pragma clang fp contract(fast)
 __device__ __forceinline__ double waveSum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
+template <typename T>
+__device__ __forceinline__ T waveSumT(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename R> struct Vec4;
+template <> struct Vec4<double> { typedef double4 type; };
+template <> struct Vec4<float> { typedef float4 type; };
+
 // tile = (destination leaf, first row); 4 waves x 4 rows; out[x] = sum over
-// sources of K0*sgm_n + K1*sgm, out[x+n] = K2*sgm_n + K3*sgm
+// sources of K0*sgm_n + K1*sgm, out[x+n] = K2*sgm_n + K3*sgm.  R is the
+// arithmetic of the coefficients, the staged densities and the sums.
+template <typename R>
 __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
                                                     const int *__restrict__ leafCount,
                                                     const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
                                                     const long long *__restrict__ coefOff,
                                                     const int *__restrict__ srcList,
-                                                    const double4 *__restrict__ coef, const double *__restrict__ x,
-                                                    double *__restrict__ out, int n) {
-  __shared__ double sS[NEAR_CHUNK], sN[NEAR_CHUNK];
+                                                    const typename Vec4<R>::type *__restrict__ coef,
+                                                    const double *__restrict__ x, R *__restrict__ out, int n) {
+  typedef typename Vec4<R>::type R4;
+  __shared__ R sS[NEAR_CHUNK], sN[NEAR_CHUNK];
   const int2 t = tiles[blockIdx.x];
   const int D = t.x, r0 = t.y;
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
   const int ncol = ncolArr[D];
   const int *src = srcList + srcOff[D];
-  const double4 *cbase = coef + coefOff[D];
+  const R4 *cbase = coef + coefOff[D];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
+  R a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
   for (int c0 = 0; c0 < ncol; c0 += NEAR_CHUNK) {
     const int cn = min(NEAR_CHUNK, ncol - c0);
     __syncthreads();
     for (int j = threadIdx.x; j < cn; j += 256) {
       int y = src[c0 + j];
-      sS[j] = x[y];
-      sN[j] = x[y + n];
+      sS[j] = (R)x[y];
+      sN[j] = (R)x[y + n];
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = w + 4 * r;
       if (row < rows) {
-        const double4 *cr = cbase + (long long)(r0 + row) * ncol + c0;
-        double s0 = 0.0, s1 = 0.0;
+        const R4 *cr = cbase + (long long)(r0 + row) * ncol + c0;
+        R s0 = 0, s1 = 0;
         int j = lane;
         for (; j + 64 < cn; j += 128) {
-          double4 c = cr[j], d = cr[j + 64];
-          double p = sS[j], m = sN[j], p2 = sS[j + 64], m2 = sN[j + 64];
+          R4 c = cr[j], d = cr[j + 64];
+          R p = sS[j], m = sN[j], p2 = sS[j + 64], m2 = sN[j + 64];
           s0 += c.x * m + c.y * p;
           s1 += c.z * m + c.w * p;
           s0 += d.x * m2 + d.y * p2;
           s1 += d.z * m2 + d.w * p2;
         }
         if (j < cn) {
-          double4 c = cr[j];
-          double p = sS[j], m = sN[j];
+          R4 c = cr[j];
+          R p = sS[j], m = sN[j];
           s0 += c.x * m + c.y * p;
           s1 += c.z * m + c.w * p;
         }
@@ -406,13 +421,19 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int row = w + 4 * r;
-    double s0 = waveSum(a0[r]), s1 = waveSum(a1[r]);
+    R s0 = waveSumT(a0[r]), s1 = waveSumT(a1[r]);
     if (row < rows && lane == 0) {
       int p = leafStart[D] + r0 + row;
       out[p] = s0;
       out[p + n] = s1;
     }
   }
+}
+
+// FP32 copies of FP64 data (the same layout, element by element)
+__global__ void k_to_f32(const double *__restrict__ in, float *__restrict__ out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
 }
 
 // -------------------------------------------------- multi-index helpers
@@ -575,17 +596,18 @@ __global__ __launch_bounds__(64) void k_q2m_mats(const Pnl *__restrict__ pnl, co
 // mom_pot = Q2M1 sgm, mom_dpdn = Q2M0 sgm_n (src/fmm.c:745-753): one block per
 // leaf, four waves split the leaf's panels (a contiguous column each), lanes own
 // multi-indices; the wave partials are added in wave order
-__global__ __launch_bounds__(256) void k_q2m(const double *__restrict__ Q0, const double *__restrict__ Q1,
+template <typename R>
+__global__ __launch_bounds__(256) void k_q2m(const R *__restrict__ Q0, const R *__restrict__ Q1,
                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
-                                             const double *__restrict__ x, double *__restrict__ mom, int n, int nm,
+                                             const double *__restrict__ x, R *__restrict__ mom, int n, int nm,
                                              int nmMax) {
-  extern __shared__ double sm[];   // [4 waves][2][nm]
+  R *sm = reinterpret_cast<R *>(dynLds);   // [4 waves][2][nm]
   const int L = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = leafStart[L], c = leafCount[L];
-  double m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
+  R m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
   for (int j = w; j < c; j += 4) {
-    const double sp = x[s + j], sn = x[s + j + n];
-    const double *q1 = Q1 + (long long)(s + j) * nm, *q0 = Q0 + (long long)(s + j) * nm;
+    const R sp = (R)x[s + j], sn = (R)x[s + j + n];
+    const R *q1 = Q1 + (long long)(s + j) * nm, *q0 = Q0 + (long long)(s + j) * nm;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int a = lane + 64 * k;
@@ -605,7 +627,7 @@ __global__ __launch_bounds__(256) void k_q2m(const double *__restrict__ Q0, cons
   }
   __syncthreads();
   for (int a = threadIdx.x; a < nm; a += 256) {
-    double v0 = 0.0, v1 = 0.0;
+    R v0 = 0, v1 = 0;
     for (int ww = 0; ww < 4; ww++) {
       v0 += sm[(2 * ww) * nm + a];
       v1 += sm[(2 * ww + 1) * nm + a];
@@ -617,19 +639,20 @@ __global__ __launch_bounds__(256) void k_q2m(const double *__restrict__ Q0, cons
 
 // src/expan.c:342-382 (transM2MWs) + convM2M (:223-247): one block per parent,
 // threads own multi-indices (two each at most); the (alpha-k, k) table in LDS
+template <typename R>
 __global__ __launch_bounds__(256) void k_m2m(int levStart, const int *__restrict__ kidStart,
                                              const int *__restrict__ nKids, const int *__restrict__ cubeLevel,
                                              const int *__restrict__ ordMom, const double *__restrict__ cubeX,
-                                             double *__restrict__ mom, int nmMax, DevTabs T) {
-  extern __shared__ double sm[];
+                                             R *__restrict__ mom, int nmMax, DevTabs T) {
+  R *sm = reinterpret_cast<R *>(dynLds);
   const int par = levStart + blockIdx.x, bd = blockDim.x, tid = threadIdx.x;
   const int ordOut = ordMom[cubeLevel[par]], nOut = NMOM(ordOut);
-  double *R = sm, *b1 = sm + nOut, *b2 = sm + 2 * nOut;
+  R *Rt = sm, *b1 = sm + nOut, *b2 = sm + 2 * nOut;
   int *ts = (int *)(sm + 3 * nOut), *tab = ts + nOut + 1;
   const int nT = T.m2mStart[nOut];
   for (int i = tid; i <= nOut; i += bd) ts[i] = T.m2mStart[i];
   for (int i = tid; i < nT; i += bd) tab[i] = T.m2m[i];
-  double acc1[2] = {0, 0}, acc2[2] = {0, 0};
+  R acc1[2] = {0, 0}, acc2[2] = {0, 0};
   for (int k = 0; k < nKids[par]; k++) {
     const int kid = kidStart[par] + k;
     const int nIn = NMOM(ordMom[cubeLevel[kid]]);
@@ -637,19 +660,19 @@ __global__ __launch_bounds__(256) void k_m2m(int levStart, const int *__restrict
     for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
     __syncthreads();
     for (int a = tid; a < nOut; a += bd) {
-      R[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
-      b1[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + a] : 0.0;
-      b2[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + nmMax + a] : 0.0;
+      Rt[a] = (R)conVect1(tr, T.mi[a], T.ifact3[a]);
+      b1[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + a] : (R)0;
+      b2[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + nmMax + a] : (R)0;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int a = tid + q * bd;
       if (a < nOut) {
-        double t1 = 0.0, t2 = 0.0;
+        R t1 = 0, t2 = 0;
         for (int e = ts[a]; e < ts[a + 1]; e++) {
           const int pr = tab[e];
-          const double rv = R[pr & 0xffff];
+          const R rv = Rt[pr & 0xffff];
           t1 += rv * b1[pr >> 16];
           t2 += rv * b2[pr >> 16];
         }
@@ -750,29 +773,30 @@ __global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairD
 // adds its running sums into the item's slot in LDS (its own, per wave).  The
 // slots are summed per alpha at the end, in item then wave order.
 #define M2L_FLUSH 0x80000000u
+template <typename R>
 __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
                                                     const int *__restrict__ cubeLevel,
                                                     const int *__restrict__ ordM2L, const int *__restrict__ ordMom,
-                                                    const double *__restrict__ G, const double *__restrict__ mom,
-                                                    double *__restrict__ lec, int nmMax, double eps, DevTabs T,
+                                                    const R *__restrict__ G, const R *__restrict__ mom,
+                                                    R *__restrict__ lec, int nmMax, R eps, DevTabs T,
                                                     const unsigned *__restrict__ planTerms,
                                                     const int *__restrict__ planIdx, const int4 *__restrict__ planOff) {
-  extern __shared__ double sm[];
+  R *sm = reinterpret_cast<R *>(dynLds);
   const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
   const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
   // plan of order p: x = first term, y = term count, z = first index entry, w = item count;
   // index entries: 65 lane starts into the terms, then nm + 1 item starts per alpha
   const int4 po = planOff[p];
   const int nTerms = po.y, nItems = po.w;
-  double *slots = sm;                                  // [wave][5][nm]
-  double *red = sm + nw * 5 * nm;                      // [wave][item][4]
+  R *slots = sm;                                       // [wave][5][nm]
+  R *red = sm + nw * 5 * nm;                           // [wave][item][4]
   unsigned *tab = (unsigned *)(red + nw * 4 * nItems);
   int *idx = (int *)(tab + nTerms);
   for (int i = tid; i < nTerms; i += blockDim.x) tab[i] = planTerms[po.x + i];
   for (int i = tid; i < 65 + nm + 1; i += blockDim.x) idx[i] = planIdx[po.z + i];
   for (int i = tid; i < nw * 4 * nItems; i += blockDim.x) red[i] = 0.0;
-  double *c1 = slots + w * 5 * nm, *c2 = c1 + nm, *v1 = c1 + 2 * nm, *v2 = c1 + 3 * nm, *v3 = c1 + 4 * nm;
-  double *myRed = red + w * 4 * nItems;
+  R *c1 = slots + w * 5 * nm, *c2 = c1 + nm, *v1 = c1 + 2 * nm, *v2 = c1 + 3 * nm, *v3 = c1 + 4 * nm;
+  R *myRed = red + w * 4 * nItems;
   const int b0 = m2lOff[c], b1 = m2lOff[c + 1];
   for (int base = b0; base < b1; base += nw) {
     const int pr = base + w;
@@ -780,13 +804,13 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
     __syncthreads();
     if (act) {
       const int src = m2lSrc[pr];
-      const double *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
-      const double *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
+      const R *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
+      const R *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
       for (int i = lane; i < nm; i += 64) {
         int sg = T.ep[i].w;
-        double a0 = g0[i], ak = gk[i];
-        c1[i] = sg * mp[i];
-        c2[i] = sg * md[i];
+        R a0 = g0[i], ak = gk[i];
+        c1[i] = (R)sg * mp[i];
+        c2[i] = (R)sg * md[i];
         v1[i] = a0 - ak;
         v2[i] = eps * ak - a0;
         v3[i] = a0 - ak / eps;
@@ -794,20 +818,20 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
     }
     __syncthreads();
     if (act) {
-      double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+      R t1 = 0, t2 = 0, t3 = 0, t4 = 0;
       const int e1 = idx[lane + 1];
       for (int e = idx[lane]; e < e1; e++) {
         const unsigned pe = tab[e];
         const int j = pe & 0xff, kk = (pe >> 8) & 0xff;
-        const double x1 = c1[j], x2 = c2[j], y1 = v1[kk];
+        const R x1 = c1[j], x2 = c2[j], y1 = v1[kk];
         t1 += x2 * y1;
         t2 += x1 * v2[kk];
         t3 += x2 * v3[kk];
         t4 += x1 * (-y1);
         if (pe & M2L_FLUSH) {
-          double *r = myRed + 4 * ((pe >> 16) & 0x7fff);
+          R *r = myRed + 4 * ((pe >> 16) & 0x7fff);
           r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
-          t1 = t2 = t3 = t4 = 0.0;
+          t1 = t2 = t3 = t4 = 0;
         }
       }
     }
@@ -817,7 +841,7 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   const int *itemStart = idx + 65;
   for (int i = tid; i < 4 * nmMax; i += blockDim.x) {
     const int k = i / nmMax, a = i - k * nmMax;
-    double v = 0.0;
+    R v = 0;
     if (a < nm && a < nmc)
       for (int it = itemStart[a]; it < itemStart[a + 1]; it++)
         for (int ww = 0; ww < nw; ww++) v += red[(ww * nItems + it) * 4 + k];
@@ -827,15 +851,16 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
 
 // src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child,
 // thread per output multi-index, (beta, alpha+beta) table in LDS
+template <typename R>
 __global__ __launch_bounds__(512) void k_l2l(int levStart, const int *__restrict__ parent,
                                              const int *__restrict__ cubeLevel, const int *__restrict__ ordMom,
-                                             const double *__restrict__ cubeX, double *__restrict__ lec, int nmMax,
+                                             const double *__restrict__ cubeX, R *__restrict__ lec, int nmMax,
                                              DevTabs T) {
-  extern __shared__ double sm[];
+  R *sm = reinterpret_cast<R *>(dynLds);
   const int kid = levStart + blockIdx.x, par = parent[kid];
   const int ordIn = ordMom[cubeLevel[par]], ordOut = ordMom[cubeLevel[kid]];
   const int nIn = NMOM(ordIn), nOut = NMOM(ordOut);
-  double *R = sm, *In = sm + nIn;   // In[4][nIn]
+  R *Rt = sm, *In = sm + nIn;   // In[4][nIn]
   int *cst = (int *)(sm + 5 * nIn), *tab = cst + nOut + 1;
   const int nT = T.convStart[nOut];
   for (int i = threadIdx.x; i <= nOut; i += blockDim.x) cst[i] = T.convStart[i];
@@ -843,17 +868,17 @@ __global__ __launch_bounds__(512) void k_l2l(int levStart, const int *__restrict
   double tr[3];
   for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
   for (int a = threadIdx.x; a < nIn; a += blockDim.x) {
-    R[a] = conVect1(tr, T.mi[a], T.ifact3[a]);
+    Rt[a] = (R)conVect1(tr, T.mi[a], T.ifact3[a]);
     for (int k = 0; k < 4; k++) In[k * nIn + a] = lec[((long long)par * 4 + k) * nmMax + a];
   }
   __syncthreads();
   for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
     const int cnt = NMOM(ordIn - deg_of(T.mi[a]));
     const int *cv = tab + cst[a];
-    double t[4] = {0, 0, 0, 0};
+    R t[4] = {0, 0, 0, 0};
     for (int e = 0; e < cnt; e++) {
       int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
-      double rv = R[j];
+      R rv = Rt[j];
       for (int k = 0; k < 4; k++) t[k] += rv * In[k * nIn + kk];
     }
     for (int k = 0; k < 4; k++) lec[((long long)kid * 4 + k) * nmMax + a] += t[k];
@@ -862,14 +887,15 @@ __global__ __launch_bounds__(512) void k_l2l(int levStart, const int *__restrict
 
 // far[p] = Q2M0^T (l1 + l2), far[p+n] = Q2M1^T (l3 + l4) (src/fmm.c:815-828):
 // one block per leaf, eight lanes per panel read its column in 64-byte pieces
-__global__ __launch_bounds__(256) void k_l2p(const double *__restrict__ Q0, const double *__restrict__ Q1,
+template <typename R>
+__global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *__restrict__ Q1,
                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
-                                             const double *__restrict__ lec, double *__restrict__ far, int n,
-                                             int nm, int nmMax) {
-  extern __shared__ double sm[];
-  double *L12 = sm, *L34 = sm + nm;
+                                             const R *__restrict__ lec, R *__restrict__ far, int n, int nm,
+                                             int nmMax) {
+  R *sm = reinterpret_cast<R *>(dynLds);
+  R *L12 = sm, *L34 = sm + nm;
   const int L = blockIdx.x;
-  const double *l = lec + (long long)L * 4 * nmMax;
+  const R *l = lec + (long long)L * 4 * nmMax;
   for (int a = threadIdx.x; a < nm; a += blockDim.x) {
     L12[a] = l[a] + l[nmMax + a];
     L34[a] = l[2 * nmMax + a] + l[3 * nmMax + a];
@@ -878,9 +904,9 @@ __global__ __launch_bounds__(256) void k_l2p(const double *__restrict__ Q0, cons
   const int g = threadIdx.x >> 3, l8 = threadIdx.x & 7, s = leafStart[L], c = leafCount[L];
   for (int j0 = 0; j0 < c; j0 += 32) {   // uniform trip count: every lane takes part in the shuffles
     const int j = j0 + g;
-    double t0 = 0.0, t1 = 0.0;
+    R t0 = 0, t1 = 0;
     if (j < c) {
-      const double *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
+      const R *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
       for (int a = l8; a < nm; a += 8) {
         t0 += q0[a] * L12[a];
         t1 += q1[a] * L34[a];
@@ -898,14 +924,15 @@ __global__ __launch_bounds__(256) void k_l2p(const double *__restrict__ Q0, cons
 }
 
 // y = alpha*(D x - far - near) + beta*y (src/fabipb.c:1271-1279)
-__global__ void k_combine(const double *__restrict__ area, const double *__restrict__ x,
-                          const double *__restrict__ far, const double *__restrict__ nearv, double *__restrict__ y,
-                          int n, double alpha, double beta, double s1, double s2, int useFar, int useNear) {
+template <typename R>
+__global__ void k_combine(const double *__restrict__ area, const double *__restrict__ x, const R *__restrict__ far,
+                          const R *__restrict__ nearv, double *__restrict__ y, int n, double alpha, double beta,
+                          double s1, double s2, int useFar, int useNear) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 2 * n) return;
   int p = i < n ? i : i - n;
   double s = i < n ? s1 : s2;
-  double k = (useFar ? far[i] : 0.0) + (useNear ? nearv[i] : 0.0);
+  double k = (useFar ? (double)far[i] : 0.0) + (useNear ? (double)nearv[i] : 0.0);
   double v = alpha * (s * area[p] * x[i] - k);
   y[i] = beta == 0.0 ? v : v + beta * y[i];
 }
@@ -1484,6 +1511,13 @@ struct fbh_ctx {
   size_t m2lLds = 0;
   int m2lWaves = 0;
   int ldsMax = 65536;
+  // FP32 operator (precision form): copies of the cached operator in FP32
+  bool fp32 = false;
+  float *Q0f = nullptr, *Q1f = nullptr, *momf = nullptr, *lecf = nullptr, *farf = nullptr, *nearf = nullptr,
+        *m2lGf = nullptr;
+  float4 *coef32 = nullptr;
+  size_t m2lLds32 = 0;
+  double *ro = nullptr, *dx = nullptr;   // refinement residual and correction
   // nearfield on a second stream, overlapped with the far-field chain
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
@@ -1542,6 +1576,8 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   if (c->hcol) hipFree(c->hcol);
   if (c->ydev) hipFree(c->ydev);
   if (c->scal) hipFree(c->scal);
+  if (c->ro) hipFree(c->ro);
+  if (c->dx) hipFree(c->dx);
   if (c->side) {
     hipStreamSynchronize(c->side);
     hipStreamDestroy(c->side);
@@ -1647,12 +1683,12 @@ static bool buildTables(fbh_ctx *c) {
 // 64 lanes, longest first; the last term of an item is flagged with its number.
 // Returns the dynamic LDS the kernel needs with `waves` waves, at the largest
 // order used by a level.
-static size_t m2lPlanLds(const fbh_ctx *c, const std::vector<int4> &off, int waves) {
+static size_t m2lPlanLds(const fbh_ctx *c, const std::vector<int4> &off, int waves, size_t elem) {
   size_t best = 0;
   for (int l = c->height; l <= c->depth; l++) {
     const int p = c->ordM2L[l], nm = NMOM(p);
     const int4 po = off[p];
-    size_t b = ((size_t)waves * 5 * nm + (size_t)waves * 4 * po.w) * sizeof(double) + (size_t)po.y * sizeof(unsigned) +
+    size_t b = ((size_t)waves * 5 * nm + (size_t)waves * 4 * po.w) * elem + (size_t)po.y * sizeof(unsigned) +
                (size_t)(65 + nm + 1) * sizeof(int);
     best = std::max(best, b);
   }
@@ -1719,9 +1755,10 @@ static bool buildM2LPlan(fbh_ctx *c) {
   }
   c->m2lWaves = 0;
   for (int wv = 4; wv >= 1; wv /= 2)
-    if (m2lPlanLds(c, off, wv) <= (size_t)c->ldsMax) { c->m2lWaves = wv; break; }
+    if (m2lPlanLds(c, off, wv, sizeof(double)) <= (size_t)c->ldsMax) { c->m2lWaves = wv; break; }
   if (!c->m2lWaves) return true;   // no plan fits: the recompute kernel is used
-  c->m2lLds = m2lPlanLds(c, off, c->m2lWaves);
+  c->m2lLds = m2lPlanLds(c, off, c->m2lWaves, sizeof(double));
+  c->m2lLds32 = m2lPlanLds(c, off, c->m2lWaves, sizeof(float));
   if (!(c->planTerms = dupload(c, terms.data(), terms.size())) || !(c->planIdx = dupload(c, idx.data(), idx.size())) ||
       !(c->planOff = dupload(c, off.data(), off.size())))
     return false;
@@ -1914,75 +1951,139 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   return c;
 }
 
-static void m2lPass(fbh_ctx *c, hipStream_t st) {
-  int pm = c->maxOrder, nm = NMOM(pm);
-  if (c->m2lG) {
-    hipLaunchKernelGGL(k_m2l_cached, dim3(c->nCube), dim3(64 * c->m2lWaves), c->m2lLds, st, c->m2lOff, c->m2lSrc,
-                       c->cubeLevelD, c->ordM2LD, c->ordMomD, c->m2lG, c->mom, c->lec, c->nmMax, c->eps, c->T,
+// the operator's buffers in one arithmetic: FP64 (plain form) or FP32 (precision form)
+template <typename R>
+struct OpBufs {
+  const R *Q0, *Q1, *G;
+  R *mom, *lec, *far, *nearv;
+  const typename Vec4<R>::type *coef;
+  size_t m2lLds;
+};
+static OpBufs<double> bufs64(fbh_ctx *c) {
+  return {c->Q0, c->Q1, c->m2lG, c->mom, c->lec, c->far, c->nearv, c->coef, c->m2lLds};
+}
+static OpBufs<float> bufs32(fbh_ctx *c) {
+  return {c->Q0f, c->Q1f, c->m2lGf, c->momf, c->lecf, c->farf, c->nearf, c->coef32, c->m2lLds32};
+}
+
+template <typename R>
+static void m2lPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
+  hipLaunchKernelGGL(k_m2l_cached<R>, dim3(c->nCube), dim3(64 * c->m2lWaves), B.m2lLds, st, c->m2lOff, c->m2lSrc,
+                     c->cubeLevelD, c->ordM2LD, c->ordMomD, B.G, B.mom, B.lec, c->nmMax, (R)c->eps, c->T,
+                     c->planTerms, c->planIdx, c->planOff);
+}
+template <>
+void m2lPass<double>(fbh_ctx *c, const OpBufs<double> &B, hipStream_t st) {
+  if (B.G) {
+    hipLaunchKernelGGL(k_m2l_cached<double>, dim3(c->nCube), dim3(64 * c->m2lWaves), B.m2lLds, st, c->m2lOff,
+                       c->m2lSrc, c->cubeLevelD, c->ordM2LD, c->ordMomD, B.G, B.mom, B.lec, c->nmMax, c->eps, c->T,
                        c->planTerms, c->planIdx, c->planOff);
   } else {
+    const int pm = c->maxOrder, nm = NMOM(pm);
     size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
     hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, st, c->m2lOff, c->m2lSrc, c->cubeLevelD, c->ordM2LD,
-                       c->ordMomD, c->cubeX, c->mom, c->lec, c->nmMax, c->kappa, c->eps, c->T);
+                       c->ordMomD, c->cubeX, B.mom, B.lec, c->nmMax, c->kappa, c->eps, c->T);
   }
 }
 
-static void q2mPass(fbh_ctx *c, const double *x, hipStream_t st) {
-  hipLaunchKernelGGL(k_q2m, dim3(c->nLeaf), dim3(256), 8 * (size_t)c->nmLeaf * sizeof(double), st, c->Q0, c->Q1,
-                     c->leafStart, c->leafCount, x, c->mom, c->n, c->nmLeaf, c->nmMax);
+template <typename R>
+static void q2mPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_q2m<R>, dim3(c->nLeaf), dim3(256), 8 * (size_t)c->nmLeaf * sizeof(R), st, B.Q0, B.Q1,
+                     c->leafStart, c->leafCount, x, B.mom, c->n, c->nmLeaf, c->nmMax);
 }
 
-static void m2mPass(fbh_ctx *c, int lev, hipStream_t st) {   // children at lev + 1 into parents at lev
+template <typename R>   // children at lev + 1 into parents at lev
+static void m2mPass(fbh_ctx *c, const OpBufs<R> &B, int lev, hipStream_t st) {
   const int nOut = NMOM(c->ordMom[lev]);
   if (c->levCount[lev] <= 0) return;
-  size_t lds = 3 * (size_t)nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut]) * sizeof(int);
-  hipLaunchKernelGGL(k_m2m, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), lds, st, c->levStart[lev],
-                     c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, c->mom, c->nmMax, c->T);
+  size_t lds = 3 * (size_t)nOut * sizeof(R) + (size_t)(nOut + 1 + c->hM2mStart[nOut]) * sizeof(int);
+  hipLaunchKernelGGL(k_m2m<R>, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), lds, st, c->levStart[lev],
+                     c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, B.mom, c->nmMax, c->T);
 }
 
-static void l2lPass(fbh_ctx *c, int lev, hipStream_t st) {   // parents at lev into children at lev + 1
+template <typename R>   // parents at lev into children at lev + 1
+static void l2lPass(fbh_ctx *c, const OpBufs<R> &B, int lev, hipStream_t st) {
   const int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
   if (c->levCount[lev + 1] <= 0) return;
-  size_t lds = 5 * (size_t)nIn * sizeof(double) + (size_t)(nOut + 1 + c->hConvStart[nOut]) * sizeof(int);
-  hipLaunchKernelGGL(k_l2l, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), lds, st, c->levStart[lev + 1],
-                     c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->lec, c->nmMax, c->T);
+  size_t lds = 5 * (size_t)nIn * sizeof(R) + (size_t)(nOut + 1 + c->hConvStart[nOut]) * sizeof(int);
+  hipLaunchKernelGGL(k_l2l<R>, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), lds, st, c->levStart[lev + 1],
+                     c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, B.lec, c->nmMax, c->T);
 }
 
-static void l2pPass(fbh_ctx *c, hipStream_t st) {
-  hipLaunchKernelGGL(k_l2p, dim3(c->nLeaf), dim3(256), 2 * (size_t)c->nmLeaf * sizeof(double), st, c->Q0, c->Q1,
-                     c->leafStart, c->leafCount, c->lec, c->far, c->n, c->nmLeaf, c->nmMax);
+template <typename R>
+static void l2pPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
+  hipLaunchKernelGGL(k_l2p<R>, dim3(c->nLeaf), dim3(256), 2 * (size_t)c->nmLeaf * sizeof(R), st, B.Q0, B.Q1,
+                     c->leafStart, c->leafCount, B.lec, B.far, c->n, c->nmLeaf, c->nmMax);
 }
 
-// far field into c->far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
-static void farPass(fbh_ctx *c, const double *x, hipStream_t st) {
-  q2mPass(c, x, st);
-  for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, lev, st);
-  m2lPass(c, st);
-  for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, lev, st);
-  l2pPass(c, st);
+// far field into B.far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
+template <typename R>
+static void farPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
+  q2mPass(c, B, x, st);
+  for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, st);
+  m2lPass(c, B, st);
+  for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, B, lev, st);
+  l2pPass(c, B, st);
 }
 
-static void nearPass(fbh_ctx *c, const double *x, hipStream_t st) {
-  hipLaunchKernelGGL(k_near_apply, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
-                     c->srcOff, c->ncol, c->coefOff, c->srcList, c->coef, x, c->nearv, c->n);
+template <typename R>
+static void nearPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_near_apply<R>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
+                     c->srcOff, c->ncol, c->coefOff, c->srcList, B.coef, x, B.nearv, c->n);
+}
+
+// y = alpha*(D - K) x + beta*y; the nearfield runs on the side stream beside
+// the far-field chain (both read x only) and the combine waits for both
+template <typename R>
+static int matvecT(fbh_ctx *c, const OpBufs<R> &B, double alpha, const double *x, double beta, double *y) {
+  HCK(hipEventRecord(c->evFork, c->stream));
+  HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
+  nearPass(c, B, x, c->side);
+  HCK(hipEventRecord(c->evJoin, c->side));
+  farPass(c, B, x, c->stream);
+  HCK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
+  double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_combine<R>, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, B.far, B.nearv, y,
+                     c->n, alpha, beta, s1, s2, 1, 1);
+  HCK(hipGetLastError());
+  return 1;
 }
 
 extern "C" int fbh_matvec_dev(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
   if (!c || !x || !y) { setErr("fbh_matvec_dev: null argument"); return 0; }
-  // the nearfield runs on the side stream beside the far-field chain; both
-  // read x only, and the combine waits for both
-  HCK(hipEventRecord(c->evFork, c->stream));
-  HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
-  nearPass(c, x, c->side);
-  HCK(hipEventRecord(c->evJoin, c->side));
-  farPass(c, x, c->stream);
-  HCK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
-  double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
-  int n2 = 2 * c->n;
-  hipLaunchKernelGGL(k_combine, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
-                     c->n, alpha, beta, s1, s2, 1, 1);
+  return matvecT(c, bufs64(c), alpha, x, beta, y);
+}
+
+extern "C" int fbh_enable_fp32(fbh_ctx *c) {
+  if (!c) { setErr("fbh_enable_fp32: null context"); return 0; }
+  if (c->fp32) return 1;
+  if (!c->m2lG) { setErr("fbh_enable_fp32: the FP32 operator needs the cached M2L tensors"); return 0; }
+  const size_t n = c->n, nq = n * c->nmLeaf, ne = (size_t)c->nCube * c->nmMax, ng = (size_t)c->nM2L * 2 * c->nmMax;
+  if (!(c->Q0f = dalloc<float>(c, nq)) || !(c->Q1f = dalloc<float>(c, nq)) || !(c->momf = dalloc<float>(c, 2 * ne)) ||
+      !(c->lecf = dalloc<float>(c, 4 * ne)) || !(c->farf = dalloc<float>(c, 2 * n)) ||
+      !(c->nearf = dalloc<float>(c, 2 * n)) || !(c->m2lGf = dalloc<float>(c, ng)) ||
+      !(c->coef32 = dalloc<float4>(c, (size_t)c->nInter)))
+    return 0;
+  auto cvt = [&](const double *in, float *out, size_t cnt) {
+    hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, c->stream, in, out, (long long)cnt);
+  };
+  cvt(c->Q0, c->Q0f, nq);
+  cvt(c->Q1, c->Q1f, nq);
+  cvt(c->m2lG, c->m2lGf, ng);
+  cvt((const double *)c->coef, (float *)c->coef32, 4 * (size_t)c->nInter);
+  HCK(hipMemsetAsync(c->momf, 0, 2 * ne * sizeof(float), c->stream));
+  HCK(hipMemsetAsync(c->lecf, 0, 4 * ne * sizeof(float), c->stream));
   HCK(hipGetLastError());
+  HCK(hipStreamSynchronize(c->stream));
+  c->fp32 = true;
   return 1;
+}
+
+extern "C" int fbh_matvec32_dev(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
+  if (!c || !x || !y) { setErr("fbh_matvec32_dev: null argument"); return 0; }
+  if (!c->fp32) { setErr("fbh_matvec32_dev: call fbh_enable_fp32 first"); return 0; }
+  return matvecT(c, bufs32(c), alpha, x, beta, y);
 }
 
 extern "C" int fbh_matvec(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
@@ -2002,8 +2103,8 @@ static int stageApply(fbh_ctx *c, double alpha, const double *sgm, double beta, 
   size_t b = 2 * (size_t)c->n * sizeof(double);
   HCK(hipMemcpyAsync(c->xs, sgm, b, hipMemcpyHostToDevice, c->stream));
   HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
-  if (useFar) farPass(c, c->xs, c->stream);
-  if (useNear) nearPass(c, c->xs, c->stream);
+  if (useFar) farPass(c, bufs64(c), c->xs, c->stream);
+  if (useNear) nearPass(c, bufs64(c), c->xs, c->stream);
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
                      beta, useFar, useNear);
@@ -2110,9 +2211,14 @@ static void rot(double *x, double *y, double c, double s) {
   *x = t;
 }
 
-extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit, int *iters,
-                             double *resid, double *hist, int *info) {
-  if (!c || !b || !x || restrt < 1 || maxit < 1) { setErr("fbh_gmres_dev: bad arguments"); return 0; }
+static int applyOp(fbh_ctx *c, bool fp32, double alpha, const double *x, double beta, double *y) {
+  return fp32 ? matvecT(c, bufs32(c), alpha, x, beta, y) : matvecT(c, bufs64(c), alpha, x, beta, y);
+}
+
+// restarted left-preconditioned GMRES on the FP64 or the FP32 operator; the
+// Krylov basis, Gram-Schmidt and Givens rotations are FP64 in both
+static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int restrt, double tol, int maxit,
+                     int *iters, double *resid, double *hist, int *info) {
   if (!ensureGmres(c, restrt)) return 0;
   const int n2 = 2 * c->n, ldh = restrt + 1, g = (n2 + 255) / 256;
   const size_t vb = (size_t)n2 * sizeof(double);
@@ -2124,7 +2230,7 @@ extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt,
   double xn, bnrm2, rn;
   HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
   if (!devNorm(c, x, &xn)) return 0;
-  if (xn != 0.0 && !fbh_matvec_dev(c, -1.0, x, 1.0, c->w)) return 0;
+  if (xn != 0.0 && !applyOp(c, fp32, -1.0, x, 1.0, c->w)) return 0;
   if (!fbh_psolve_diag_dev(c, c->r, c->w)) return 0;
   if (!devNorm(c, b, &bnrm2)) return 0;
   if (bnrm2 == 0.0) bnrm2 = 1.0;
@@ -2146,7 +2252,7 @@ extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt,
       ++i;
       ++*iters;
       double *vi = c->V + (size_t)(i - 1) * n2, *vn = c->V + (size_t)i * n2;
-      if (!fbh_matvec_dev(c, 1.0, vi, 0.0, c->w)) return 0;
+      if (!applyOp(c, fp32, 1.0, vi, 0.0, c->w)) return 0;
       if (!fbh_psolve_diag_dev(c, c->w, c->w)) return 0;
       for (int k = 0; k < i; k++) {   // modified Gram-Schmidt
         const double *vk = c->V + (size_t)k * n2;
@@ -2178,7 +2284,7 @@ extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt,
     HCK(hipStreamSynchronize(c->stream));
     if (*resid <= tol) return 1;
     HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
-    if (!fbh_matvec_dev(c, -1.0, x, 1.0, c->w)) return 0;
+    if (!applyOp(c, fp32, -1.0, x, 1.0, c->w)) return 0;
     if (!fbh_psolve_diag_dev(c, c->r, c->w)) return 0;
     if (!devNorm(c, c->r, &rn)) return 0;
     *resid = rn / bnrm2;
@@ -2186,6 +2292,76 @@ extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt,
     if (*resid <= tol) return 1;
     if (*iters == maxit) { *info = 1; return 1; }
   }
+}
+
+extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit, int *iters,
+                             double *resid, double *hist, int *info) {
+  if (!c || !b || !x || restrt < 1 || maxit < 1) { setErr("fbh_gmres_dev: bad arguments"); return 0; }
+  return gmresCore(c, false, b, x, restrt, tol, maxit, iters, resid, hist, info);
+}
+
+__global__ void k_axpy(double *__restrict__ y, const double *__restrict__ x, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+// Precision form: iterative refinement around GMRES on the FP32 operator.
+// Each correction solves A32 d = r with d0 = 0 to a relative residual of
+// half the outer tolerance measured against ||b||; x += d; the residual that
+// decides convergence is r = b - A64 x in FP64 with the same test as the plain
+// solve, ||M^-1 r|| / ||b|| <= tol.  The first residual (x = x0) is taken with
+// the FP32 operator.  iters counts inner iterations (FP32 products), outer the
+// corrections, mv64 the FP64 products.
+extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit,
+                                     int maxOuter, int *iters, double *resid, double *hist, int *info, int *outer,
+                                     int *mv64) {
+  if (!c || !b || !x || restrt < 1 || maxit < 1 || maxOuter < 1) {
+    setErr("fbh_gmres_refined_dev: bad arguments");
+    return 0;
+  }
+  if (!c->fp32 && !fbh_enable_fp32(c)) return 0;
+  if (!ensureGmres(c, restrt)) return 0;
+  const int n2 = 2 * c->n, g = (n2 + 255) / 256;
+  const size_t vb = (size_t)n2 * sizeof(double);
+  if (!c->ro) HCK(hipMalloc(&c->ro, vb));
+  if (!c->dx) HCK(hipMalloc(&c->dx, vb));
+  *iters = 0;
+  *outer = 0;
+  *mv64 = 0;
+  *info = 0;
+  int nh = 0;
+  double bnrm2, xn;
+  if (!devNorm(c, b, &bnrm2)) return 0;
+  if (bnrm2 == 0.0) bnrm2 = 1.0;
+  if (!devNorm(c, x, &xn)) return 0;
+  HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
+  if (xn != 0.0 && !applyOp(c, true, -1.0, x, 1.0, c->ro)) return 0;
+  for (;;) {
+    double rr;
+    if (!devNorm(c, c->ro, &rr)) return 0;
+    if (!(rr > 0.0)) break;
+    HCK(hipMemsetAsync(c->dx, 0, vb, c->stream));
+    int it = 0, inf = 0;
+    double ir;
+    if (!gmresCore(c, true, c->ro, c->dx, restrt, 0.5 * tol * bnrm2 / rr, maxit - *iters, &it, &ir, nullptr, &inf))
+      return 0;
+    *iters += it;
+    ++*outer;
+    hipLaunchKernelGGL(k_axpy, dim3(g), dim3(256), 0, c->stream, x, c->dx, n2);
+    HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
+    if (!applyOp(c, false, -1.0, x, 1.0, c->ro)) return 0;
+    ++*mv64;
+    if (!fbh_psolve_diag_dev(c, c->w, c->ro)) return 0;
+    double rn;
+    if (!devNorm(c, c->w, &rn)) return 0;
+    *resid = rn / bnrm2;
+    if (hist) hist[nh++] = *resid;
+    if (rn != rn) { *info = 2; return 1; }
+    if (*resid <= tol) return 1;
+    if (*iters >= maxit || *outer >= maxOuter) { *info = 1; return 1; }
+  }
+  *resid = 0.0;
+  return 1;
 }
 
 extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b_dev) {
@@ -2347,19 +2523,20 @@ extern "C" int fbh_profile(fbh_ctx *c, const double *x, double *y, int iters, do
   int n2 = 2 * c->n;
   for (int it = 0; it < iters; it++) {
     hipEventRecord(ev[0], c->stream);
-    nearPass(c, x, c->stream);
+    const OpBufs<double> B = bufs64(c);
+    nearPass(c, B, x, c->stream);
     hipEventRecord(ev[1], c->stream);
-    q2mPass(c, x, c->stream);
+    q2mPass(c, B, x, c->stream);
     hipEventRecord(ev[2], c->stream);
-    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, lev, c->stream);
+    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
     hipEventRecord(ev[3], c->stream);
-    m2lPass(c, c->stream);
+    m2lPass(c, B, c->stream);
     hipEventRecord(ev[4], c->stream);
-    for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, lev, c->stream);
+    for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, B, lev, c->stream);
     hipEventRecord(ev[5], c->stream);
-    l2pPass(c, c->stream);
+    l2pPass(c, B, c->stream);
     hipEventRecord(ev[6], c->stream);
-    hipLaunchKernelGGL(k_combine, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
+    hipLaunchKernelGGL(k_combine<double>, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far, c->nearv, y,
                        c->n, 1.0, 0.0, s1, s2, 1, 1);
     hipEventRecord(ev[7], c->stream);
     HCK(hipEventSynchronize(ev[7]));
@@ -2411,7 +2588,7 @@ extern "C" int fbh_copy_d2d(fbh_ctx *c, double *d, const double *s, long long n)
 extern "C" int fbh_q2m_host(fbh_ctx *c, const double *sgm, double *mom) {
   if (!c || !sgm || !mom) { setErr("fbh_q2m_host: null argument"); return 0; }
   HCK(hipMemcpyAsync(c->xs, sgm, 2 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  q2mPass(c, c->xs, c->stream);
+  q2mPass(c, bufs64(c), c->xs, c->stream);
   HCK(hipGetLastError());
   HCK(hipMemcpyAsync(mom, c->mom, (size_t)c->nLeaf * 2 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
@@ -2421,7 +2598,7 @@ extern "C" int fbh_q2m_host(fbh_ctx *c, const double *sgm, double *mom) {
 extern "C" int fbh_m2l_host(fbh_ctx *c, const double *mom, double *lec) {
   if (!c || !mom || !lec) { setErr("fbh_m2l_host: null argument"); return 0; }
   HCK(hipMemcpyAsync(c->mom, mom, (size_t)c->nCube * 2 * c->nmMax * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  m2lPass(c, c->stream);
+  m2lPass(c, bufs64(c), c->stream);
   HCK(hipGetLastError());
   HCK(hipMemcpyAsync(lec, c->lec, (size_t)c->nCube * 4 * c->nmMax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
@@ -2433,7 +2610,7 @@ extern "C" int fbh_l2p_host(fbh_ctx *c, const double *lec, double alpha, double 
   size_t b = 2 * (size_t)c->n * sizeof(double);
   HCK(hipMemcpyAsync(c->lec, lec, (size_t)c->nLeaf * 4 * c->nmMax * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
-  l2pPass(c, c->stream);
+  l2pPass(c, bufs64(c), c->stream);
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha, beta,
                      1, 0);

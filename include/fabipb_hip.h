@@ -117,6 +117,26 @@ int fbh_psolve_diag_dev(fbh_ctx *ctx, double *out, const double *in);
 int fbh_gmres_dev(fbh_ctx *ctx, const double *b, double *x, int restart, double tol,
                   int maxit, int *iters, double *resid, double *hist, int *info);
 
+/*
+ * Precision form (no counterpart in the reference, whose solve is FP64 only).
+ * fbh_enable_fp32 makes FP32 copies of the cached operator: nearfield
+ * coefficients (16 bytes per interaction), leaf Q2M/L2P matrices and M2L
+ * derivative tensors; expansions, translations and the nearfield sums then
+ * run in FP32.  Needs the M2L tensor cache (it is kept whenever it fits).
+ * fbh_matvec32_dev is fbh_matvec_dev on that operator (x, y stay FP64).
+ * fbh_gmres_refined_dev solves to the same criterion as fbh_gmres_dev
+ * (||M^-1 (b - A x)|| / ||b|| <= tol, A the FP64 operator) by iterative
+ * refinement: corrections from GMRES on the FP32 operator (FP64 Krylov basis,
+ * Gram-Schmidt and rotations), residuals in FP64.  iters = inner iterations,
+ * outer = corrections (at most maxOuter), mv64 = FP64 products; hist (optional,
+ * maxOuter entries) receives the outer residuals; info as for fbh_gmres_dev.
+ */
+int fbh_enable_fp32(fbh_ctx *ctx);
+int fbh_matvec32_dev(fbh_ctx *ctx, double alpha, const double *x, double beta, double *y);
+int fbh_gmres_refined_dev(fbh_ctx *ctx, const double *b, double *x, int restart, double tol,
+                          int maxit, int maxOuter, int *iters, double *resid, double *hist,
+                          int *info, int *outer, int *mv64);
+
 /* direct right-hand side on host arrays: b[2n] = fac * sum_q ... (setupRHS direct loop,
  * src/fabipb.c:789-798; gpuSetupRHS, gpu_backend.h:21), fac = 1/(4 pi eps1) */
 int fbh_rhs(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b);

@@ -202,6 +202,37 @@ def test_orders_against_oracle(fa, meshdir, name, order):
     assert abs(e - ref) <= 1e-12 * abs(ref)
 
 
+@pytest.mark.parametrize("name", ["glob4", "cfg1_sphere"])
+def test_fp32_operator(fa, meshdir, name):
+    """The FP32 operator of the precision form against the FP64 one.  Tolerance 1e-5:
+    FP32 coefficients and expansions (unit roundoff 6e-8) summed over ~1e3 near
+    sources per row, with the cancellation of D x - K x."""
+    mesh, tree, op = case_op(fa, meshdir, name)
+    op.enable_fp32()
+    for seed in (91, 92):
+        x = splitmix_vec(seed, 2 * tree.n)
+        y64 = op.matvec(x)
+        y32 = op.matvec32(x)
+        print(f"\n[{name}] fp32 matvec rel {rel(y32, y64):.3e}", flush=True)
+        assert rel(y32, y64) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["glob4", "cfg1_sphere"])
+def test_fp32_refined_solve(fa, meshdir, name):
+    """Precision form: FP64-refined solve on the FP32 operator reaches the plain solve's
+    residual test in FP64 and its energy within 1e-6 relative."""
+    mesh, tree, op = case_op(fa, meshdir, name)
+    kw = dict(kappa=FL["k"], eps1=FL["eps1"], eps2=FL["eps2"], qorder=FL["q"], tol=FL["o"],
+              restart=FL["a"], maxit=FL["i"], height=FL["H"], op=op, tree=tree, mesh=mesh)
+    r64 = fa.solve(None, tree.depth, **kw)
+    r32 = fa.solve(None, tree.depth, precision="fp32", **kw)
+    err = abs(r32["energy_raw"] - r64["energy_raw"]) / abs(r64["energy_raw"])
+    print(f"\n[{name}] fp64 its={r64['iters']} fp32 its={r32['iters']} outer={r32['outer']} "
+          f"mv64={r32['mv64']} resid={r32['resid']:.3e} energy rel err={err:.3e}", flush=True)
+    assert r32["info"] == 0 and r32["resid"] <= FL["o"]
+    assert err <= 1e-6
+
+
 def test_matvec_linearity_full_size(fa, meshdir):
     """Size-independent property at the bench size: A(a x + b z) = a A x + b A z."""
     mesh, tree, op = case_op(fa, meshdir, "cfg2_globule")
