@@ -265,6 +265,178 @@ done:
   return (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
 }
 
+/* ---------------- merging of small faces ---------------- */
+/* Edge collapse of faces below area amin: for each such face, in index order,
+ * its edges shortest first are collapsed to the surface point over the mean of
+ * the two directions.  A collapse is skipped when it would break the closed
+ * manifold (the two end vertices must share exactly the two opposite
+ * vertices), leave a vertex of valence below 4, or flip or flatten a face.  A
+ * collapse locks the vertices of every face it touched for the rest of the
+ * pass; passes repeat until one makes no collapse.  Fully deterministic. */
+static void tri_normal(const double *a, const double *b, const double *c, double *n) {
+  double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+static double tri_area(const double *a, const double *b, const double *c) {
+  double n[3];
+  tri_normal(a, b, c, n);
+  return 0.5 * sqrt(dot3(n, n));
+}
+
+static int merge_small(mesh_t *m, const shape_t *s, double amin, double *pos) {
+  int nv = m->nv, nf = m->nf, collapsed = 0;
+  char *alive = malloc(nf), *lock = malloc(nv);
+  int *off = malloc(sizeof(int) * (size_t)(nv + 1)), *adj = malloc(sizeof(int) * 3 * (size_t)nf);
+  memset(alive, 1, nf);
+  for (int pass = 0; pass < 100; pass++) {
+    int done = 0;
+    memset(lock, 0, nv);
+    memset(off, 0, sizeof(int) * (size_t)(nv + 1));
+    for (int f = 0; f < nf; f++)
+      if (alive[f])
+        for (int k = 0; k < 3; k++) off[m->f[3 * f + k] + 1]++;
+    for (int v = 0; v < nv; v++) off[v + 1] += off[v];
+    int *fill = malloc(sizeof(int) * (size_t)nv);
+    memcpy(fill, off, sizeof(int) * (size_t)nv);
+    for (int f = 0; f < nf; f++)
+      if (alive[f])
+        for (int k = 0; k < 3; k++) adj[fill[m->f[3 * f + k]]++] = f;
+    free(fill);
+    for (int f = 0; f < nf; f++) {
+      if (!alive[f]) continue;
+      int *F = &m->f[3 * f];
+      if (tri_area(&pos[3 * F[0]], &pos[3 * F[1]], &pos[3 * F[2]]) >= amin) continue;
+      int order[3] = {0, 1, 2};
+      double len[3];
+      for (int k = 0; k < 3; k++) {
+        const double *a = &pos[3 * F[k]], *b = &pos[3 * F[(k + 1) % 3]];
+        double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+        len[k] = dot3(d, d);
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++)
+          if (len[order[j]] < len[order[i]]) { int t = order[i]; order[i] = order[j]; order[j] = t; }
+      for (int oi = 0; oi < 3; oi++) {
+        const int u = F[order[oi]], v = F[(order[oi] + 1) % 3];
+        if (lock[u] || lock[v]) continue;
+        /* faces on the edge and the opposite vertices */
+        int ef[2], ne = 0, opp[2];
+        for (int q = off[u]; q < off[u + 1]; q++) {
+          const int g = adj[q], *G = &m->f[3 * g];
+          if (!alive[g]) continue;
+          if (G[0] == v || G[1] == v || G[2] == v) {
+            if (ne < 2) { ef[ne] = g; opp[ne] = G[0] != u && G[0] != v ? G[0] : G[1] != u && G[1] != v ? G[1] : G[2]; }
+            ne++;
+          }
+        }
+        if (ne != 2) continue;
+        /* link condition: common neighbours of u and v are exactly opp[0], opp[1] */
+        int nbu[64], nu = 0, bad = 0, common = 0;
+        for (int q = off[u]; q < off[u + 1] && !bad; q++) {
+          const int *G = &m->f[3 * adj[q]];
+          if (!alive[adj[q]]) continue;
+          for (int k = 0; k < 3; k++) {
+            int w = G[k], seen = 0;
+            if (w == u) continue;
+            for (int t = 0; t < nu; t++) seen |= nbu[t] == w;
+            if (!seen) { if (nu == 64) { bad = 1; break; } nbu[nu++] = w; }
+          }
+        }
+        int nvv = 0, seenv[64];
+        for (int q = off[v]; q < off[v + 1] && !bad; q++) {
+          const int *G = &m->f[3 * adj[q]];
+          if (!alive[adj[q]]) continue;
+          for (int k = 0; k < 3; k++) {
+            int w = G[k], seen = 0;
+            if (w == v) continue;
+            for (int t = 0; t < nvv; t++) seen |= seenv[t] == w;
+            if (seen) continue;
+            if (nvv == 64) { bad = 1; break; }
+            seenv[nvv++] = w;
+            if (w == u) continue;
+            for (int t = 0; t < nu; t++)
+              if (nbu[t] == w) { common++; if (w != opp[0] && w != opp[1]) bad = 1; }
+          }
+        }
+        if (bad || common != 2 || nu + nvv - 4 < 4) continue;
+        int valOpp = 1;
+        for (int t = 0; t < 2; t++) {
+          int cnt = 0;
+          for (int q = off[opp[t]]; q < off[opp[t] + 1]; q++) cnt += alive[adj[q]];
+          if (cnt - 1 < 4) valOpp = 0;
+        }
+        if (!valOpp) continue;
+        /* new point and the face checks */
+        double dir[3] = {m->v[3 * u] + m->v[3 * v], m->v[3 * u + 1] + m->v[3 * v + 1], m->v[3 * u + 2] + m->v[3 * v + 2]};
+        normalize3(dir);
+        double pn[3];
+        shape_point(s, dir, pn);
+        int okf = 1;
+        for (int side = 0; side < 2 && okf; side++) {
+          const int x = side ? v : u;
+          for (int q = off[x]; q < off[x + 1] && okf; q++) {
+            const int g = adj[q];
+            if (!alive[g] || g == ef[0] || g == ef[1]) continue;
+            const int *G = &m->f[3 * g];
+            const double *P[3], *Q[3];
+            for (int k = 0; k < 3; k++) {
+              P[k] = &pos[3 * G[k]];
+              Q[k] = (G[k] == u || G[k] == v) ? pn : P[k];
+            }
+            double n0[3], n1[3];
+            tri_normal(P[0], P[1], P[2], n0);
+            tri_normal(Q[0], Q[1], Q[2], n1);
+            double c = dot3(n0, n1), l0 = sqrt(dot3(n0, n0)), l1 = sqrt(dot3(n1, n1));
+            if (!(c > 0.5 * l0 * l1) || !(l1 > 1e-8)) okf = 0;
+          }
+        }
+        if (!okf) continue;
+        /* collapse v into u */
+        for (int k = 0; k < 3; k++) { m->v[3 * u + k] = dir[k]; pos[3 * u + k] = pn[k]; }
+        alive[ef[0]] = alive[ef[1]] = 0;
+        for (int side = 0; side < 2; side++) {
+          const int x = side ? v : u;
+          for (int q = off[x]; q < off[x + 1]; q++) {
+            const int g = adj[q];
+            int *G = &m->f[3 * g];
+            for (int k = 0; k < 3; k++) {
+              if (alive[g] && G[k] == v) G[k] = u;
+              lock[G[k]] = 1;
+            }
+          }
+        }
+        lock[u] = lock[v] = 1;
+        done++;
+        break;
+      }
+    }
+    collapsed += done;
+    if (!done) break;
+  }
+  /* compact: live faces, used vertices, in their original order */
+  int *map = malloc(sizeof(int) * (size_t)nv), nf2 = 0, nv2 = 0;
+  for (int v = 0; v < nv; v++) map[v] = -1;
+  for (int f = 0; f < nf; f++)
+    if (alive[f]) {
+      for (int k = 0; k < 3; k++) map[m->f[3 * f + k]] = 0;
+      for (int k = 0; k < 3; k++) m->f[3 * nf2 + k] = m->f[3 * f + k];
+      nf2++;
+    }
+  for (int v = 0; v < nv; v++)
+    if (map[v] == 0) {
+      map[v] = nv2;
+      for (int k = 0; k < 3; k++) { m->v[3 * nv2 + k] = m->v[3 * v + k]; pos[3 * nv2 + k] = pos[3 * v + k]; }
+      nv2++;
+    }
+  for (int i = 0; i < 3 * nf2; i++) m->f[i] = map[m->f[i]];
+  m->nv = nv2;
+  m->nf = nf2;
+  free(map); free(alive); free(lock); free(off); free(adj);
+  return collapsed;
+}
+
 /* ---------------- output ---------------- */
 static FILE *open_out(const char *base, const char *ext) {
   char path[4096];
@@ -278,7 +450,8 @@ static void usage(void) {
   fprintf(stderr,
           "usage: fbmesh <sphere|globule> <out-base> [--level L] [--radius R]\n"
           "              [--amp A] [--lmax L] [--charges ball|center|inside]\n"
-          "              [--nq N] [--rq r] [--q0 q] [--sigma s] [--depth d] [--seed S]\n");
+          "              [--nq N] [--rq r] [--q0 q] [--sigma s] [--depth d] [--seed S]\n"
+          "              [--merge A]   (collapse faces below area A, square Angstrom)\n");
   exit(2);
 }
 
@@ -286,7 +459,7 @@ int main(int argc, char **argv) {
   if (argc < 3) usage();
   const char *kind = argv[1], *base = argv[2];
   int level = 3, lmax = 6, nq = 1;
-  double R = 10.0, amp = 0.08, rq = 8.0, q0 = 1.0, sigma = 0.4, depth = 1.5;
+  double R = 10.0, amp = 0.08, rq = 8.0, q0 = 1.0, sigma = 0.4, depth = 1.5, merge = 0.0;
   const char *charges = "center";
   uint64_t seed = 1;
   for (int i = 3; i < argc; i++) {
@@ -304,6 +477,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(a, "--sigma")) sigma = atof(v);
     else if (!strcmp(a, "--depth")) depth = atof(v);
     else if (!strcmp(a, "--seed")) seed = strtoull(v, NULL, 10);
+    else if (!strcmp(a, "--merge")) merge = atof(v);
     else usage();
   }
   if (level < 0 || level > 9 || nq < 1) usage();
@@ -325,10 +499,12 @@ int main(int argc, char **argv) {
   icosphere(&m, level);
   double *pos = malloc(sizeof(double) * 3 * (size_t)m.nv);
   double *nrm = malloc(sizeof(double) * 3 * (size_t)m.nv);
-  for (int i = 0; i < m.nv; i++) {
-    shape_point(&shape, &m.v[3 * i], &pos[3 * i]);
-    shape_normal(&shape, &m.v[3 * i], &nrm[3 * i]);
+  for (int i = 0; i < m.nv; i++) shape_point(&shape, &m.v[3 * i], &pos[3 * i]);
+  if (merge > 0.0) {
+    int nf0 = m.nf, k = merge_small(&m, &shape, merge, pos);
+    printf("fbmesh: merged faces below %g A^2: %d collapses, %d -> %d faces\n", merge, k, nf0, m.nf);
   }
+  for (int i = 0; i < m.nv; i++) shape_normal(&shape, &m.v[3 * i], &nrm[3 * i]);
 
   /* charges */
   double *qx = malloc(sizeof(double) * 3 * (size_t)nq), *qq = malloc(sizeof(double) * (size_t)nq);
@@ -358,6 +534,28 @@ int main(int argc, char **argv) {
       }
     }
     double reach = depth + lmaxEdge, reach2 = reach * reach, depth2 = depth * depth;
+    /* uniform grid of face centroids, cell = reach: a point's candidates lie in
+       the 27 cells around it */
+    const int G = (int)fmin(256.0, fmax(1.0, floor(2.0 * rmax / reach)));
+    const double cell = 2.0 * rmax / G;
+    int *gstart = calloc((size_t)G * G * G + 1, sizeof(int)), *gface = malloc(sizeof(int) * (size_t)m.nf);
+    int *gcell = malloc(sizeof(int) * (size_t)m.nf);
+    for (int f = 0; f < m.nf; f++) {
+      int ci[3];
+      for (int k = 0; k < 3; k++) {
+        ci[k] = (int)floor((cen[3 * f + k] + rmax) / cell);
+        ci[k] = ci[k] < 0 ? 0 : ci[k] >= G ? G - 1 : ci[k];
+      }
+      gcell[f] = (ci[0] * G + ci[1]) * G + ci[2];
+      gstart[gcell[f] + 1]++;
+    }
+    for (int i = 0; i < G * G * G; i++) gstart[i + 1] += gstart[i];
+    {
+      int *fill = malloc(sizeof(int) * (size_t)G * G * G);
+      memcpy(fill, gstart, sizeof(int) * (size_t)G * G * G);
+      for (int f = 0; f < m.nf; f++) gface[fill[gcell[f]]++] = f;
+      free(fill);
+    }
     long tries = 0;
     for (int j = 0; j < nq;) {
       if (++tries > 1000000000L) { fprintf(stderr, "fbmesh: charge placement failed\n"); return 1; }
@@ -367,12 +565,21 @@ int main(int argc, char **argv) {
       if (rho < 1e-12) continue;
       for (int k = 0; k < 3; k++) d[k] = p[k] / rho;
       if (rho > shape_r(&shape, d) - depth * 0.5) continue;  /* outside, or too close radially */
-      int ok = 1;
-      for (int f = 0; f < m.nf && ok; f++) {
-        double dx = p[0] - cen[3 * f], dy = p[1] - cen[3 * f + 1], dz = p[2] - cen[3 * f + 2];
-        if (dx * dx + dy * dy + dz * dz > reach2) continue;
-        if (pt_tri_dist2(p, &pos[3 * m.f[3 * f]], &pos[3 * m.f[3 * f + 1]], &pos[3 * m.f[3 * f + 2]]) < depth2) ok = 0;
-      }
+      int ok = 1, pc[3];
+      for (int k = 0; k < 3; k++) pc[k] = (int)floor((p[k] + rmax) / cell);
+      for (int i0 = pc[0] - 1; i0 <= pc[0] + 1 && ok; i0++)
+        for (int i1 = pc[1] - 1; i1 <= pc[1] + 1 && ok; i1++)
+          for (int i2 = pc[2] - 1; i2 <= pc[2] + 1 && ok; i2++) {
+            if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= G || i1 >= G || i2 >= G) continue;
+            const int cc = (i0 * G + i1) * G + i2;
+            for (int q = gstart[cc]; q < gstart[cc + 1] && ok; q++) {
+              const int f = gface[q];
+              double dx = p[0] - cen[3 * f], dy = p[1] - cen[3 * f + 1], dz = p[2] - cen[3 * f + 2];
+              if (dx * dx + dy * dy + dz * dz > reach2) continue;
+              if (pt_tri_dist2(p, &pos[3 * m.f[3 * f]], &pos[3 * m.f[3 * f + 1]], &pos[3 * m.f[3 * f + 2]]) < depth2)
+                ok = 0;
+            }
+          }
       if (!ok) continue;
       for (int k = 0; k < 3; k++) qx[3 * j + k] = p[k];
       qq[j] = sigma * rng_normal(&rng);
@@ -382,7 +589,7 @@ int main(int argc, char **argv) {
     for (int j = 0; j < nq; j++) s += qq[j];
     double shift = (nearbyint(s) - s) / nq;
     for (int j = 0; j < nq; j++) qq[j] += shift;
-    free(cen);
+    free(cen); free(gstart); free(gface); free(gcell);
   } else {
     usage();
   }
