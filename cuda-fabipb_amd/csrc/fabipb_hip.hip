@@ -990,6 +990,47 @@ __global__ __launch_bounds__(256) void k_rhs(const Pnl *__restrict__ pnl, int n,
   }
 }
 
+#pragma clang fp contract(fast)
+// the direct right-hand side at q = 1: one quadrature point per panel, so the
+// point and its weight 2 A t w w leave the charge loop; 1/r from rsqrt
+__global__ __launch_bounds__(256) void k_rhs_q1(const Pnl *__restrict__ pnl, int n, int nChar,
+                                                const double *__restrict__ pos, const double *__restrict__ chr,
+                                                double *__restrict__ b, double fac, double t, double ww) {
+  __shared__ double4 sp[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xq[3] = {0, 0, 0}, nr[3] = {0, 0, 0}, W = 0.0;
+  if (i < n) {
+    const Pnl &P = pnl[i];
+    for (int k = 0; k < 3; k++) {
+      xq[k] = P.v[k] + t * (P.a[6 + k] + t * P.a[k]);
+      nr[k] = P.nr[k];
+    }
+    W = ww * 2.0 * P.area * fac;
+  }
+  double b0 = 0.0, b1 = 0.0;
+  for (int j0 = 0; j0 < nChar; j0 += 256) {
+    __syncthreads();
+    const int j = j0 + threadIdx.x;
+    if (j < nChar) sp[threadIdx.x] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], chr[j]);
+    __syncthreads();
+    const int cnt = min(256, nChar - j0);
+#pragma unroll 4
+    for (int jj = 0; jj < cnt; jj++) {
+      const double4 c = sp[jj];
+      const double rx = xq[0] - c.x, ry = xq[1] - c.y, rz = xq[2] - c.z;
+      const double r2 = rx * rx + ry * ry + rz * rz, ri = rsqrt(r2);
+      const double ip = nr[0] * rx + nr[1] * ry + nr[2] * rz;
+      b0 += c.w * ri;
+      b1 -= c.w * ip * (ri * ri * ri);
+    }
+  }
+  if (i < n) {
+    b[i] = b0 * W;
+    b[i + n] = b1 * W;
+  }
+}
+#pragma clang fp contract(off)
+
 // ------------------------------------------------------------ charge tree
 
 // leaf moments of the charges (src/chargeTree.c:313-337): block per leaf
@@ -2372,8 +2413,12 @@ extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const doubl
   HCK(hipMemcpy(dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
   HCK(hipMemcpy(dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
   double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
-  hipLaunchKernelGGL(k_rhs, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc, b_dev,
-                     fac, c->Q, c->qorder);
+  if (c->qorder == 1)
+    hipLaunchKernelGGL(k_rhs_q1, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc,
+                       b_dev, fac, c->Q.t[0], c->Q.t[0] * c->Q.w[0] * c->Q.w[0]);
+  else
+    hipLaunchKernelGGL(k_rhs, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc, b_dev,
+                       fac, c->Q, c->qorder);
   hipError_t e = hipStreamSynchronize(c->stream);
   hipFree(dp);
   hipFree(dc);

@@ -100,5 +100,10 @@ def cli_cases():
     return [k for k, v in CASES["cases"].items() if v.get("cli")]
 
 
+def case_flags(name):
+    """The common solve flags with the case's own overrides (GMRES restart -a, limit -i)."""
+    return dict(CASES["flags"], **CASES["cases"][name].get("flags", {}))
+
+
 def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))

@@ -45,7 +45,6 @@ def main():
     cfg = json.load(open(os.path.join(HERE, "cases.json")))
     run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8"], cwd=ROOT)
     run(["make", "-s", "-C", "cuda-fabipb_amd", "bin/fbmesh"], cwd=ROOT)
-    fl = cfg["flags"]
     want = sys.argv[1:] or list(cfg["cases"])
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
                VECLIB_MAXIMUM_THREADS="1", BLIS_NUM_THREADS="1", FABIPB_REUSE_MESH="1")
@@ -53,6 +52,7 @@ def main():
     try:
         for name in want:
             case = cfg["cases"][name]
+            fl = dict(cfg["flags"], **case.get("flags", {}))   # per-case overrides (restart, iteration limit)
             base = os.path.join(tmp, name)
             run([os.path.join(ROOT, "cuda-fabipb_amd/bin/fbmesh"), case["mesh"][0], base] + case["mesh"][1:])
             out = os.path.join(HERE, name)

@@ -125,6 +125,27 @@ def test_gmres_solve_matches_reference(oracles, name):
     assert abs(e - cli["panel_tree_raw"]) <= 1e-11 * abs(cli["panel_tree_raw"])
 
 
+@pytest.mark.parametrize("name", ["glob4_a3", "glob4_a3_i5"])
+def test_gmres_restart_and_limit_match_reference(oracles, name):
+    """GMRES(3) with restarts, and the iteration limit (info 1), against the reference's
+    log on the glob4 mesh: iterations, info, the residual after each step and after each
+    restart (src/gmres.c:295-340), and the energy of the returned iterate."""
+    from conftest import CASES, case_flags
+    F, o = oracles(CASES["cases"][name]["same_as"])
+    cli = fixture(name)["cli"]
+    fl = case_flags(name)
+    b = F["rhs"]
+    x, info, its, res, hist = o.gmres(b, b.copy(), restart=fl["a"], tol=fl["o"], maxit=fl["i"])
+    assert info == cli["info"] and its == cli["iterations"]
+    h = np.array(cli["resid_history"])
+    assert len(h) > its   # restarts add the recomputed residual to the log
+    assert np.allclose(hist[:len(h)], h, rtol=1e-5, atol=0)
+    depth = len(F["lev_start"]) - 1
+    ct = golden_io.ctree_from_fixture(F, depth)
+    e = o.energy(ct, F["pos"].reshape(-1, 3), F["chr"], x)
+    assert abs(e - cli["panel_tree_raw"]) <= 1e-11 * abs(cli["panel_tree_raw"])
+
+
 def test_cfg1_matvec(fa, meshdir):
     """20,480-panel sphere: oracle nearfield and matvec on the product's tree, whose lists
     and panel order equal the reference's, against the recorded vectors."""
