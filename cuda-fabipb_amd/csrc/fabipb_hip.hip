@@ -1559,6 +1559,9 @@ struct fbh_ctx {
   float4 *coef32 = nullptr;
   size_t m2lLds32 = 0;
   double *ro = nullptr, *dx = nullptr;   // refinement residual and correction
+  void *enArena = nullptr;                // energy: charge tree, moments and scratch
+  size_t enArenaBytes = 0;
+  std::vector<unsigned char> enStage;
   // nearfield on a second stream, overlapped with the far-field chain
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
@@ -1618,6 +1621,7 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   if (c->ydev) hipFree(c->ydev);
   if (c->scal) hipFree(c->scal);
   if (c->ro) hipFree(c->ro);
+  if (c->enArena) hipFree(c->enArena);
   if (c->dx) hipFree(c->dx);
   if (c->side) {
     hipStreamSynchronize(c->side);
@@ -2466,97 +2470,104 @@ extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const do
   }
   ord[depth + 1] = maxOrd + 1;
   const int nmMax = c->nmMax;
-  int *dLevel, *dKid, *dNK, *dCS, *dNC, *dCI, *dOrd;
-  double *dX, *dR, *dPos, *dChr, *dMom, *dPart, *dOut, *dRed;
-  std::vector<void *> tmp;
-  auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
-    if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
-    tmp.push_back(*dst);
-    return !src || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  // one expansion order over the levels the walk can accept: the wave walk
+  // with per-cluster polynomial tables; otherwise the wave-per-panel walk
+  const int c0 = lvStart[c->height], P = ord[c->height];
+  bool uni = c0 >= 0 && P >= 1 && P <= 9;
+  for (int l = c->height; l <= depth && uni; l++) uni = ord[l] == P;
+  std::vector<int4> dsc;
+  const int SZ = uni ? enPolySize(P) : 0, nClus = uni ? nc - c0 : 0;
+  if (uni) {
+    int o = 0;
+    for (int a = P; a >= 0; a--)   // the order enFar reads them in
+      for (int b = P - a; b >= 0; b--)
+        for (int g3 = P - a - b; g3 >= 0; g3--)
+          for (int j = 0; j <= (P - a - b - g3) / 2; j++) dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 0, o++));
+    for (int a = P + 1; a >= 0; a--)
+      for (int b = P + 1 - a; b >= 0; b--)
+        for (int g3 = P + 1 - a - b; g3 >= 0; g3--)
+          for (int j = 0; j <= (P + 1 - a - b - g3) / 2; j++, o += 3)
+            dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 1, o));
+    if (o != SZ) { setErr("fbh_energy: coefficient table has %d of %d entries", o, SZ); return 0; }
+  }
+  // inputs staged in one host block and copied once into a device arena kept
+  // by the context; scratch follows the inputs in the same arena
+  struct Seg { const void *src; size_t bytes; size_t off; };
+  std::vector<Seg> seg;
+  size_t top = 0;
+  auto add = [&](const void *src, size_t bytes) {
+    seg.push_back({src, bytes, top});
+    top += (bytes + 255) & ~(size_t)255;
+    return seg.size() - 1;
   };
-  bool ok = up((void **)&dLevel, ctLevel, nc * sizeof(int)) && up((void **)&dKid, ctKidStart, nc * sizeof(int)) &&
-            up((void **)&dNK, ctNKids, nc * sizeof(int)) && up((void **)&dCS, ctChgStart, nc * sizeof(int)) &&
-            up((void **)&dNC, ctNChgs, nc * sizeof(int)) && up((void **)&dCI, ctChgIdx, nChar * sizeof(int)) &&
-            up((void **)&dOrd, ord.data(), ord.size() * sizeof(int)) &&
-            up((void **)&dX, ctX, 3 * (size_t)nc * sizeof(double)) && up((void **)&dR, ctERad, nc * sizeof(double)) &&
-            up((void **)&dPos, pos, 3 * (size_t)nChar * sizeof(double)) &&
-            up((void **)&dChr, chr, nChar * sizeof(double)) &&
-            up((void **)&dMom, nullptr, (size_t)nc * nmMax * sizeof(double)) &&
-            up((void **)&dPart, nullptr, (size_t)c->n * sizeof(double)) && up((void **)&dOut, nullptr, sizeof(double)) &&
-            up((void **)&dRed, nullptr, RED_BLOCKS * sizeof(double));
-  int rc = 0;
-  bool tablesFailed = false;
-  if (ok) {
-    // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416)
-    if (lvCount[depth] > 0)
-      hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
-                         lvStart[depth], dCS, dNC, dCI, dX, dPos, dChr, ord[depth], dMom, nmMax, c->T);
-    for (int l = depth - 1; l >= c->height; l--) {
-      if (lvCount[l] <= 0) continue;
-      int nOut = NMOM(ord[l]);
-      hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), 2 * nOut * sizeof(double),
-                         c->stream, lvStart[l], dKid, dNK, dX, dOrd, l, dMom, nmMax, c->T);
-    }
-    // one expansion order over the levels the walk can accept: the wave walk
-    // with per-cluster polynomial tables; otherwise the wave-per-panel walk
-    const int c0 = lvStart[c->height], P = ord[c->height];
-    bool uni = c0 >= 0 && P >= 1 && P <= 9;
-    for (int l = c->height; l <= depth && uni; l++) uni = ord[l] == P;
-    if (uni) {
-      std::vector<int4> dsc;
-      int o = 0;
-      for (int a = P; a >= 0; a--)   // the order enFar reads them in
-        for (int b = P - a; b >= 0; b--)
-          for (int g3 = P - a - b; g3 >= 0; g3--)
-            for (int j = 0; j <= (P - a - b - g3) / 2; j++) dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 0, o++));
-      for (int a = P + 1; a >= 0; a--)
-        for (int b = P + 1 - a; b >= 0; b--)
-          for (int g3 = P + 1 - a - b; g3 >= 0; g3--)
-            for (int j = 0; j <= (P + 1 - a - b - g3) / 2; j++, o += 3)
-              dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 1, o));
-      const int SZ = enPolySize(P), nClus = nc - c0;
-      int4 *dDesc;
-      double *dPoly;
-      if (o != SZ || !up((void **)&dDesc, dsc.data(), dsc.size() * sizeof(int4)) ||
-          !up((void **)&dPoly, nullptr, (size_t)nClus * SZ * sizeof(double))) {
-        setErr("fbh_energy: coefficient tables (%d of %d entries) could not be set up", o, SZ);
-        ok = false;
-        tablesFailed = true;
-      } else {
-        hipLaunchKernelGGL(k_ct_poly, dim3(nClus), dim3(256), 0, c->stream, c0, dMom, nmMax, dDesc, (int)dsc.size(), SZ,
-                           dPoly);
-        const int blocks = ((c->n + 63) / 64 + 3) / 4;
+  const size_t iLevel = add(ctLevel, nc * sizeof(int)), iKid = add(ctKidStart, nc * sizeof(int)),
+               iNK = add(ctNKids, nc * sizeof(int)), iCS = add(ctChgStart, nc * sizeof(int)),
+               iNC = add(ctNChgs, nc * sizeof(int)), iCI = add(ctChgIdx, nChar * sizeof(int)),
+               iOrd = add(ord.data(), ord.size() * sizeof(int)), iX = add(ctX, 3 * (size_t)nc * sizeof(double)),
+               iR = add(ctERad, nc * sizeof(double)), iPos = add(pos, 3 * (size_t)nChar * sizeof(double)),
+               iChr = add(chr, nChar * sizeof(double)), iDesc = add(dsc.data(), dsc.size() * sizeof(int4));
+  const size_t inBytes = top;
+  const size_t iMom = add(nullptr, (size_t)nc * nmMax * sizeof(double)),
+               iPart = add(nullptr, (size_t)c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
+               iRed = add(nullptr, RED_BLOCKS * sizeof(double)),
+               iPoly = add(nullptr, (size_t)nClus * SZ * sizeof(double));
+  if (c->enArenaBytes < top) {
+    if (c->enArena) hipFree(c->enArena);
+    c->enArena = nullptr;
+    c->enArenaBytes = 0;
+    if (hipMalloc(&c->enArena, top) != hipSuccess) { setErr("fbh_energy: device arena of %zu bytes", top); return 0; }
+    c->enArenaBytes = top;
+  }
+  c->enStage.resize(inBytes);
+  for (const Seg &g : seg)
+    if (g.src && g.bytes) memcpy(c->enStage.data() + g.off, g.src, g.bytes);
+  unsigned char *A = (unsigned char *)c->enArena;
+  HCK(hipMemcpyAsync(A, c->enStage.data(), inBytes, hipMemcpyHostToDevice, c->stream));
+  int *dLevel = (int *)(A + seg[iLevel].off), *dKid = (int *)(A + seg[iKid].off), *dNK = (int *)(A + seg[iNK].off),
+      *dCS = (int *)(A + seg[iCS].off), *dNC = (int *)(A + seg[iNC].off), *dCI = (int *)(A + seg[iCI].off),
+      *dOrd = (int *)(A + seg[iOrd].off);
+  double *dX = (double *)(A + seg[iX].off), *dR = (double *)(A + seg[iR].off), *dPos = (double *)(A + seg[iPos].off),
+         *dChr = (double *)(A + seg[iChr].off), *dMom = (double *)(A + seg[iMom].off),
+         *dPart = (double *)(A + seg[iPart].off), *dOut = (double *)(A + seg[iOut].off),
+         *dRed = (double *)(A + seg[iRed].off), *dPoly = (double *)(A + seg[iPoly].off);
+  const int4 *dDesc = (const int4 *)(A + seg[iDesc].off);
+
+  // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416)
+  if (lvCount[depth] > 0)
+    hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
+                       lvStart[depth], dCS, dNC, dCI, dX, dPos, dChr, ord[depth], dMom, nmMax, c->T);
+  for (int l = depth - 1; l >= c->height; l--) {
+    if (lvCount[l] <= 0) continue;
+    int nOut = NMOM(ord[l]);
+    hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), 2 * nOut * sizeof(double),
+                       c->stream, lvStart[l], dKid, dNK, dX, dOrd, l, dMom, nmMax, c->T);
+  }
+  if (uni) {
+    hipLaunchKernelGGL(k_ct_poly, dim3(nClus), dim3(256), 0, c->stream, c0, dMom, nmMax, dDesc, (int)dsc.size(), SZ,
+                       dPoly);
+    const int blocks = ((c->n + 63) / 64 + 3) / 4;
 #define EN_WAVE(PP)                                                                                               \
   case PP:                                                                                                        \
     hipLaunchKernelGGL(k_energy_wave<PP>, dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, \
                        dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta, c->kappa,   \
                        c->eps, c->Q, c->qorder, dPart);                                                           \
     break;
-        switch (P) { EN_WAVE(1) EN_WAVE(2) EN_WAVE(3) EN_WAVE(4) EN_WAVE(5) EN_WAVE(6) EN_WAVE(7) EN_WAVE(8) EN_WAVE(9) }
+    switch (P) { EN_WAVE(1) EN_WAVE(2) EN_WAVE(3) EN_WAVE(4) EN_WAVE(5) EN_WAVE(6) EN_WAVE(7) EN_WAVE(8) EN_WAVE(9) }
 #undef EN_WAVE
-      }
-    } else {
-      int P1 = ord[depth + 1];
-      size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
-      hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, dR, dKid,
-                         dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
-                         c->Q, c->qorder, c->T, dPart);
-    }
+  } else {
+    int P1 = ord[depth + 1];
+    size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
+    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, dR, dKid,
+                       dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
+                       c->Q, c->qorder, c->T, dPart);
   }
-  if (ok) {
-    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, c->n, dRed);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) setErr("energy kernels failed: %s", hipGetErrorString(e));
-    else rc = 1;
-  } else if (!tablesFailed) {
-    setErr("fbh_energy: device allocation/upload failed");
-  }
-  hipStreamSynchronize(c->stream);
-  for (void *p : tmp) hipFree(p);
-  return rc;
+  hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, c->n, dRed);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) { setErr("energy kernels failed: %s", hipGetErrorString(e)); return 0; }
+  return 1;
 }
 
 extern "C" int fbh_profile(fbh_ctx *c, const double *x, double *y, int iters, double *ms) {
