@@ -437,11 +437,20 @@ static void leafMoments(const fbo_layout *L, int leaf, int order, int job, doubl
 }
 
 /* ----------------------------------------------------------- context */
+typedef struct {
+  int start, m;              /* first panel (idx order) and panel count of the cube */
+  double *a;                 /* 2m x 2m, LU factors in LAPACK's column-major view */
+  int *piv;
+} pblock;
+
 struct fbo_ctx {
   fbo_layout L;
   int nmMax;                 /* nMom(maxOrder) */
   double **q2m0, **q2m1;     /* per leaf, nMom(ordMom[depth]) x count, column per panel */
   double *mom, *lec;         /* per FMM cube: 2 / 4 arrays of nmMax */
+  int precMode;              /* 3 diagonal, 2 block LU */
+  int nPB;
+  pblock *pb;
 };
 
 fbo_ctx *fbo_create(const fbo_layout *L) {
@@ -462,11 +471,15 @@ fbo_ctx *fbo_create(const fbo_layout *L) {
   }
   c->mom = calloc((size_t)L->nCube * 2 * c->nmMax, sizeof(double));
   c->lec = calloc((size_t)L->nCube * 4 * c->nmMax, sizeof(double));
+  c->precMode = 3;
   return c;
 }
 
+static void block_free(fbo_ctx *c);
+
 void fbo_destroy(fbo_ctx *c) {
   if (!c) return;
+  block_free(c);
   for (int l = 0; l < c->L.nLeaf; l++) { free(c->q2m0[l]); free(c->q2m1[l]); }
   free(c->q2m0); free(c->q2m1); free(c->mom); free(c->lec); free(c);
 }
@@ -691,6 +704,111 @@ void fbo_psolve_diag(fbo_ctx *c, double *out, const double *in) {
   }
 }
 
+/* ----------------------------------------------- block preconditioner */
+/* src/precond_fmm.c:553-652 (setupPreconditioning), :147-268 (block assembly),
+ * :861-953 (PtVfmmCachedLU).  One dense block per cube at level depth-1 over
+ * its m panels, unknowns [sigma(m) | sigma_n(m)].  The reference fills the
+ * array B row-major, B[i][j] from KER = panelIA0(X = panel j, Y = panel i):
+ *   B[i][j] = -KER1, B[i][j+m] = -KER0, B[i+m][j] = -KER3, B[i+m][j+m] = -KER2,
+ *   B[i][i] += s1 area_i, B[i+m][i+m] += s2 area_i,
+ * and hands it to dgetrf/dgetrs ('N'), which read it column-major: the
+ * operator solved is A(r, c) = B[c][r].  LU with partial pivoting as dgetf2
+ * (first largest pivot, row swaps over the whole row). */
+static void block_free(fbo_ctx *c) {
+  for (int b = 0; b < c->nPB; b++) { free(c->pb[b].a); free(c->pb[b].piv); }
+  free(c->pb);
+  c->pb = NULL;
+  c->nPB = 0;
+}
+
+int fbo_set_precond(fbo_ctx *c, int mode) {
+  const fbo_layout *L = &c->L;
+  block_free(c);
+  if (mode == 3) { c->precMode = 3; return 0; }
+  if (mode < 0 || mode > 2) return -1;
+  const int lev = L->depth - 1;
+  if (lev < L->height) return -1;
+  const double s1 = (1.0 + L->eps) / 2.0, s2 = (1.0 + 1.0 / L->eps) / 2.0;
+  c->nPB = L->levCount[lev];
+  c->pb = calloc((size_t)c->nPB, sizeof(pblock));
+  for (int b = 0; b < c->nPB; b++) {
+    const int cb = L->levStart[lev] + b, k0 = L->kidStart[cb];
+    pblock *P = &c->pb[b];
+    P->start = L->leafStart[k0];
+    P->m = 0;
+    for (int k = 0; k < L->nKids[cb]; k++) P->m += L->leafCount[k0 + k];
+    const int m = P->m, M = 2 * m;
+    P->a = calloc((size_t)M * (size_t)M, sizeof(double));
+    P->piv = calloc((size_t)M, sizeof(int));
+    double *B = P->a, ker[4];
+    for (int i = 0; i < m; i++) {
+      for (int j = 0; j < m; j++) {
+        fbo_panel_ia0(L, P->start + j, P->start + i, ker);
+        B[(size_t)i * M + j] = -ker[1];
+        B[(size_t)i * M + j + m] = -ker[0];
+        B[(size_t)(i + m) * M + j] = -ker[3];
+        B[(size_t)(i + m) * M + j + m] = -ker[2];
+      }
+      B[(size_t)i * M + i] += s1 * L->area[P->start + i];
+      B[(size_t)(i + m) * M + i + m] += s2 * L->area[P->start + i];
+    }
+    /* LU of the column-major view A(r, c) = B[c*M + r] */
+#define A_(r, cc) B[(size_t)(cc) * M + (r)]
+    for (int j = 0; j < M; j++) {
+      int p = j;
+      double best = fabs(A_(j, j));
+      for (int r = j + 1; r < M; r++)
+        if (fabs(A_(r, j)) > best) { best = fabs(A_(r, j)); p = r; }
+      P->piv[j] = p;
+      if (A_(p, j) == 0.0) return -2;
+      if (p != j)
+        for (int cc = 0; cc < M; cc++) { double t = A_(j, cc); A_(j, cc) = A_(p, cc); A_(p, cc) = t; }
+      const double inv = 1.0 / A_(j, j);
+      for (int r = j + 1; r < M; r++) A_(r, j) *= inv;
+      for (int cc = j + 1; cc < M; cc++) {
+        const double u = A_(j, cc);
+        if (u != 0.0)
+          for (int r = j + 1; r < M; r++) A_(r, cc) -= A_(r, j) * u;
+      }
+    }
+#undef A_
+  }
+  c->precMode = 2;
+  return 0;
+}
+
+/* the block solves of PtVfmmCachedLU; in-place safe */
+static void psolve_block(fbo_ctx *c, double *out, const double *in) {
+  const int n = c->L.n;
+  for (int b = 0; b < c->nPB; b++) {
+    const pblock *P = &c->pb[b];
+    const int m = P->m, M = 2 * m;
+    const double *B = P->a;
+    double *x = malloc(sizeof(double) * (size_t)M);
+    for (int i = 0; i < m; i++) { x[i] = in[P->start + i]; x[i + m] = in[n + P->start + i]; }
+#define A_(r, cc) B[(size_t)(cc) * M + (r)]
+    for (int j = 0; j < M; j++)
+      if (P->piv[j] != j) { double t = x[j]; x[j] = x[P->piv[j]]; x[P->piv[j]] = t; }
+    for (int j = 0; j < M; j++)
+      if (x[j] != 0.0)
+        for (int r = j + 1; r < M; r++) x[r] -= x[j] * A_(r, j);
+    for (int j = M - 1; j >= 0; j--) {
+      if (x[j] != 0.0) {
+        x[j] /= A_(j, j);
+        for (int r = 0; r < j; r++) x[r] -= x[j] * A_(r, j);
+      }
+    }
+#undef A_
+    for (int i = 0; i < m; i++) { out[P->start + i] = x[i]; out[n + P->start + i] = x[i + m]; }
+    free(x);
+  }
+}
+
+void fbo_psolve(fbo_ctx *c, double *out, const double *in) {
+  if (c->precMode == 2) psolve_block(c, out, in);
+  else fbo_psolve_diag(c, out, in);
+}
+
 /* ------------------------------------------------------------- GMRES */
 static double dot(int n, const double *a, const double *b) {
   double s = 0.0;
@@ -725,7 +843,7 @@ int fbo_gmres(fbo_ctx *c, const double *b, double *x, int restrt, double tol, in
   *iters = 0;
   memcpy(w, b, sizeof(double) * n);
   if (nrm2(n, x) != 0.0) fbo_matvec(c, -1.0, x, 1.0, w);
-  fbo_psolve_diag(c, r, w);
+  fbo_psolve(c, r, w);
   double bnrm2 = nrm2(n, b);
   if (bnrm2 == 0.0) bnrm2 = 1.0;
   *resid = nrm2(n, r) / bnrm2;
@@ -740,7 +858,7 @@ int fbo_gmres(fbo_ctx *c, const double *b, double *x, int restrt, double tol, in
     for (;;) {
       ++i; ++*iters;
       fbo_matvec(c, 1.0, &V[(size_t)(i - 1) * n], 0.0, w);
-      fbo_psolve_diag(c, w, w);
+      fbo_psolve(c, w, w);
       double *h = &H[(size_t)(i - 1) * ldh];
       for (k = 0; k < i; k++) {          /* modified Gram-Schmidt */
         h[k] = dot(n, w, &V[(size_t)k * n]);
@@ -771,7 +889,7 @@ int fbo_gmres(fbo_ctx *c, const double *b, double *x, int restrt, double tol, in
     if (*resid <= tol) goto out;
     memcpy(w, b, sizeof(double) * n);
     fbo_matvec(c, -1.0, x, 1.0, w);
-    fbo_psolve_diag(c, r, w);
+    fbo_psolve(c, r, w);
     *resid = nrm2(n, r) / bnrm2;
     if (hist) hist[nh++] = *resid;
     if (*resid <= tol) goto out;

@@ -1,7 +1,8 @@
 /*
  * fbo.h - the CPU oracle: a plain-C restatement of the reference FABIPB
  * Galerkin BIE matvec and its neighbours (kernels, Galerkin quadrature,
- * Taylor moments and translations, FMM apply, diagonal preconditioner,
+ * Taylor moments and translations, FMM apply, diagonal and block-LU
+ * preconditioners,
  * restarted GMRES, direct right-hand side, panel-tree solvation energy).
  *
  * TEST INFRASTRUCTURE ONLY.  It is the checker the HIP path is compared with;
@@ -79,6 +80,11 @@ void fbo_expansions(fbo_ctx *c, double *mom, double *lec);
 /* y = alpha*((D - K) x) + beta*y, D = diag((1+eps)/2*area, (1+1/eps)/2*area) */
 void fbo_matvec(fbo_ctx *c, double alpha, const double *x, double beta, double *y);
 void fbo_psolve_diag(fbo_ctx *c, double *out, const double *in);
+/* preconditioner for fbo_psolve / fbo_gmres: 3 diagonal (default); 0, 1, 2 the
+ * leaf-parent block LU (src/precond_fmm.c: the three modes solve the same
+ * blocks).  Returns 0, -1 if the layout has no level depth-1, -2 on a zero pivot. */
+int fbo_set_precond(fbo_ctx *c, int mode);
+void fbo_psolve(fbo_ctx *c, double *out, const double *in);
 /* restarted left-preconditioned GMRES; returns info; hist (maxit+1) optional */
 int fbo_gmres(fbo_ctx *c, const double *b, double *x, int restart, double tol, int maxit,
               int *iters, double *resid, double *hist);

@@ -63,6 +63,9 @@ def lib():
         L.fbo_apply.argtypes = [C.c_void_p, C.c_double, _dp, C.c_double, _dp, C.c_int]
         L.fbo_matvec.argtypes = [C.c_void_p, C.c_double, _dp, C.c_double, _dp]
         L.fbo_psolve_diag.argtypes = [C.c_void_p, _dp, _dp]
+        L.fbo_psolve.argtypes = [C.c_void_p, _dp, _dp]
+        L.fbo_set_precond.restype = C.c_int
+        L.fbo_set_precond.argtypes = [C.c_void_p, C.c_int]
         L.fbo_expansions.argtypes = [C.c_void_p, _dp, _dp]
         L.fbo_gmres.restype = C.c_int
         L.fbo_gmres.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
@@ -148,6 +151,19 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(2 * self.n) if y is None else np.array(y, dtype=np.float64)
         lib().fbo_matvec(self.ctx, alpha, dptr(x), beta, dptr(y))
+        return y
+
+    def set_precond(self, mode):
+        """3: diagonal; 0, 1, 2: the leaf-parent block LU (src/precond_fmm.c)."""
+        rc = lib().fbo_set_precond(self.ctx, mode)
+        if rc != 0:
+            raise RuntimeError(f"fbo_set_precond({mode}) = {rc}")
+
+    def psolve_any(self, x):
+        """The preconditioner chosen by set_precond."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        lib().fbo_psolve(self.ctx, dptr(y), dptr(x))
         return y
 
     def psolve(self, x):
