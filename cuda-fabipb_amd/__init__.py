@@ -99,6 +99,10 @@ def hip_lib():
         L.fbh_gmres_dev.restype = C.c_int
         L.fbh_gmres_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
                                     _ip, _dp, _dp, _ip]
+        L.fbh_set_precond.restype = C.c_int
+        L.fbh_set_precond.argtypes = [_vp, C.c_int]
+        L.fbh_psolve_dev.restype = C.c_int
+        L.fbh_psolve_dev.argtypes = [_vp, _vp, _vp]
         L.fbh_enable_fp32.restype = C.c_int
         L.fbh_enable_fp32.argtypes = [_vp]
         L.fbh_matvec32_dev.restype = C.c_int
@@ -316,6 +320,14 @@ class Operator:
         self._ck(hip_lib().fbh_matvec_dev(self.h, alpha, C.c_void_p(x.data_ptr()), beta,
                                           C.c_void_p(y.data_ptr())))
 
+    def set_precond(self, mode):
+        """GMRES preconditioner (src/fabipb.c:1289-1309): 3 diagonal; 0, 1, 2 the
+        leaf-parent block LU (src/precond_fmm.c), built as explicit block inverses."""
+        self._ck(hip_lib().fbh_set_precond(self.h, mode))
+
+    def psolve_dev(self, x, y):
+        self._ck(hip_lib().fbh_psolve_dev(self.h, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr())))
+
     def enable_fp32(self):
         """FP32 copies of the operator for the precision form (fbh_enable_fp32)."""
         self._ck(hip_lib().fbh_enable_fp32(self.h))
@@ -376,17 +388,20 @@ class Operator:
 
 
 def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,
-          restart=30, maxit=100, height=2, x0="rhs", precision="fp64", op=None, tree=None, mesh=None):
+          restart=30, maxit=100, height=2, x0="rhs", precision="fp64", op=None, tree=None, mesh=None,
+          precond=3):
     """One solve (src/fabipb.c:1067-1210): returns energies, iterations and times.
     precision "fp64" is the plain form (GMRES on the FP64 operator); "fp32" is the
     precision form (refinement around GMRES on the FP32 operator, FP64 residuals).
-    An existing mesh/tree/operator may be passed in to time the solve alone."""
+    precond is the -P mode (3 diagonal; 0, 1, 2 block LU).  An existing
+    mesh/tree/operator may be passed in to time the solve alone."""
     torch = _torch()
     t0 = time.perf_counter()
     mesh = mesh if mesh is not None else Mesh(base)
     tree = tree if tree is not None else Tree(mesh, depth, order=order, height=height, sep=sep)
     t1 = time.perf_counter()
     op = op if op is not None else Operator(tree, kappa, eps1, eps2, qorder)
+    op.set_precond(precond)
     if precision == "fp32":
         op.enable_fp32()
     elif precision != "fp64":

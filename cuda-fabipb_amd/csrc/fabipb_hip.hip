@@ -35,6 +35,8 @@
 //   GMRES helpers: fixed-grid deterministic dot products, MGS update,
 //                normalisation, diagonal preconditioner, x += V y
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -341,6 +343,100 @@ __global__ __launch_bounds__(256) void k_near_build(const Pnl *__restrict__ pnl,
     double k[4];
     panelIA0(X, Y, x == y, kappa, eps, Q, q, k);
     out[j] = make_double4(k[0], k[1], k[2], k[3]);
+  }
+}
+
+// ------------------------------------------------- block preconditioner
+// Leaf-parent blocks (src/precond_fmm.c:147-268, 553-652): block b covers the
+// m panels [start, start+m) of one cube at level depth-1, unknowns
+// [sigma | sigma_n].  The reference fills B[i][j] from panelIA0(X = j, Y = i)
+// and LAPACK factors B read column-major, i.e. the operator
+//   M(r, c) = B[c][r]:  M(r, c) = -KER1(r, c), M(r, c+m) = -KER3(r, c),
+//   M(r+m, c) = -KER0(r, c), M(r+m, c+m) = -KER2(r, c), KER = panelIA0(X = r, Y = c),
+//   plus s1 area_r, s2 area_r on the diagonal.
+// k_bp_assemble writes M row-major into a group buffer padded to Mp with the
+// identity; the batched LU and inverse read that column-major (as M^T), so the
+// inverse they leave, read row-major, is M^-1.
+__global__ __launch_bounds__(256) void k_bp_assemble(const Pnl *__restrict__ pnl, const int *__restrict__ grpBlock,
+                                                     const int *__restrict__ bStart, const int *__restrict__ bM,
+                                                     int Mp, double *__restrict__ T, double kappa, double eps,
+                                                     double s1, double s2, QTab Q, int q) {
+  const int g = blockIdx.x, r = blockIdx.y, b = grpBlock[g];
+  const int m = bM[b], start = bStart[b];
+  double *A = T + (long long)g * Mp * Mp;
+  if (r >= m) {   // padding rows (r >= 2m is reached through r - m below)
+    return;
+  }
+  const Pnl X = pnl[start + r];
+  for (int c = threadIdx.x; c < m; c += blockDim.x) {
+    const Pnl Y = pnl[start + c];
+    double k[4];
+    panelIA0(X, Y, r == c, kappa, eps, Q, q, k);
+    A[(long long)r * Mp + c] = -k[1] + (r == c ? s1 * X.area : 0.0);
+    A[(long long)r * Mp + c + m] = -k[3];
+    A[(long long)(r + m) * Mp + c] = -k[0];
+    A[(long long)(r + m) * Mp + c + m] = -k[2] + (r == c ? s2 * X.area : 0.0);
+  }
+}
+
+// identity on the padding of every block of a group (rows and columns >= 2m)
+__global__ void k_bp_pad(const int *__restrict__ grpBlock, const int *__restrict__ bM, int Mp, double *__restrict__ T) {
+  const int g = blockIdx.x, M = 2 * bM[grpBlock[g]];
+  double *A = T + (long long)g * Mp * Mp;
+  for (int r = M + threadIdx.x; r < Mp; r += blockDim.x) A[(long long)r * Mp + r] = 1.0;
+}
+
+// the M x M inverse out of the padded group buffer into the packed store
+__global__ void k_bp_unpad(const int *__restrict__ grpBlock, const int *__restrict__ bM,
+                           const long long *__restrict__ bOff, int Mp, const double *__restrict__ T,
+                           double *__restrict__ inv) {
+  const int g = blockIdx.x, b = grpBlock[g], M = 2 * bM[b];
+  const double *A = T + (long long)g * Mp * Mp;
+  double *o = inv + bOff[b];
+  for (int r = blockIdx.y; r < M; r += gridDim.y)
+    for (int c = threadIdx.x; c < M; c += blockDim.x) o[(long long)r * M + c] = A[(long long)r * Mp + c];
+}
+
+// y = M^-1 x per block, row-major inverse streamed once: tile = (block, first
+// row), 4 waves x 4 rows, the block's x gathered into LDS
+__global__ __launch_bounds__(256) void k_bp_apply(const int2 *__restrict__ tiles, const int *__restrict__ bStart,
+                                                  const int *__restrict__ bM, const long long *__restrict__ bOff,
+                                                  const double *__restrict__ inv, const double *__restrict__ x,
+                                                  double *__restrict__ y, int n) {
+  __shared__ double sx[NEAR_CHUNK];
+  const int2 t = tiles[blockIdx.x];
+  const int b = t.x, r0 = t.y, m = bM[b], M = 2 * m, start = bStart[b];
+  const int rows = min(16, M - r0);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[4] = {0, 0, 0, 0};
+  for (int c0 = 0; c0 < M; c0 += NEAR_CHUNK) {
+    const int cn = min(NEAR_CHUNK, M - c0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < cn; j += 256) {
+      const int cc = c0 + j;
+      sx[j] = cc < m ? x[start + cc] : x[n + start + cc - m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int row = w + 4 * rr;
+      if (row < rows) {
+        const double *a = inv + bOff[b] + (long long)(r0 + row) * M + c0;
+        double s = 0.0;
+        for (int j = lane; j < cn; j += 64) s += a[j] * sx[j];
+        acc[rr] += s;
+      }
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < 4; rr++) {
+    const int row = w + 4 * rr;
+    double v = acc[rr];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (row < rows && lane == 0) {
+      const int r = r0 + row;
+      y[r < m ? start + r : n + start + r - m] = v;
+    }
   }
 }
 
@@ -1562,6 +1658,16 @@ struct fbh_ctx {
   void *enArena = nullptr;                // energy: charge tree, moments and scratch
   size_t enArenaBytes = 0;
   std::vector<unsigned char> enStage;
+  // preconditioner: 3 diagonal, 2 leaf-parent block LU (explicit inverses)
+  int precMode = 3;
+  std::vector<int> hKidStart, hNKids, hLeafStart, hLeafCount;
+  int nPB = 0, nPBTiles = 0;
+  int *pbStart = nullptr, *pbM = nullptr;
+  long long *pbOff = nullptr;
+  int2 *pbTiles = nullptr;
+  double *pbInv = nullptr, *pbTmp = nullptr;
+  size_t pbBytes = 0;
+  rocblas_handle blas = nullptr;
   // nearfield on a second stream, overlapped with the far-field chain
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
@@ -1622,6 +1728,10 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   if (c->scal) hipFree(c->scal);
   if (c->ro) hipFree(c->ro);
   if (c->enArena) hipFree(c->enArena);
+  for (void *q : {(void *)c->pbStart, (void *)c->pbM, (void *)c->pbOff, (void *)c->pbTiles, (void *)c->pbInv,
+                  (void *)c->pbTmp})
+    if (q) hipFree(q);
+  if (c->blas) rocblas_destroy_handle(c->blas);
   if (c->dx) hipFree(c->dx);
   if (c->side) {
     hipStreamSynchronize(c->side);
@@ -1874,6 +1984,10 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   c->ordMom.assign(ordMom, ordMom + depth + 1);
   c->ordM2L.assign(ordM2L, ordM2L + depth + 1);
   c->cubeLevel.assign(cubeLevel, cubeLevel + nCube);
+  c->hKidStart.assign(kidStart, kidStart + nCube);
+  c->hNKids.assign(nKids, nKids + nCube);
+  c->hLeafStart.assign(leafStart, leafStart + nLeaf);
+  c->hLeafCount.assign(leafCount, leafCount + nLeaf);
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {
@@ -2215,6 +2329,150 @@ extern "C" int fbh_psolve_diag_dev(fbh_ctx *c, double *out, const double *in) {
   return 1;
 }
 
+static void freePB(fbh_ctx *c) {
+  for (void **q : {(void **)&c->pbStart, (void **)&c->pbM, (void **)&c->pbOff, (void **)&c->pbTiles,
+                   (void **)&c->pbInv, (void **)&c->pbTmp}) {
+    if (*q) hipFree(*q);
+    *q = nullptr;
+  }
+  c->nPB = c->nPBTiles = 0;
+  c->pbBytes = 0;
+}
+
+// the preconditioner of GMRES (PtVmain, src/fabipb.c:1289-1309): 3 is the
+// diagonal; 0, 1 and 2 are the reference's three ways to apply the same
+// leaf-parent block LU (recomputed, cached blocks, cached factors), here built
+// once as explicit block inverses: batched LU and inverse per group of blocks
+// padded to the same size (rocSOLVER getrf/getri), then one HBM-bound pass
+// over the packed inverses per application
+extern "C" int fbh_set_precond(fbh_ctx *c, int mode) {
+  if (!c) { setErr("fbh_set_precond: null context"); return 0; }
+  if (mode == 3 || mode == -1) { freePB(c); c->precMode = mode; return 1; }
+  if (mode < -1 || mode > 3) { setErr("fbh_set_precond: mode %d not in -1..3", mode); return 0; }
+  if (c->precMode == 2 && c->pbInv) return 1;
+  const int lev = c->depth - 1;
+  if (lev < c->height) {
+    setErr("fbh_set_precond: the block preconditioner needs a level depth-1 >= height (depth %d, height %d)",
+           c->depth, c->height);
+    return 0;
+  }
+  freePB(c);
+  const int nb = c->levCount[lev], lv0 = c->levStart[lev];
+  std::vector<int> st(nb), mm(nb);
+  std::vector<long long> off(nb + 1, 0);
+  std::vector<int2> tiles;
+  for (int b = 0; b < nb; b++) {
+    const int cb = lv0 + b, k0 = c->hKidStart[cb];
+    st[b] = c->hLeafStart[k0];
+    mm[b] = 0;
+    for (int k = 0; k < c->hNKids[cb]; k++) mm[b] += c->hLeafCount[k0 + k];
+    const int M = 2 * mm[b];
+    off[b + 1] = off[b] + (long long)M * M;
+    for (int r = 0; r < M; r += 16) tiles.push_back(make_int2(b, r));
+  }
+  // groups of blocks padded to the same size (multiples of 64)
+  std::vector<std::vector<int>> groups;
+  std::vector<int> gMp;
+  {
+    std::vector<int> order(nb);
+    for (int b = 0; b < nb; b++) order[b] = b;
+    auto pad = [&](int b) { return ((2 * mm[b] + 63) / 64) * 64; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pad(x) < pad(y); });
+    for (int b : order) {
+      if (gMp.empty() || gMp.back() != pad(b)) { gMp.push_back(pad(b)); groups.emplace_back(); }
+      groups.back().push_back(b);
+    }
+  }
+  size_t tmpMax = 0;
+  int cntMax = 0;
+  for (size_t g = 0; g < groups.size(); g++) {
+    tmpMax = std::max(tmpMax, groups[g].size() * (size_t)gMp[g] * gMp[g]);
+    cntMax = std::max(cntMax, (int)groups[g].size());
+  }
+  double *T = nullptr;
+  int *ipiv = nullptr, *info = nullptr, *grp = nullptr;
+  const size_t n2 = 2 * (size_t)c->n;
+  auto fail = [&](const char *what) {
+    setErr("fbh_set_precond: %s", what);
+    if (T) hipFree(T);
+    if (ipiv) hipFree(ipiv);
+    if (info) hipFree(info);
+    if (grp) hipFree(grp);
+    freePB(c);
+    return 0;
+  };
+  if (hipMalloc(&c->pbStart, nb * sizeof(int)) != hipSuccess || hipMalloc(&c->pbM, nb * sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->pbOff, (nb + 1) * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->pbTiles, tiles.size() * sizeof(int2)) != hipSuccess ||
+      hipMalloc(&c->pbInv, std::max<long long>(off[nb], 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->pbTmp, n2 * sizeof(double)) != hipSuccess || hipMalloc(&T, tmpMax * sizeof(double)) != hipSuccess ||
+      hipMalloc(&ipiv, (size_t)cntMax * gMp.back() * sizeof(int)) != hipSuccess ||
+      hipMalloc(&info, cntMax * sizeof(int)) != hipSuccess || hipMalloc(&grp, nb * sizeof(int)) != hipSuccess)
+    return fail("device allocation failed");
+  c->pbBytes = off[nb] * sizeof(double);
+  if (hipMemcpy(c->pbStart, st.data(), nb * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->pbM, mm.data(), nb * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->pbOff, off.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->pbTiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+    return fail("upload failed");
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) return fail("rocblas handle");
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) return fail("rocblas stream");
+  const double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
+  std::vector<int> hinfo(cntMax);
+  for (size_t g = 0; g < groups.size(); g++) {
+    const int cnt = (int)groups[g].size(), Mp = gMp[g];
+    int mMax = 0;
+    for (int b : groups[g]) mMax = std::max(mMax, mm[b]);
+    if (hipMemcpyAsync(grp, groups[g].data(), cnt * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemsetAsync(T, 0, (size_t)cnt * Mp * Mp * sizeof(double), c->stream) != hipSuccess)
+      return fail("group staging failed");
+    hipLaunchKernelGGL(k_bp_pad, dim3(cnt), dim3(64), 0, c->stream, grp, c->pbM, Mp, T);
+    hipLaunchKernelGGL(k_bp_assemble, dim3(cnt, mMax), dim3(256), 0, c->stream, c->pnl, grp, c->pbStart, c->pbM, Mp,
+                       T, c->kappa, c->eps, s1, s2, c->Q, c->qorder);
+    if (rocsolver_dgetrf_strided_batched(c->blas, Mp, Mp, T, Mp, (rocblas_stride)Mp * Mp, ipiv, Mp, info, cnt) !=
+            rocblas_status_success ||
+        rocsolver_dgetri_strided_batched(c->blas, Mp, T, Mp, (rocblas_stride)Mp * Mp, ipiv, Mp, info, cnt) !=
+            rocblas_status_success)
+      return fail("batched LU/inverse failed");
+    if (hipMemcpyAsync(hinfo.data(), info, cnt * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return fail("LU status read failed");
+    for (int k = 0; k < cnt; k++)
+      if (hinfo[k] != 0) {
+        setErr("fbh_set_precond: block %d is singular (info %d)", groups[g][k], hinfo[k]);
+        return fail("singular block");
+      }
+    hipLaunchKernelGGL(k_bp_unpad, dim3(cnt, 64), dim3(256), 0, c->stream, grp, c->pbM, c->pbOff, Mp, T, c->pbInv);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(T);
+  hipFree(ipiv);
+  hipFree(info);
+  hipFree(grp);
+  T = nullptr; ipiv = info = grp = nullptr;
+  if (e != hipSuccess) return fail(hipGetErrorString(e));
+  c->nPB = nb;
+  c->nPBTiles = (int)tiles.size();
+  c->precMode = 2;
+  return 1;
+}
+
+// the current preconditioner; out may equal in
+extern "C" int fbh_psolve_dev(fbh_ctx *c, double *out, const double *in) {
+  if (!c || !out || !in) { setErr("fbh_psolve_dev: null argument"); return 0; }
+  if (c->precMode == -1) {   // PtVmain with the preconditioner disabled: a copy
+    if (out != in) HCK(hipMemcpyAsync(out, in, 2 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return 1;
+  }
+  if (c->precMode != 2) return fbh_psolve_diag_dev(c, out, in);
+  HCK(hipMemcpyAsync(c->pbTmp, in, 2 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_bp_apply, dim3(c->nPBTiles), dim3(256), 0, c->stream, c->pbTiles, c->pbStart, c->pbM, c->pbOff,
+                     c->pbInv, c->pbTmp, out, c->n);
+  HCK(hipGetLastError());
+  return 1;
+}
+
 static int ensureGmres(fbh_ctx *c, int restart) {
   if (c->V && c->gmRestart >= restart) return 1;
   if (c->V) { hipFree(c->V); c->V = nullptr; }
@@ -2276,7 +2534,7 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
   HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
   if (!devNorm(c, x, &xn)) return 0;
   if (xn != 0.0 && !applyOp(c, fp32, -1.0, x, 1.0, c->w)) return 0;
-  if (!fbh_psolve_diag_dev(c, c->r, c->w)) return 0;
+  if (!fbh_psolve_dev(c, c->r, c->w)) return 0;
   if (!devNorm(c, b, &bnrm2)) return 0;
   if (bnrm2 == 0.0) bnrm2 = 1.0;
   if (!devNorm(c, c->r, &rn)) return 0;
@@ -2298,7 +2556,7 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
       ++*iters;
       double *vi = c->V + (size_t)(i - 1) * n2, *vn = c->V + (size_t)i * n2;
       if (!applyOp(c, fp32, 1.0, vi, 0.0, c->w)) return 0;
-      if (!fbh_psolve_diag_dev(c, c->w, c->w)) return 0;
+      if (!fbh_psolve_dev(c, c->w, c->w)) return 0;
       for (int k = 0; k < i; k++) {   // modified Gram-Schmidt
         const double *vk = c->V + (size_t)k * n2;
         hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, vk, n2, c->part);
@@ -2330,7 +2588,7 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
     if (*resid <= tol) return 1;
     HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
     if (!applyOp(c, fp32, -1.0, x, 1.0, c->w)) return 0;
-    if (!fbh_psolve_diag_dev(c, c->r, c->w)) return 0;
+    if (!fbh_psolve_dev(c, c->r, c->w)) return 0;
     if (!devNorm(c, c->r, &rn)) return 0;
     *resid = rn / bnrm2;
     if (hist) hist[nh++] = *resid;
@@ -2396,7 +2654,7 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
     HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
     if (!applyOp(c, false, -1.0, x, 1.0, c->ro)) return 0;
     ++*mv64;
-    if (!fbh_psolve_diag_dev(c, c->w, c->ro)) return 0;
+    if (!fbh_psolve_dev(c, c->w, c->ro)) return 0;
     double rn;
     if (!devNorm(c, c->w, &rn)) return 0;
     *resid = rn / bnrm2;

@@ -6,13 +6,13 @@
  *
  * Sequence (as src/fabipb.c): read <base>.pqr/.vert/.face, build the octree
  * and interaction lists on the host, build the device operator, set up the
- * right-hand side, run GMRES with the diagonal preconditioner, then the
- * panel-tree solvation energy.  Every numeric stage runs on the GPU through
- * include/fabipb_hip.h; the CPU path of the reference (-g=0) is not part of
- * this build, nor are meshing (MSMS/NanoShaper), the dense direct operators
- * (-r=1/2) and the block preconditioners (-P=0/1/2).  Those options are
- * parsed and validated exactly as the reference does, then refused with a
- * message, so scripts fail loudly rather than run something else.
+ * right-hand side, run GMRES with the -P preconditioner (diagonal, block LU
+ * or none), then the panel-tree solvation energy.  Every numeric stage runs on
+ * the GPU through include/fabipb_hip.h; the CPU path of the reference (-g=0)
+ * is not part of this build, nor are meshing (MSMS/NanoShaper) and the dense
+ * direct operators (-r=1/2).  Those options are parsed and validated exactly
+ * as the reference does, then refused with a message, so scripts fail loudly
+ * rather than run something else.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -73,7 +73,7 @@ static void usage(const char *prog) {
   printf("  -S=<val>   separation ratio    -q=<ord> panel quadrature order (1..10)\n");
   printf("  -k=<val>   Debye kappa         -eps1=<val> -eps2=<val> dielectrics (-e1/-e2)\n");
   printf("  -o=<val>   GMRES tolerance     -a=<n> restart   -i=<n> max iterations\n");
-  printf("  -P=-1|3    none or diagonal preconditioner\n");
+  printf("  -P=-1..3   preconditioner: -1 none, 0/1/2 leaf-parent block LU, 3 diagonal (default auto)\n");
   printf("Environment: FABIPB_GMRES_INITIAL=rhs|zero FABIPB_GMRES_LOG_RESID=1 FABIPB_WRITE_SOLUTION=<csv>\n");
   printf("             FABIPB_ENERGY_MODE=panel-tree|compare FABIPB_STOP_AFTER_RHS FABIPB_STOP_AFTER_GMRES\n");
 }
@@ -148,15 +148,15 @@ int main(int argc, char **argv) {
     printf("Mesh filtered panels: kept=%d area=%f\n", mesh->n, s);
   }
   if (meshOnly) { printf("Mesh-only mode: stopping after mesh generation.\n"); return 0; }
-  /* preconditioner policy (src/fabipb_options.c:198-265): the block-LU mode
-   * the auto policy picks at high contrast is not available yet */
+  /* preconditioner policy (src/fabipb_options.c:85-88, 198-265): capsid-scale
+   * charge counts take the diagonal, high dielectric contrast the block LU */
   if (!precondExplicit) {
-    int hi = eps1 <= 2.0 && eps2 / eps1 >= 40.0 && mesh->nChar <= 5000000;
-    if (hi) printf("Resolved preconditioner mode=2 (auto: high dielectric contrast) is not available in this build; using 3 (diagonal)\n");
-    precond = 3;
-  } else if (precond >= 0 && precond <= 2) {
-    fprintf(stderr, "Error: -P=%d (block preconditioner) is not part of this build; use -P=3 or -P=-1\n", precond);
-    return 1;
+    const char *ev = getenv("FABIPB_HUGE_CAPSID_ATOMS");
+    if (!ev || !*ev) ev = getenv("FABIPB_Q2M_HUGE_CAPSID_ATOMS");
+    unsigned long long huge = (ev && *ev) ? strtoull(ev, NULL, 10) : 5000000ULL;
+    int hugeCapsid = (unsigned long long)mesh->nChar > huge;
+    int hi = eps1 > 0.0 && eps1 <= 2.0 && eps2 / eps1 >= 40.0;
+    precond = hugeCapsid ? 3 : hi ? 2 : 3;
   }
   double tLoad = now() - tStage;
   if (bench) {
@@ -165,7 +165,10 @@ int main(int argc, char **argv) {
     printf("GMRES variables: tol=%1.e arnoldiSz=%d maxIt=%d\n", tol, arnoldiSz, numItr);
     printf("kappa=%f, eps1=%f, eps2=%f\n", kappa, eps1, eps2);
     printf("GPU mode=1 (0=CPU, 1=GPU)\n");
-    printf("Preconditioner mode=%d (-1=disabled, 3=diagonal)\n", precond);
+    if (precondExplicit)
+      printf("Preconditioner mode=%d (-1=disabled, 0=original, 1=cached-blocks, 2=cached-LU, 3=diagonal)\n", precond);
+    else
+      printf("Preconditioner mode=auto (-1=disabled, 0=original, 1=cached-blocks, 2=cached-LU, 3=diagonal)\n");
   }
 
   tStage = now();
@@ -216,7 +219,7 @@ int main(int argc, char **argv) {
   int iters = 0, info = 0;
   double resid = 0.0, *hist = calloc((size_t)2 * numItr + 2, sizeof(double));
   tStage = now();
-  if (precond != 3) { fprintf(stderr, "Error: only the diagonal preconditioner runs on the device here\n"); return 1; }
+  if (!fbh_set_precond(ctx, precond)) die("preconditioner setup");
   if (!fbh_gmres_dev(ctx, b, x, arnoldiSz, tol, numItr, &iters, &resid, hist, &info)) die("GMRES");
   double tGmres = now() - tStage;
   if (getenv("FABIPB_GMRES_LOG_RESID") && atoi(getenv("FABIPB_GMRES_LOG_RESID")) > 0)

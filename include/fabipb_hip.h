@@ -107,8 +107,23 @@ int fbh_near_coeff(fbh_ctx *ctx, int x, int y, double *ker4);
 int fbh_psolve_diag_dev(fbh_ctx *ctx, double *out, const double *in);
 
 /*
- * Restarted left-preconditioned GMRES with the diagonal preconditioner, all
- * vectors in HBM (src/gmres.c:153-373, modified Gram-Schmidt :527-621;
+ * The preconditioner GMRES uses, as the -P option picks it (PtVmain,
+ * src/fabipb.c:1289-1309): 3 diagonal (the default of a new context), -1
+ * none (a copy); 0, 1 and 2 the leaf-parent block LU of
+ * setupPreconditioning / PtVfmm / PtVfmmCached / PtVfmmCachedLU
+ * (src/precond_fmm.c:553-953; the three modes solve the same blocks).  One dense 2m x 2m block per cube at level depth-1,
+ * built on the device and inverted once (batched LU); applying it streams the
+ * inverses.  Replaces gpuBuildPrecondPairsBatched / gpuBuildPrecondDisjointBlock
+ * (gpu_backend.h:38-45), which only assembled the blocks.  Fails if depth-1 <
+ * height or a block is singular (the reference exits there).
+ * fbh_psolve_dev applies the current one (out may equal in).
+ */
+int fbh_set_precond(fbh_ctx *ctx, int mode);
+int fbh_psolve_dev(fbh_ctx *ctx, double *out, const double *in);
+
+/*
+ * Restarted left-preconditioned GMRES with the preconditioner of
+ * fbh_set_precond, all vectors in HBM (src/gmres.c:153-373, modified Gram-Schmidt :527-621;
  * convergence on ||M^-1 r|| / ||b||).  b, x are device pointers; x holds the
  * initial guess on entry.  hist (optional, host, maxit+restarts entries)
  * receives the residual history.  Returns info as the reference does

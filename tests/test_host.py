@@ -149,5 +149,5 @@ def test_cli_options(built, meshdir):
     assert r.returncode == 1 and "outside the accepted range" in r.stderr
     r = subprocess.run([exe, "-g=0", str(meshdir / "sph3_ball")], stderr=subprocess.PIPE, text=True)
     assert r.returncode == 1 and "CPU path" in r.stderr
-    r = subprocess.run([exe, "-P=2", str(meshdir / "sph3_ball")], stderr=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
-    assert r.returncode == 1
+    r = subprocess.run([exe, "-P=4", str(meshdir / "sph3_ball")], stderr=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+    assert r.returncode == 1 and "outside the accepted range" in r.stderr
