@@ -701,6 +701,7 @@ __global__ __launch_bounds__(256) void k_q2m(const R *__restrict__ Q0, const R *
   const int L = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = leafStart[L], c = leafCount[L];
   R m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
+#pragma unroll 4
   for (int j = w; j < c; j += 4) {
     const R sp = (R)x[s + j], sn = (R)x[s + j + n];
     const R *q1 = Q1 + (long long)(s + j) * nm, *q0 = Q0 + (long long)(s + j) * nm;
@@ -733,57 +734,77 @@ __global__ __launch_bounds__(256) void k_q2m(const R *__restrict__ Q0, const R *
   }
 }
 
+// LDS writes of a wave become visible to all its lanes (no block barrier)
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // src/expan.c:342-382 (transM2MWs) + convM2M (:223-247): one block per parent,
-// threads own multi-indices (two each at most); the (alpha-k, k) table in LDS
+// one wave per child (at most eight), lanes over multi-indices.  Each wave
+// forms its child's translation monomials and moments in LDS and convolves
+// them; the children's contributions are added in child order at the end.
+// The (alpha-k, k) table sits in LDS.
 template <typename R>
-__global__ __launch_bounds__(256) void k_m2m(int levStart, const int *__restrict__ kidStart,
+__global__ __launch_bounds__(512) void k_m2m(int levStart, const int *__restrict__ kidStart,
                                              const int *__restrict__ nKids, const int *__restrict__ cubeLevel,
                                              const int *__restrict__ ordMom, const double *__restrict__ cubeX,
                                              R *__restrict__ mom, int nmMax, DevTabs T) {
   R *sm = reinterpret_cast<R *>(dynLds);
-  const int par = levStart + blockIdx.x, bd = blockDim.x, tid = threadIdx.x;
-  const int ordOut = ordMom[cubeLevel[par]], nOut = NMOM(ordOut);
-  R *Rt = sm, *b1 = sm + nOut, *b2 = sm + 2 * nOut;
-  int *ts = (int *)(sm + 3 * nOut), *tab = ts + nOut + 1;
+  const int par = levStart + blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int nk = nKids[par], ordOut = ordMom[cubeLevel[par]], nOut = NMOM(ordOut);
+  R *Rt = sm + 3 * w * nOut, *b1 = Rt + nOut, *b2 = Rt + 2 * nOut;   // this wave's child
+  int *ts = (int *)(sm + 24 * nOut), *tab = ts + nOut + 1;
   const int nT = T.m2mStart[nOut];
-  for (int i = tid; i <= nOut; i += bd) ts[i] = T.m2mStart[i];
-  for (int i = tid; i < nT; i += bd) tab[i] = T.m2m[i];
-  R acc1[2] = {0, 0}, acc2[2] = {0, 0};
-  for (int k = 0; k < nKids[par]; k++) {
-    const int kid = kidStart[par] + k;
+  for (int i = tid; i <= nOut; i += blockDim.x) ts[i] = T.m2mStart[i];
+  for (int i = tid; i < nT; i += blockDim.x) tab[i] = T.m2m[i];
+  if (w < nk) {
+    const int kid = kidStart[par] + w;
     const int nIn = NMOM(ordMom[cubeLevel[kid]]);
     double tr[3];
     for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
-    __syncthreads();
-    for (int a = tid; a < nOut; a += bd) {
+    for (int a = lane; a < nOut; a += 64) {
       Rt[a] = (R)conVect1(tr, T.mi[a], T.ifact3[a]);
       b1[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + a] : (R)0;
       b2[a] = a < nIn ? mom[(long long)kid * 2 * nmMax + nmMax + a] : (R)0;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  R t1[4] = {0, 0, 0, 0}, t2[4] = {0, 0, 0, 0};
+  if (w < nk) {
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int a = tid + q * bd;
-      if (a < nOut) {
-        R t1 = 0, t2 = 0;
+    for (int q = 0; q < 4; q++) {
+      const int a = lane + 64 * q;
+      if (a < nOut)
         for (int e = ts[a]; e < ts[a + 1]; e++) {
           const int pr = tab[e];
           const R rv = Rt[pr & 0xffff];
-          t1 += rv * b1[pr >> 16];
-          t2 += rv * b2[pr >> 16];
+          t1[q] += rv * b1[pr >> 16];
+          t2[q] += rv * b2[pr >> 16];
         }
-        acc1[q] += t1;
-        acc2[q] += t2;
+    }
+  }
+  __syncthreads();
+  if (w < nk) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int a = lane + 64 * q;
+      if (a < nOut) {
+        b1[a] = t1[q];
+        b2[a] = t2[q];
       }
     }
   }
-#pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int a = tid + q * bd;
-    if (a < nOut) {
-      mom[(long long)par * 2 * nmMax + a] = acc1[q];
-      mom[(long long)par * 2 * nmMax + nmMax + a] = acc2[q];
+  __syncthreads();
+  for (int a = tid; a < nOut; a += blockDim.x) {
+    R s1 = 0, s2 = 0;
+    for (int k = 0; k < nk; k++) {
+      s1 += sm[3 * k * nOut + nOut + a];
+      s2 += sm[3 * k * nOut + 2 * nOut + a];
     }
+    mom[(long long)par * 2 * nmMax + a] = s1;
+    mom[(long long)par * 2 * nmMax + nmMax + a] = s2;
   }
 }
 
@@ -861,7 +882,8 @@ __global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairD
 
 // M2L with cached derivative tensors (src/fmm.c:774-787, src/expan.c:258-290,
 // 471-494): one block of nw waves (four unless LDS is short) per destination
-// cube; wave w takes pairs w, w + nw, ... of the cube's interaction list.  The
+// cube; wave w takes pairs w, w + nw, ... of the cube's interaction list, on
+// its own (wave barriers only, next pair prefetched into registers).  The
 // (alpha, beta) terms of the convolution are cut into items (short runs of one alpha)
 // that the host spreads over the 64 lanes by term count, so every lane walks a
 // list of about 1/64 of the terms instead of one lane carrying alpha = 0's long
@@ -890,47 +912,70 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   int *idx = (int *)(tab + nTerms);
   for (int i = tid; i < nTerms; i += blockDim.x) tab[i] = planTerms[po.x + i];
   for (int i = tid; i < 65 + nm + 1; i += blockDim.x) idx[i] = planIdx[po.z + i];
-  for (int i = tid; i < nw * 4 * nItems; i += blockDim.x) red[i] = 0.0;
-  R *c1 = slots + w * 5 * nm, *c2 = c1 + nm, *v1 = c1 + 2 * nm, *v2 = c1 + 3 * nm, *v3 = c1 + 4 * nm;
-  R *myRed = red + w * 4 * nItems;
-  const int b0 = m2lOff[c], b1 = m2lOff[c + 1];
-  for (int base = b0; base < b1; base += nw) {
-    const int pr = base + w;
-    const bool act = pr < b1;
-    __syncthreads();
-    if (act) {
-      const int src = m2lSrc[pr];
-      const R *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
-      const R *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
-      for (int i = lane; i < nm; i += 64) {
-        int sg = T.ep[i].w;
-        R a0 = g0[i], ak = gk[i];
-        c1[i] = (R)sg * mp[i];
-        c2[i] = (R)sg * md[i];
-        v1[i] = a0 - ak;
-        v2[i] = eps * ak - a0;
-        v3[i] = a0 - ak / eps;
+  for (int i = tid; i < nw * 4 * nItems; i += blockDim.x) red[i] = 0;
+  __syncthreads();
+  // each wave walks its own pairs without block barriers: the next pair's
+  // tensors and moments are loaded into registers while the current one is
+  // contracted out of this wave's LDS slot
+  R *__restrict__ c1 = slots + w * 5 * nm;
+  R *__restrict__ c2 = c1 + nm;
+  R *__restrict__ v1 = c1 + 2 * nm;
+  R *__restrict__ v2 = c1 + 3 * nm;
+  R *__restrict__ v3 = c1 + 4 * nm;
+  R *__restrict__ myRed = red + w * 4 * nItems;
+  const int e0 = idx[lane], e1 = idx[lane + 1];
+  R sg[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) sg[q] = lane + 64 * q < nm ? (R)T.ep[lane + 64 * q].w : (R)0;
+  R ga[4], gb[4], ma[4], mb[4];
+  auto fetch = [&](int pr) {
+    const int src = m2lSrc[pr];
+    const R *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
+    const R *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = lane + 64 * q;
+      if (i < nm) {
+        ga[q] = g0[i];
+        gb[q] = gk[i];
+        ma[q] = mp[i];
+        mb[q] = md[i];
       }
     }
-    __syncthreads();
-    if (act) {
-      R t1 = 0, t2 = 0, t3 = 0, t4 = 0;
-      const int e1 = idx[lane + 1];
-      for (int e = idx[lane]; e < e1; e++) {
-        const unsigned pe = tab[e];
-        const int j = pe & 0xff, kk = (pe >> 8) & 0xff;
-        const R x1 = c1[j], x2 = c2[j], y1 = v1[kk];
-        t1 += x2 * y1;
-        t2 += x1 * v2[kk];
-        t3 += x2 * v3[kk];
-        t4 += x1 * (-y1);
-        if (pe & M2L_FLUSH) {
-          R *r = myRed + 4 * ((pe >> 16) & 0x7fff);
-          r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
-          t1 = t2 = t3 = t4 = 0;
-        }
+  };
+  const int b1 = m2lOff[c + 1];
+  int pr = m2lOff[c] + w;
+  if (pr < b1) fetch(pr);
+  for (; pr < b1; pr += nw) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = lane + 64 * q;
+      if (i < nm) {
+        c1[i] = sg[q] * ma[q];
+        c2[i] = sg[q] * mb[q];
+        v1[i] = ga[q] - gb[q];
+        v2[i] = eps * gb[q] - ga[q];
+        v3[i] = ga[q] - gb[q] / eps;
       }
     }
+    waveSync();
+    if (pr + nw < b1) fetch(pr + nw);
+    R t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+    for (int e = e0; e < e1; e++) {
+      const unsigned pe = tab[e];
+      const int j = pe & 0xff, kk = (pe >> 8) & 0xff;
+      const R x1 = c1[j], x2 = c2[j], y1 = v1[kk];
+      t1 += x2 * y1;
+      t2 += x1 * v2[kk];
+      t3 += x2 * v3[kk];
+      t4 += x1 * (-y1);
+      if (pe & M2L_FLUSH) {
+        R *r = myRed + 4 * ((pe >> 16) & 0x7fff);
+        r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
+        t1 = t2 = t3 = t4 = 0;
+      }
+    }
+    waveSync();
   }
   __syncthreads();
   const int nmc = NMOM(ordMom[cubeLevel[c]]);
@@ -2155,8 +2200,8 @@ template <typename R>   // children at lev + 1 into parents at lev
 static void m2mPass(fbh_ctx *c, const OpBufs<R> &B, int lev, hipStream_t st) {
   const int nOut = NMOM(c->ordMom[lev]);
   if (c->levCount[lev] <= 0) return;
-  size_t lds = 3 * (size_t)nOut * sizeof(R) + (size_t)(nOut + 1 + c->hM2mStart[nOut]) * sizeof(int);
-  hipLaunchKernelGGL(k_m2m<R>, dim3(c->levCount[lev]), dim3(blk64((nOut + 1) / 2, 256)), lds, st, c->levStart[lev],
+  size_t lds = 24 * (size_t)nOut * sizeof(R) + (size_t)(nOut + 1 + c->hM2mStart[nOut]) * sizeof(int);
+  hipLaunchKernelGGL(k_m2m<R>, dim3(c->levCount[lev]), dim3(512), lds, st, c->levStart[lev],
                      c->kidStart, c->nKids, c->cubeLevelD, c->ordMomD, c->cubeX, B.mom, c->nmMax, c->T);
 }
 
