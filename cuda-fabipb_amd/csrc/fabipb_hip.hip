@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fabipb_hip.h"
@@ -891,7 +892,14 @@ __global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairD
 // adds its running sums into the item's slot in LDS (its own, per wave).  The
 // slots are summed per alpha at the end, in item then wave order.
 #define M2L_FLUSH 0x80000000u
-template <typename R>
+template <typename R> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+// MAXT > 0: every lane's term list (at most MAXT entries, the same for every
+// pair) is held in registers and the contraction is unrolled, so the LDS reads
+// of different terms are in flight together; MAXT = 0 reads the list from LDS
+template <typename R, int MAXT>
 __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
                                                     const int *__restrict__ cubeLevel,
                                                     const int *__restrict__ ordM2L, const int *__restrict__ ordMom,
@@ -899,6 +907,8 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
                                                     R *__restrict__ lec, int nmMax, R eps, DevTabs T,
                                                     const unsigned *__restrict__ planTerms,
                                                     const int *__restrict__ planIdx, const int4 *__restrict__ planOff) {
+  typedef typename Vec2<R>::type R2;
+  typedef typename Vec4<R>::type R4;
   R *sm = reinterpret_cast<R *>(dynLds);
   const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
   const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
@@ -906,8 +916,9 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   // index entries: 65 lane starts into the terms, then nm + 1 item starts per alpha
   const int4 po = planOff[p];
   const int nTerms = po.y, nItems = po.w;
-  R *slots = sm;                                       // [wave][5][nm]
-  R *red = sm + nw * 5 * nm;                           // [wave][item][4]
+  const int nmP = (nm + 1) & ~1;                       // even, so every slot starts 16-byte aligned
+  R *slots = sm;                                       // per wave: nmP x (c1 c2), nmP x (v1 v2 v3 -)
+  R *red = sm + nw * 6 * nmP;                          // [wave][item][4]
   unsigned *tab = (unsigned *)(red + nw * 4 * nItems);
   int *idx = (int *)(tab + nTerms);
   for (int i = tid; i < nTerms; i += blockDim.x) tab[i] = planTerms[po.x + i];
@@ -917,13 +928,15 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   // each wave walks its own pairs without block barriers: the next pair's
   // tensors and moments are loaded into registers while the current one is
   // contracted out of this wave's LDS slot
-  R *__restrict__ c1 = slots + w * 5 * nm;
-  R *__restrict__ c2 = c1 + nm;
-  R *__restrict__ v1 = c1 + 2 * nm;
-  R *__restrict__ v2 = c1 + 3 * nm;
-  R *__restrict__ v3 = c1 + 4 * nm;
+  R2 *__restrict__ cc = reinterpret_cast<R2 *>(slots + w * 6 * nmP);          // c1 = sgn M_pot, c2 = sgn M_dpdn
+  R4 *__restrict__ vv = reinterpret_cast<R4 *>(slots + w * 6 * nmP + 2 * nmP);  // v1, v2, v3
   R *__restrict__ myRed = red + w * 4 * nItems;
   const int e0 = idx[lane], e1 = idx[lane + 1];
+  unsigned pe[MAXT > 0 ? MAXT : 1];
+  if constexpr (MAXT > 0) {
+#pragma unroll
+    for (int k = 0; k < MAXT; k++) pe[k] = e0 + k < e1 ? tab[e0 + k] : 0u;
+  }
   R sg[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) sg[q] = lane + 64 * q < nm ? (R)T.ep[lane + 64 * q].w : (R)0;
@@ -943,6 +956,19 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
       }
     }
   };
+  auto term = [&](unsigned e, R &t1, R &t2, R &t3, R &t4) {
+    const R2 x = cc[e & 0xff];
+    const R4 y = vv[(e >> 8) & 0xff];
+    t1 += x.y * y.x;
+    t2 += x.x * y.y;
+    t3 += x.y * y.z;
+    t4 += x.x * (-y.x);
+    if (e & M2L_FLUSH) {
+      R *r = myRed + 4 * ((e >> 16) & 0x7fff);
+      r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
+      t1 = t2 = t3 = t4 = 0;
+    }
+  };
   const int b1 = m2lOff[c + 1];
   int pr = m2lOff[c] + w;
   if (pr < b1) fetch(pr);
@@ -951,29 +977,27 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
     for (int q = 0; q < 4; q++) {
       const int i = lane + 64 * q;
       if (i < nm) {
-        c1[i] = sg[q] * ma[q];
-        c2[i] = sg[q] * mb[q];
-        v1[i] = ga[q] - gb[q];
-        v2[i] = eps * gb[q] - ga[q];
-        v3[i] = ga[q] - gb[q] / eps;
+        R2 x;
+        x.x = sg[q] * ma[q];
+        x.y = sg[q] * mb[q];
+        cc[i] = x;
+        R4 y;
+        y.x = ga[q] - gb[q];
+        y.y = eps * gb[q] - ga[q];
+        y.z = ga[q] - gb[q] / eps;
+        y.w = 0;
+        vv[i] = y;
       }
     }
     waveSync();
     if (pr + nw < b1) fetch(pr + nw);
     R t1 = 0, t2 = 0, t3 = 0, t4 = 0;
-    for (int e = e0; e < e1; e++) {
-      const unsigned pe = tab[e];
-      const int j = pe & 0xff, kk = (pe >> 8) & 0xff;
-      const R x1 = c1[j], x2 = c2[j], y1 = v1[kk];
-      t1 += x2 * y1;
-      t2 += x1 * v2[kk];
-      t3 += x2 * v3[kk];
-      t4 += x1 * (-y1);
-      if (pe & M2L_FLUSH) {
-        R *r = myRed + 4 * ((pe >> 16) & 0x7fff);
-        r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
-        t1 = t2 = t3 = t4 = 0;
-      }
+    if constexpr (MAXT > 0) {
+#pragma unroll
+      for (int k = 0; k < MAXT; k++)
+        if (e0 + k < e1) term(pe[k], t1, t2, t3, t4);
+    } else {
+      for (int e = e0; e < e1; e++) term(tab[e], t1, t2, t3, t4);
     }
     waveSync();
   }
@@ -1692,6 +1716,7 @@ struct fbh_ctx {
   int4 *planOff = nullptr;
   size_t m2lLds = 0;
   int m2lWaves = 0;
+  int m2lMaxT = 0;   // register-resident term lists (16, 32) or 0
   int ldsMax = 65536;
   // FP32 operator (precision form): copies of the cached operator in FP32
   bool fp32 = false;
@@ -1888,7 +1913,7 @@ static size_t m2lPlanLds(const fbh_ctx *c, const std::vector<int4> &off, int wav
   for (int l = c->height; l <= c->depth; l++) {
     const int p = c->ordM2L[l], nm = NMOM(p);
     const int4 po = off[p];
-    size_t b = ((size_t)waves * 5 * nm + (size_t)waves * 4 * po.w) * elem + (size_t)po.y * sizeof(unsigned) +
+    size_t b = ((size_t)waves * 6 * ((nm + 1) & ~1) + (size_t)waves * 4 * po.w) * elem + (size_t)po.y * sizeof(unsigned) +
                (size_t)(65 + nm + 1) * sizeof(int);
     best = std::max(best, b);
   }
@@ -1899,6 +1924,7 @@ static bool buildM2LPlan(fbh_ctx *c) {
   std::vector<unsigned> terms;
   std::vector<int> idx;
   std::vector<int4> off(c->maxOrder + 1);
+  std::vector<int> laneTerms(c->maxOrder + 1, 0);
   std::vector<int> deg(NMOM(c->maxOrder));
   for (int d = 0, g = 0; d <= c->maxOrder; d++)
     for (int k = 0; k < (d + 1) * (d + 2) / 2; k++) deg[g++] = d;
@@ -1949,6 +1975,7 @@ static bool buildM2LPlan(fbh_ctx *c) {
     }
     laneStart[64] = (int)terms.size() - po.x;
     po.y = laneStart[64];
+    for (int l = 0; l < 64; l++) laneTerms[p] = std::max(laneTerms[p], laneStart[l + 1] - laneStart[l]);
     idx.insert(idx.end(), laneStart.begin(), laneStart.end());
     idx.insert(idx.end(), itemStart.begin(), itemStart.end());
     off[p] = po;
@@ -1959,6 +1986,9 @@ static bool buildM2LPlan(fbh_ctx *c) {
   if (!c->m2lWaves) return true;   // no plan fits: the recompute kernel is used
   c->m2lLds = m2lPlanLds(c, off, c->m2lWaves, sizeof(double));
   c->m2lLds32 = m2lPlanLds(c, off, c->m2lWaves, sizeof(float));
+  int mt = 0;   // longest lane term list at the orders the levels use
+  for (int l = c->height; l <= c->depth; l++) mt = std::max(mt, laneTerms[c->ordM2L[l]]);
+  c->m2lMaxT = mt <= 16 ? 16 : mt <= 32 ? 32 : 0;
   if (!(c->planTerms = dupload(c, terms.data(), terms.size())) || !(c->planIdx = dupload(c, idx.data(), idx.size())) ||
       !(c->planOff = dupload(c, off.data(), off.size())))
     return false;
@@ -2170,19 +2200,22 @@ static OpBufs<float> bufs32(fbh_ctx *c) {
   return {c->Q0f, c->Q1f, c->m2lGf, c->momf, c->lecf, c->farf, c->nearf, c->coef32, c->m2lLds32};
 }
 
-template <typename R>
-static void m2lPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
-  hipLaunchKernelGGL(k_m2l_cached<R>, dim3(c->nCube), dim3(64 * c->m2lWaves), B.m2lLds, st, c->m2lOff, c->m2lSrc,
-                     c->cubeLevelD, c->ordM2LD, c->ordMomD, B.G, B.mom, B.lec, c->nmMax, (R)c->eps, c->T,
+template <typename R, int MT>
+static void m2lLaunch(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
+  hipLaunchKernelGGL((k_m2l_cached<R, MT>), dim3(c->nCube), dim3(64 * c->m2lWaves), B.m2lLds, st, c->m2lOff,
+                     c->m2lSrc, c->cubeLevelD, c->ordM2LD, c->ordMomD, B.G, B.mom, B.lec, c->nmMax, (R)c->eps, c->T,
                      c->planTerms, c->planIdx, c->planOff);
 }
-template <>
-void m2lPass<double>(fbh_ctx *c, const OpBufs<double> &B, hipStream_t st) {
+
+template <typename R>
+static void m2lPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
   if (B.G) {
-    hipLaunchKernelGGL(k_m2l_cached<double>, dim3(c->nCube), dim3(64 * c->m2lWaves), B.m2lLds, st, c->m2lOff,
-                       c->m2lSrc, c->cubeLevelD, c->ordM2LD, c->ordMomD, B.G, B.mom, B.lec, c->nmMax, c->eps, c->T,
-                       c->planTerms, c->planIdx, c->planOff);
-  } else {
+    if (c->m2lMaxT == 16) m2lLaunch<R, 16>(c, B, st);
+    else if (c->m2lMaxT == 32) m2lLaunch<R, 32>(c, B, st);
+    else m2lLaunch<R, 0>(c, B, st);
+    return;
+  }
+  if constexpr (std::is_same<R, double>::value) {   // the FP32 operator always has the cache
     const int pm = c->maxOrder, nm = NMOM(pm);
     size_t lds = (2 * (size_t)derivScratch(pm) + 7 * (size_t)nm) * sizeof(double);
     hipLaunchKernelGGL(k_m2l, dim3(c->nCube), dim3(64), lds, st, c->m2lOff, c->m2lSrc, c->cubeLevelD, c->ordM2LD,
