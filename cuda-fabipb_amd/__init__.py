@@ -99,6 +99,10 @@ def hip_lib():
         L.fbh_gmres_dev.restype = C.c_int
         L.fbh_gmres_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
                                     _ip, _dp, _dp, _ip]
+        L.fbh_set_direct.restype = C.c_int
+        L.fbh_set_direct.argtypes = [_vp, C.c_int]
+        L.fbh_direct_apply.restype = C.c_int
+        L.fbh_direct_apply.argtypes = [_vp, C.c_double, C.c_double, _dp, _dp]
         L.fbh_set_precond.restype = C.c_int
         L.fbh_set_precond.argtypes = [_vp, C.c_int]
         L.fbh_psolve_dev.restype = C.c_int
@@ -320,6 +324,10 @@ class Operator:
         self._ck(hip_lib().fbh_matvec_dev(self.h, alpha, C.c_void_p(x.data_ptr()), beta,
                                           C.c_void_p(y.data_ptr())))
 
+    def set_direct(self, on):
+        """-r=1/2: the dense direct operator (every pair's panelIA0) in place of the FMM."""
+        self._ck(hip_lib().fbh_set_direct(self.h, int(on)))
+
     def set_precond(self, mode):
         """GMRES preconditioner (src/fabipb.c:1289-1309): 3 diagonal; 0, 1, 2 the
         leaf-parent block LU (src/precond_fmm.c), built as explicit block inverses."""
@@ -389,11 +397,12 @@ class Operator:
 
 def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,
           restart=30, maxit=100, height=2, x0="rhs", precision="fp64", op=None, tree=None, mesh=None,
-          precond=3):
+          precond=3, direct=False):
     """One solve (src/fabipb.c:1067-1210): returns energies, iterations and times.
     precision "fp64" is the plain form (GMRES on the FP64 operator); "fp32" is the
     precision form (refinement around GMRES on the FP32 operator, FP64 residuals).
-    precond is the -P mode (3 diagonal; 0, 1, 2 block LU).  An existing
+    precond is the -P mode (3 diagonal; 0, 1, 2 block LU); direct the -r=1/2 dense
+    operator in place of the FMM.  An existing
     mesh/tree/operator may be passed in to time the solve alone."""
     torch = _torch()
     t0 = time.perf_counter()
@@ -402,6 +411,7 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
     t1 = time.perf_counter()
     op = op if op is not None else Operator(tree, kappa, eps1, eps2, qorder)
     op.set_precond(precond)
+    op.set_direct(direct)
     if precision == "fp32":
         op.enable_fp32()
     elif precision != "fp64":

@@ -1738,6 +1738,11 @@ struct fbh_ctx {
   double *pbInv = nullptr, *pbTmp = nullptr;
   size_t pbBytes = 0;
   rocblas_handle blas = nullptr;
+  // dense direct operator (-r=1/2): every panel pair through the nearfield store
+  bool direct = false;
+  int *dSrcOff = nullptr, *dNcol = nullptr, *dSrcList = nullptr;
+  long long *dCoefOff = nullptr;
+  double4 *dCoef = nullptr;
   // nearfield on a second stream, overlapped with the far-field chain
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
@@ -1802,6 +1807,8 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
                   (void *)c->pbTmp})
     if (q) hipFree(q);
   if (c->blas) rocblas_destroy_handle(c->blas);
+  for (void *q : {(void *)c->dSrcOff, (void *)c->dNcol, (void *)c->dSrcList, (void *)c->dCoefOff, (void *)c->dCoef})
+    if (q) hipFree(q);
   if (c->dx) hipFree(c->dx);
   if (c->side) {
     hipStreamSynchronize(c->side);
@@ -2269,10 +2276,26 @@ static void nearPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_
                      c->srcOff, c->ncol, c->coefOff, c->srcList, B.coef, x, B.nearv, c->n);
 }
 
+// the dense direct operator: the nearfield kernel over every panel pair
+static void directPass(fbh_ctx *c, const double *x, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
+                     c->dSrcOff, c->dNcol, c->dCoefOff, c->dSrcList, c->dCoef, x, out, c->n);
+}
+
 // y = alpha*(D - K) x + beta*y; the nearfield runs on the side stream beside
 // the far-field chain (both read x only) and the combine waits for both
 template <typename R>
 static int matvecT(fbh_ctx *c, const OpBufs<R> &B, double alpha, const double *x, double beta, double *y) {
+  if (c->direct) {   // MtVmain with -r=1/2 (src/fabipb.c:1251-1266): no far field
+    if (!std::is_same<R, double>::value) { setErr("the FP32 operator has no direct (-r) form"); return 0; }
+    directPass(c, x, c->nearv, c->stream);
+    const double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
+    const int n2 = 2 * c->n;
+    hipLaunchKernelGGL(k_combine<double>, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far,
+                       c->nearv, y, c->n, alpha, beta, s1, s2, 0, 1);
+    HCK(hipGetLastError());
+    return 1;
+  }
   HCK(hipEventRecord(c->evFork, c->stream));
   HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
   nearPass(c, B, x, c->side);
@@ -2345,6 +2368,76 @@ static int stageApply(fbh_ctx *c, double alpha, const double *sgm, double beta, 
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
                      beta, useFar, useNear);
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(pot, c->ys, b, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+// the dense direct operator (cpuDirectApply, src/direct_backend.c:93-208;
+// gpuDirectApply, gpu_backend.h:15): every pair's panelIA0 in one store of
+// n*n KER[4] records, rows contiguous and summed in panel order
+extern "C" int fbh_set_direct(fbh_ctx *c, int on) {
+  if (!c) { setErr("fbh_set_direct: null context"); return 0; }
+  if (!on) {
+    for (void **q : {(void **)&c->dSrcOff, (void **)&c->dNcol, (void **)&c->dSrcList, (void **)&c->dCoefOff,
+                     (void **)&c->dCoef}) {
+      if (*q) hipFree(*q);
+      *q = nullptr;
+    }
+    c->direct = false;
+    return 1;
+  }
+  if (c->direct) return 1;
+  const size_t n = c->n, need = n * n * sizeof(double4);
+  size_t freeB = 0, totalB = 0;
+  if (hipMemGetInfo(&freeB, &totalB) != hipSuccess || need + ((size_t)1 << 30) > freeB) {
+    setErr("fbh_set_direct: the dense operator needs %.1f GB of device memory, %.1f GB free", need / 1e9,
+           freeB / 1e9);
+    return 0;
+  }
+  std::vector<int> srcOff(c->nLeaf + 1, 0), ncol(c->nLeaf, c->n), srcList(n);
+  std::vector<long long> coefOff(c->nLeaf + 1);
+  for (int l = 0; l < c->nLeaf; l++) coefOff[l] = (long long)c->hLeafStart[l] * (long long)n;
+  coefOff[c->nLeaf] = (long long)(n * n);
+  for (size_t i = 0; i < n; i++) srcList[i] = (int)i;
+  auto fail = [&](const char *what) {
+    setErr("fbh_set_direct: %s", what);
+    fbh_set_direct(c, 0);
+    return 0;
+  };
+  if (hipMalloc(&c->dSrcOff, srcOff.size() * sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->dNcol, ncol.size() * sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->dSrcList, n * sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->dCoefOff, coefOff.size() * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->dCoef, need) != hipSuccess)
+    return fail("device allocation failed");
+  if (hipMemcpy(c->dSrcOff, srcOff.data(), srcOff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->dNcol, ncol.data(), ncol.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->dSrcList, srcList.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->dCoefOff, coefOff.data(), coefOff.size() * sizeof(long long), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return fail("upload failed");
+  hipLaunchKernelGGL(k_near_build, dim3(c->n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart, c->dSrcOff,
+                     c->dNcol, c->dCoefOff, c->dSrcList, c->dCoef, c->kappa, c->eps, c->Q, c->qorder);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(hipGetErrorString(e));
+  c->direct = true;
+  return 1;
+}
+
+// pot = beta*pot + alpha*K_direct*sgm on host arrays (gpuDirectApply)
+extern "C" int fbh_direct_apply(fbh_ctx *c, double alpha, double beta, const double *sgm, double *pot) {
+  if (!c || !sgm || !pot) { setErr("fbh_direct_apply: null argument"); return 0; }
+  if (!c->direct && !fbh_set_direct(c, 1)) return 0;
+  size_t b = 2 * (size_t)c->n * sizeof(double);
+  HCK(hipMemcpyAsync(c->xs, sgm, b, hipMemcpyHostToDevice, c->stream));
+  HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
+  directPass(c, c->xs, c->nearv, c->stream);
+  int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
+                     beta, 0, 1);
   HCK(hipGetLastError());
   HCK(hipMemcpyAsync(pot, c->ys, b, hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));

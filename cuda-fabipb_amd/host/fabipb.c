@@ -9,10 +9,10 @@
  * right-hand side, run GMRES with the -P preconditioner (diagonal, block LU
  * or none), then the panel-tree solvation energy.  Every numeric stage runs on
  * the GPU through include/fabipb_hip.h; the CPU path of the reference (-g=0)
- * is not part of this build, nor are meshing (MSMS/NanoShaper) and the dense
- * direct operators (-r=1/2).  Those options are parsed and validated exactly
- * as the reference does, then refused with a message, so scripts fail loudly
- * rather than run something else.
+ * is not part of this build, nor is meshing (MSMS/NanoShaper).  -r=1/2 (the
+ * dense direct operator) both run it on the device.  Refused options are
+ * parsed and validated exactly as the reference does, then refused with a
+ * message, so scripts fail loudly rather than run something else.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -129,7 +129,6 @@ int main(int argc, char **argv) {
   if (!panelfile[0]) { fprintf(stderr, "Error: no input given\n"); usage(argv[0]); return 1; }
   if (depth < 1 || height < 0 || height > depth) { printf("Bad FMM level range: height=%d depth=%d\n", height, depth); return 0; }
   if (gpuMode == 0) { fprintf(stderr, "Error: -g=0 (the CPU path) is not part of this build; it runs on the GPU only\n"); return 1; }
-  if (matvecMode != 0) { fprintf(stderr, "Error: -r=%d (dense direct operator) is not part of this build\n", matvecMode); return 1; }
   if (!fbh_available()) die("GPU backend unavailable");
 
   /* inputs: <base>.pqr/.vert/.face; a path without '/' is looked up in test_proteins/ (src/input.c:24-36) */
@@ -220,6 +219,7 @@ int main(int argc, char **argv) {
   double resid = 0.0, *hist = calloc((size_t)2 * numItr + 2, sizeof(double));
   tStage = now();
   if (!fbh_set_precond(ctx, precond)) die("preconditioner setup");
+  if (matvecMode != 0 && !fbh_set_direct(ctx, 1)) die("dense direct operator");   /* -r=1/2 */
   if (!fbh_gmres_dev(ctx, b, x, arnoldiSz, tol, numItr, &iters, &resid, hist, &info)) die("GMRES");
   double tGmres = now() - tStage;
   if (getenv("FABIPB_GMRES_LOG_RESID") && atoi(getenv("FABIPB_GMRES_LOG_RESID")) > 0)

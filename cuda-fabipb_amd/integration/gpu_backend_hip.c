@@ -218,11 +218,10 @@ int gpuPanelChargeTreeEnergy(ssystem *sys, const double *sgm, double *pot) {
   return rc;
 }
 
-/* not implemented on the device in this build: the reference runs its CPU path */
+/* pot = beta*pot + alpha*K_direct*sgm, the dense operator of -r=1 (src/fabipb.c:1251-1258) */
 int gpuDirectApply(ssystem *sys, double alpha, double beta, const double *sgm, double *pot) {
-  (void)sys; (void)alpha; (void)beta; (void)sgm; (void)pot;
-  snprintf(gMsg, sizeof(gMsg), "dense direct operator not in this build");
-  return 0;
+  if (!ensure(sys)) return 0;
+  return fbh_direct_apply(gCtx, alpha, beta, sgm, pot) ? 1 : fail("direct operator");
 }
 int gpuRhsReuseCoulombDot(ssystem *sys, const double *rhs, const double *sgm, double *dot) {
   (void)sys; (void)rhs; (void)sgm; (void)dot;

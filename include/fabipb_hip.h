@@ -94,6 +94,17 @@ int fbh_q2m_host(fbh_ctx *ctx, const double *sgm, double *mom);
 int fbh_m2l_host(fbh_ctx *ctx, const double *mom, double *lec);
 int fbh_l2p_host(fbh_ctx *ctx, const double *lec, double alpha, double beta, double *pot);
 
+/*
+ * The dense direct operator of -r=1/2 (cpuDirectApply, src/direct_backend.c:
+ * 93-208; gpuDirectApply, gpu_backend.h:15): every panel pair's panelIA0 kept in
+ * one n*n store of KER[4] records (32 n^2 bytes; refused when it would not fit)
+ * and applied row by row in panel order.  While on, fbh_matvec_dev and the
+ * solvers use it in place of the FMM (no far field).  fbh_direct_apply is
+ * gpuDirectApply on host arrays: pot = beta*pot + alpha*K*sgm (turns it on).
+ */
+int fbh_set_direct(fbh_ctx *ctx, int on);
+int fbh_direct_apply(fbh_ctx *ctx, double alpha, double beta, const double *sgm, double *pot);
+
 /* device expansions after the last far-field pass: nCube*2*nMom(maxOrder)
  * moments and nCube*4*nMom(maxOrder) locals (the reference's cube->mom_*, lec_k*) */
 int fbh_expansions(fbh_ctx *ctx, double *mom, double *lec);

@@ -449,6 +449,7 @@ struct fbo_ctx {
   double **q2m0, **q2m1;     /* per leaf, nMom(ordMom[depth]) x count, column per panel */
   double *mom, *lec;         /* per FMM cube: 2 / 4 arrays of nmMax */
   int precMode;              /* 3 diagonal, 2 block LU */
+  double *dense;             /* -r=1/2: KER[4] of every (d, s) pair, row-major, or NULL */
   int nPB;
   pblock *pb;
 };
@@ -480,6 +481,7 @@ static void block_free(fbo_ctx *c);
 void fbo_destroy(fbo_ctx *c) {
   if (!c) return;
   block_free(c);
+  free(c->dense);
   for (int l = 0; l < c->L.nLeaf; l++) { free(c->q2m0[l]); free(c->q2m1[l]); }
   free(c->q2m0); free(c->q2m1); free(c->mom); free(c->lec); free(c);
 }
@@ -681,12 +683,42 @@ void fbo_apply(fbo_ctx *c, double alpha, const double *sgm, double beta, double 
   }
 }
 
+/* -r=1/2: the dense direct operator (src/direct_backend.c:57-78 builds every
+ * pair's panelIA0, :172-208 applies it row by row in panel order) */
+int fbo_set_direct(fbo_ctx *c, int on) {
+  const fbo_layout *L = &c->L;
+  free(c->dense);
+  c->dense = NULL;
+  if (!on) return 0;
+  const size_t n = (size_t)L->n;
+  c->dense = malloc(sizeof(double) * 4 * n * n);
+  if (!c->dense) return -1;
+  for (size_t d = 0; d < n; d++)
+    for (size_t s = 0; s < n; s++) fbo_panel_ia0(L, (int)d, (int)s, &c->dense[4 * (d * n + s)]);
+  return 0;
+}
+
+static void direct_apply(fbo_ctx *c, double alpha, const double *sgm, double beta, double *pot) {
+  const int n = c->L.n;
+  for (int d = 0; d < n; d++) {
+    const double *k = &c->dense[4 * (size_t)d * n];
+    double sp = 0.0, sd = 0.0;
+    for (int s = 0; s < n; s++) {
+      sp += k[4 * s] * sgm[s + n] + k[4 * s + 1] * sgm[s];
+      sd += k[4 * s + 2] * sgm[s + n] + k[4 * s + 3] * sgm[s];
+    }
+    pot[d] = beta * pot[d] + alpha * sp;
+    pot[d + n] = beta * pot[d + n] + alpha * sd;
+  }
+}
+
 /* src/fabipb.c:1237-1287 (MtVmain) */
 void fbo_matvec(fbo_ctx *c, double alpha, const double *x, double beta, double *y) {
   const fbo_layout *L = &c->L;
   int n = L->n;
   double s1 = (1.0 + L->eps) / 2.0 * alpha, s2 = (1.0 + 1.0 / L->eps) / 2.0 * alpha;
-  fbo_apply(c, alpha, x, -beta, y, 3);
+  if (c->dense) direct_apply(c, alpha, x, -beta, y);
+  else fbo_apply(c, alpha, x, -beta, y, 3);
   for (int i = 0; i < n; i++) {
     y[i] = (s1 * L->area[i] * x[i] - y[i]);
     y[i + n] = s2 * L->area[i] * x[i + n] - y[i + n];

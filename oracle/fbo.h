@@ -85,6 +85,10 @@ void fbo_psolve_diag(fbo_ctx *c, double *out, const double *in);
  * blocks).  Returns 0, -1 if the layout has no level depth-1, -2 on a zero pivot. */
 int fbo_set_precond(fbo_ctx *c, int mode);
 void fbo_psolve(fbo_ctx *c, double *out, const double *in);
+/* -r=1/2: fbo_matvec / fbo_gmres use the dense direct operator (every pair's
+ * panelIA0, n*n*32 bytes) instead of the FMM; 0 turns it off.  Returns 0, -1 if
+ * the table cannot be allocated. */
+int fbo_set_direct(fbo_ctx *c, int on);
 /* restarted left-preconditioned GMRES; returns info; hist (maxit+1) optional */
 int fbo_gmres(fbo_ctx *c, const double *b, double *x, int restart, double tol, int maxit,
               int *iters, double *resid, double *hist);

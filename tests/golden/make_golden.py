@@ -72,9 +72,12 @@ def main():
                 sol = os.path.join(tmp, name + "_sol.csv")
                 e2 = dict(env, FABIPB_ENERGY_MODE="compare", FABIPB_GMRES_LOG_RESID="1",
                           FABIPB_WRITE_SOLUTION=sol)
-                txt = run([os.path.join(ROOT, "oracle/_ref/fabipb_ref"), "-g=0", "-B=1", f"-o={fl['o']}",
-                           f"-P={fl['P']}", f"-a={fl['a']}", f"-i={fl['i']}"] + common + [base], env=e2)
-                res = {"flags": common + [f"-o={fl['o']}", f"-P={fl['P']}", f"-a={fl['a']}", f"-i={fl['i']}"]}
+                solve = [f"-o={fl['o']}", f"-P={fl['P']}", f"-a={fl['a']}", f"-i={fl['i']}"]
+                if "r" in fl:   # dense direct operator instead of the FMM
+                    solve.append(f"-r={fl['r']}")
+                txt = run([os.path.join(ROOT, "oracle/_ref/fabipb_ref"), "-g=0", "-B=1"] + solve + common + [base],
+                          env=e2)
+                res = {"flags": common + solve}
                 m = re.search(r"GMRES status: info=(\d+) iterations=(\d+) final-residual=(\S+)", txt)
                 res["info"], res["iterations"], res["final_residual"] = int(m[1]), int(m[2]), float(m[3])
                 res["resid_history"] = [float(v) for v in re.findall(r"GMRES residual: iter=\d+ resid=(\S+)", txt)]
@@ -89,9 +92,11 @@ def main():
                 m = re.search(r"GMRES call counts: matvec=(\d+)", txt)
                 res["matvecs"] = int(m[1])
                 m = re.search(r"FMM stage avg/call \(ms\): (.*)", txt)
-                res["stage_ms"] = {k: float(v) for k, v in re.findall(r"(\w+)=([\d.]+)", m[1])}
+                if m:   # FMM runs only (not the dense direct operator)
+                    res["stage_ms"] = {k: float(v) for k, v in re.findall(r"(\w+)=([\d.]+)", m[1])}
                 m = re.search(r"FMM matvec stats: calls=(\d+) total=([\d.]+) s", txt)
-                res["matvec_calls"], res["matvec_total_s"] = int(m[1]), float(m[2])
+                if m:
+                    res["matvec_calls"], res["matvec_total_s"] = int(m[1]), float(m[2])
                 res["gmres_s"] = float(re.search(r"gmres=([\d.]+)", txt)[1])
                 res["host_threads"] = os.cpu_count()
                 if case.get("solution"):

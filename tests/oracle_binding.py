@@ -66,6 +66,8 @@ def lib():
         L.fbo_psolve.argtypes = [C.c_void_p, _dp, _dp]
         L.fbo_set_precond.restype = C.c_int
         L.fbo_set_precond.argtypes = [C.c_void_p, C.c_int]
+        L.fbo_set_direct.restype = C.c_int
+        L.fbo_set_direct.argtypes = [C.c_void_p, C.c_int]
         L.fbo_expansions.argtypes = [C.c_void_p, _dp, _dp]
         L.fbo_gmres.restype = C.c_int
         L.fbo_gmres.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
@@ -158,6 +160,11 @@ class Oracle:
         rc = lib().fbo_set_precond(self.ctx, mode)
         if rc != 0:
             raise RuntimeError(f"fbo_set_precond({mode}) = {rc}")
+
+    def set_direct(self, on):
+        """-r=1/2: matvec and GMRES on the dense direct operator instead of the FMM."""
+        if lib().fbo_set_direct(self.ctx, int(on)) != 0:
+            raise RuntimeError("fbo_set_direct failed")
 
     def psolve_any(self, x):
         """The preconditioner chosen by set_precond."""

@@ -139,14 +139,14 @@ def test_energy_walk(fa, meshdir, name):
 @pytest.mark.parametrize("name", [c for c in cli_cases() if c != "cfg2_globule"])
 def test_solve_matches_reference(fa, meshdir, name):
     """Full plain-FP64 solve: energy rel. error <= 1e-9, GMRES iterations within +-1, the
-    same info (the cases include GMRES(3) restarts, an iteration limit and the -P=2
-    block-LU preconditioner)."""
+    same info (the cases include GMRES(3) restarts, an iteration limit, the -P=2
+    block-LU preconditioner and the -r=2 dense direct operator)."""
     F = fixture(name)
     cli = F["cli"]
     fl = case_flags(name)
     r = fa.solve(str(meshdir / name), CASES["cases"][name]["t"], order=fl["p"], sep=fl["S"], kappa=fl["k"],
                  eps1=fl["eps1"], eps2=fl["eps2"], qorder=fl["q"], tol=fl["o"], restart=fl["a"],
-                 maxit=fl["i"], height=fl["H"], precond=fl["P"])
+                 maxit=fl["i"], height=fl["H"], precond=fl["P"], direct=fl.get("r", 0) > 0)
     print(f"\n[{name}] its={r['iters']} (ref {cli['iterations']}) raw={r['energy_raw']!r} "
           f"(ref {cli['panel_tree_raw']!r})", flush=True)
     assert abs(r["iters"] - cli["iterations"]) <= 1 and r["info"] == cli["info"]
@@ -302,10 +302,34 @@ def test_reference_host_with_hip_backend(meshdir, name):
     assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
 
 
-@pytest.mark.parametrize("name", ["glob4", "glob4_p2", "glob4_a3"])
+@pytest.mark.parametrize("name", ["sph3_ball_r2", "glob4_r2"])
+def test_reference_host_direct_on_gpu(meshdir, name):
+    """Drop-in, dense direct operator: the reference host with -r=1 calls gpuDirectApply
+    (the binding's fbh_direct_apply) every matvec; its solve must reproduce the
+    reference's own -r=2 CPU solve (same operator, same order of summation)."""
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle/_ref/fabipb_ref_hip")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in binary not built (needs the reference sources)")
+    cli = fixture(name)["cli"]
+    flags = [f if f != "-r=2" else "-r=1" for f in cli["flags"]]
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
+               FABIPB_ENERGY_MODE="compare")
+    r = subprocess.run([exe, "-g=1", "-B=1"] + flags + [str(meshdir / name)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    print(r.stdout[-1200:], flush=True)
+    assert r.returncode == 0 and "Direct GPU matvec unavailable" not in r.stdout
+    m = re.search(r"GMRES status: info=(\d+) iterations=(\d+)", r.stdout)
+    assert int(m[1]) == cli["info"] and abs(int(m[2]) - cli["iterations"]) <= 1
+    raw = float(re.search(r"panel-tree raw=(\S+)", r.stdout)[1])
+    assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+
+
+@pytest.mark.parametrize("name", ["glob4", "glob4_p2", "glob4_a3", "glob4_r2"])
 def test_fabipb_cli_solve(meshdir, name):
     """bin/fabipb (the C host driver) on the GPU with the reference's recorded flags
-    (diagonal, -P=2 block LU, GMRES(3)): iterations and 17-digit energy."""
+    (diagonal, -P=2 block LU, GMRES(3), -r=2 dense direct): iterations and 17-digit energy."""
     import re
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

@@ -170,6 +170,30 @@ def test_gmres_block_precond_matches_reference(oracles, name):
     assert abs(e - cli["panel_tree_raw"]) <= 1e-11 * abs(cli["panel_tree_raw"])
 
 
+def test_gmres_direct_operator_matches_reference(oracles):
+    """-r=2: GMRES on the dense direct operator (src/direct_backend.c) against the
+    reference's log: iterations, residual history and energy."""
+    from conftest import CASES, case_flags
+    name = "sph3_ball_r2"
+    F, o = oracles(CASES["cases"][name]["same_as"])
+    cli = fixture(name)["cli"]
+    fl = case_flags(name)
+    o.set_direct(True)
+    try:
+        b = F["rhs"]
+        x, info, its, res, hist = o.gmres(b, b.copy(), restart=fl["a"], tol=fl["o"], maxit=fl["i"])
+    finally:
+        o.set_direct(False)
+    assert info == cli["info"] and its == cli["iterations"]
+    h = np.array(cli["resid_history"])
+    assert np.allclose(hist[:len(h)], h, rtol=1e-5, atol=0)
+    depth = len(F["lev_start"]) - 1
+    ct = golden_io.ctree_from_fixture(F, depth)
+    e = o.energy(ct, F["pos"].reshape(-1, 3), F["chr"], x)
+    assert abs(e - cli["panel_tree_raw"]) <= 1e-11 * abs(cli["panel_tree_raw"])
+    assert rel(x, fixture(name)["solution"]) <= 1e-10
+
+
 def test_block_precond_inverts_its_blocks(oracles):
     """The block solve inverts the operator restricted to each leaf-parent cube: for x
     supported on one cube, M^-1 (M x) = x, with M x formed from panelIA0 directly."""
