@@ -121,6 +121,9 @@ def hip_lib():
         L.fbh_energy_dev.restype = C.c_int
         L.fbh_energy_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip,
                                      _ip, _ip, _ip, _ip, C.c_double, _vp, _dp]
+        L.fbh_rhs_tree_dev.restype = C.c_int
+        L.fbh_rhs_tree_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip,
+                                       _ip, _ip, _ip, _ip, C.c_double, _vp]
         L.fbh_profile.restype = C.c_int
         L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
         L.fbh_stats.restype = C.c_int
@@ -388,6 +391,17 @@ class Operator:
             theta, C.c_void_p(sigma.data_ptr()), C.byref(e)))
         return e.value
 
+    def rhs_tree_dev(self, ct, pos, chr_, b, theta=0.8):
+        """Charge-tree right-hand side into the device tensor b (setupRHSTree)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        keep = {k: np.ascontiguousarray(v) for k, v in ct.items() if isinstance(v, np.ndarray)}
+        self._ck(hip_lib().fbh_rhs_tree_dev(
+            self.h, len(chr_), _d(pos), _d(chr_), len(keep["level"]), _i(keep["level"]),
+            _d(keep["x"]), _d(keep["eRad"]), _i(keep["kidStart"]), _i(keep["nKids"]),
+            _i(keep["chgStart"]), _i(keep["nChgs"]), _i(keep["chgIdx"]), _i(keep["order"]),
+            theta, C.c_void_p(b.data_ptr())))
+
     def profile(self, x, y, iters=10):
         ms = np.zeros(8)
         self._ck(hip_lib().fbh_profile(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
@@ -397,12 +411,13 @@ class Operator:
 
 def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,
           restart=30, maxit=100, height=2, x0="rhs", precision="fp64", op=None, tree=None, mesh=None,
-          precond=3, direct=False):
+          precond=3, direct=False, rhs="auto"):
     """One solve (src/fabipb.c:1067-1210): returns energies, iterations and times.
     precision "fp64" is the plain form (GMRES on the FP64 operator); "fp32" is the
     precision form (refinement around GMRES on the FP32 operator, FP64 residuals).
     precond is the -P mode (3 diagonal; 0, 1, 2 block LU); direct the -r=1/2 dense
-    operator in place of the FMM.  An existing
+    operator in place of the FMM; rhs "direct", "tree" or "auto" (the reference's
+    pair-count policy, FABIPB_FORCE_TREE_RHS).  An existing
     mesh/tree/operator may be passed in to time the solve alone."""
     torch = _torch()
     t0 = time.perf_counter()
@@ -420,7 +435,13 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
     t2 = time.perf_counter()
     n2 = 2 * tree.n
     b = torch.empty(n2, dtype=torch.float64, device="cuda")
-    op.rhs_dev(mesh.pos, mesh.chr, b)
+    ct = tree.charge_tree()
+    if rhs == "auto":   # src/fabipb_options.c:174-183
+        rhs = "tree" if host_lib().fb_should_use_tree_rhs(tree.n, len(mesh.chr), 1) else "direct"
+    if rhs == "tree":
+        op.rhs_tree_dev(ct, mesh.pos, mesh.chr, b)
+    else:
+        op.rhs_dev(mesh.pos, mesh.chr, b)
     x = b.clone() if x0 == "rhs" else torch.zeros_like(b)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
@@ -430,7 +451,6 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
         g = op.gmres_dev(b, x, restart=restart, tol=tol, maxit=maxit)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
-    ct = tree.charge_tree()
     raw = op.energy_dev(ct, mesh.pos, mesh.chr, x)
     torch.cuda.synchronize()
     t5 = time.perf_counter()

@@ -1352,6 +1352,17 @@ __device__ inline void enDirect(const double *qp, const double *nr, const double
   y1 += qj * (eps * dGkdn - dG0dn);
 }
 
+// one charge of an unaccepted leaf in the tree right-hand side
+// (src/treecode.c:398-411): y0 += q / r, y1 += -q (n.r) / r^3
+__device__ inline void rhsDirect(const double *qp, const double *nr, const double *pj, double qj, double &y0,
+                                 double &y1) {
+  double x[3] = {qp[0] - pj[0], qp[1] - pj[1], qp[2] - pj[2]};
+  double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], ri = 1.0 / sqrt(r2), r3i = ri / r2;
+  double ip = nr[0] * x[0] + nr[1] * x[1] + nr[2] * x[2];
+  y0 += qj * ri;
+  y1 += qj * (-ip * r3i);
+}
+
 #pragma clang fp contract(fast)
 // ------------------------------------- panel-tree energy, wave-coherent walk
 //
@@ -1431,17 +1442,28 @@ __global__ __launch_bounds__(256) void k_ct_poly(int c0, const double *__restric
   }
 }
 
-template <int P>
+// MODE 0: the energy's kernels G0 - Gk and eps Gk - G0 (src/treecode.c:581-615);
+// MODE 1: the right-hand side's Coulomb kernel 1/r (:283-379), both halves
+template <int P, int MODE>
 __device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, double ry, double rz, double nx,
                                       double ny, double nz, double kappa, double eps, double &y0, double &y1) {
   const double rho = sqrt(rx * rx + ry * ry + rz * rz);
-  double g0[P + 2], gk[P + 2];
-  radial(kappa, rho, P + 1, g0, 1, gk, 1);
   double h[P + 2], u[P + 2];
+  if constexpr (MODE == 0) {
+    double g0[P + 2], gk[P + 2];
+    radial(kappa, rho, P + 1, g0, 1, gk, 1);
 #pragma unroll
-  for (int k = 0; k <= P + 1; k++) {
-    h[k] = g0[k] - gk[k];
-    u[k] = eps * gk[k] - g0[k];
+    for (int k = 0; k <= P + 1; k++) {
+      h[k] = g0[k] - gk[k];
+      u[k] = eps * gk[k] - g0[k];
+    }
+  } else {
+    const double nr2 = -1.0 / (rho * rho);
+    h[0] = 1.0 / rho;
+#pragma unroll
+    for (int k = 0; k <= P; k++) h[k + 1] = (2 * k + 1) * nr2 * h[k];
+#pragma unroll
+    for (int k = 0; k <= P + 1; k++) u[k] = h[k];
   }
   // nested Horner in x (outer), y, z (inner); a coefficient of r^g is the
   // j-sum over the radial values of degree |g| + j
@@ -1491,7 +1513,8 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, 
   y1 += s1;
 }
 
-// Panel-tree energy (src/treecode.c:617-695, 581-615) with one lane per panel:
+// Panel-tree energy (src/treecode.c:617-695, 581-615; MODE 0) and the
+// charge-tree right-hand side (src/treecode.c:381-450; MODE 1) with one lane per panel:
 // the 64 panels of a wave (consecutive in leaf order, so close together) walk
 // the charge tree together, depth first with kids in order, carrying the mask
 // of lanes still descending.  A lane's contributions arrive in the order of its
@@ -1499,7 +1522,7 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, 
 // unaccepted leaves are summed directly.  Cluster data are wave-uniform, so the
 // coefficient tables come in through the scalar cache.
 #define EN_WSTACK 256
-template <int P>
+template <int P, int MODE>
 __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
                                                      const int *__restrict__ level, const double *__restrict__ cx,
                                                      const double *__restrict__ eRad,
@@ -1549,12 +1572,18 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
         const bool acc = on && eRad[cb] < theta * dist && lev >= height;
         const bool dir = on && !acc && lev == depth;
         const unsigned long long down = __ballot(on && !acc && lev < depth);
-        if (acc) enFar<P>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps, y0, y1);
+        if (acc) enFar<P, MODE>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps, y0, y1);
         if (__ballot(dir) != 0ull) {
           const int s = chgStart[cb], nq = nChgs[cb];
           for (int k = 0; k < nq; k++) {
             const int j = chgIdx[s + k];
-            if (dir) enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
+            if (dir) {
+              if constexpr (MODE == 0) {
+                enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
+              } else {
+                rhsDirect(qp, nr, pos + 3 * j, chr[j], y0, y1);
+              }
+            }
           }
         }
         if (down != 0ull) {
@@ -1575,7 +1604,14 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
     }
   o0 *= 2.0 * area;
   o1 *= 2.0 * area;
-  if (live) part[p] = o0 * sgm[p + n] + o1 * sgm[p];
+  if (live) {
+    if constexpr (MODE == 0) {
+      part[p] = o0 * sgm[p + n] + o1 * sgm[p];
+    } else {   // part is the right-hand side b[2n], fac = 1/(4 pi eps1) passed as eps
+      part[p] = eps * o0;
+      part[p + n] = eps * o1;
+    }
+  }
 }
 
 // block partials of sum a[i] over a fixed grid (deterministic order)
@@ -2865,11 +2901,17 @@ extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *c
   return 1;
 }
 
-extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
-                              const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
-                              const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
-                              const int *ctOrder, double theta, const double *sgm_dev, double *energy_raw) {
-  if (!c || nChar <= 0 || nc <= 0 || !energy_raw) { setErr("fbh_energy: bad arguments"); return 0; }
+// the charge-tree walks: mode 0 the panel-tree energy of sgm_dev into
+// *energy_raw, mode 1 the tree right-hand side into b_dev
+static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, const double *chr, int nc,
+                          const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                          const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                          const int *ctOrder, double theta, const double *sgm_dev, double *b_dev,
+                          double *energy_raw) {
+  if (!c || nChar <= 0 || nc <= 0 || (mode == 0 && (!energy_raw || !sgm_dev)) || (mode == 1 && !b_dev)) {
+    setErr("fbh_energy/fbh_rhs_tree: bad arguments");
+    return 0;
+  }
   const int depth = c->depth;
   // validate the tree on the host: every index the walk dereferences
   std::vector<int> lvStart(depth + 2, -1), lvCount(depth + 1, 0);
@@ -2975,14 +3017,21 @@ extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const do
     hipLaunchKernelGGL(k_ct_poly, dim3(nClus), dim3(256), 0, c->stream, c0, dMom, nmMax, dDesc, (int)dsc.size(), SZ,
                        dPoly);
     const int blocks = ((c->n + 63) / 64 + 3) / 4;
-#define EN_WAVE(PP)                                                                                               \
+#define EN_WAVE(PP, MM)                                                                                           \
   case PP:                                                                                                        \
-    hipLaunchKernelGGL(k_energy_wave<PP>, dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, \
-                       dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta, c->kappa,   \
-                       c->eps, c->Q, c->qorder, dPart);                                                           \
+    hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
+                       dLevel, dX, dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta,  \
+                       c->kappa, MM ? kFourPiI / c->eps1 : c->eps, c->Q, c->qorder, MM ? b_dev : dPart);           \
     break;
-    switch (P) { EN_WAVE(1) EN_WAVE(2) EN_WAVE(3) EN_WAVE(4) EN_WAVE(5) EN_WAVE(6) EN_WAVE(7) EN_WAVE(8) EN_WAVE(9) }
+    if (mode == 0) {
+      switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
+    } else {
+      switch (P) { EN_WAVE(1, 1) EN_WAVE(2, 1) EN_WAVE(3, 1) EN_WAVE(4, 1) EN_WAVE(5, 1) EN_WAVE(6, 1) EN_WAVE(7, 1) EN_WAVE(8, 1) EN_WAVE(9, 1) }
+    }
 #undef EN_WAVE
+  } else if (mode == 1) {
+    setErr("fbh_rhs_tree: the accepted levels must share one expansion order");
+    return 0;
   } else {
     int P1 = ord[depth + 1];
     size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
@@ -2990,13 +3039,34 @@ extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const do
                        dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
                        c->Q, c->qorder, c->T, dPart);
   }
-  hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, c->n, dRed);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipSuccess;
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, c->n, dRed);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  } else {
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) { setErr("energy kernels failed: %s", hipGetErrorString(e)); return 0; }
   return 1;
+}
+
+extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                              const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                              const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                              const int *ctOrder, double theta, const double *sgm_dev, double *energy_raw) {
+  return chargeTreeWalk(c, 0, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
+                        ctChgIdx, ctOrder, theta, sgm_dev, nullptr, energy_raw);
+}
+
+extern "C" int fbh_rhs_tree_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                                const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                                const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                                const int *ctOrder, double theta, double *b_dev) {
+  return chargeTreeWalk(c, 1, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
+                        ctChgIdx, ctOrder, theta, nullptr, b_dev, nullptr);
 }
 
 extern "C" int fbh_profile(fbh_ctx *c, const double *x, double *y, int iters, double *ms) {

@@ -83,6 +83,19 @@ static void die(const char *what) {
   exit(1);
 }
 
+/* acceptance ratio of the charge-tree walks: FABIPB_RHS_TREE_THETA /
+ * FABIPB_ENERGY_TREE_THETA in (0, 1], else 0.8 (src/treecode.c:75-158) */
+static double envTheta(const char *name) {
+  const char *e = getenv(name);
+  if (e && *e) {
+    char *end = NULL;
+    double v = strtod(e, &end);
+    if (end != e && *end == '\0' && v > 0.0 && v <= 1.0) return v;
+    fprintf(stderr, "Warning: ignoring invalid %s='%s'\n", name, e);
+  }
+  return 0.8;
+}
+
 int main(int argc, char **argv) {
   char panelfile[80] = {0};
   int order = -1, orderMom = 0, numItr = 100, arnoldiSz = 30, depth = 5, height = 2, qOrd = 1;
@@ -196,14 +209,26 @@ int main(int argc, char **argv) {
   double *b = fbh_dev_alloc(ctx, n2), *x = fbh_dev_alloc(ctx, n2);
   if (!b || !x) die("device vectors");
   tStage = now();
-  if (fb_should_use_tree_rhs(n, mesh->nChar, 1)) {
-    fprintf(stderr, "Error: %lld panel-charge pairs need the tree right-hand side, which is not part of this build yet\n",
-            (long long)n * mesh->nChar);
-    return 1;
+  /* charge octree for the tree right-hand side and the energy (src/chargeTree.c:228-307) */
+  fb_ctree *ct = fb_ctree_build(mesh, depth);
+  if (!ct) die("charge tree");
+  int *ord = malloc(sizeof(int) * (size_t)(depth + 1));
+  for (int l = 0; l <= depth; l++) ord[l] = fb_charge_tree_order(t, l);
+  const int treeRhs = fb_should_use_tree_rhs(n, mesh->nChar, 1);
+  const double rhsTheta = envTheta("FABIPB_RHS_TREE_THETA"), energyTheta = envTheta("FABIPB_ENERGY_TREE_THETA");
+  if (bench) {
+    printf("setupRHS direct pairs: panels=%d charges=%d pairs=%llu mode=%s", n, mesh->nChar,
+           (unsigned long long)n * (unsigned long long)mesh->nChar, treeRhs ? "tree" : "direct");
+    if (treeRhs) printf(" theta=%g", rhsTheta);
+    printf("\n");
   }
-  if (bench) printf("setupRHS direct pairs: panels=%d charges=%d pairs=%llu mode=direct\n", n, mesh->nChar,
-                    (unsigned long long)n * (unsigned long long)mesh->nChar);
-  if (!fbh_rhs_dev(ctx, mesh->nChar, mesh->pos, mesh->chr, b)) die("setupRHS");
+  if (treeRhs) {   /* src/fabipb.c:733-742 (setupRHSTree) */
+    if (!fbh_rhs_tree_dev(ctx, mesh->nChar, mesh->pos, mesh->chr, ct->nc, ct->level, ct->x, ct->eRad, ct->kidStart,
+                          ct->nKids, ct->chgStart, ct->nChgs, ct->chgIdx, ord, rhsTheta, b))
+      die("setupRHS (tree)");
+  } else if (!fbh_rhs_dev(ctx, mesh->nChar, mesh->pos, mesh->chr, b)) {
+    die("setupRHS");
+  }
   double tRhs = now() - tStage;
   if (getenv("FABIPB_STOP_AFTER_RHS")) {
     printf("FABIPB_STOP_AFTER_RHS set: stopping after setupRHS.\n");
@@ -250,12 +275,9 @@ int main(int argc, char **argv) {
     return 0;
   }
   tStage = now();
-  fb_ctree *ct = fb_ctree_build(mesh, depth);
-  int *ord = malloc(sizeof(int) * (size_t)(depth + 1));
-  for (int l = 0; l <= depth; l++) ord[l] = fb_charge_tree_order(t, l);
   double raw = 0.0;
   if (!fbh_energy_dev(ctx, mesh->nChar, mesh->pos, mesh->chr, ct->nc, ct->level, ct->x, ct->eRad, ct->kidStart,
-                      ct->nKids, ct->chgStart, ct->nChgs, ct->chgIdx, ord, 0.8, x, &raw))
+                      ct->nKids, ct->chgStart, ct->nChgs, ct->chgIdx, ord, energyTheta, x, &raw))
     die("panel-tree energy");
   double tEnergy = now() - tStage;
   const char *em = getenv("FABIPB_ENERGY_MODE");

@@ -30,6 +30,7 @@
 extern double kappa, epsilon, epsilon1, epsilon2;   /* src/fabipb.c:34 */
 int rhsChargeExpansionOrder(ssystem *sys, int level);  /* src/chargeTree.c:117 */
 double energyTreeTheta(void);                          /* src/treecode.c:133 */
+double rhsTreeTheta(void);                             /* src/treecode.c:75 */
 
 static fbh_ctx *gCtx;
 static ssystem *gSys;
@@ -167,54 +168,94 @@ int gpuSetupRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
   return fbh_rhs(gCtx, sys->nChar, sys->pos, sys->chr, sgm) ? 1 : fail("setupRHS");
 }
 
-int gpuPanelChargeTreeEnergy(ssystem *sys, const double *sgm, double *pot) {
-  if (!sys->chgCubeList || !ensure(sys)) return 0;
-  int depth = sys->chgDepth, nc = 0, lev, i;
+/* the reference's charge tree (sys->chgCubeList, src/chargeTree.c:228-307) as flat arrays */
+typedef struct {
+  int nc, *level, *kidStart, *nKids, *chgStart, *nChgs, *chgIdx, *ord;
+  double *x, *eRad;
+} flatCT;
+
+static void freeCT(flatCT *t) {
+  free(t->level); free(t->kidStart); free(t->nKids); free(t->chgStart); free(t->nChgs); free(t->chgIdx);
+  free(t->ord); free(t->x); free(t->eRad);
+}
+
+static int flattenCT(ssystem *sys, flatCT *t) {
+  int depth = sys->chgDepth, lev, i, j = 0, cur = 0;
+  memset(t, 0, sizeof(*t));
   for (lev = 0; lev <= depth; lev++)
-    for (cube *c = sys->chgCubeList[lev]; c; c = c->next) nc++;
-  int *level = malloc(sizeof(int) * (size_t)nc), *kidStart = malloc(sizeof(int) * (size_t)nc);
-  int *nKids = malloc(sizeof(int) * (size_t)nc), *chgStart = calloc((size_t)nc, sizeof(int));
-  int *nChgs = malloc(sizeof(int) * (size_t)nc), *chgIdx = malloc(sizeof(int) * (size_t)sys->nChar);
-  int *ord = malloc(sizeof(int) * (size_t)(depth + 1));
-  double *x = malloc(sizeof(double) * 3 * (size_t)nc), *eRad = malloc(sizeof(double) * (size_t)nc);
-  int j = 0, kidNext = 0, cur = 0, rc = 0;
+    for (cube *c = sys->chgCubeList[lev]; c; c = c->next) t->nc++;
+  int nc = t->nc;
+  t->level = malloc(sizeof(int) * (size_t)nc); t->kidStart = malloc(sizeof(int) * (size_t)nc);
+  t->nKids = malloc(sizeof(int) * (size_t)nc); t->chgStart = calloc((size_t)nc, sizeof(int));
+  t->nChgs = malloc(sizeof(int) * (size_t)nc); t->chgIdx = malloc(sizeof(int) * (size_t)sys->nChar);
+  t->ord = malloc(sizeof(int) * (size_t)(depth + 1));
+  t->x = malloc(sizeof(double) * 3 * (size_t)nc); t->eRad = malloc(sizeof(double) * (size_t)nc);
+  if (!t->level || !t->kidStart || !t->nKids || !t->chgStart || !t->nChgs || !t->chgIdx || !t->ord || !t->x ||
+      !t->eRad) {
+    freeCT(t);
+    return 0;
+  }
   for (lev = 0; lev <= depth; lev++) {
-    ord[lev] = rhsChargeExpansionOrder(sys, lev);
-    int first = j;
+    t->ord[lev] = rhsChargeExpansionOrder(sys, lev);
     for (cube *c = sys->chgCubeList[lev]; c; c = c->next, j++) {
-      level[j] = lev;
-      for (i = 0; i < 3; i++) x[3 * j + i] = c->x[i];
-      eRad[j] = c->eRad;
-      nKids[j] = lev < depth ? c->nKids : 0;
-      nChgs[j] = c->nChgs;
+      t->level[j] = lev;
+      for (i = 0; i < 3; i++) t->x[3 * j + i] = c->x[i];
+      t->eRad[j] = c->eRad;
+      t->nKids[j] = lev < depth ? c->nKids : 0;
+      t->nChgs[j] = c->nChgs;
       if (lev == depth) {
-        chgStart[j] = cur;
-        for (i = 0; i < c->nChgs; i++) chgIdx[cur++] = c->chgIdx[i];
+        t->chgStart[j] = cur;
+        for (i = 0; i < c->nChgs; i++) t->chgIdx[cur++] = c->chgIdx[i];
       }
     }
-    (void)first;
   }
   /* kids of consecutive parents are consecutive in the next level's list */
   j = 0;
   for (lev = 0; lev <= depth; lev++) {
-    int nextStart = j;
-    for (cube *c = sys->chgCubeList[lev]; c; c = c->next) nextStart++;
-    kidNext = nextStart;
+    int kidNext = j;
+    for (cube *c = sys->chgCubeList[lev]; c; c = c->next) kidNext++;
     for (cube *c = sys->chgCubeList[lev]; c; c = c->next, j++) {
-      kidStart[j] = nKids[j] ? kidNext : -1;
-      kidNext += nKids[j];
+      t->kidStart[j] = t->nKids[j] ? kidNext : -1;
+      kidNext += t->nKids[j];
     }
   }
-  int n2 = 2 * sys->nPnls;
+  return 1;
+}
+
+int gpuPanelChargeTreeEnergy(ssystem *sys, const double *sgm, double *pot) {
+  if (!sys->chgCubeList || !ensure(sys)) return 0;
+  flatCT t;
+  if (!flattenCT(sys, &t)) return 0;
+  int rc = 0, n2 = 2 * sys->nPnls;
   double *d = fbh_dev_alloc(gCtx, n2);
   if (d && fbh_copy_h2d(gCtx, d, sgm, n2) &&
-      fbh_energy_dev(gCtx, sys->nChar, sys->pos, sys->chr, nc, level, x, eRad, kidStart, nKids, chgStart, nChgs,
-                     chgIdx, ord, energyTreeTheta(), d, pot))
+      fbh_energy_dev(gCtx, sys->nChar, sys->pos, sys->chr, t.nc, t.level, t.x, t.eRad, t.kidStart, t.nKids,
+                     t.chgStart, t.nChgs, t.chgIdx, t.ord, energyTreeTheta(), d, pot))
     rc = 1;
   else
     fail("panel-tree energy");
   fbh_dev_free(gCtx, d);
-  free(level); free(kidStart); free(nKids); free(chgStart); free(nChgs); free(chgIdx); free(ord); free(x); free(eRad);
+  freeCT(&t);
+  return rc;
+}
+
+/* sgm = the charge-tree right-hand side (setupRHSTree, src/fabipb.c:733-742); fac
+ * is 1/(4 pi eps1) there as in fbh_rhs_tree_dev, qOrder the operator's own */
+int gpuChargeTreeRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
+  (void)fac;
+  if (!sys->chgCubeList || !ensure(sys) || qOrder != sys->maxQuadOrder) return 0;
+  flatCT t;
+  if (!flattenCT(sys, &t)) return 0;
+  int rc = 0, n2 = 2 * sys->nPnls;
+  double *d = fbh_dev_alloc(gCtx, n2);
+  if (d && fbh_rhs_tree_dev(gCtx, sys->nChar, sys->pos, sys->chr, t.nc, t.level, t.x, t.eRad, t.kidStart,
+                            t.nKids, t.chgStart, t.nChgs, t.chgIdx, t.ord, rhsTreeTheta(), d) &&
+      fbh_copy_d2h(gCtx, sgm, d, n2))
+    rc = 1;
+  else
+    fail("tree right-hand side");
+  fbh_dev_free(gCtx, d);
+  freeCT(&t);
   return rc;
 }
 
@@ -229,10 +270,6 @@ int gpuRhsReuseCoulombDot(ssystem *sys, const double *rhs, const double *sgm, do
 }
 int gpuRhsReusePanelChargeTreeEnergy(ssystem *sys, const double *rhs, const double *sgm, double *pot) {
   (void)sys; (void)rhs; (void)sgm; (void)pot;
-  return 0;
-}
-int gpuChargeTreeRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
-  (void)sys; (void)qOrder; (void)fac; (void)sgm;
   return 0;
 }
 int gpuBuildPrecondPairsBatched(ssystem *sys, const int *pairSrc, const int *pairDst, long long nPairs, double *k0,

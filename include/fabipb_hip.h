@@ -182,6 +182,20 @@ int fbh_energy_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr
                    const int *ctNChgs, const int *ctChgIdx, const int *ctOrder,
                    double theta, const double *sgm_dev, double *energy_raw);
 
+/*
+ * Charge-tree right-hand side into b_dev (2n, device): the walk of the energy
+ * with the Coulomb kernel (setupRHSTree, src/fabipb.c:633-742, and
+ * rhsTreeWalkLocal / chgClusterEvalLocal, src/treecode.c:283-450; replaces
+ * gpuChargeTreeRHS, gpu_backend.h:36).  Same charge-tree arrays as
+ * fbh_energy_dev; theta is FABIPB_RHS_TREE_THETA's value (0.8).  The levels
+ * the walk can accept must share one expansion order (true for fixed -p).
+ */
+int fbh_rhs_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                     int nc, const int *ctLevel, const double *ctX, const double *ctERad,
+                     const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                     const int *ctNChgs, const int *ctChgIdx, const int *ctOrder,
+                     double theta, double *b_dev);
+
 /* per-stage device timing over `iters` matvecs (HIP events on the context stream):
  * ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole matvec */
 int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);

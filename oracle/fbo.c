@@ -1067,6 +1067,82 @@ static void energyWalk(ewalk *W, int cb, const double *qp, const double *nrm, do
   for (int k = 0; k < T->nKids[cb]; k++) energyWalk(W, T->kidStart[cb] + k, qp, nrm, y);
 }
 
+/* src/treecode.c:381-416 (rhsTreeWalkLocal) with :349-379 (chgClusterEvalLocal):
+ * the Coulomb kernel only; the derivative tensors of 1/(4 pi r) are those of
+ * setupCoulombDerivsLocal (:283-347), the G0 half of setupDerivs */
+static void rhsWalk(ewalk *W, int cb, const double *qp, const double *nrm, double theta, double *y) {
+  const fbo_layout *L = &W->c->L;
+  const fbo_ctree *T = W->T;
+  double r[3];
+  for (int k = 0; k < 3; k++) r[k] = qp[k] - T->x[3 * cb + k];
+  double dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  int lev = T->level[cb];
+  if (T->eRad[cb] < theta * dist && lev >= L->height) {
+    int order = ctOrder(L, lev);
+    fbo_derivs(L->kappa, order + 1, r, W->g0, W->gk);
+    const double *mom = &W->mom[(size_t)cb * W->c->nmMax];
+    double y0 = 0.0, y1 = 0.0;
+    int i = 0;
+    for (int n = 0; n <= order; n++)
+      for (int i1 = 0; i1 <= n; i1++)
+        for (int i2 = 0; i2 <= n - i1; i2++, i++) {
+          int i3 = n - i1 - i2;
+          y0 += gSgn3[i] * mom[i] * W->g0[i];
+          double dn = nrm[0] * W->g0[IDX(i1 + 1, i2, i3)] + nrm[1] * W->g0[IDX(i1, i2 + 1, i3)] +
+                      nrm[2] * W->g0[IDX(i1, i2, i3 + 1)];
+          y1 += gSgn3[i] * mom[i] * dn;
+        }
+    y[0] += y0 / kFourPiI;
+    y[1] += y1 / kFourPiI;
+    return;
+  }
+  if (lev == T->depth) {
+    for (int i = 0; i < T->nChgs[cb]; i++) {
+      int j = T->chgIdx[T->chgStart[cb] + i];
+      double x[3];
+      for (int k = 0; k < 3; k++) x[k] = qp[k] - T->pos[3 * j + k];
+      double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], ri = 1.0 / sqrt(r2), r3i = ri / r2;
+      double ip = nrm[0] * x[0] + nrm[1] * x[1] + nrm[2] * x[2];
+      y[0] += T->chr[j] * ri;
+      y[1] += T->chr[j] * (-ip * r3i);
+    }
+    return;
+  }
+  for (int k = 0; k < T->nKids[cb]; k++) rhsWalk(W, T->kidStart[cb] + k, qp, nrm, theta, y);
+}
+
+/* src/fabipb.c:633-649 (rhsTreeSetupWorker) with src/treecode.c:418-450
+ * (panelRHSTreeWorkspace): b = fac * (2 A sum_q t w w y), fac = 1/(4 pi eps1) */
+void fbo_rhs_tree(fbo_ctx *c, const fbo_ctree *T, double theta, double *b) {
+  const fbo_layout *L = &c->L;
+  int n = L->n, q = L->qorder;
+  const double *t = kT[q], *w = kW[q];
+  const double fac = kFourPiI / L->eps1;
+  ewalk *W = calloc(1, sizeof(ewalk));
+  double *mom = malloc(sizeof(double) * (size_t)T->nc * c->nmMax);
+  fbo_ctree_moments(c, T, mom);
+  W->c = c; W->T = T; W->mom = mom;
+  for (int p = 0; p < n; p++) {
+    const double *v = &L->vtx[9 * p];
+    double a[3][3], qp[3], y[2], o0 = 0.0, o1 = 0.0;
+    edges(v, a);
+    for (int ix = 0; ix < q; ix++)
+      for (int jx = 0; jx < q; jx++) {
+        for (int k = 0; k < 3; k++) qp[k] = v[k] + t[ix] * (a[2][k] + t[jx] * a[0][k]);
+        y[0] = y[1] = 0.0;
+        rhsWalk(W, 0, qp, &L->nrm[3 * p], theta, y);
+        o0 += y[0] * t[ix] * w[ix] * w[jx];
+        o1 += y[1] * t[ix] * w[ix] * w[jx];
+      }
+    o0 *= 2.0 * L->area[p];
+    o1 *= 2.0 * L->area[p];
+    b[p] = fac * o0;
+    b[p + n] = fac * o1;
+  }
+  free(mom);
+  free(W);
+}
+
 /* src/treecode.c:663-695 (panelEnergyTree) and :869-1017 (the panel sum) */
 double fbo_energy_tree(fbo_ctx *c, const fbo_ctree *T, const double *sgm) {
   const fbo_layout *L = &c->L;

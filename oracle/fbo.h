@@ -95,6 +95,9 @@ int fbo_gmres(fbo_ctx *c, const double *b, double *x, int restart, double tol, i
 void fbo_rhs_direct(fbo_ctx *c, int nChar, const double *pos, const double *chr, double *b);
 /* raw panel-tree energy (multiply by 2*pi*332.0716 for kcal/mol) */
 double fbo_energy_tree(fbo_ctx *c, const fbo_ctree *T, const double *sgm);
+/* charge-tree right-hand side (src/fabipb.c:633-742, src/treecode.c:283-450):
+ * b[2n], the same walk as the energy with the Coulomb kernel, acceptance theta */
+void fbo_rhs_tree(fbo_ctx *c, const fbo_ctree *T, double theta, double *b);
 /* charge-tree moments as computed for the walk: nMom(maxOrder) per cube */
 void fbo_ctree_moments(fbo_ctx *c, const fbo_ctree *T, double *mom);
 

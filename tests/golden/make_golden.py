@@ -8,7 +8,8 @@ Runs in the development container only (it needs /root/reference):
      - runs oracle/_ref/ref_harness: layout, kernels, one matvec, RHS, energy
        of a seeded density, charge tree -> tests/golden/<case>/*.bin
      - runs oracle/_ref/fabipb_ref (-g=0 -B=1, FABIPB_ENERGY_MODE=compare,
-       FABIPB_GMRES_LOG_RESID=1, FABIPB_WRITE_SOLUTION) and keeps the
+       FABIPB_GMRES_LOG_RESID=1, FABIPB_WRITE_SOLUTION; with "tree_rhs" also
+       FABIPB_FORCE_TREE_RHS=1 and the RHS sample of every row) and keeps the
        iteration count, residual history, 17-digit panel-tree energy, layout
        counts, matvec timings and (where asked) the solution vector
        -> tests/golden/<case>/cli.json (+ solution.bin)
@@ -72,6 +73,9 @@ def main():
                 sol = os.path.join(tmp, name + "_sol.csv")
                 e2 = dict(env, FABIPB_ENERGY_MODE="compare", FABIPB_GMRES_LOG_RESID="1",
                           FABIPB_WRITE_SOLUTION=sol)
+                rhsCsv = os.path.join(tmp, name + "_rhs.csv")
+                if case.get("tree_rhs"):   # the charge-tree right-hand side, recorded row by row
+                    e2.update(FABIPB_FORCE_TREE_RHS="1", FABIPB_RHS_SAMPLE_PATH=rhsCsv, FABIPB_RHS_SAMPLE_STRIDE="1")
                 solve = [f"-o={fl['o']}", f"-P={fl['P']}", f"-a={fl['a']}", f"-i={fl['i']}"]
                 if "r" in fl:   # dense direct operator instead of the FMM
                     solve.append(f"-r={fl['r']}")
@@ -106,6 +110,11 @@ def main():
                     res["solution_face_order"] = "idx"
                     np.concatenate([a[:, 6], a[:, 7]]).astype(np.float64).tofile(os.path.join(out, "solution.bin"))
                     a[:, 1].astype(np.int32).tofile(os.path.join(out, "solution_face.bin"))
+                if case.get("tree_rhs"):
+                    res["rhs_mode"] = re.search(r"setupRHS direct pairs: .* mode=(\w+)", txt)[1]
+                    a = np.loadtxt(rhsCsv, delimiter=",", skiprows=1)
+                    a = a[np.argsort(a[:, 0])]
+                    np.concatenate([a[:, 9], a[:, 10]]).astype(np.float64).tofile(os.path.join(out, "rhs_tree.bin"))
                 json.dump(res, open(os.path.join(out, "cli.json"), "w"), indent=1)
                 print(f"[{name}] its={res['iterations']} raw={res['panel_tree_raw']!r}", flush=True)
     finally:

@@ -66,6 +66,7 @@ def lib():
         L.fbo_psolve.argtypes = [C.c_void_p, _dp, _dp]
         L.fbo_set_precond.restype = C.c_int
         L.fbo_set_precond.argtypes = [C.c_void_p, C.c_int]
+        L.fbo_rhs_tree.argtypes = [C.c_void_p, C.c_void_p, C.c_double, _dp]
         L.fbo_set_direct.restype = C.c_int
         L.fbo_set_direct.argtypes = [C.c_void_p, C.c_int]
         L.fbo_expansions.argtypes = [C.c_void_p, _dp, _dp]
@@ -237,6 +238,12 @@ class Oracle:
         T, keep = self._ctree(ct, pos, chr_, theta)
         s = np.ascontiguousarray(sigma, dtype=np.float64)
         return lib().fbo_energy_tree(self.ctx, C.byref(T), dptr(s))
+
+    def rhs_tree(self, ct, pos, chr_, theta=0.8):
+        T, keep = self._ctree(ct, pos, chr_, theta)
+        b = np.zeros(2 * self.n)
+        lib().fbo_rhs_tree(self.ctx, C.byref(T), theta, dptr(b))
+        return b
 
     def ctree_moments(self, ct, pos, chr_):
         T, keep = self._ctree(ct, pos, chr_)

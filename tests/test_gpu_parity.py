@@ -146,7 +146,8 @@ def test_solve_matches_reference(fa, meshdir, name):
     fl = case_flags(name)
     r = fa.solve(str(meshdir / name), CASES["cases"][name]["t"], order=fl["p"], sep=fl["S"], kappa=fl["k"],
                  eps1=fl["eps1"], eps2=fl["eps2"], qorder=fl["q"], tol=fl["o"], restart=fl["a"],
-                 maxit=fl["i"], height=fl["H"], precond=fl["P"], direct=fl.get("r", 0) > 0)
+                 maxit=fl["i"], height=fl["H"], precond=fl["P"], direct=fl.get("r", 0) > 0,
+                 rhs="tree" if CASES["cases"][name].get("tree_rhs") else "direct")
     print(f"\n[{name}] its={r['iters']} (ref {cli['iterations']}) raw={r['energy_raw']!r} "
           f"(ref {cli['panel_tree_raw']!r})", flush=True)
     assert abs(r["iters"] - cli["iterations"]) <= 1 and r["info"] == cli["info"]
@@ -234,6 +235,20 @@ def test_fp32_refined_solve(fa, meshdir, name):
           f"mv64={r32['mv64']} resid={r32['resid']:.3e} energy rel err={err:.3e}", flush=True)
     assert r32["info"] == 0 and r32["resid"] <= FL["o"]
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["sph3_ball_tr", "glob4_tr"])
+def test_tree_rhs_matches_reference(fa, meshdir, name):
+    """The charge-tree right-hand side on the device against the reference's RHS sample
+    (FABIPB_FORCE_TREE_RHS=1), row by row."""
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, CASES["cases"][name]["same_as"])
+    ref = np.fromfile(os.path.join(os.path.dirname(__file__), "golden", name, "rhs_tree.bin"))
+    b = torch.empty(2 * tree.n, dtype=torch.float64, device="cuda")
+    op.rhs_tree_dev(tree.charge_tree(), mesh.pos, mesh.chr, b)
+    got = b.cpu().numpy()
+    print(f"\n[{name}] tree rhs max rel {np.max(np.abs(got - ref)) / np.max(np.abs(ref)):.2e}", flush=True)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
 @pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
@@ -326,16 +341,43 @@ def test_reference_host_direct_on_gpu(meshdir, name):
     assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
 
 
-@pytest.mark.parametrize("name", ["glob4", "glob4_p2", "glob4_a3", "glob4_r2"])
+def test_reference_host_tree_rhs_on_gpu(meshdir):
+    """Drop-in, charge-tree right-hand side: the reference host with
+    FABIPB_FORCE_TREE_RHS=1 and -g=1 calls gpuChargeTreeRHS (the binding's
+    fbh_rhs_tree_dev); its solve must reproduce the reference's CPU tree-RHS solve."""
+    import re
+    import subprocess
+    name = "glob4_tr"
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle/_ref/fabipb_ref_hip")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in binary not built (needs the reference sources)")
+    cli = fixture(name)["cli"]
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
+               FABIPB_ENERGY_MODE="compare", FABIPB_FORCE_TREE_RHS="1")
+    r = subprocess.run([exe, "-g=1", "-B=1"] + cli["flags"] + [str(meshdir / name)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    print(r.stdout[-1200:], flush=True)
+    assert r.returncode == 0 and "mode=tree" in r.stdout
+    m = re.search(r"GMRES status: info=(\d+) iterations=(\d+)", r.stdout)
+    assert int(m[1]) == cli["info"] and abs(int(m[2]) - cli["iterations"]) <= 1
+    raw = float(re.search(r"panel-tree raw=(\S+)", r.stdout)[1])
+    assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+
+
+@pytest.mark.parametrize("name", ["glob4", "glob4_p2", "glob4_a3", "glob4_r2", "glob4_tr"])
 def test_fabipb_cli_solve(meshdir, name):
     """bin/fabipb (the C host driver) on the GPU with the reference's recorded flags
-    (diagonal, -P=2 block LU, GMRES(3), -r=2 dense direct): iterations and 17-digit energy."""
+    (diagonal, -P=2 block LU, GMRES(3), -r=2 dense direct, tree RHS): iterations and
+    17-digit energy."""
     import re
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     cli = fixture(name)["cli"]
+    env = dict(os.environ, FABIPB_ENERGY_MODE="compare")
+    if CASES["cases"][name].get("tree_rhs"):
+        env["FABIPB_FORCE_TREE_RHS"] = "1"
     r = subprocess.run([os.path.join(root, "cuda-fabipb_amd/bin/fabipb"), "-g=1", "-B=1"] + cli["flags"] +
-                       [str(meshdir / name)], env=dict(os.environ, FABIPB_ENERGY_MODE="compare"),
+                       [str(meshdir / name)], env=env,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
     print(r.stdout[-1200:], flush=True)
     assert r.returncode == 0

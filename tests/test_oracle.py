@@ -194,6 +194,28 @@ def test_gmres_direct_operator_matches_reference(oracles):
     assert rel(x, fixture(name)["solution"]) <= 1e-10
 
 
+@pytest.mark.parametrize("name", ["sph3_ball_tr", "glob4_tr"])
+def test_tree_rhs_matches_reference(oracles, name):
+    """The charge-tree right-hand side (FABIPB_FORCE_TREE_RHS=1) row by row against the
+    reference's RHS sample, and GMRES + energy from it against the reference solve."""
+    from conftest import CASES, case_flags
+    F, o = oracles(CASES["cases"][name]["same_as"])
+    G = fixture(name)
+    cli = G["cli"]
+    fl = case_flags(name)
+    depth = len(F["lev_start"]) - 1
+    ct = golden_io.ctree_from_fixture(F, depth)
+    pos, chr_ = F["pos"].reshape(-1, 3), F["chr"]
+    b = o.rhs_tree(ct, pos, chr_)
+    ref = np.fromfile(os.path.join(os.path.dirname(__file__), "golden", name, "rhs_tree.bin"))
+    assert np.max(np.abs(b - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert rel(b, F["rhs"]) > 1e-4   # really the tree, not the direct sum
+    x, info, its, res, hist = o.gmres(b, b.copy(), restart=fl["a"], tol=fl["o"], maxit=fl["i"])
+    assert info == cli["info"] and its == cli["iterations"]
+    e = o.energy(ct, pos, chr_, x)
+    assert abs(e - cli["panel_tree_raw"]) <= 1e-11 * abs(cli["panel_tree_raw"])
+
+
 def test_block_precond_inverts_its_blocks(oracles):
     """The block solve inverts the operator restricted to each leaf-parent cube: for x
     supported on one cube, M^-1 (M x) = x, with M x formed from panelIA0 directly."""
