@@ -702,7 +702,6 @@ __global__ __launch_bounds__(256) void k_q2m(const R *__restrict__ Q0, const R *
   const int L = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = leafStart[L], c = leafCount[L];
   R m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
-#pragma unroll 4
   for (int j = w; j < c; j += 4) {
     const R sp = (R)x[s + j], sn = (R)x[s + j + n];
     const R *q1 = Q1 + (long long)(s + j) * nm, *q0 = Q0 + (long long)(s + j) * nm;
@@ -908,7 +907,6 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
                                                     const unsigned *__restrict__ planTerms,
                                                     const int *__restrict__ planIdx, const int4 *__restrict__ planOff) {
   typedef typename Vec2<R>::type R2;
-  typedef typename Vec4<R>::type R4;
   R *sm = reinterpret_cast<R *>(dynLds);
   const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
   const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
@@ -917,8 +915,8 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   const int4 po = planOff[p];
   const int nTerms = po.y, nItems = po.w;
   const int nmP = (nm + 1) & ~1;                       // even, so every slot starts 16-byte aligned
-  R *slots = sm;                                       // per wave: nmP x (c1 c2), nmP x (v1 v2 v3 -)
-  R *red = sm + nw * 6 * nmP;                          // [wave][item][4]
+  R *slots = sm;                                       // per wave: nmP x (c1 c2), nmP x (a0 ak)
+  R *red = sm + nw * 4 * nmP;                          // [wave][item][4]
   unsigned *tab = (unsigned *)(red + nw * 4 * nItems);
   int *idx = (int *)(tab + nTerms);
   for (int i = tid; i < nTerms; i += blockDim.x) tab[i] = planTerms[po.x + i];
@@ -928,8 +926,8 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   // each wave walks its own pairs without block barriers: the next pair's
   // tensors and moments are loaded into registers while the current one is
   // contracted out of this wave's LDS slot
-  R2 *__restrict__ cc = reinterpret_cast<R2 *>(slots + w * 6 * nmP);          // c1 = sgn M_pot, c2 = sgn M_dpdn
-  R4 *__restrict__ vv = reinterpret_cast<R4 *>(slots + w * 6 * nmP + 2 * nmP);  // v1, v2, v3
+  R2 *__restrict__ cc = reinterpret_cast<R2 *>(slots + w * 4 * nmP);          // c1 = sgn M_pot, c2 = sgn M_dpdn
+  R2 *__restrict__ gg = reinterpret_cast<R2 *>(slots + w * 4 * nmP + 2 * nmP);  // a0 = D G0, ak = D Gk
   R *__restrict__ myRed = red + w * 4 * nItems;
   const int e0 = idx[lane], e1 = idx[lane + 1];
   unsigned pe[MAXT > 0 ? MAXT : 1];
@@ -956,16 +954,22 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
       }
     }
   };
+  // four products per (alpha, beta): c1 a0, c1 ak, c2 a0, c2 ak; an item's
+  // sums combine into the four locals when it ends:
+  //   l1 += c2*(a0 - ak), l2 += c1*(eps ak - a0), l3 += c2*(a0 - ak/eps), l4 += c1*(ak - a0)
   auto term = [&](unsigned e, R &t1, R &t2, R &t3, R &t4) {
     const R2 x = cc[e & 0xff];
-    const R4 y = vv[(e >> 8) & 0xff];
-    t1 += x.y * y.x;
-    t2 += x.x * y.y;
-    t3 += x.y * y.z;
-    t4 += x.x * (-y.x);
+    const R2 g = gg[(e >> 8) & 0xff];
+    t1 += x.x * g.x;
+    t2 += x.x * g.y;
+    t3 += x.y * g.x;
+    t4 += x.y * g.y;
     if (e & M2L_FLUSH) {
       R *r = myRed + 4 * ((e >> 16) & 0x7fff);
-      r[0] += t1; r[1] += t2; r[2] += t3; r[3] += t4;
+      r[0] += t3 - t4;
+      r[1] += eps * t2 - t1;
+      r[2] += t3 - t4 / eps;
+      r[3] += t2 - t1;
       t1 = t2 = t3 = t4 = 0;
     }
   };
@@ -977,16 +981,13 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
     for (int q = 0; q < 4; q++) {
       const int i = lane + 64 * q;
       if (i < nm) {
-        R2 x;
+        R2 x, g;
         x.x = sg[q] * ma[q];
         x.y = sg[q] * mb[q];
         cc[i] = x;
-        R4 y;
-        y.x = ga[q] - gb[q];
-        y.y = eps * gb[q] - ga[q];
-        y.z = ga[q] - gb[q] / eps;
-        y.w = 0;
-        vv[i] = y;
+        g.x = ga[q];
+        g.y = gb[q];
+        gg[i] = g;
       }
     }
     waveSync();
@@ -1956,7 +1957,7 @@ static size_t m2lPlanLds(const fbh_ctx *c, const std::vector<int4> &off, int wav
   for (int l = c->height; l <= c->depth; l++) {
     const int p = c->ordM2L[l], nm = NMOM(p);
     const int4 po = off[p];
-    size_t b = ((size_t)waves * 6 * ((nm + 1) & ~1) + (size_t)waves * 4 * po.w) * elem + (size_t)po.y * sizeof(unsigned) +
+    size_t b = ((size_t)waves * 4 * ((nm + 1) & ~1) + (size_t)waves * 4 * po.w) * elem + (size_t)po.y * sizeof(unsigned) +
                (size_t)(65 + nm + 1) * sizeof(int);
     best = std::max(best, b);
   }
