@@ -213,6 +213,17 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(base)
 
+    # measured HBM traffic per k_near_apply launch on this workload (rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE, scaled by a 1 GiB copy), from the committed counter summary
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_near_apply_traffic_globule81920.md")
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("{") and '"fetch_kib"' in line:
+                t = json.loads(line)
+                if t["interactions"] == st["near_interactions"]:
+                    traffic = t["fetch_kib"] * 1024 * t["fetch_scale"] + t["write_kib"] * 1024 * t["write_scale"]
+
     out = {
         "metric": METRIC,
         "value": args.steps / dt,
@@ -231,7 +242,7 @@ def main():
                    "panels": tree.n, "charges": int(len(mesh.chr)), "depth": DEPTH, "order": ORDER, "mac": MAC,
                    "near_interactions": st["near_interactions"], "m2l_pairs": st["m2l_pairs"]},
         "roofline": {"bound": "hbm", "kernel": "k_near_apply", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes": near_bytes, "kernel_ms": near_ms},
         "cpu_baseline": cpu,
         "stages_ms": stages,
