@@ -126,6 +126,8 @@ def hip_lib():
                                        _ip, _ip, _ip, _ip, C.c_double, _vp]
         L.fbh_profile.restype = C.c_int
         L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
+        L.fbh_profile32.restype = C.c_int
+        L.fbh_profile32.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
         L.fbh_stats.restype = C.c_int
         L.fbh_stats.argtypes = [_vp, C.POINTER(C.c_longlong)]
         L.fbh_stream.restype = _vp
@@ -402,10 +404,10 @@ class Operator:
             _i(keep["chgStart"]), _i(keep["nChgs"]), _i(keep["chgIdx"]), _i(keep["order"]),
             theta, C.c_void_p(b.data_ptr())))
 
-    def profile(self, x, y, iters=10):
+    def profile(self, x, y, iters=10, precision="fp64"):
         ms = np.zeros(8)
-        self._ck(hip_lib().fbh_profile(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
-                                       iters, _d(ms)))
+        f = hip_lib().fbh_profile32 if precision == "fp32" else hip_lib().fbh_profile
+        self._ck(f(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), iters, _d(ms)))
         return dict(zip(["near", "q2m", "m2m", "m2l", "l2l", "l2p", "combine", "total"], ms))
 
 

@@ -199,6 +199,8 @@ int fbh_rhs_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *c
 /* per-stage device timing over `iters` matvecs (HIP events on the context stream):
  * ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole matvec */
 int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
+/* the same for the FP32 operator (after fbh_enable_fp32) */
+int fbh_profile32(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
 
 /* sizes: out[0]=n, [1]=nearfield interactions, [2]=M2L pairs, [3]=nMom(maxOrder),
  * [4]=nMom(ordMom[depth]), [5]=device bytes held, [6]=M2M/L2L links, [7]=nCube */
