@@ -124,6 +124,15 @@ def hip_lib():
         L.fbh_rhs_tree_dev.restype = C.c_int
         L.fbh_rhs_tree_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip,
                                        _ip, _ip, _ip, _ip, C.c_double, _vp]
+        L.fbh_matvec_t_dev.restype = C.c_int
+        L.fbh_matvec_t_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp]
+        L.fbh_gmres_t_dev.restype = C.c_int
+        L.fbh_gmres_t_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _ip, _dp, _dp, _ip]
+        L.fbh_energy_direct_dev.restype = C.c_int
+        L.fbh_energy_direct_dev.argtypes = [_vp, C.c_int, _dp, _dp, _vp, _vp, _dp]
+        L.fbh_gradient_dev.restype = C.c_int
+        L.fbh_gradient_dev.argtypes = [_vp, C.c_int, _dp, _dp, _vp, _vp, C.c_int, C.c_double, C.c_int,
+                                       _dp, _dp, _ip, _dp, _ip]
         L.fbh_profile.restype = C.c_int
         L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
         L.fbh_profile32.restype = C.c_int
@@ -367,6 +376,41 @@ class Operator:
         return dict(iters=it.value, resid=res.value, info=info.value, outer=outer.value, mv64=mv64.value,
                     hist=hist[:outer.value])
 
+    def matvec_t_dev(self, x, y, alpha=1.0, beta=0.0):
+        """y = alpha A^T x + beta y (the adjoint operator)."""
+        self._ck(hip_lib().fbh_matvec_t_dev(self.h, alpha, C.c_void_p(x.data_ptr()), beta,
+                                            C.c_void_p(y.data_ptr())))
+
+    def gmres_t_dev(self, b, x, restart=30, tol=1e-6, maxit=100):
+        it, res, info = C.c_int(0), C.c_double(0.0), C.c_int(0)
+        hist = np.zeros(maxit + 2)
+        self._ck(hip_lib().fbh_gmres_t_dev(self.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+                                           restart, tol, maxit, C.byref(it), C.byref(res), _d(hist),
+                                           C.byref(info)))
+        return dict(iters=it.value, resid=res.value, info=info.value)
+
+    def energy_direct_dev(self, pos, chr_, sgm, g=None):
+        """Raw direct energy g . sgm (times 2 pi 332.0716 for kcal/mol); g (2n tensor) optional out."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        e = C.c_double(0.0)
+        self._ck(hip_lib().fbh_energy_direct_dev(
+            self.h, len(chr_), _d(pos), _d(chr_), C.c_void_p(sgm.data_ptr()),
+            C.c_void_p(g.data_ptr()) if g is not None else None, C.byref(e)))
+        return e.value
+
+    def gradient_dev(self, pos, chr_, sgm, lam, restart=30, tol=1e-6, maxit=100):
+        """Adjoint gradient of the raw direct energy: (dE/dq [nChar], dE/dkappa, solve info); lam in/out."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        dq = np.zeros(len(chr_))
+        dk, res = C.c_double(0.0), C.c_double(0.0)
+        it, info = C.c_int(0), C.c_int(0)
+        self._ck(hip_lib().fbh_gradient_dev(
+            self.h, len(chr_), _d(pos), _d(chr_), C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()),
+            restart, tol, maxit, _d(dq), C.byref(dk), C.byref(it), C.byref(res), C.byref(info)))
+        return dq, dk.value, dict(iters=it.value, resid=res.value, info=info.value)
+
     def rhs_dev(self, pos, chr_, b):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
@@ -409,6 +453,82 @@ class Operator:
         f = hip_lib().fbh_profile32 if precision == "fp32" else hip_lib().fbh_profile
         self._ck(f(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), iters, _d(ms)))
         return dict(zip(["near", "q2m", "m2m", "m2l", "l2l", "l2p", "combine", "total"], ms))
+
+
+class Problem:
+    """Differentiation form: a fixed mesh and tree; E(q, kappa) through the direct
+    right-hand side, GMRES on the FP64 operator and the direct energy
+    (Operator.energy_direct_dev), and its adjoint gradient (Operator.gradient_dev).
+    Operators are built per kappa and kept (the nearfield coefficients and M2L
+    tensors depend on kappa)."""
+
+    def __init__(self, mesh, depth, order=6, sep=0.8, eps1=1.0, eps2=80.0, qorder=1, height=2, tol=1e-6,
+                 restart=30, maxit=100, precond=3):
+        self.mesh = mesh
+        self.tree = Tree(mesh, depth, order=order, height=height, sep=sep)
+        self.eps1, self.eps2, self.qorder = eps1, eps2, qorder
+        self.tol, self.restart, self.maxit, self.precond = tol, restart, maxit, precond
+        self._ops = {}
+
+    def operator(self, kappa):
+        kappa = float(kappa)
+        if kappa not in self._ops:
+            op = Operator(self.tree, kappa, self.eps1, self.eps2, self.qorder)
+            op.set_precond(self.precond)
+            self._ops[kappa] = op
+        return self._ops[kappa]
+
+    def solve(self, q, kappa):
+        """Solution u (device tensor) and the raw direct energy for charges q."""
+        torch = _torch()
+        op = self.operator(kappa)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        b = torch.empty(2 * self.tree.n, dtype=torch.float64, device="cuda")
+        op.rhs_dev(self.mesh.pos, q, b)
+        x = b.clone()
+        g = op.gmres_dev(b, x, restart=self.restart, tol=self.tol, maxit=self.maxit)
+        if g["info"] != 0:
+            raise RuntimeError(f"forward GMRES did not converge: {g}")
+        return x, op.energy_direct_dev(self.mesh.pos, q, x), g
+
+    def energy(self, q, kappa):
+        """Solvation energy in kcal/mol."""
+        return self.solve(q, kappa)[1] * ENERGY_SCALE
+
+    def gradient(self, q, kappa, u=None):
+        """(E, dE/dq, dE/dkappa) in kcal/mol units; u: a solution already at hand."""
+        torch = _torch()
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if u is None:
+            u, e, _ = self.solve(q, kappa)
+        else:
+            e = self.operator(kappa).energy_direct_dev(self.mesh.pos, q, u)
+        lam = torch.zeros_like(u)
+        dq, dk, info = self.operator(kappa).gradient_dev(self.mesh.pos, q, u, lam, restart=self.restart,
+                                                         tol=self.tol, maxit=self.maxit)
+        if info["info"] != 0:
+            raise RuntimeError(f"adjoint GMRES did not converge: {info}")
+        return e * ENERGY_SCALE, dq * ENERGY_SCALE, dk * ENERGY_SCALE
+
+
+def solvation_energy(problem, q, kappa):
+    """E(q, kappa) as a torch function with the adjoint gradient as its backward:
+    q a float64 tensor of charges, kappa a float64 scalar tensor."""
+    torch = _torch()
+
+    class _SolvationEnergy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, qt, kt):
+            u, e, _ = problem.solve(qt.detach().cpu().numpy(), float(kt))
+            ctx.u, ctx.q, ctx.kappa = u, qt.detach().cpu().numpy(), float(kt)
+            return torch.tensor(e * ENERGY_SCALE, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gE):
+            _, dq, dk = problem.gradient(ctx.q, ctx.kappa, u=ctx.u)
+            return gE * torch.tensor(dq, dtype=torch.float64), gE * torch.tensor(dk, dtype=torch.float64)
+
+    return _SolvationEnergy.apply(q, kappa)
 
 
 def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,

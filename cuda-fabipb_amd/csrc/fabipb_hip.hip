@@ -156,10 +156,40 @@ __device__ __forceinline__ void ker4(double kappa, double eps, const double *x, 
   y[3] = ddGk - ddG0;
 }
 
+// d/dkappa of the four kernelKER4 components (the Galerkin rules are linear
+// in the kernel, so the same rules give d KER / d kappa):
+//   d(G0 - Gk) = e^{-kappa r} / (4 pi),  d coef = -kappa r^2 e^{-kappa r},
+//   d y1 = eps dcoef dG0dy,  d y2 = dcoef dG0dx / eps,
+//   d y3 = dcoef ddG0 - kappa e^{-kappa r} (2 - kappa r) (n_x.r) dG0dy
+__device__ __forceinline__ void ker4dk(double kappa, double eps, const double *x, const double *nX,
+                                       const double *nY, double *y) {
+  double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  double r = sqrt(r2);
+  double expKa = exp(-kappa * r);
+  double ip0 = nX[0] * nY[0] + nX[1] * nY[1] + nX[2] * nY[2];
+  double ipX = nX[0] * x[0] + nX[1] * x[1] + nX[2] * x[2];
+  double ipY = nY[0] * x[0] + nY[1] * x[1] + nY[2] * x[2];
+  double G0 = 1.0 / r * kFourPiI;
+  double dG0dy = ipY * G0 / r2;
+  double dG0dx = ipX * G0 / r2;
+  double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) / r2;
+  double dcoef = -kappa * r2 * expKa;
+  y[0] = expKa * kFourPiI;
+  y[1] = eps * dcoef * dG0dy;
+  y[2] = dcoef * dG0dx / eps;
+  y[3] = dcoef * ddG0 - kappa * expKa * (2.0 - kappa * r) * ipX * dG0dy;
+}
+
 struct KC {
   double kappa, eps;
   const double *nX, *nY;
+  bool dk;   // integrate d ker4 / d kappa instead of ker4
 };
+
+__device__ __forceinline__ void kerK(const KC &K, const double *x, double *y) {
+  if (K.dk) ker4dk(K.kappa, K.eps, x, K.nX, K.nY, y);
+  else ker4(K.kappa, K.eps, x, K.nX, K.nY, y);
+}
 
 // src/numQuad.c:586-618 (pnlNil0): disjoint panels, Duffy squares
 __device__ void dNil0(const KC &K, const QTab &Q, int q, const double *r0, const double *ax2, const double *ax0,
@@ -173,7 +203,7 @@ __device__ void dNil0(const KC &K, const QTab &Q, int q, const double *r0, const
       for (int a = 0; a < q; a++)
         for (int b = 0; b < q; b++) {
           for (int k = 0; k < 3; k++) r[k] = r1[k] - Q.t[a] * (ay1[k] + Q.t[b] * ay2[k]);
-          ker4(K.kappa, K.eps, r, K.nX, K.nY, f);
+          kerK(K, r, f);
           for (int i = 0; i < 4; i++) iy[i] += f[i] * Q.t[a] * Q.w[a] * Q.w[b];
         }
       for (int i = 0; i < 4; i++) in[i] += iy[i] * Q.t[ix] * Q.w[ix] * Q.w[jx];
@@ -198,8 +228,8 @@ __device__ void dOne0(const KC &K, const QTab &Q, int q, const double *a2, const
             r1[k] = omg * (a2[k] + eta2 * a0[k] - eta1 * (b2[k] + eta3 * b0[k]));
             r2[k] = omg * (eta1 * (a2[k] + eta3 * a0[k]) - (b2[k] + eta2 * b0[k]));
           }
-          ker4(K.kappa, K.eps, r1, K.nX, K.nY, f1);
-          ker4(K.kappa, K.eps, r2, K.nX, K.nY, f2);
+          kerK(K, r1, f1);
+          kerK(K, r2, f2);
           for (int i = 0; i < 4; i++) iy[i] += (f1[i] + f2[i]) * Q.w[c] * wij;
         }
       }
@@ -233,7 +263,7 @@ __device__ void dTwo0(const KC &K, const QTab &Q, int q, const double *a, const 
           double s[4] = {0, 0, 0, 0};
           const int ord[6] = {0, 2, 3, 1, 4, 5};
           for (int m = 0; m < 6; m++) {
-            ker4(K.kappa, K.eps, r[ord[m]], K.nX, K.nY, f);
+            kerK(K, r[ord[m]], f);
             for (int i = 0; i < 4; i++) s[i] = (m == 0) ? f[i] : s[i] + f[i];
           }
           for (int i = 0; i < 4; i++) iy[i] += (s[i] * eta1) * Q.w[ia] * Q.w[jy];
@@ -265,7 +295,7 @@ __device__ void dThr0(const KC &K, const QTab &Q, int q, const double *a2, const
           double s[4] = {0, 0, 0, 0};
           const int ord[6] = {0, 2, 3, 1, 4, 5};
           for (int m = 0; m < 6; m++) {
-            ker4(K.kappa, K.eps, r[ord[m]], K.nX, K.nY, f);
+            kerK(K, r[ord[m]], f);
             for (int i = 0; i < 4; i++) s[i] = (m == 0) ? f[i] : s[i] + f[i];
           }
           for (int i = 0; i < 4; i++) iy[i] += s[i] * Q.w[d];
@@ -300,8 +330,8 @@ __device__ int commonVtx(const double *p, const double *q, int *ix, int *iy) {
 
 // src/numQuad.c:629-697 (panelIA0): KER = 4 A_x A_y * (rule for the case)
 __device__ void panelIA0(const Pnl &X, const Pnl &Y, bool same, double kappa, double eps, const QTab &Q, int q,
-                         double *out) {
-  KC K = {kappa, eps, X.nr, Y.nr};
+                         double *out, bool dk = false) {
+  KC K = {kappa, eps, X.nr, Y.nr, dk};
   int ix[3], iy[3];
   int nv = same ? 3 : commonVtx(X.v, Y.v, ix, iy);
   double in[4];
@@ -344,6 +374,30 @@ __global__ __launch_bounds__(256) void k_near_build(const Pnl *__restrict__ pnl,
     double k[4];
     panelIA0(X, Y, x == y, kappa, eps, Q, q, k);
     out[j] = make_double4(k[0], k[1], k[2], k[3]);
+  }
+}
+
+// The transposed nearfield in the same row layout: row r, column c holds the
+// 2x2 block of A^T, i.e. of KER = panelIA0(X = c, Y = r) read transposed:
+//   out[r] += KER3 sgm_n[c] + KER1 sgm[c],  out[r+n] += KER2 sgm_n[c] + KER0 sgm[c]
+// (valid because the near leaf lists are symmetric: c is near r's leaf
+// exactly when r is near c's leaf).
+__global__ __launch_bounds__(256) void k_near_build_t(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
+                                                      const int *__restrict__ leafStart,
+                                                      const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                                      const long long *__restrict__ coefOff,
+                                                      const int *__restrict__ srcList, double4 *__restrict__ coef,
+                                                      double kappa, double eps, QTab Q, int q) {
+  const int r = blockIdx.x, D = leafOf[r], row = r - leafStart[D], ncol = ncolArr[D];
+  const Pnl R = pnl[r];
+  double4 *out = coef + coefOff[D] + (long long)row * ncol;
+  const int *src = srcList + srcOff[D];
+  for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
+    const int c = src[j];
+    const Pnl C = pnl[c];
+    double k[4];
+    panelIA0(C, R, c == r, kappa, eps, Q, q, k);
+    out[j] = make_double4(k[3], k[1], k[2], k[0]);
   }
 }
 
@@ -396,6 +450,16 @@ __global__ void k_bp_unpad(const int *__restrict__ grpBlock, const int *__restri
   double *o = inv + bOff[b];
   for (int r = blockIdx.y; r < M; r += gridDim.y)
     for (int c = threadIdx.x; c < M; c += blockDim.x) o[(long long)r * M + c] = A[(long long)r * Mp + c];
+}
+
+// the transposed inverse of every block (for the adjoint solve)
+__global__ void k_bp_transpose(const int *__restrict__ bM, const long long *__restrict__ bOff,
+                               const double *__restrict__ inv, double *__restrict__ invT) {
+  const int b = blockIdx.x, M = 2 * bM[b];
+  const double *a = inv + bOff[b];
+  double *t = invT + bOff[b];
+  for (int r = blockIdx.y; r < M; r += gridDim.y)
+    for (int c = threadIdx.x; c < M; c += blockDim.x) t[(long long)c * M + r] = a[(long long)r * M + c];
 }
 
 // y = M^-1 x per block, row-major inverse streamed once: tile = (block, first
@@ -581,29 +645,20 @@ __global__ __launch_bounds__(256) void k_spec_count(const Pnl *__restrict__ pnl,
   if (threadIdx.x == 0) cnt[x] = part[0];
 }
 
-__global__ __launch_bounds__(256) void k_spec_fill(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
-                                                   const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
-                                                   const int *__restrict__ srcList,
-                                                   const long long *__restrict__ specOff, int *__restrict__ specCol,
-                                                   float4 *__restrict__ specVal, double kappa, double eps, QTab Q,
-                                                   int q) {
+// the column indices of each row's corrected pairs, in near-list order
+__global__ __launch_bounds__(256) void k_spec_index(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
+                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                                    const int *__restrict__ srcList,
+                                                    const long long *__restrict__ specOff, int *__restrict__ specCol) {
   __shared__ int scan[256];
   const int x = blockIdx.x, D = leafOf[x], nc = ncolArr[D];
   const int *src = srcList + srcOff[D];
   const Pnl X = pnl[x];
-  const double t = Q.t[0];
-  double px[3];
-  for (int k = 0; k < 3; k++) px[k] = X.v[k] + t * (X.a[6 + k] + t * X.a[k]);
   long long base = specOff[x];
   for (int j0 = 0; j0 < nc; j0 += 256) {   // ordered compaction, 256 columns at a time
     const int j = j0 + threadIdx.x;
     const int y = j < nc ? src[j] : 0;
-    Pnl Y;
-    bool sp = false;
-    if (j < nc) {
-      Y = pnl[y];
-      sp = flySpecial(X, Y, x, y);
-    }
+    const bool sp = j < nc && flySpecial(X, pnl[y], x, y);
     scan[threadIdx.x] = sp ? 1 : 0;
     __syncthreads();
     for (int o = 1; o < 256; o <<= 1) {   // inclusive scan
@@ -612,21 +667,39 @@ __global__ __launch_bounds__(256) void k_spec_fill(const Pnl *__restrict__ pnl, 
       scan[threadIdx.x] += v;
       __syncthreads();
     }
-    if (sp) {
-      double k[4], f[4] = {0, 0, 0, 0};
-      panelIA0(X, Y, x == y, kappa, eps, Q, q, k);
-      if (x != y) {
-        double r[3];
-        for (int d = 0; d < 3; d++) r[d] = px[d] - (Y.v[d] + t * (Y.a[6 + d] + t * Y.a[d]));
-        ker4(kappa, eps, r, X.nr, Y.nr, f);
-        for (int i = 0; i < 4; i++) f[i] *= X.area * Y.area;
-      }
-      const long long at = base + scan[threadIdx.x] - 1;
-      specCol[at] = y;
-      specVal[at] = make_float4((float)(k[0] - f[0]), (float)(k[1] - f[1]), (float)(k[2] - f[2]), (float)(k[3] - f[3]));
-    }
+    if (sp) specCol[base + scan[threadIdx.x] - 1] = y;
     base += scan[255];
     __syncthreads();
+  }
+}
+
+// correction of each listed pair: the Galerkin rule minus the one-point
+// formula at q = 1 (the whole rule for the panel itself); DK: of d/dkappa.
+// One block per row; the result goes to v32 (FP32) or v64.
+template <bool DK>
+__global__ __launch_bounds__(64) void k_spec_vals(const Pnl *__restrict__ pnl, const long long *__restrict__ specOff,
+                                                  const int *__restrict__ specCol, float4 *__restrict__ v32,
+                                                  double4 *__restrict__ v64, double kappa, double eps, QTab Q,
+                                                  int q) {
+  const int x = blockIdx.x;
+  const Pnl X = pnl[x];
+  const double t = Q.t[0];
+  double px[3];
+  for (int k = 0; k < 3; k++) px[k] = X.v[k] + t * (X.a[6 + k] + t * X.a[k]);
+  for (long long e = specOff[x] + threadIdx.x; e < specOff[x + 1]; e += 64) {
+    const int y = specCol[e];
+    const Pnl Y = pnl[y];
+    double k[4], f[4] = {0, 0, 0, 0};
+    panelIA0(X, Y, x == y, kappa, eps, Q, q, k, DK);
+    if (x != y) {
+      double r[3];
+      for (int d = 0; d < 3; d++) r[d] = px[d] - (Y.v[d] + t * (Y.a[6 + d] + t * Y.a[d]));
+      if (DK) ker4dk(kappa, eps, r, X.nr, Y.nr, f);
+      else ker4(kappa, eps, r, X.nr, Y.nr, f);
+      for (int i = 0; i < 4; i++) f[i] *= X.area * Y.area;
+    }
+    if (v32) v32[e] = make_float4((float)(k[0] - f[0]), (float)(k[1] - f[1]), (float)(k[2] - f[2]), (float)(k[3] - f[3]));
+    else v64[e] = make_double4(k[0] - f[0], k[1] - f[1], k[2] - f[2], k[3] - f[3]);
   }
 }
 
@@ -784,11 +857,19 @@ __device__ void radial(double kappa, double r, int p, double *G0, int s0, double
 // computed by the whole block degree by degree: block d holds entries
 // (q, j), q = 0..p-d, j < (d+1)(d+2)/2, at offset off(d) + q*len(d) + j.
 // Leaves dG[0][alpha] in o0/ok.  Ends with a barrier.
+// dk: o0 receives d/dkappa of the Gk tensors instead of the G0 tensors.  The
+// radial seeds f_q = ((1/r) d/dr)^q Gk satisfy d f_q / d kappa = kappa f_(q-1)
+// for q >= 1 (d/dkappa commutes with (1/r) d/dr and (1/r) d/dr e^{-kappa r}
+// = -kappa e^{-kappa r}/r), and d f_0 / d kappa = -e^{-kappa r}/(4 pi).
 __device__ void derivs_block(double kappa, int p, const double *r, double *s0, double *sk, double *o0, double *ok,
-                             const int4 *mi, int tid, int nt) {
+                             const int4 *mi, int tid, int nt, bool dk = false) {
   if (tid == 0) {
     double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     radial(kappa, rn, p, s0, 1, sk, 1);
+    if (dk) {
+      for (int q = p; q >= 1; q--) s0[q] = kappa * sk[q - 1];
+      s0[0] = -exp(-kappa * rn) * kFourPiI;
+    }
     o0[0] = s0[0];
     ok[0] = sk[0];
   }
@@ -911,13 +992,14 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_q2m(const R *__restrict__ Q0, const R *__restrict__ Q1,
                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
                                              const double *__restrict__ x, R *__restrict__ mom, int n, int nm,
-                                             int nmMax) {
+                                             int nmMax, int swap) {
   R *sm = reinterpret_cast<R *>(dynLds);   // [4 waves][2][nm]
   const int L = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = leafStart[L], c = leafCount[L];
+  const int oP = swap ? n : 0, oN = swap ? 0 : n;   // swap: the transposed L2P (Q1 x_n, Q0 x)
   R m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
   for (int j = w; j < c; j += 4) {
-    const R sp = (R)x[s + j], sn = (R)x[s + j + n];
+    const R sp = (R)x[s + j + oP], sn = (R)x[s + j + oN];
     const R *q1 = Q1 + (long long)(s + j) * nm, *q0 = Q0 + (long long)(s + j) * nm;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -1112,7 +1194,13 @@ template <> struct Vec2<float> { typedef float2 type; };
 // MAXT > 0: every lane's term list (at most MAXT entries, the same for every
 // pair) is held in registers and the contraction is unrolled, so the LDS reads
 // of different terms are in flight together; MAXT = 0 reads the list from LDS
-template <typename R, int MAXT>
+// TR: the transposed pass of the adjoint operator.  mom then holds the
+// adjoint locals (b = Q1 y_n | a = Q0 y, carried up the tree) and lec receives
+// the adjoint moments (M_pot | M_dpdn), two arrays per cube:
+//   M_pot += a*(eps Gk - G0) + b*(Gk - G0),  M_dpdn += a*(G0 - Gk) + b*(G0 - Gk/eps)
+// with the convolutions taken over the cube's own list (the lists are
+// symmetric and D^a G(x_S - x_D) = sgn_a D^a G(x_D - x_S)).
+template <typename R, int MAXT, bool TR = false>
 __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
                                                     const int *__restrict__ cubeLevel,
                                                     const int *__restrict__ ordM2L, const int *__restrict__ ordMom,
@@ -1180,10 +1268,15 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
     t4 += x.y * g.y;
     if (e & M2L_FLUSH) {
       R *r = myRed + 4 * ((e >> 16) & 0x7fff);
-      r[0] += t3 - t4;
-      r[1] += eps * t2 - t1;
-      r[2] += t3 - t4 / eps;
-      r[3] += t2 - t1;
+      if constexpr (TR) {   // x = (sgn b, sgn a): t1 = b*G0, t2 = b*Gk, t3 = a*G0, t4 = a*Gk
+        r[0] += eps * t4 - t3 + t2 - t1;
+        r[1] += t3 - t4 + t1 - t2 / eps;
+      } else {
+        r[0] += t3 - t4;
+        r[1] += eps * t2 - t1;
+        r[2] += t3 - t4 / eps;
+        r[3] += t2 - t1;
+      }
       t1 = t2 = t3 = t4 = 0;
     }
   };
@@ -1219,19 +1312,22 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   __syncthreads();
   const int nmc = NMOM(ordMom[cubeLevel[c]]);
   const int *itemStart = idx + 65;
-  for (int i = tid; i < 4 * nmMax; i += blockDim.x) {
+  constexpr int NO = TR ? 2 : 4;
+  for (int i = tid; i < NO * nmMax; i += blockDim.x) {
     const int k = i / nmMax, a = i - k * nmMax;
     R v = 0;
     if (a < nm && a < nmc)
       for (int it = itemStart[a]; it < itemStart[a + 1]; it++)
         for (int ww = 0; ww < nw; ww++) v += red[(ww * nItems + it) * 4 + k];
-    lec[((long long)c * 4 + k) * nmMax + a] = v;
+    lec[((long long)c * NO + k) * nmMax + a] = v;
   }
 }
 
 // src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child,
 // thread per output multi-index, (beta, alpha+beta) table in LDS
-template <typename R>
+// NA = 4: the local expansions l1..l4; NA = 2: the transposed M2M of the
+// adjoint operator (same translation, two arrays)
+template <typename R, int NA = 4>
 __global__ __launch_bounds__(512) void k_l2l(int levStart, const int *__restrict__ parent,
                                              const int *__restrict__ cubeLevel, const int *__restrict__ ordMom,
                                              const double *__restrict__ cubeX, R *__restrict__ lec, int nmMax,
@@ -1249,25 +1345,27 @@ __global__ __launch_bounds__(512) void k_l2l(int levStart, const int *__restrict
   for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
   for (int a = threadIdx.x; a < nIn; a += blockDim.x) {
     Rt[a] = (R)conVect1(tr, T.mi[a], T.ifact3[a]);
-    for (int k = 0; k < 4; k++) In[k * nIn + a] = lec[((long long)par * 4 + k) * nmMax + a];
+    for (int k = 0; k < NA; k++) In[k * nIn + a] = lec[((long long)par * NA + k) * nmMax + a];
   }
   __syncthreads();
   for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
     const int cnt = NMOM(ordIn - deg_of(T.mi[a]));
     const int *cv = tab + cst[a];
-    R t[4] = {0, 0, 0, 0};
+    R t[NA] = {};
     for (int e = 0; e < cnt; e++) {
       int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
       R rv = Rt[j];
-      for (int k = 0; k < 4; k++) t[k] += rv * In[k * nIn + kk];
+      for (int k = 0; k < NA; k++) t[k] += rv * In[k * nIn + kk];
     }
-    for (int k = 0; k < 4; k++) lec[((long long)kid * 4 + k) * nmMax + a] += t[k];
+    for (int k = 0; k < NA; k++) lec[((long long)kid * NA + k) * nmMax + a] += t[k];
   }
 }
 
 // far[p] = Q2M0^T (l1 + l2), far[p+n] = Q2M1^T (l3 + l4) (src/fmm.c:815-828):
 // one block per leaf, eight lanes per panel read its column in 64-byte pieces
-template <typename R>
+// TR: the transposed Q2M of the adjoint operator, from the adjoint moments
+// (M_pot | M_dpdn): far[p] = Q2M1^T M_pot, far[p+n] = Q2M0^T M_dpdn
+template <typename R, bool TR = false>
 __global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *__restrict__ Q1,
                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
                                              const R *__restrict__ lec, R *__restrict__ far, int n, int nm,
@@ -1275,10 +1373,18 @@ __global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *
   R *sm = reinterpret_cast<R *>(dynLds);
   R *L12 = sm, *L34 = sm + nm;
   const int L = blockIdx.x;
-  const R *l = lec + (long long)L * 4 * nmMax;
-  for (int a = threadIdx.x; a < nm; a += blockDim.x) {
-    L12[a] = l[a] + l[nmMax + a];
-    L34[a] = l[2 * nmMax + a] + l[3 * nmMax + a];
+  if constexpr (TR) {
+    const R *l = lec + (long long)L * 2 * nmMax;
+    for (int a = threadIdx.x; a < nm; a += blockDim.x) {
+      L12[a] = l[nmMax + a];
+      L34[a] = l[a];
+    }
+  } else {
+    const R *l = lec + (long long)L * 4 * nmMax;
+    for (int a = threadIdx.x; a < nm; a += blockDim.x) {
+      L12[a] = l[a] + l[nmMax + a];
+      L34[a] = l[2 * nmMax + a] + l[3 * nmMax + a];
+    }
   }
   __syncthreads();
   const int g = threadIdx.x >> 3, l8 = threadIdx.x & 7, s = leafStart[L], c = leafCount[L];
@@ -1297,8 +1403,8 @@ __global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *
       t1 += __shfl_xor(t1, o, 64);
     }
     if (j < c && l8 == 0) {
-      far[s + j] = t0;
-      far[s + j + n] = t1;
+      far[s + j + (TR ? n : 0)] = t0;
+      far[s + j + (TR ? 0 : n)] = t1;
     }
   }
 }
@@ -1408,6 +1514,279 @@ __global__ __launch_bounds__(256) void k_rhs_q1(const Pnl *__restrict__ pnl, int
     b[i] = b0 * W;
     b[i + n] = b1 * W;
   }
+}
+
+// ------------------------------------------------ differentiation form
+// Direct energy weights: the panel-tree energy (src/treecode.c:581-695,
+// enDirect below) with every charge taken directly, as a vector,
+//   E_raw = g . [sgm | sgm_n],  g[p] = 2A sum_pts t w w y1,  g[p+n] = 2A sum_pts t w w y0,
+// y0 = sum_j q_j (G0 - Gk), y1 = sum_j q_j (eps dGk/dn - dG0/dn); gk = dg/dkappa:
+//   d(G0 - Gk)/dkappa = e^{-kappa r}/(4 pi),  d(eps dGk/dn)/dkappa = eps dG0/dn (-kappa r^2 e^{-kappa r})
+__global__ __launch_bounds__(256) void k_energy_g(const Pnl *__restrict__ pnl, int n, int nChar,
+                                                  const double *__restrict__ pos, const double *__restrict__ chr,
+                                                  double *__restrict__ g, double *__restrict__ gk, double kappa,
+                                                  double eps, QTab Q, int q) {
+  __shared__ double4 sp[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const Pnl P = pnl[min(i, n - 1)];
+  double o0 = 0.0, o1 = 0.0, d0 = 0.0, d1 = 0.0;
+  for (int j0 = 0; j0 < nChar; j0 += 256) {
+    __syncthreads();
+    const int j = j0 + threadIdx.x;
+    if (j < nChar) sp[threadIdx.x] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], chr[j]);
+    __syncthreads();
+    const int cnt = min(256, nChar - j0);
+    for (int ix = 0; ix < q; ix++)
+      for (int jx = 0; jx < q; jx++) {
+        double qp[3];
+        for (int k = 0; k < 3; k++) qp[k] = P.v[k] + Q.t[ix] * (P.a[6 + k] + Q.t[jx] * P.a[k]);
+        const double wq = Q.t[ix] * Q.w[ix] * Q.w[jx];
+        double y0 = 0.0, y1 = 0.0, e0 = 0.0, e1 = 0.0;
+        for (int jj = 0; jj < cnt; jj++) {
+          const double4 c = sp[jj];
+          const double x0 = qp[0] - c.x, x1 = qp[1] - c.y, x2 = qp[2] - c.z;
+          const double r2 = x0 * x0 + x1 * x1 + x2 * x2, rn = sqrt(r2);
+          const double G0 = kFourPiI / rn, ek = exp(-kappa * rn), Gk = ek * G0;
+          const double coef = (kappa * rn + 1.0) * ek;
+          const double ip = P.nr[0] * x0 + P.nr[1] * x1 + P.nr[2] * x2;
+          const double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+          y0 += c.w * (G0 - Gk);
+          y1 += c.w * (eps * dGkdn - dG0dn);
+          e0 += c.w * (ek * kFourPiI);
+          e1 += c.w * (eps * dG0dn * (-kappa * r2 * ek));
+        }
+        o0 += wq * y0;
+        o1 += wq * y1;
+        d0 += wq * e0;
+        d1 += wq * e1;
+      }
+  }
+  if (i < n) {
+    const double a2 = 2.0 * P.area;
+    g[i] = a2 * o1;
+    g[i + n] = a2 * o0;
+    if (gk) {
+      gk[i] = a2 * d1;
+      gk[i + n] = a2 * d0;
+    }
+  }
+}
+
+// dE/dq_j = (G^T u)_j + (R^T lam)_j: the energy weights of charge j against
+// the solution u = [sgm | sgm_n] plus the direct right-hand side
+// (src/numQuad.c:786-819, kernelRHS src/kernel.c:152-167) transposed against
+// the adjoint solution lam.  Block (charge tile, panel chunk); partial sums
+// part[chunk][j] are added in chunk order by k_grad_sum.
+#define GRAD_PCH 2048
+__global__ __launch_bounds__(256) void k_grad_q(const Pnl *__restrict__ pnl, int n, int nChar,
+                                                const double *__restrict__ pos, const double *__restrict__ u,
+                                                const double *__restrict__ lam, double *__restrict__ part,
+                                                double kappa, double eps, double fac, QTab Q, int q) {
+  __shared__ double sv[256][16];   // v0, a2, a0, n, 2A sgm, 2A sgm_n, fac 2A lam, fac 2A lam_n
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int p0 = blockIdx.y * GRAD_PCH, p1 = min(n, p0 + GRAD_PCH);
+  double c[3] = {0, 0, 0};
+  if (j < nChar)
+    for (int k = 0; k < 3; k++) c[k] = pos[3 * j + k];
+  double acc = 0.0;
+  for (int b0 = p0; b0 < p1; b0 += 256) {
+    __syncthreads();
+    const int pi = b0 + threadIdx.x;
+    if (pi < p1) {
+      const Pnl &P = pnl[pi];
+      double *v = sv[threadIdx.x];
+      for (int k = 0; k < 3; k++) {
+        v[k] = P.v[k];
+        v[3 + k] = P.a[6 + k];
+        v[6 + k] = P.a[k];
+        v[9 + k] = P.nr[k];
+      }
+      const double a2 = 2.0 * P.area;
+      v[12] = a2 * u[pi];
+      v[13] = a2 * u[pi + n];
+      v[14] = fac * a2 * lam[pi];
+      v[15] = fac * a2 * lam[pi + n];
+    }
+    __syncthreads();
+    const int cnt = min(256, p1 - b0);
+    for (int jj = 0; jj < cnt; jj++) {
+      const double *v = sv[jj];
+      for (int ix = 0; ix < q; ix++)
+        for (int jx = 0; jx < q; jx++) {
+          const double wq = Q.t[ix] * Q.w[ix] * Q.w[jx];
+          double x[3];
+          for (int k = 0; k < 3; k++) x[k] = v[k] + Q.t[ix] * (v[3 + k] + Q.t[jx] * v[6 + k]) - c[k];
+          const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2), ri = 1.0 / rn;
+          const double G0 = kFourPiI * ri, ek = exp(-kappa * rn), Gk = ek * G0;
+          const double coef = (kappa * rn + 1.0) * ek;
+          const double ip = v[9] * x[0] + v[10] * x[1] + v[11] * x[2];
+          const double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+          acc += wq * ((G0 - Gk) * v[13] + (eps * dGkdn - dG0dn) * v[12] + ri * v[14] - ip * ri * ri * ri * v[15]);
+        }
+    }
+  }
+  if (j < nChar) part[(long long)blockIdx.y * nChar + j] = acc;
+}
+
+__global__ void k_grad_sum(const double *__restrict__ part, int nChunk, int nChar, double *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nChar) return;
+  double v = 0.0;
+  for (int k = 0; k < nChunk; k++) v += part[(long long)k * nChar + j];
+  out[j] = v;
+}
+
+// near part of lam^T (dK/dkappa) u at q = 1: every source of the row's near
+// list but the panel itself by the one-point rule A_x A_y dker4(p_x - p_y),
+// FP64, plus the stored corrections of the self / shared-vertex pairs.
+// Tile as k_near_apply; part[x] = A_x (lam_x s0 + lam_{x+n} s1) + corrections.
+__global__ __launch_bounds__(256) void k_near_dk(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
+                                                 const int *__restrict__ leafCount, const int *__restrict__ srcOff,
+                                                 const int *__restrict__ ncolArr, const int *__restrict__ srcList,
+                                                 const Pnl *__restrict__ pnl, const double *__restrict__ u,
+                                                 const double *__restrict__ lam,
+                                                 const long long *__restrict__ specOff,
+                                                 const int *__restrict__ specCol,
+                                                 const double4 *__restrict__ specDk, double *__restrict__ part,
+                                                 int n, double kappa, double eps, double t) {
+  __shared__ double4 sP[256], sN[256];
+  __shared__ int sY[256];
+  const int2 tl = tiles[blockIdx.x];
+  const int D = tl.x, r0 = tl.y;
+  const int rows = min(NEAR_ROWS, leafCount[D] - r0);
+  const int ncol = ncolArr[D];
+  const int *src = srcList + srcOff[D];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
+  for (int c0 = 0; c0 < ncol; c0 += 256) {
+    const int cn = min(256, ncol - c0);
+    __syncthreads();
+    if (threadIdx.x < cn) {
+      const int y = src[c0 + threadIdx.x];
+      const Pnl &Y = pnl[y];
+      double q[3];
+      for (int k = 0; k < 3; k++) q[k] = Y.v[k] + t * (Y.a[6 + k] + t * Y.a[k]);
+      sP[threadIdx.x] = make_double4(q[0], q[1], q[2], Y.area * u[y]);
+      sN[threadIdx.x] = make_double4(Y.nr[0], Y.nr[1], Y.nr[2], Y.area * u[y + n]);
+      sY[threadIdx.x] = y;
+    }
+    __syncthreads();
+    for (int r = 0; r < 4; r++) {
+      const int row = w + 4 * r;
+      if (row >= rows) continue;
+      const int xi = leafStart[D] + r0 + row;
+      const Pnl &X = pnl[xi];
+      double px[3];
+      for (int k = 0; k < 3; k++) px[k] = X.v[k] + t * (X.a[6 + k] + t * X.a[k]);
+      double s0 = 0.0, s1 = 0.0;
+      for (int j = lane; j < cn; j += 64) {
+        if (sY[j] == xi) continue;
+        const double4 P = sP[j], N = sN[j];
+        const double rr[3] = {px[0] - P.x, px[1] - P.y, px[2] - P.z}, ny[3] = {N.x, N.y, N.z};
+        double f[4];
+        ker4dk(kappa, eps, rr, X.nr, ny, f);
+        s0 += f[0] * N.w + f[1] * P.w;
+        s1 += f[2] * N.w + f[3] * P.w;
+      }
+      a0[r] += s0;
+      a1[r] += s1;
+    }
+  }
+  for (int r = 0; r < 4; r++) {
+    const int row = w + 4 * r;
+    const int xi = leafStart[D] + r0 + min(row, rows - 1);
+    const double A = pnl[xi].area;
+    double t0 = A * a0[r], t1 = A * a1[r];
+    if (row < rows)
+      for (long long e = specOff[xi] + lane; e < specOff[xi + 1]; e += 64) {
+        const int y = specCol[e];
+        const double4 d = specDk[e];
+        t0 += d.x * u[y + n] + d.y * u[y];
+        t1 += d.z * u[y + n] + d.w * u[y];
+      }
+    const double v = waveSumT(lam[xi] * t0 + lam[xi + n] * t1);
+    if (row < rows && lane == 0) part[xi] = v;
+  }
+}
+
+// the same over every near pair with the full Galerkin rules (any q)
+__global__ __launch_bounds__(256) void k_near_dk_rules(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
+                                                       const int *__restrict__ srcOff,
+                                                       const int *__restrict__ ncolArr,
+                                                       const int *__restrict__ srcList, const double *__restrict__ u,
+                                                       const double *__restrict__ lam, double *__restrict__ part,
+                                                       int n, double kappa, double eps, QTab Q, int q) {
+  __shared__ double red[256];
+  const int x = blockIdx.x, D = leafOf[x], nc = ncolArr[D];
+  const int *src = srcList + srcOff[D];
+  const Pnl X = pnl[x];
+  double v = 0.0;
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+    const int y = src[j];
+    const Pnl Y = pnl[y];
+    double k[4];
+    panelIA0(X, Y, x == y, kappa, eps, Q, q, k, true);
+    v += lam[x] * (k[0] * u[y + n] + k[1] * u[y]) + lam[x + n] * (k[2] * u[y + n] + k[3] * u[y]);
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[x] = red[0];
+}
+
+// far part of lam^T (dK/dkappa) u: per M2L pair (D <- S) the Gk terms of the
+// pair differentiated in kappa, contracted with the adjoint locals of D
+// (adjL = b | a) and the moments of S:
+//   sum_a (eps a + b)[a] (c1 * dGk)[a] - (a + b/eps)[a] (c2 * dGk)[a],
+// c1 = sgn M_pot, c2 = sgn M_dpdn, (c * T)[a] = sum_b c[b] T[a+b].  One wave
+// per destination cube; part[D] is its sum.
+__global__ __launch_bounds__(64) void k_m2l_dk(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
+                                               const int *__restrict__ cubeLevel, const int *__restrict__ ordM2L,
+                                               const int *__restrict__ ordMom, const double *__restrict__ cubeX,
+                                               const double *__restrict__ mom, const double *__restrict__ adjL,
+                                               double *__restrict__ part, int nmMax, double kappa, double eps,
+                                               DevTabs T) {
+  extern __shared__ double sm[];
+  const int c = blockIdx.x, tid = threadIdx.x;
+  const int p = ordM2L[cubeLevel[c]], nm = NMOM(p), dsz = derivScratch(p);
+  const int nmc = NMOM(ordMom[cubeLevel[c]]);
+  double *s0 = sm, *sk = s0 + dsz, *Tk = sk + dsz, *Gk = Tk + nm, *c1 = Gk + nm, *c2 = c1 + nm;
+  double *A1 = c2 + nm, *A2 = A1 + nm;
+  for (int i = tid; i < nm; i += 64) {
+    const double b = adjL[(long long)c * 2 * nmMax + i], a = adjL[(long long)c * 2 * nmMax + nmMax + i];
+    A1[i] = i < nmc ? eps * a + b : 0.0;
+    A2[i] = i < nmc ? a + b / eps : 0.0;
+  }
+  double acc = 0.0;
+  for (int pr = m2lOff[c]; pr < m2lOff[c + 1]; pr++) {
+    const int src = m2lSrc[pr];
+    double r[3];
+    for (int d = 0; d < 3; d++) r[d] = cubeX[3 * c + d] - cubeX[3 * src + d];
+    derivs_block(kappa, p, r, s0, sk, Tk, Gk, T.mi, tid, 64, true);
+    for (int i = tid; i < nm; i += 64) {
+      const double sg = T.ep[i].w;
+      c1[i] = sg * mom[(long long)src * 2 * nmMax + i];
+      c2[i] = sg * mom[(long long)src * 2 * nmMax + nmMax + i];
+    }
+    __syncthreads();
+    for (int a = tid; a < nm; a += 64) {
+      const int cnt = NMOM(p - deg_of(T.mi[a]));
+      const int *cv = T.conv + T.convStart[a];
+      double t2 = 0.0, t4 = 0.0;
+      for (int e = 0; e < cnt; e++) {
+        const int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
+        t2 += c1[j] * Tk[kk];
+        t4 += c2[j] * Tk[kk];
+      }
+      acc += A1[a] * t2 - A2[a] * t4;
+    }
+    __syncthreads();
+  }
+  acc = waveSumT(acc);
+  if (tid == 0) part[c] = acc;
 }
 #pragma clang fp contract(off)
 
@@ -1993,8 +2372,13 @@ struct fbh_ctx {
   int *pbStart = nullptr, *pbM = nullptr;
   long long *pbOff = nullptr;
   int2 *pbTiles = nullptr;
-  double *pbInv = nullptr, *pbTmp = nullptr;
+  double *pbInv = nullptr, *pbTmp = nullptr, *pbInvT = nullptr;
   size_t pbBytes = 0;
+  // adjoint operator A^T: transposed near store, adjoint locals / moments
+  bool nearSym = false, m2lSym = false;
+  double4 *coefT = nullptr;
+  double *adjL = nullptr, *adjM = nullptr;
+  double4 *specDk = nullptr;   // d/dkappa corrections of the special pairs
   rocblas_handle blas = nullptr;
   // dense direct operator (-r=1/2): every panel pair through the nearfield store
   bool direct = false;
@@ -2062,7 +2446,7 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   if (c->ro) hipFree(c->ro);
   if (c->enArena) hipFree(c->enArena);
   for (void *q : {(void *)c->pbStart, (void *)c->pbM, (void *)c->pbOff, (void *)c->pbTiles, (void *)c->pbInv,
-                  (void *)c->pbTmp})
+                  (void *)c->pbTmp, (void *)c->pbInvT})
     if (q) hipFree(q);
   if (c->blas) rocblas_destroy_handle(c->blas);
   for (void *q : {(void *)c->dSrcOff, (void *)c->dNcol, (void *)c->dSrcList, (void *)c->dCoefOff, (void *)c->dCoef})
@@ -2422,6 +2806,22 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   }
   c->nInter = coefOff[nLeaf];
   c->nTiles = (int)tiles.size();
+  {   // the adjoint operator relies on symmetric near and M2L lists
+    auto symmetric = [](int cnt, const int *off, const int *lst) {
+      std::vector<std::pair<int, int>> e;
+      for (int a = 0; a < cnt; a++)
+        for (int k = off[a]; k < off[a + 1]; k++) e.push_back({a, lst[k]});
+      std::sort(e.begin(), e.end());
+      for (auto &pr : e)
+        if (!std::binary_search(e.begin(), e.end(), std::make_pair(pr.second, pr.first))) return false;
+      return true;
+    };
+    c->nearSym = symmetric(nLeaf, nearOff, nearSrc);
+    c->m2lSym = symmetric(nCube, m2lOff, m2lSrc);
+    for (int cb = 0; cb < nCube && c->m2lSym; cb++)   // pairs at one level share their orders
+      for (int k = m2lOff[cb]; k < m2lOff[cb + 1]; k++)
+        if (cubeLevel[m2lSrc[k]] != cubeLevel[cb]) { c->m2lSym = false; break; }
+  }
   if (!(c->srcOff = dupload(c, srcOff.data(), nLeaf + 1)) || !(c->ncol = dupload(c, ncol.data(), nLeaf)) ||
       !(c->coefOff = dupload(c, coefOff.data(), nLeaf + 1)) ||
       !(c->srcList = dupload(c, srcList.data(), srcList.size())) ||
@@ -2491,7 +2891,7 @@ static void m2lPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
 template <typename R>
 static void q2mPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
   hipLaunchKernelGGL(k_q2m<R>, dim3(c->nLeaf), dim3(256), 8 * (size_t)c->nmLeaf * sizeof(R), st, B.Q0, B.Q1,
-                     c->leafStart, c->leafCount, x, B.mom, c->n, c->nmLeaf, c->nmMax);
+                     c->leafStart, c->leafCount, x, B.mom, c->n, c->nmLeaf, c->nmMax, 0);
 }
 
 template <typename R>   // children at lev + 1 into parents at lev
@@ -2508,7 +2908,7 @@ static void l2lPass(fbh_ctx *c, const OpBufs<R> &B, int lev, hipStream_t st) {
   const int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
   if (c->levCount[lev + 1] <= 0) return;
   size_t lds = 5 * (size_t)nIn * sizeof(R) + (size_t)(nOut + 1 + c->hConvStart[nOut]) * sizeof(int);
-  hipLaunchKernelGGL(k_l2l<R>, dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), lds, st, c->levStart[lev + 1],
+  hipLaunchKernelGGL((k_l2l<R, 4>), dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), lds, st, c->levStart[lev + 1],
                      c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, B.lec, c->nmMax, c->T);
 }
 
@@ -2581,6 +2981,108 @@ extern "C" int fbh_matvec_dev(fbh_ctx *c, double alpha, const double *x, double 
   return matvecT(c, bufs64(c), alpha, x, beta, y);
 }
 
+// ---------------------------------------------------------- adjoint operator
+// A^T = D - K_near^T - K_far^T.  K_far = L2P L2L M2L M2M Q2M, so
+// K_far^T = Q2M^T M2M^T M2L^T L2L^T L2P^T, and each transposed pass is one of
+// the forward kernels: L2P^T is a Q2M of (y_n, y) (adjoint locals
+// l1 = l2 = Q0 y, l3 = l4 = Q1 y_n), L2L^T is an M2M of those two arrays,
+// M2L^T the cached M2L with the transposed combination, M2M^T an L2L of two
+// arrays, Q2M^T an L2P from the adjoint moments.
+static int ensureAdjoint(fbh_ctx *c) {
+  if (c->coefT) return 1;
+  if (!c->nearSym || !c->m2lSym) {
+    setErr("the adjoint operator needs symmetric near and M2L interaction lists");
+    return 0;
+  }
+  if (!c->m2lG) { setErr("the adjoint operator needs the cached M2L tensors"); return 0; }
+  const size_t ne = (size_t)c->nCube * 2 * c->nmMax;
+  if (!(c->coefT = dalloc<double4>(c, (size_t)c->nInter)) || !(c->adjL = dalloc<double>(c, ne)) ||
+      !(c->adjM = dalloc<double>(c, ne)))
+    return 0;
+  HCK(hipMemsetAsync(c->adjL, 0, ne * sizeof(double), c->stream));
+  HCK(hipMemsetAsync(c->adjM, 0, ne * sizeof(double), c->stream));
+  hipLaunchKernelGGL(k_near_build_t, dim3(c->n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart,
+                     c->srcOff, c->ncol, c->coefOff, c->srcList, c->coefT, c->kappa, c->eps, c->Q, c->qorder);
+  HCK(hipGetLastError());
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+// the transposed far chain from y into c->far (adjL: adjoint locals, adjM: adjoint moments)
+static void farPassT(fbh_ctx *c, const double *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_q2m<double>, dim3(c->nLeaf), dim3(256), 8 * (size_t)c->nmLeaf * sizeof(double), st, c->Q0,
+                     c->Q1, c->leafStart, c->leafCount, y, c->adjL, c->n, c->nmLeaf, c->nmMax, 1);
+  OpBufs<double> B = bufs64(c);
+  B.mom = c->adjL;
+  for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, st);
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(c->nCube), dim3(64 * c->m2lWaves), c->m2lLds, st, c->m2lOff, c->m2lSrc,
+                       c->cubeLevelD, c->ordM2LD, c->ordMomD, (const double *)c->m2lG, (const double *)c->adjL,
+                       c->adjM, c->nmMax, c->eps, c->T, c->planTerms, c->planIdx, c->planOff);
+  };
+  if (c->m2lMaxT == 16) launch(k_m2l_cached<double, 16, true>);
+  else if (c->m2lMaxT == 32) launch(k_m2l_cached<double, 32, true>);
+  else launch(k_m2l_cached<double, 0, true>);
+  for (int lev = c->height; lev < c->depth; lev++) {
+    const int nIn = NMOM(c->ordMom[lev]), nOut = NMOM(c->ordMom[lev + 1]);
+    if (c->levCount[lev + 1] <= 0) continue;
+    size_t lds = 5 * (size_t)nIn * sizeof(double) + (size_t)(nOut + 1 + c->hConvStart[nOut]) * sizeof(int);
+    hipLaunchKernelGGL((k_l2l<double, 2>), dim3(c->levCount[lev + 1]), dim3(blk64(nOut, 512)), lds, st,
+                       c->levStart[lev + 1], c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, c->adjM, c->nmMax,
+                       c->T);
+  }
+  hipLaunchKernelGGL((k_l2p<double, true>), dim3(c->nLeaf), dim3(256), 2 * (size_t)c->nmLeaf * sizeof(double), st,
+                     c->Q0, c->Q1, c->leafStart, c->leafCount, c->adjM, c->far, c->n, c->nmLeaf, c->nmMax);
+}
+
+// y = alpha*(D - K)^T x + beta*y
+static int matvecTr(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
+  if (c->direct) { setErr("the adjoint operator has no direct (-r) form"); return 0; }
+  if (!ensureAdjoint(c)) return 0;
+  HCK(hipEventRecord(c->evFork, c->stream));
+  HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
+  hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
+                     c->leafCount, c->srcOff, c->ncol, c->coefOff, c->srcList, c->coefT, x, c->nearv, c->n);
+  HCK(hipEventRecord(c->evJoin, c->side));
+  farPassT(c, x, c->stream);
+  HCK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
+  const double s1 = (1.0 + c->eps) / 2.0, s2 = (1.0 + 1.0 / c->eps) / 2.0;
+  const int n2 = 2 * c->n;
+  hipLaunchKernelGGL(k_combine<double>, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, c->far,
+                     c->nearv, y, c->n, alpha, beta, s1, s2, 1, 1);
+  HCK(hipGetLastError());
+  return 1;
+}
+
+extern "C" int fbh_matvec_t_dev(fbh_ctx *c, double alpha, const double *x, double beta, double *y) {
+  if (!c || !x || !y) { setErr("fbh_matvec_t_dev: null argument"); return 0; }
+  return matvecTr(c, alpha, x, beta, y);
+}
+
+// the self / shared-vertex pairs of every row's near list (CSR, near-list order)
+static int ensureSpec(fbh_ctx *c) {
+  if (c->specCol) return 1;
+  const size_t n = c->n;
+  int *cnt = dalloc<int>(c, n);
+  if (!cnt || !(c->specOff = dalloc<long long>(c, n + 1))) return 0;
+  hipLaunchKernelGGL(k_spec_count, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
+                     c->srcList, cnt);
+  HCK(hipGetLastError());
+  std::vector<int> hc(n);
+  std::vector<long long> ho(n + 1, 0);
+  HCK(hipMemcpyAsync(hc.data(), cnt, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; i++) ho[i + 1] = ho[i] + hc[i];
+  c->nSpec = ho[n];
+  if (!(c->specCol = dalloc<int>(c, (size_t)std::max<long long>(c->nSpec, 1)))) return 0;
+  HCK(hipMemcpyAsync(c->specOff, ho.data(), (n + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_spec_index, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
+                     c->srcList, c->specOff, c->specCol);
+  HCK(hipGetLastError());
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
 extern "C" int fbh_enable_fp32(fbh_ctx *c) {
   if (!c) { setErr("fbh_enable_fp32: null context"); return 0; }
   if (c->fp32) return 1;
@@ -2599,27 +3101,14 @@ extern "C" int fbh_enable_fp32(fbh_ctx *c) {
   const char *fe = getenv("FABIPB_FP32_NEAR");   // "stream": FP32 copies of the stored coefficients
   c->fly = c->qorder == 1 && !(fe && !strcmp(fe, "stream"));
   if (c->fly) {
-    int *cnt = dalloc<int>(c, n);
     if (!(c->flyPA = dalloc<float4>(c, n)) || !(c->flyPN = dalloc<float4>(c, n)) ||
-        !(c->flyPL = dalloc<float4>(c, n)) ||
-        !(c->specOff = dalloc<long long>(c, n + 1)) || !cnt)
+        !(c->flyPL = dalloc<float4>(c, n)) || !ensureSpec(c) ||
+        !(c->specVal = dalloc<float4>(c, (size_t)c->nSpec)))
       return 0;
     hipLaunchKernelGGL(k_fly_panels, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->pnl, (int)n, c->Q.t[0],
                        c->flyPA, c->flyPN, c->flyPL);
-    hipLaunchKernelGGL(k_spec_count, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
-                       c->srcList, cnt);
-    HCK(hipGetLastError());
-    std::vector<int> hc(n);
-    std::vector<long long> ho(n + 1, 0);
-    HCK(hipMemcpyAsync(hc.data(), cnt, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HCK(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < n; i++) ho[i + 1] = ho[i] + hc[i];
-    c->nSpec = ho[n];
-    if (!(c->specCol = dalloc<int>(c, (size_t)c->nSpec)) || !(c->specVal = dalloc<float4>(c, (size_t)c->nSpec)))
-      return 0;
-    HCK(hipMemcpyAsync(c->specOff, ho.data(), (n + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_spec_fill, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
-                       c->srcList, c->specOff, c->specCol, c->specVal, c->kappa, c->eps, c->Q, c->qorder);
+    hipLaunchKernelGGL(k_spec_vals<false>, dim3(n), dim3(64), 0, c->stream, c->pnl, c->specOff, c->specCol,
+                       c->specVal, (double4 *)nullptr, c->kappa, c->eps, c->Q, c->qorder);
   } else {
     if (!(c->coef32 = dalloc<float4>(c, (size_t)c->nInter))) return 0;
     cvt((const double *)c->coef, (float *)c->coef32, 4 * (size_t)c->nInter);
@@ -2794,7 +3283,7 @@ extern "C" int fbh_psolve_diag_dev(fbh_ctx *c, double *out, const double *in) {
 
 static void freePB(fbh_ctx *c) {
   for (void **q : {(void **)&c->pbStart, (void **)&c->pbM, (void **)&c->pbOff, (void **)&c->pbTiles,
-                   (void **)&c->pbInv, (void **)&c->pbTmp}) {
+                   (void **)&c->pbInv, (void **)&c->pbTmp, (void **)&c->pbInvT}) {
     if (*q) hipFree(*q);
     *q = nullptr;
   }
@@ -2977,13 +3466,34 @@ static void rot(double *x, double *y, double c, double s) {
   *x = t;
 }
 
-static int applyOp(fbh_ctx *c, bool fp32, double alpha, const double *x, double beta, double *y) {
-  return fp32 ? matvecT(c, bufs32(c), alpha, x, beta, y) : matvecT(c, bufs64(c), alpha, x, beta, y);
+enum OpKind { OP64 = 0, OP32 = 1, OPT64 = 2 };   // FP64 A, FP32 A, FP64 A^T
+
+static int applyOp(fbh_ctx *c, int kind, double alpha, const double *x, double beta, double *y) {
+  if (kind == OPT64) return matvecTr(c, alpha, x, beta, y);
+  return kind == OP32 ? matvecT(c, bufs32(c), alpha, x, beta, y) : matvecT(c, bufs64(c), alpha, x, beta, y);
 }
 
-// restarted left-preconditioned GMRES on the FP64 or the FP32 operator; the
-// Krylov basis, Gram-Schmidt and Givens rotations are FP64 in both
-static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int restrt, double tol, int maxit,
+// the preconditioner of the operator kind: P for A, P^T for A^T
+static int precApply(fbh_ctx *c, int kind, double *out, const double *in) {
+  if (kind != OPT64 || c->precMode != 2) return fbh_psolve_dev(c, out, in);   // none and diagonal: symmetric
+  if (!c->pbInvT) {
+    long long tot = 0;
+    HCK(hipMemcpy(&tot, c->pbOff + c->nPB, sizeof(long long), hipMemcpyDeviceToHost));
+    HCK(hipMalloc(&c->pbInvT, std::max<long long>(tot, 1) * sizeof(double)));
+    hipLaunchKernelGGL(k_bp_transpose, dim3(c->nPB, 64), dim3(256), 0, c->stream, c->pbM, c->pbOff, c->pbInv,
+                       c->pbInvT);
+    HCK(hipGetLastError());
+  }
+  HCK(hipMemcpyAsync(c->pbTmp, in, 2 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_bp_apply, dim3(c->nPBTiles), dim3(256), 0, c->stream, c->pbTiles, c->pbStart, c->pbM, c->pbOff,
+                     c->pbInvT, c->pbTmp, out, c->n);
+  HCK(hipGetLastError());
+  return 1;
+}
+
+// restarted left-preconditioned GMRES on the FP64, the FP32 or the transposed
+// operator; the Krylov basis, Gram-Schmidt and Givens rotations are FP64 in all
+static int gmresCore(fbh_ctx *c, int kind, const double *b, double *x, int restrt, double tol, int maxit,
                      int *iters, double *resid, double *hist, int *info) {
   if (!ensureGmres(c, restrt)) return 0;
   const int n2 = 2 * c->n, ldh = restrt + 1, g = (n2 + 255) / 256;
@@ -2996,8 +3506,8 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
   double xn, bnrm2, rn;
   HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
   if (!devNorm(c, x, &xn)) return 0;
-  if (xn != 0.0 && !applyOp(c, fp32, -1.0, x, 1.0, c->w)) return 0;
-  if (!fbh_psolve_dev(c, c->r, c->w)) return 0;
+  if (xn != 0.0 && !applyOp(c, kind, -1.0, x, 1.0, c->w)) return 0;
+  if (!precApply(c, kind, c->r, c->w)) return 0;
   if (!devNorm(c, b, &bnrm2)) return 0;
   if (bnrm2 == 0.0) bnrm2 = 1.0;
   if (!devNorm(c, c->r, &rn)) return 0;
@@ -3018,8 +3528,8 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
       ++i;
       ++*iters;
       double *vi = c->V + (size_t)(i - 1) * n2, *vn = c->V + (size_t)i * n2;
-      if (!applyOp(c, fp32, 1.0, vi, 0.0, c->w)) return 0;
-      if (!fbh_psolve_dev(c, c->w, c->w)) return 0;
+      if (!applyOp(c, kind, 1.0, vi, 0.0, c->w)) return 0;
+      if (!precApply(c, kind, c->w, c->w)) return 0;
       for (int k = 0; k < i; k++) {   // modified Gram-Schmidt
         const double *vk = c->V + (size_t)k * n2;
         hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, vk, n2, c->part);
@@ -3050,8 +3560,8 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
     HCK(hipStreamSynchronize(c->stream));
     if (*resid <= tol) return 1;
     HCK(hipMemcpyAsync(c->w, b, vb, hipMemcpyDeviceToDevice, c->stream));
-    if (!applyOp(c, fp32, -1.0, x, 1.0, c->w)) return 0;
-    if (!fbh_psolve_dev(c, c->r, c->w)) return 0;
+    if (!applyOp(c, kind, -1.0, x, 1.0, c->w)) return 0;
+    if (!precApply(c, kind, c->r, c->w)) return 0;
     if (!devNorm(c, c->r, &rn)) return 0;
     *resid = rn / bnrm2;
     if (hist) hist[nh++] = *resid;
@@ -3063,7 +3573,15 @@ static int gmresCore(fbh_ctx *c, bool fp32, const double *b, double *x, int rest
 extern "C" int fbh_gmres_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit, int *iters,
                              double *resid, double *hist, int *info) {
   if (!c || !b || !x || restrt < 1 || maxit < 1) { setErr("fbh_gmres_dev: bad arguments"); return 0; }
-  return gmresCore(c, false, b, x, restrt, tol, maxit, iters, resid, hist, info);
+  return gmresCore(c, OP64, b, x, restrt, tol, maxit, iters, resid, hist, info);
+}
+
+// adjoint solve A^T x = b (left-preconditioned with P^T), same stopping rule
+extern "C" int fbh_gmres_t_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit,
+                               int *iters, double *resid, double *hist, int *info) {
+  if (!c || !b || !x || restrt < 1 || maxit < 1) { setErr("fbh_gmres_t_dev: bad arguments"); return 0; }
+  if (!ensureAdjoint(c)) return 0;
+  return gmresCore(c, OPT64, b, x, restrt, tol, maxit, iters, resid, hist, info);
 }
 
 __global__ void k_axpy(double *__restrict__ y, const double *__restrict__ x, int n) {
@@ -3102,7 +3620,7 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
   const double braw = bnrm2;
   if (!devNorm(c, x, &xn)) return 0;
   HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
-  if (xn != 0.0 && !applyOp(c, true, -1.0, x, 1.0, c->ro)) return 0;
+  if (xn != 0.0 && !applyOp(c, OP32, -1.0, x, 1.0, c->ro)) return 0;
   for (;;) {
     double rr;
     if (!devNorm(c, c->ro, &rr)) return 0;
@@ -3110,7 +3628,7 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
     HCK(hipMemsetAsync(c->dx, 0, vb, c->stream));
     int it = 0, inf = 0;
     double ir;
-    if (!gmresCore(c, true, c->ro, c->dx, restrt, 0.5 * tol * bnrm2 / rr, maxit - *iters, &it, &ir, nullptr, &inf))
+    if (!gmresCore(c, OP32, c->ro, c->dx, restrt, 0.5 * tol * bnrm2 / rr, maxit - *iters, &it, &ir, nullptr, &inf))
       return 0;
     *iters += it;
     ++*outer;
@@ -3120,10 +3638,10 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
       // ro -= A dx needs the operator only to the accuracy of dx itself, and
       // the FP32 operator's error on it, eps32 * |A dx| ~ eps32 * |ro|, stays
       // four orders below tol * |b|
-      if (!applyOp(c, true, -1.0, c->dx, 1.0, c->ro)) return 0;
+      if (!applyOp(c, OP32, -1.0, c->dx, 1.0, c->ro)) return 0;
     } else {
       HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
-      if (!applyOp(c, false, -1.0, x, 1.0, c->ro)) return 0;
+      if (!applyOp(c, OP64, -1.0, x, 1.0, c->ro)) return 0;
       ++*mv64;
     }
     if (!fbh_psolve_dev(c, c->w, c->ro)) return 0;
@@ -3158,6 +3676,142 @@ extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const doubl
   hipFree(dc);
   if (e != hipSuccess) { setErr("k_rhs failed: %s", hipGetErrorString(e)); return 0; }
   return 1;
+}
+
+// ------------------------------------------------ differentiation form
+static int devDot(fbh_ctx *c, const double *a, const double *b, int m, double *out) {
+  hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, b, m, c->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->part, RED_BLOCKS, c->scal);
+  HCK(hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+static int devSum(fbh_ctx *c, const double *a, int m, double *out) {
+  hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, a, m, c->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->part, RED_BLOCKS, c->scal);
+  HCK(hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+// device copies of the charges (freed by the caller)
+static int uploadCharges(int nChar, const double *pos, const double *chr, double **dp, double **dc) {
+  *dp = *dc = nullptr;
+  if (hipMalloc(dp, 3 * (size_t)nChar * sizeof(double)) != hipSuccess ||
+      hipMalloc(dc, (size_t)nChar * sizeof(double)) != hipSuccess ||
+      hipMemcpy(*dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(*dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    if (*dp) hipFree(*dp);
+    if (*dc) hipFree(*dc);
+    setErr("uploading %d charges failed", nChar);
+    return 0;
+  }
+  return 1;
+}
+
+extern "C" int fbh_energy_direct_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr,
+                                     const double *sgm_dev, double *g_dev, double *energy_raw) {
+  if (!c || nChar <= 0 || !pos || !chr || !sgm_dev || !energy_raw) {
+    setErr("fbh_energy_direct_dev: bad arguments");
+    return 0;
+  }
+  if (!ensureGmres(c, 1)) return 0;
+  double *dp, *dc, *g = g_dev;
+  if (!uploadCharges(nChar, pos, chr, &dp, &dc)) return 0;
+  if (!g && hipMalloc(&g, 2 * (size_t)c->n * sizeof(double)) != hipSuccess) {
+    hipFree(dp);
+    hipFree(dc);
+    setErr("fbh_energy_direct_dev: allocation failed");
+    return 0;
+  }
+  hipLaunchKernelGGL(k_energy_g, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc, g,
+                     (double *)nullptr, c->kappa, c->eps, c->Q, c->qorder);
+  int ok = hipGetLastError() == hipSuccess && devDot(c, g, sgm_dev, 2 * c->n, energy_raw);
+  hipStreamSynchronize(c->stream);
+  if (!g_dev) hipFree(g);
+  hipFree(dp);
+  hipFree(dc);
+  if (!ok) setErr("fbh_energy_direct_dev: %s", gErr[0] ? gErr : "kernel failed");
+  return ok;
+}
+
+// dE/dq and dE/dkappa of E_raw = g(q, kappa) . u with A(kappa) u = b(q):
+//   A^T lam = g,  dE/dq = G^T u + R^T lam,  dE/dkappa = (dg/dkappa) . u + lam^T (dK/dkappa) u
+// (b = R q does not depend on kappa, A = D - K with D free of kappa)
+extern "C" int fbh_gradient_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, const double *sgm_dev,
+                                double *lam_dev, int restrt, double tol, int maxit, double *dEdq,
+                                double *dEdkappa, int *iters, double *resid, int *info) {
+  if (!c || nChar <= 0 || !pos || !chr || !sgm_dev || !lam_dev || !dEdq || !dEdkappa || !iters || !resid ||
+      !info || restrt < 1 || maxit < 1) {
+    setErr("fbh_gradient_dev: bad arguments");
+    return 0;
+  }
+  if (!ensureAdjoint(c) || !ensureGmres(c, restrt)) return 0;
+  const int n = c->n, n2 = 2 * n;
+  const int nChunk = (n + GRAD_PCH - 1) / GRAD_PCH;
+  double *dp = nullptr, *dc = nullptr, *g = nullptr, *gk = nullptr, *part = nullptr, *dq = nullptr;
+  if (!uploadCharges(nChar, pos, chr, &dp, &dc)) return 0;
+  size_t partN = std::max<size_t>((size_t)nChunk * nChar, std::max(n, c->nCube));
+  int ok = hipMalloc(&g, n2 * sizeof(double)) == hipSuccess && hipMalloc(&gk, n2 * sizeof(double)) == hipSuccess &&
+           hipMalloc(&part, partN * sizeof(double)) == hipSuccess &&
+           hipMalloc(&dq, (size_t)nChar * sizeof(double)) == hipSuccess;
+  if (!ok) setErr("fbh_gradient_dev: allocation failed");
+  double eK = 0.0, eNear = 0.0, eFar = 0.0;
+  if (ok) {
+    hipLaunchKernelGGL(k_energy_g, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->pnl, n, nChar, dp, dc, g, gk,
+                       c->kappa, c->eps, c->Q, c->qorder);
+    ok = hipGetLastError() == hipSuccess && gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid,
+                                                      nullptr, info);
+  }
+  if (ok) {   // dE/dq
+    const double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
+    hipLaunchKernelGGL(k_grad_q, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar, dp,
+                       sgm_dev, lam_dev, part, c->kappa, c->eps, fac, c->Q, c->qorder);
+    hipLaunchKernelGGL(k_grad_sum, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, part, nChunk, nChar, dq);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(dEdq, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) setErr("fbh_gradient_dev: charge gradient kernels failed");
+  }
+  if (ok) ok = devDot(c, gk, sgm_dev, n2, &eK);
+  if (ok) {   // near part of lam^T dK/dkappa u
+    const char *er = getenv("FABIPB_DK_RULES");   // "1": the full rules for every pair (check path)
+    if (c->qorder == 1 && !(er && er[0] == '1')) {
+      if (!c->specDk) {
+        ok = ensureSpec(c) && (c->specDk = dalloc<double4>(c, (size_t)c->nSpec)) != nullptr;
+        if (ok)
+          hipLaunchKernelGGL(k_spec_vals<true>, dim3(n), dim3(64), 0, c->stream, c->pnl, c->specOff, c->specCol,
+                             (float4 *)nullptr, c->specDk, c->kappa, c->eps, c->Q, c->qorder);
+      }
+      if (ok)
+        hipLaunchKernelGGL(k_near_dk, dim3(c->nTiles), dim3(256), 0, c->stream, c->tiles, c->leafStart,
+                           c->leafCount, c->srcOff, c->ncol, c->srcList, c->pnl, sgm_dev, lam_dev, c->specOff,
+                           c->specCol, c->specDk, part, n, c->kappa, c->eps, c->Q.t[0]);
+    } else {
+      hipLaunchKernelGGL(k_near_dk_rules, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
+                         c->srcList, sgm_dev, lam_dev, part, n, c->kappa, c->eps, c->Q, c->qorder);
+    }
+    ok = ok && hipGetLastError() == hipSuccess && devSum(c, part, n, &eNear);
+  }
+  if (ok) {   // far part: moments of u, adjoint locals of lam, the kappa-differentiated M2L between them
+    OpBufs<double> B = bufs64(c);
+    q2mPass(c, B, sgm_dev, c->stream);
+    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
+    hipLaunchKernelGGL(k_q2m<double>, dim3(c->nLeaf), dim3(256), 8 * (size_t)c->nmLeaf * sizeof(double), c->stream,
+                       c->Q0, c->Q1, c->leafStart, c->leafCount, lam_dev, c->adjL, n, c->nmLeaf, c->nmMax, 1);
+    B.mom = c->adjL;
+    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
+    const size_t lds = (2 * (size_t)derivScratch(c->maxOrder) + 6 * (size_t)NMOM(c->maxOrder)) * sizeof(double);
+    hipLaunchKernelGGL(k_m2l_dk, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
+                       c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->adjL, part, c->nmMax, c->kappa, c->eps, c->T);
+    ok = hipGetLastError() == hipSuccess && devSum(c, part, c->nCube, &eFar);
+    if (!ok) setErr("fbh_gradient_dev: kappa-derivative kernels failed");
+  }
+  *dEdkappa = eK + eNear + eFar;
+  hipStreamSynchronize(c->stream);
+  for (double *q : {dp, dc, g, gk, part, dq})
+    if (q) hipFree(q);
+  return ok;
 }
 
 extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b) {

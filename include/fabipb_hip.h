@@ -196,6 +196,32 @@ int fbh_rhs_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *c
                      const int *ctNChgs, const int *ctChgIdx, const int *ctOrder,
                      double theta, double *b_dev);
 
+/* Differentiation form (no counterpart in the reference, which has no
+ * adjoint; it differentiates what MtVmain, setupRHS and the panel-tree energy
+ * compute: src/fabipb.c:1237-1287, 744-800, src/treecode.c:581-695).
+ *
+ * fbh_matvec_t_dev: y = alpha*A^T x + beta*y for the operator of
+ * fbh_matvec_dev: the transposed nearfield (its own coefficient store, built
+ * on first use) and the transposed FMM chain (Q2M^T M2M^T M2L^T L2L^T L2P^T).
+ * Needs symmetric near and M2L lists and the M2L tensor cache; not the -r form.
+ * fbh_gmres_t_dev: GMRES on A^T with the transposed preconditioner, the
+ * stopping rule of fbh_gmres_dev.
+ * fbh_energy_direct_dev: E_raw = g . sgm with the direct energy weights
+ * g (2n; the panel-tree energy with every charge taken directly), g_dev may be
+ * NULL.  Scale by 2*pi*332.0716 for kcal/mol, as the panel-tree energy.
+ * fbh_gradient_dev: for E_raw(q, kappa) = g(q, kappa) . sgm, A sgm = b(q)
+ * (direct right-hand side), solves A^T lam = g (lam_dev: initial guess in,
+ * solution out) and returns dE_raw/dq (host array, nChar) and dE_raw/dkappa.
+ */
+int fbh_matvec_t_dev(fbh_ctx *ctx, double alpha, const double *x, double beta, double *y);
+int fbh_gmres_t_dev(fbh_ctx *ctx, const double *b, double *x, int restart, double tol, int maxit,
+                    int *iters, double *resid, double *hist, int *info);
+int fbh_energy_direct_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                          const double *sgm_dev, double *g_dev, double *energy_raw);
+int fbh_gradient_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                     const double *sgm_dev, double *lam_dev, int restart, double tol, int maxit,
+                     double *dEdq, double *dEdkappa, int *iters, double *resid, int *info);
+
 /* per-stage device timing over `iters` matvecs (HIP events on the context stream):
  * ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole matvec */
 int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
