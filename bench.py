@@ -19,7 +19,12 @@ error against it), per-stage device times (HIP events on the launch stream),
 the nearfield kernel's roofline position, and the reference CPU path timed on
 the host cores.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calib-copy]
+Two further blocks: the precision form at ~317k panels (R = 40 A level-7
+globule, 8000 charges, depth 5) and the differentiation form (the level-6
+globule with 5000 charges: adjoint gradient dE/dq, dE/dkappa against one
+forward solve).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-large] [--no-grad] [--calib-copy]
 """
 import argparse
 import importlib.util
@@ -37,6 +42,13 @@ METRIC = "PB solve wall time & matvec/s at 1e-6 energy err, FP32-refined vs FP64
 MESH = ["globule", "--level", "6", "--radius", "22", "--charges", "inside", "--nq", "2000", "--sigma", "0.4",
         "--depth", "1.5", "--seed", "2"]
 DEPTH, ORDER, MAC, KAPPA, EPS1, EPS2, Q, TOL, RESTART, MAXIT = 4, 6, 0.8, 0.1257, 1.0, 80.0, 1, 1e-6, 30, 100
+# precision form at scale: R = 40 A level-7 globule, faces under 0.05 A^2 merged, 8000 charges, depth 5
+MESH_LARGE = ["globule", "--level", "7", "--radius", "40", "--merge", "0.05", "--charges", "inside", "--nq", "8000",
+              "--sigma", "0.4", "--depth", "1.5", "--seed", "3"]
+DEPTH_LARGE = 5
+# differentiation form: the level-6 globule with 5000 charges
+MESH_GRAD = ["globule", "--level", "6", "--radius", "22", "--charges", "inside", "--nq", "5000", "--sigma", "0.4",
+             "--depth", "1.5", "--seed", "4"]
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 CPU_CORES = 16             # the GPU box's host share for one GPU
 COEF_BYTES = 32            # nearfield: KER[4] doubles streamed per (dst, src) panel pair
@@ -87,6 +99,97 @@ def calib_copy(torch, mib=1024):
     print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written per launch", file=sys.stderr, flush=True)
 
 
+def timed_solve(fa, torch, op, n, mesh, ct, precision):
+    """RHS + GMRES + panel-tree energy, each stage synchronised and timed."""
+    b = torch.empty(2 * n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    op.rhs_dev(mesh.pos, mesh.chr, b)
+    xs = b.clone()
+    torch.cuda.synchronize()
+    tr = time.perf_counter()
+    if precision == "fp32":
+        gm = op.gmres_refined_dev(b, xs, restart=RESTART, tol=TOL, maxit=MAXIT)
+    else:
+        gm = op.gmres_dev(b, xs, restart=RESTART, tol=TOL, maxit=MAXIT)
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    raw = op.energy_dev(ct, mesh.pos, mesh.chr, xs)
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    return {"wall_s": te - ts, "rhs_s": tr - ts, "gmres_s": tg - tr, "energy_s": te - tg, "iterations": gm["iters"],
+            "final_residual": gm["resid"], "energy_raw": raw, "outer": gm.get("outer"), "fp64_matvecs": gm.get("mv64")}
+
+
+def precision_large(fa, torch, tmp):
+    """The precision form at ~317k panels: FP32-refined against plain FP64 solve wall time."""
+    base = os.path.join(tmp, "glob7")
+    subprocess.check_call([os.path.join(ROOT, "cuda-fabipb_amd", "bin", "fbmesh"), MESH_LARGE[0], base] + MESH_LARGE[1:],
+                          stdout=subprocess.DEVNULL)
+    mesh = fa.Mesh(base)
+    tree = fa.Tree(mesh, DEPTH_LARGE, order=ORDER, sep=MAC)
+    op = fa.Operator(tree, KAPPA, EPS1, EPS2, Q)
+    op.enable_fp32()
+    ct = tree.charge_tree()
+    r = {}
+    for prec in ("fp64", "fp32"):
+        timed_solve(fa, torch, op, tree.n, mesh, ct, prec)
+        r[prec] = timed_solve(fa, torch, op, tree.n, mesh, ct, prec)
+    st = op.stats()
+    out = {"panels": tree.n, "charges": int(len(mesh.chr)), "depth": DEPTH_LARGE,
+           "near_interactions": st["near_interactions"], "m2l_pairs": st["m2l_pairs"],
+           "fp64": r["fp64"], "fp32_refined": r["fp32"],
+           "wall_speedup": r["fp64"]["wall_s"] / r["fp32"]["wall_s"],
+           "energy_rel_err": abs(r["fp32"]["energy_raw"] - r["fp64"]["energy_raw"]) / abs(r["fp64"]["energy_raw"])}
+    print(f"[bench] precision form at {tree.n} panels: fp64 {r['fp64']['wall_s'] * 1e3:.2f} ms, fp32-refined "
+          f"{r['fp32']['wall_s'] * 1e3:.2f} ms ({out['wall_speedup']:.2f}x), err {out['energy_rel_err']:.2e}",
+          file=sys.stderr, flush=True)
+    del op
+    torch.cuda.empty_cache()
+    return out
+
+
+def differentiation(fa, torch, np, tmp):
+    """Adjoint gradient dE/dq (every charge) and dE/dkappa against one forward solve (direct energy)."""
+    base = os.path.join(tmp, "glob6q")
+    subprocess.check_call([os.path.join(ROOT, "cuda-fabipb_amd", "bin", "fbmesh"), MESH_GRAD[0], base] + MESH_GRAD[1:],
+                          stdout=subprocess.DEVNULL)
+    mesh = fa.Mesh(base)
+    P = fa.Problem(mesh, DEPTH, order=ORDER, sep=MAC, eps1=EPS1, eps2=EPS2, qorder=Q, tol=TOL, restart=RESTART,
+                   maxit=MAXIT)
+    op = P.operator(KAPPA)
+    q = np.ascontiguousarray(mesh.chr, dtype=np.float64)
+
+    def forward():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, e, g = P.solve(q, KAPPA)
+        torch.cuda.synchronize()
+        return u, e, g, time.perf_counter() - t0
+
+    def adjoint(u):
+        lam = torch.zeros_like(u)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dq, dk, info = op.gradient_dev(mesh.pos, q, u, lam, restart=RESTART, tol=TOL, maxit=MAXIT)
+        torch.cuda.synchronize()
+        return dq, dk, info, time.perf_counter() - t0
+
+    u, _, _, _ = forward()
+    adjoint(u)   # first call builds the transposed store and the special-pair corrections
+    torch.cuda.synchronize()
+    u, e, g, t_fwd = forward()
+    dq, dk, info, t_adj = adjoint(u)
+    out = {"panels": P.tree.n, "charges": int(len(q)), "energy_kcal_per_mol": e * fa.ENERGY_SCALE,
+           "forward_s": t_fwd, "forward_iterations": g["iters"], "adjoint_s": t_adj,
+           "adjoint_iterations": info["iters"], "adjoint_over_forward": t_adj / t_fwd,
+           "dEdkappa_kcal_per_mol_A": dk * fa.ENERGY_SCALE,
+           "dEdq_norm_kcal_per_mol_e": float(np.linalg.norm(dq)) * fa.ENERGY_SCALE}
+    print(f"[bench] differentiation at {P.tree.n} panels, {len(q)} charges: forward {t_fwd * 1e3:.2f} ms, adjoint "
+          f"gradient {t_adj * 1e3:.2f} ms ({out['adjoint_over_forward']:.2f}x)", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +198,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU (one MI355X per box); --gpus must be 1", file=sys.stderr)
@@ -208,6 +313,12 @@ def main():
               f"outer={precision['outer']} err={precision['energy_rel_err_vs_fp64']:.2e} "
               f"matvec32={ms32:.3f} ms", file=sys.stderr, flush=True)
 
+    large = grad = None
+    if not args.no_solve and not args.no_large:
+        large = precision_large(fa, torch, tmp)
+    if not args.no_solve and not args.no_grad:
+        grad = differentiation(fa, torch, np, tmp)
+
     if args.calib_copy:
         calib_copy(torch)
 
@@ -249,6 +360,8 @@ def main():
         "setup_s": {"tree": t_tree, "device_operator": t_setup},
         "solve": solve,
         "precision_form": precision,
+        "precision_form_large": large,
+        "differentiation_form": grad,
     }
     print(json.dumps(out), flush=True)
 
