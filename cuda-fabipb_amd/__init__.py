@@ -532,15 +532,18 @@ def solvation_energy(problem, q, kappa):
 
 
 def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qorder=1, tol=1e-6,
-          restart=30, maxit=100, height=2, x0="rhs", precision="fp64", op=None, tree=None, mesh=None,
+          restart=30, maxit=100, height=2, x0="auto", precision="fp64", op=None, tree=None, mesh=None,
           precond=3, direct=False, rhs="auto"):
     """One solve (src/fabipb.c:1067-1210): returns energies, iterations and times.
     precision "fp64" is the plain form (GMRES on the FP64 operator); "fp32" is the
     precision form (refinement around GMRES on the FP32 operator, FP64 residuals).
     precond is the -P mode (3 diagonal; 0, 1, 2 block LU); direct the -r=1/2 dense
     operator in place of the FMM; rhs "direct", "tree" or "auto" (the reference's
-    pair-count policy, FABIPB_FORCE_TREE_RHS).  An existing
-    mesh/tree/operator may be passed in to time the solve alone."""
+    pair-count policy, FABIPB_FORCE_TREE_RHS).  x0 "rhs" starts GMRES from b (the
+    reference's default), "zero" from 0; "auto" is "rhs" for the plain form and
+    "zero" for the precision form, whose first correction then solves A d = b
+    without an initial residual matvec.  An existing mesh/tree/operator may be
+    passed in to time the solve alone."""
     torch = _torch()
     t0 = time.perf_counter()
     mesh = mesh if mesh is not None else Mesh(base)
@@ -564,6 +567,8 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
         op.rhs_tree_dev(ct, mesh.pos, mesh.chr, b)
     else:
         op.rhs_dev(mesh.pos, mesh.chr, b)
+    if x0 == "auto":
+        x0 = "zero" if precision == "fp32" else "rhs"
     x = b.clone() if x0 == "rhs" else torch.zeros_like(b)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
