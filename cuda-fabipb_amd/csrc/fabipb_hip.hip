@@ -377,28 +377,48 @@ __global__ __launch_bounds__(256) void k_near_build(const Pnl *__restrict__ pnl,
   }
 }
 
-// The transposed nearfield in the same row layout: row r, column c holds the
-// 2x2 block of A^T, i.e. of KER = panelIA0(X = c, Y = r) read transposed:
-//   out[r] += KER3 sgm_n[c] + KER1 sgm[c],  out[r+n] += KER2 sgm_n[c] + KER0 sgm[c]
-// (valid because the near leaf lists are symmetric: c is near r's leaf
-// exactly when r is near c's leaf).
-__global__ __launch_bounds__(256) void k_near_build_t(const Pnl *__restrict__ pnl, const int *__restrict__ leafOf,
-                                                      const int *__restrict__ leafStart,
-                                                      const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
-                                                      const long long *__restrict__ coefOff,
-                                                      const int *__restrict__ srcList, double4 *__restrict__ coef,
-                                                      double kappa, double eps, QTab Q, int q) {
-  const int r = blockIdx.x, D = leafOf[r], row = r - leafStart[D], ncol = ncolArr[D];
-  const Pnl R = pnl[r];
-  double4 *out = coef + coefOff[D] + (long long)row * ncol;
+// the stored coefficient blocks of the streamed tiles (tileCoef[t] >= 0):
+// one block per (tile, row), threads over the row's source columns
+// TR: the transposed store, (KER3, KER1, KER2, KER0) of panelIA0(X = y, Y = x)
+template <bool TR = false>
+__global__ __launch_bounds__(256) void k_near_build_tiles(const Pnl *__restrict__ pnl, const int2 *__restrict__ tiles,
+                                                          const int *__restrict__ leafStart,
+                                                          const int *__restrict__ leafCount,
+                                                          const int *__restrict__ srcOff,
+                                                          const int *__restrict__ ncolArr,
+                                                          const long long *__restrict__ tileCoef,
+                                                          const int *__restrict__ srcList, double4 *__restrict__ coef,
+                                                          double kappa, double eps, QTab Q, int q) {
+  const int t = blockIdx.x, row = blockIdx.y;
+  const long long off = tileCoef[t];
+  if (off < 0) return;
+  const int2 tl = tiles[t];
+  const int D = tl.x, r0 = tl.y;
+  if (row >= min(NEAR_ROWS, leafCount[D] - r0)) return;
+  const int x = leafStart[D] + r0 + row, ncol = ncolArr[D];
+  const Pnl X = pnl[x];
+  double4 *out = coef + off + (long long)row * ncol;
   const int *src = srcList + srcOff[D];
   for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
-    const int c = src[j];
-    const Pnl C = pnl[c];
+    const int y = src[j];
+    const Pnl Y = pnl[y];
     double k[4];
-    panelIA0(C, R, c == r, kappa, eps, Q, q, k);
-    out[j] = make_double4(k[3], k[1], k[2], k[0]);
+    if (TR) {
+      panelIA0(Y, X, x == y, kappa, eps, Q, q, k);
+      out[j] = make_double4(k[3], k[1], k[2], k[0]);
+    } else {
+      panelIA0(X, Y, x == y, kappa, eps, Q, q, k);
+      out[j] = make_double4(k[0], k[1], k[2], k[3]);
+    }
   }
+}
+
+// one pair's coefficients (a query of the stage ABI)
+__global__ void k_near_pair(const Pnl *__restrict__ pnl, int x, int y, double kappa, double eps, QTab Q, int q,
+                            double *__restrict__ out) {
+  double k[4];
+  panelIA0(pnl[x], pnl[y], x == y, kappa, eps, Q, q, k);
+  for (int i = 0; i < 4; i++) out[i] = k[i];
 }
 
 // ------------------------------------------------- block preconditioner
@@ -525,23 +545,25 @@ template <> struct Vec4<float> { typedef float4 type; };
 
 // tile = (destination leaf, first row); 4 waves x 4 rows; out[x] = sum over
 // sources of K0*sgm_n + K1*sgm, out[x+n] = K2*sgm_n + K3*sgm.  R is the
-// arithmetic of the coefficients, the staged densities and the sums.
+// arithmetic of the coefficients, the staged densities and the sums.  The
+// tile's coefficient rows start at tileCoef[t] (row stride = the leaf's
+// column count).
 template <typename R>
-__global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
-                                                    const int *__restrict__ leafCount,
-                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
-                                                    const long long *__restrict__ coefOff,
-                                                    const int *__restrict__ srcList,
-                                                    const typename Vec4<R>::type *__restrict__ coef,
-                                                    const double *__restrict__ x, R *__restrict__ out, int n) {
+__device__ __forceinline__ void nearStreamTile(int t, const int2 *__restrict__ tiles,
+                                               const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                               const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                               const long long *__restrict__ tileCoef,
+                                               const int *__restrict__ srcList,
+                                               const typename Vec4<R>::type *__restrict__ coef,
+                                               const double *__restrict__ x, R *__restrict__ out, int n, R *sS,
+                                               R *sN) {
   typedef typename Vec4<R>::type R4;
-  __shared__ R sS[NEAR_CHUNK], sN[NEAR_CHUNK];
-  const int2 t = tiles[blockIdx.x];
-  const int D = t.x, r0 = t.y;
+  const int2 tl = tiles[t];
+  const int D = tl.x, r0 = tl.y;
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
   const int ncol = ncolArr[D];
   const int *src = srcList + srcOff[D];
-  const R4 *cbase = coef + coefOff[D];
+  const R4 *cbase = coef + tileCoef[t];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   R a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
   for (int c0 = 0; c0 < ncol; c0 += NEAR_CHUNK) {
@@ -557,7 +579,7 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
     for (int r = 0; r < 4; r++) {
       const int row = w + 4 * r;
       if (row < rows) {
-        const R4 *cr = cbase + (long long)(r0 + row) * ncol + c0;
+        const R4 *cr = cbase + (long long)row * ncol + c0;
         R s0 = 0, s1 = 0;
         int j = lane;
         for (; j + 64 < cn; j += 128) {
@@ -591,6 +613,19 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
+                                                    const int *__restrict__ leafCount,
+                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                                    const long long *__restrict__ tileCoef,
+                                                    const int *__restrict__ srcList,
+                                                    const typename Vec4<R>::type *__restrict__ coef,
+                                                    const double *__restrict__ x, R *__restrict__ out, int n) {
+  __shared__ R sS[NEAR_CHUNK], sN[NEAR_CHUNK];
+  nearStreamTile<R>(blockIdx.x, tiles, leafStart, leafCount, srcOff, ncolArr, tileCoef, srcList, coef, x, out, n,
+                    sS, sN);
+}
+
 // ------------------------------------------- FP64 nearfield computed on the fly
 // The plain form's nearfield without the coefficient stream: at q = 1 every
 // source but the panel itself by KER = A_x A_y ker4(p_x - p_y) in FP64, plus
@@ -599,18 +634,20 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
 // staged in LDS.  The values differ from the stored coefficients only by the
 // rounding of the formula's operation order.
 #define FLY64_CHUNK 512
-__global__ __launch_bounds__(256) void k_near_fly64(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
-                                                    const int *__restrict__ leafCount,
-                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
-                                                    const int *__restrict__ srcList, const double4 *__restrict__ PA,
-                                                    const double4 *__restrict__ PN, const double *__restrict__ x,
-                                                    const long long *__restrict__ specOff,
-                                                    const int *__restrict__ specCol,
-                                                    const double4 *__restrict__ specVal, double *__restrict__ out,
-                                                    int n, double kappa, double eps) {
-  __shared__ double4 sP[FLY64_CHUNK], sN[FLY64_CHUNK];
-  __shared__ int sY[FLY64_CHUNK];
-  const int2 tl = tiles[blockIdx.x];
+// TR: the transposed operator's row x, i.e. KER(y, x) read transposed: with
+// r, n_x, n_y as below, y0 and y3 are symmetric in (x, y) and
+//   y1(y,x) = -(eps coef - 1) dG0dx,  y2(y,x) = -(coef/eps - 1) dG0dy,
+//   out[x] += y3 A sgm_n + y1(y,x) A sgm,  out[x+n] += y2(y,x) A sgm_n + y0 A sgm
+template <bool TR>
+__device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ tiles,
+                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                              const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                              const int *__restrict__ srcList, const double4 *__restrict__ PA,
+                                              const double4 *__restrict__ PN, const double *__restrict__ x,
+                                              const long long *__restrict__ specOff, const int *__restrict__ specCol,
+                                              const double4 *__restrict__ specVal, double *__restrict__ out, int n,
+                                              double kappa, double eps, double4 *sP, double4 *sN, int *sY) {
+  const int2 tl = tiles[t];
   const int D = tl.x, r0 = tl.y;
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
   const int ncol = ncolArr[D];
@@ -648,11 +685,18 @@ __global__ __launch_bounds__(256) void k_near_fly64(const int2 *__restrict__ til
         const double dG0dy = ipY * G0 * ri2, dG0dx = ipX * G0 * ri2;
         const double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) * ri2;
         const double y0 = G0 - Gk;
-        const double y1 = eps * (coef * dG0dy) - dG0dy;
-        const double y2 = coef * dG0dx * ieps - dG0dx;
         const double y3 = (coef * ddG0 - kap2 * e * ipX * dG0dy) - ddG0;
-        s0 += y0 * N.w + y1 * P.w;
-        s1 += y2 * N.w + y3 * P.w;
+        if constexpr (TR) {
+          const double y1 = -(eps * (coef * dG0dx) - dG0dx);
+          const double y2 = -(coef * dG0dy * ieps - dG0dy);
+          s0 += y3 * N.w + y1 * P.w;
+          s1 += y2 * N.w + y0 * P.w;
+        } else {
+          const double y1 = eps * (coef * dG0dy) - dG0dy;
+          const double y2 = coef * dG0dx * ieps - dG0dx;
+          s0 += y0 * N.w + y1 * P.w;
+          s1 += y2 * N.w + y3 * P.w;
+        }
       }
       a0[r] += s0;
       a1[r] += s1;
@@ -675,6 +719,37 @@ __global__ __launch_bounds__(256) void k_near_fly64(const int2 *__restrict__ til
       out[xi] = t0;
       out[xi + n] = t1;
     }
+  }
+}
+
+// The mixed nearfield: tiles with a stored coefficient block (tileCoef >= 0)
+// stream it (HBM-bound), the others compute their pairs on the fly
+// (VALU-bound).  The host interleaves the two kinds in block order, so every
+// CU holds both at once and the stream and the arithmetic overlap.
+#define MIX64_LDS (2 * FLY64_CHUNK * 32 + FLY64_CHUNK * 4)
+static_assert(MIX64_LDS >= 2 * NEAR_CHUNK * 8, "the stream tiles' LDS must fit the union");
+template <bool TR = false>
+__global__ __launch_bounds__(256) void k_near_mix64(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
+                                                    const int *__restrict__ leafCount,
+                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                                    const long long *__restrict__ tileCoef,
+                                                    const int *__restrict__ srcList,
+                                                    const double4 *__restrict__ coef, const double4 *__restrict__ PA,
+                                                    const double4 *__restrict__ PN, const double *__restrict__ x,
+                                                    const long long *__restrict__ specOff,
+                                                    const int *__restrict__ specCol,
+                                                    const double4 *__restrict__ specVal, double *__restrict__ out,
+                                                    int n, double kappa, double eps) {
+  __shared__ __align__(16) unsigned char lds[MIX64_LDS];
+  const int t = blockIdx.x;
+  if (tileCoef[t] >= 0) {
+    double *sS = reinterpret_cast<double *>(lds);
+    nearStreamTile<double>(t, tiles, leafStart, leafCount, srcOff, ncolArr, tileCoef, srcList, coef, x, out, n, sS,
+                           sS + NEAR_CHUNK);
+  } else {
+    double4 *sP = reinterpret_cast<double4 *>(lds);
+    nearFly64Tile<TR>(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, x, specOff, specCol, specVal,
+                      out, n, kappa, eps, sP, sP + FLY64_CHUNK, reinterpret_cast<int *>(sP + 2 * FLY64_CHUNK));
   }
 }
 
@@ -774,7 +849,8 @@ __global__ __launch_bounds__(256) void k_spec_index(const Pnl *__restrict__ pnl,
 // correction of each listed pair: the Galerkin rule minus the one-point
 // formula at q = 1 (the whole rule for the panel itself); DK: of d/dkappa.
 // One block per row; the result goes to v32 (FP32) or v64.
-template <bool DK>
+// TR: of the transposed operator, (KER3, KER1, KER2, KER0) of the pair (y, x)
+template <bool DK, bool TR = false>
 __global__ __launch_bounds__(64) void k_spec_vals(const Pnl *__restrict__ pnl, const long long *__restrict__ specOff,
                                                   const int *__restrict__ specCol, float4 *__restrict__ v32,
                                                   double4 *__restrict__ v64, double kappa, double eps, QTab Q,
@@ -788,16 +864,27 @@ __global__ __launch_bounds__(64) void k_spec_vals(const Pnl *__restrict__ pnl, c
     const int y = specCol[e];
     const Pnl Y = pnl[y];
     double k[4], f[4] = {0, 0, 0, 0};
-    panelIA0(X, Y, x == y, kappa, eps, Q, q, k, DK);
+    if (TR) panelIA0(Y, X, x == y, kappa, eps, Q, q, k, DK);
+    else panelIA0(X, Y, x == y, kappa, eps, Q, q, k, DK);
     if (x != y) {
       double r[3];
       for (int d = 0; d < 3; d++) r[d] = px[d] - (Y.v[d] + t * (Y.a[6 + d] + t * Y.a[d]));
-      if (DK) ker4dk(kappa, eps, r, X.nr, Y.nr, f);
-      else ker4(kappa, eps, r, X.nr, Y.nr, f);
+      if (TR)
+        for (int d = 0; d < 3; d++) r[d] = -r[d];
+      const double *na = TR ? Y.nr : X.nr, *nb = TR ? X.nr : Y.nr;
+      if (DK) ker4dk(kappa, eps, r, na, nb, f);
+      else ker4(kappa, eps, r, na, nb, f);
       for (int i = 0; i < 4; i++) f[i] *= X.area * Y.area;
     }
-    if (v32) v32[e] = make_float4((float)(k[0] - f[0]), (float)(k[1] - f[1]), (float)(k[2] - f[2]), (float)(k[3] - f[3]));
-    else v64[e] = make_double4(k[0] - f[0], k[1] - f[1], k[2] - f[2], k[3] - f[3]);
+    double d[4];
+    for (int i = 0; i < 4; i++) d[i] = k[i] - f[i];
+    if (TR) {
+      const double d0 = d[0];
+      d[0] = d[3];
+      d[3] = d0;
+    }
+    if (v32) v32[e] = make_float4((float)d[0], (float)d[1], (float)d[2], (float)d[3]);
+    else v64[e] = make_double4(d[0], d[1], d[2], d[3]);
   }
 }
 
@@ -816,19 +903,17 @@ __device__ __forceinline__ void flyExp(float t, float e, float &omE, float &cm1)
 }
 
 #define FLY_CHUNK 512
-__global__ __launch_bounds__(256) void k_near_fly32(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
-                                                    const int *__restrict__ leafCount,
-                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
-                                                    const int *__restrict__ srcList, const float4 *__restrict__ PA,
-                                                    const float4 *__restrict__ PN, const float4 *__restrict__ PL,
-                                                    const double *__restrict__ x,
-                                                    const long long *__restrict__ specOff,
-                                                    const int *__restrict__ specCol,
-                                                    const float4 *__restrict__ specVal, float *__restrict__ out, int n,
-                                                    float kappa, float eps) {
-  // (p hi, A sigma), (n, A sigma_n), (p lo, source panel index)
-  __shared__ float4 sP[FLY_CHUNK], sN[FLY_CHUNK], sL[FLY_CHUNK];
-  const int2 tl = tiles[blockIdx.x];
+// LDS: (p hi, A sigma), (n, A sigma_n), (p lo, source panel index), FLY_CHUNK each
+__device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ tiles,
+                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                              const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                              const int *__restrict__ srcList, const float4 *__restrict__ PA,
+                                              const float4 *__restrict__ PN, const float4 *__restrict__ PL,
+                                              const double *__restrict__ x, const long long *__restrict__ specOff,
+                                              const int *__restrict__ specCol, const float4 *__restrict__ specVal,
+                                              float *__restrict__ out, int n, float kappa, float eps, float4 *sP,
+                                              float4 *sN, float4 *sL) {
+  const int2 tl = tiles[t];
   const int D = tl.x, r0 = tl.y;
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
   const int ncol = ncolArr[D];
@@ -909,6 +994,33 @@ __global__ __launch_bounds__(256) void k_near_fly32(const int2 *__restrict__ til
       out[xi[r]] = t0;
       out[xi[r] + n] = t1;
     }
+  }
+}
+
+#define MIX32_LDS (3 * FLY_CHUNK * 16)
+static_assert(MIX32_LDS >= 2 * NEAR_CHUNK * 4, "the stream tiles' LDS must fit the union");
+__global__ __launch_bounds__(256) void k_near_mix32(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
+                                                    const int *__restrict__ leafCount,
+                                                    const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
+                                                    const long long *__restrict__ tileCoef,
+                                                    const int *__restrict__ srcList,
+                                                    const float4 *__restrict__ coef, const float4 *__restrict__ PA,
+                                                    const float4 *__restrict__ PN, const float4 *__restrict__ PL,
+                                                    const double *__restrict__ x,
+                                                    const long long *__restrict__ specOff,
+                                                    const int *__restrict__ specCol,
+                                                    const float4 *__restrict__ specVal, float *__restrict__ out, int n,
+                                                    float kappa, float eps) {
+  __shared__ __align__(16) unsigned char lds[MIX32_LDS];
+  const int t = blockIdx.x;
+  if (tileCoef[t] >= 0) {
+    float *sS = reinterpret_cast<float *>(lds);
+    nearStreamTile<float>(t, tiles, leafStart, leafCount, srcOff, ncolArr, tileCoef, srcList, coef, x, out, n, sS,
+                          sS + NEAR_CHUNK);
+  } else {
+    float4 *sP = reinterpret_cast<float4 *>(lds);
+    nearFly32Tile(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, PL, x, specOff, specCol, specVal,
+                  out, n, kappa, eps, sP, sP + FLY_CHUNK, sP + 2 * FLY_CHUNK);
   }
 }
 
@@ -2444,7 +2556,9 @@ struct fbh_ctx {
       *ordM2LD;
   long long nInter;
   int *srcOff, *ncol, *srcList;
-  long long *coefOff;
+  long long *tileCoef = nullptr, *dTileCoef = nullptr;   // per tile
+  long long nStored = 0;       // interactions of the streamed tiles (the coefficient store)
+  int nFlyTiles = 0;           // tiles computed on the fly
   double4 *coef;
   int2 *tiles;
   int nTiles;
@@ -2479,7 +2593,7 @@ struct fbh_ctx {
   long long nSpec = 0;
   // FP64 nearfield on the fly (q = 1): panel data and the corrections in FP64
   bool fly64 = false;
-  double4 *fly64PA = nullptr, *fly64PN = nullptr, *specVal64 = nullptr;
+  double4 *fly64PA = nullptr, *fly64PN = nullptr, *specVal64 = nullptr, *specValT64 = nullptr;
   double *ro = nullptr, *dx = nullptr;   // refinement residual and correction
   void *enArena = nullptr;                // energy: charge tree, moments and scratch
   size_t enArenaBytes = 0;
@@ -2568,7 +2682,8 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
                   (void *)c->pbTmp, (void *)c->pbInvT})
     if (q) hipFree(q);
   if (c->blas) rocblas_destroy_handle(c->blas);
-  for (void *q : {(void *)c->dSrcOff, (void *)c->dNcol, (void *)c->dSrcList, (void *)c->dCoefOff, (void *)c->dCoef})
+  for (void *q : {(void *)c->dSrcOff, (void *)c->dNcol, (void *)c->dSrcList, (void *)c->dCoefOff, (void *)c->dCoef,
+                  (void *)c->dTileCoef})
     if (q) hipFree(q);
   if (c->dx) hipFree(c->dx);
   if (c->side) {
@@ -2943,14 +3058,49 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
       for (int k = m2lOff[cb]; k < m2lOff[cb + 1]; k++)
         if (cubeLevel[m2lSrc[k]] != cubeLevel[cb]) { c->m2lSym = false; break; }
   }
+  // Tiles that stream stored coefficients and tiles computed on the fly (q = 1
+  // only), interleaved in tile order so the HBM stream and the arithmetic run
+  // side by side: a tile streams while the streamed share of the interactions
+  // so far stays within the fraction f (FABIPB_NEAR_STREAM; FABIPB_NEAR=stream
+  // or fly for all of one kind).
+  double f = 0.45;
+  {
+    const char *fs = getenv("FABIPB_NEAR_STREAM"), *fm = getenv("FABIPB_NEAR");
+    if (fs) f = std::min(1.0, std::max(0.0, atof(fs)));
+    if (fm && !strcmp(fm, "stream")) f = 1.0;
+    if (fm && !strcmp(fm, "fly")) f = 0.0;
+    if (qorder != 1) f = 1.0;
+  }
+  std::vector<long long> tileCoef(tiles.size());
+  {
+    long long seen = 0;
+    c->nStored = 0;
+    c->nFlyTiles = 0;
+    for (size_t t = 0; t < tiles.size(); t++) {
+      const int D = tiles[t].x, r0 = tiles[t].y, rows = std::min(NEAR_ROWS, leafCount[D] - r0);
+      const long long inter = (long long)rows * ncol[D];
+      if ((double)(c->nStored + inter) <= f * (double)(seen + inter)) {
+        tileCoef[t] = c->nStored;
+        c->nStored += inter;
+      } else {
+        tileCoef[t] = -1;
+        c->nFlyTiles++;
+      }
+      seen += inter;
+    }
+  }
   if (!(c->srcOff = dupload(c, srcOff.data(), nLeaf + 1)) || !(c->ncol = dupload(c, ncol.data(), nLeaf)) ||
-      !(c->coefOff = dupload(c, coefOff.data(), nLeaf + 1)) ||
       !(c->srcList = dupload(c, srcList.data(), srcList.size())) ||
-      !(c->tiles = dupload(c, tiles.data(), tiles.size())) || !(c->coef = dalloc<double4>(c, (size_t)c->nInter)))
+      !(c->tiles = dupload(c, tiles.data(), tiles.size())) ||
+      !(c->tileCoef = dupload(c, tileCoef.data(), tiles.size())) ||
+      !(c->coef = dalloc<double4>(c, (size_t)std::max<long long>(c->nStored, 1))))
     FAIL();
-  hipLaunchKernelGGL(k_near_build, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart, c->srcOff,
-                     c->ncol, c->coefOff, c->srcList, c->coef, kappa, c->eps, c->Q, qorder);
-  if (hipGetLastError() != hipSuccess) { setErr("k_near_build launch failed"); FAIL(); }
+  if (c->nStored > 0) {
+    hipLaunchKernelGGL(k_near_build_tiles<false>, dim3(c->nTiles, NEAR_ROWS), dim3(256), 0, c->stream, c->pnl, c->tiles,
+                       c->leafStart, c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coef, kappa,
+                       c->eps, c->Q, qorder);
+    if (hipGetLastError() != hipSuccess) { setErr("k_near_build_tiles launch failed"); FAIL(); }
+  }
   // M2L derivative tensors per pair, cached when they fit (1344 B per pair at p=6)
   c->m2lG = nullptr;
   if (!buildM2LPlan(c)) FAIL();
@@ -2967,10 +3117,7 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
     if (hipGetLastError() != hipSuccess) { setErr("k_m2l_derivs launch failed"); FAIL(); }
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { setErr("setup kernels failed: %s", hipGetErrorString(hipGetLastError())); FAIL(); }
-  {
-    const char *ne = getenv("FABIPB_NEAR");   // "fly": the FP64 nearfield computed on the fly (q = 1)
-    if (qorder == 1 && ne && !strcmp(ne, "fly") && !enableFly64(c)) FAIL();
-  }
+  if (c->nFlyTiles > 0 && !enableFly64(c)) FAIL();
 #undef FAIL
   return c;
 }
@@ -3055,30 +3202,25 @@ static void farPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t
 
 template <typename R>
 static void nearPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
-  if constexpr (std::is_same<R, double>::value) {
-    if (c->fly64) {
-      hipLaunchKernelGGL(k_near_fly64, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
-                         c->srcOff, c->ncol, c->srcList, c->fly64PA, c->fly64PN, x, c->specOff, c->specCol,
-                         c->specVal64, B.nearv, c->n, c->kappa, c->eps);
-      return;
-    }
-  }
-  if constexpr (std::is_same<R, float>::value) {
-    if (c->fly) {
-      hipLaunchKernelGGL(k_near_fly32, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
-                         c->srcOff, c->ncol, c->srcList, c->flyPA, c->flyPN, c->flyPL, x, c->specOff, c->specCol, c->specVal,
-                         B.nearv, c->n, (float)c->kappa, (float)c->eps);
-      return;
-    }
+  if (c->nFlyTiles > 0) {
+    if constexpr (std::is_same<R, double>::value)
+      hipLaunchKernelGGL(k_near_mix64<false>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
+                         c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->fly64PA, c->fly64PN, x, c->specOff,
+                         c->specCol, c->specVal64, B.nearv, c->n, c->kappa, c->eps);
+    else
+      hipLaunchKernelGGL(k_near_mix32, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
+                         c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->flyPA, c->flyPN, c->flyPL, x,
+                         c->specOff, c->specCol, c->specVal, B.nearv, c->n, (float)c->kappa, (float)c->eps);
+    return;
   }
   hipLaunchKernelGGL(k_near_apply<R>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
-                     c->srcOff, c->ncol, c->coefOff, c->srcList, B.coef, x, B.nearv, c->n);
+                     c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, x, B.nearv, c->n);
 }
 
 // the dense direct operator: the nearfield kernel over every panel pair
 static void directPass(fbh_ctx *c, const double *x, double *out, hipStream_t st) {
   hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
-                     c->dSrcOff, c->dNcol, c->dCoefOff, c->dSrcList, c->dCoef, x, out, c->n);
+                     c->dSrcOff, c->dNcol, c->dTileCoef, c->dSrcList, c->dCoef, x, out, c->n);
 }
 
 // y = alpha*(D - K) x + beta*y; the nearfield runs on the side stream beside
@@ -3129,13 +3271,21 @@ static int ensureAdjoint(fbh_ctx *c) {
   }
   if (!c->m2lG) { setErr("the adjoint operator needs the cached M2L tensors"); return 0; }
   const size_t ne = (size_t)c->nCube * 2 * c->nmMax;
-  if (!(c->coefT = dalloc<double4>(c, (size_t)c->nInter)) || !(c->adjL = dalloc<double>(c, ne)) ||
-      !(c->adjM = dalloc<double>(c, ne)))
+  if (!(c->coefT = dalloc<double4>(c, (size_t)std::max<long long>(c->nStored, 1))) ||
+      !(c->adjL = dalloc<double>(c, ne)) || !(c->adjM = dalloc<double>(c, ne)))
     return 0;
   HCK(hipMemsetAsync(c->adjL, 0, ne * sizeof(double), c->stream));
   HCK(hipMemsetAsync(c->adjM, 0, ne * sizeof(double), c->stream));
-  hipLaunchKernelGGL(k_near_build_t, dim3(c->n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart,
-                     c->srcOff, c->ncol, c->coefOff, c->srcList, c->coefT, c->kappa, c->eps, c->Q, c->qorder);
+  // the streamed tiles' transposed blocks, the others' transposed corrections
+  if (c->nStored > 0)
+    hipLaunchKernelGGL(k_near_build_tiles<true>, dim3(c->nTiles, NEAR_ROWS), dim3(256), 0, c->stream, c->pnl,
+                       c->tiles, c->leafStart, c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT,
+                       c->kappa, c->eps, c->Q, c->qorder);
+  if (c->nFlyTiles > 0) {
+    if (!(c->specValT64 = dalloc<double4>(c, (size_t)c->nSpec))) return 0;
+    hipLaunchKernelGGL((k_spec_vals<false, true>), dim3(c->n), dim3(64), 0, c->stream, c->pnl, c->specOff,
+                       c->specCol, (float4 *)nullptr, c->specValT64, c->kappa, c->eps, c->Q, c->qorder);
+  }
   HCK(hipGetLastError());
   HCK(hipStreamSynchronize(c->stream));
   return 1;
@@ -3174,8 +3324,13 @@ static int matvecTr(fbh_ctx *c, double alpha, const double *x, double beta, doub
   if (!ensureAdjoint(c)) return 0;
   HCK(hipEventRecord(c->evFork, c->stream));
   HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
-  hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
-                     c->leafCount, c->srcOff, c->ncol, c->coefOff, c->srcList, c->coefT, x, c->nearv, c->n);
+  if (c->nFlyTiles > 0)
+    hipLaunchKernelGGL(k_near_mix64<true>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
+                       c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT, c->fly64PA, c->fly64PN, x,
+                       c->specOff, c->specCol, c->specValT64, c->nearv, c->n, c->kappa, c->eps);
+  else
+    hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
+                       c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT, x, c->nearv, c->n);
   HCK(hipEventRecord(c->evJoin, c->side));
   farPassT(c, x, c->stream);
   HCK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
@@ -3216,8 +3371,7 @@ static int ensureSpec(fbh_ctx *c) {
   return 1;
 }
 
-// FP64 nearfield on the fly (q = 1; FABIPB_NEAR=fly selects it, the stored
-// coefficients stay the default)
+// panel data and special-pair corrections of the tiles computed on the fly
 static int enableFly64(fbh_ctx *c) {
   const size_t n = c->n;
   if (!(c->fly64PA = dalloc<double4>(c, n)) || !(c->fly64PN = dalloc<double4>(c, n)) || !ensureSpec(c) ||
@@ -3248,8 +3402,13 @@ extern "C" int fbh_enable_fp32(fbh_ctx *c) {
   cvt(c->Q0, c->Q0f, nq);
   cvt(c->Q1, c->Q1f, nq);
   cvt(c->m2lG, c->m2lGf, ng);
-  const char *fe = getenv("FABIPB_FP32_NEAR");   // "stream": FP32 copies of the stored coefficients
-  c->fly = c->qorder == 1 && !(fe && !strcmp(fe, "stream"));
+  // the streamed tiles read FP32 copies of their stored coefficients, the
+  // others compute in FP32 on the fly
+  c->fly = c->nFlyTiles > 0;
+  if (c->nStored > 0) {
+    if (!(c->coef32 = dalloc<float4>(c, (size_t)c->nStored))) return 0;
+    cvt((const double *)c->coef, (float *)c->coef32, 4 * (size_t)c->nStored);
+  }
   if (c->fly) {
     if (!(c->flyPA = dalloc<float4>(c, n)) || !(c->flyPN = dalloc<float4>(c, n)) ||
         !(c->flyPL = dalloc<float4>(c, n)) || !ensureSpec(c) ||
@@ -3259,9 +3418,6 @@ extern "C" int fbh_enable_fp32(fbh_ctx *c) {
                        c->flyPA, c->flyPN, c->flyPL);
     hipLaunchKernelGGL(k_spec_vals<false>, dim3(n), dim3(64), 0, c->stream, c->pnl, c->specOff, c->specCol,
                        c->specVal, (double4 *)nullptr, c->kappa, c->eps, c->Q, c->qorder);
-  } else {
-    if (!(c->coef32 = dalloc<float4>(c, (size_t)c->nInter))) return 0;
-    cvt((const double *)c->coef, (float *)c->coef32, 4 * (size_t)c->nInter);
   }
   HCK(hipMemsetAsync(c->momf, 0, 2 * ne * sizeof(float), c->stream));
   HCK(hipMemsetAsync(c->lecf, 0, 4 * ne * sizeof(float), c->stream));
@@ -3312,7 +3468,7 @@ extern "C" int fbh_set_direct(fbh_ctx *c, int on) {
   if (!c) { setErr("fbh_set_direct: null context"); return 0; }
   if (!on) {
     for (void **q : {(void **)&c->dSrcOff, (void **)&c->dNcol, (void **)&c->dSrcList, (void **)&c->dCoefOff,
-                     (void **)&c->dCoef}) {
+                     (void **)&c->dCoef, (void **)&c->dTileCoef}) {
       if (*q) hipFree(*q);
       *q = nullptr;
     }
@@ -3332,21 +3488,28 @@ extern "C" int fbh_set_direct(fbh_ctx *c, int on) {
   for (int l = 0; l < c->nLeaf; l++) coefOff[l] = (long long)c->hLeafStart[l] * (long long)n;
   coefOff[c->nLeaf] = (long long)(n * n);
   for (size_t i = 0; i < n; i++) srcList[i] = (int)i;
+  std::vector<long long> tileOff;   // the tiles of fbh_create, in its order
+  for (int l = 0; l < c->nLeaf; l++)
+    for (int r = 0; r < c->hLeafCount[l]; r += NEAR_ROWS) tileOff.push_back(coefOff[l] + (long long)r * (long long)n);
   auto fail = [&](const char *what) {
     setErr("fbh_set_direct: %s", what);
     fbh_set_direct(c, 0);
     return 0;
   };
+  if (tileOff.size() != (size_t)c->nTiles) return fail("tile layout mismatch");
   if (hipMalloc(&c->dSrcOff, srcOff.size() * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->dNcol, ncol.size() * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->dSrcList, n * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->dCoefOff, coefOff.size() * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->dTileCoef, tileOff.size() * sizeof(long long)) != hipSuccess ||
       hipMalloc(&c->dCoef, need) != hipSuccess)
     return fail("device allocation failed");
   if (hipMemcpy(c->dSrcOff, srcOff.data(), srcOff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->dNcol, ncol.data(), ncol.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->dSrcList, srcList.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->dCoefOff, coefOff.data(), coefOff.size() * sizeof(long long), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->dTileCoef, tileOff.data(), tileOff.size() * sizeof(long long), hipMemcpyHostToDevice) !=
           hipSuccess)
     return fail("upload failed");
   hipLaunchKernelGGL(k_near_build, dim3(c->n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->leafStart, c->dSrcOff,
@@ -3404,19 +3567,23 @@ extern "C" int fbh_near_coeff(fbh_ctx *c, int x, int y, double *ker4) {
   HCK(hipStreamSynchronize(c->stream));
   int D = 0;
   std::vector<int> ls(c->nLeaf), lc(c->nLeaf), so(c->nLeaf + 1), nc(c->nLeaf);
-  std::vector<long long> co(c->nLeaf + 1);
   HCK(hipMemcpy(ls.data(), c->leafStart, c->nLeaf * sizeof(int), hipMemcpyDeviceToHost));
   HCK(hipMemcpy(lc.data(), c->leafCount, c->nLeaf * sizeof(int), hipMemcpyDeviceToHost));
   HCK(hipMemcpy(so.data(), c->srcOff, (c->nLeaf + 1) * sizeof(int), hipMemcpyDeviceToHost));
   HCK(hipMemcpy(nc.data(), c->ncol, c->nLeaf * sizeof(int), hipMemcpyDeviceToHost));
-  HCK(hipMemcpy(co.data(), c->coefOff, (c->nLeaf + 1) * sizeof(long long), hipMemcpyDeviceToHost));
   while (D < c->nLeaf && !(x >= ls[D] && x < ls[D] + lc[D])) D++;
   std::vector<int> sl(nc[D]);
   HCK(hipMemcpy(sl.data(), c->srcList + so[D], nc[D] * sizeof(int), hipMemcpyDeviceToHost));
   for (int j = 0; j < nc[D]; j++)
     if (sl[j] == y) {
-      HCK(hipMemcpy(ker4, c->coef + co[D] + (long long)(x - ls[D]) * nc[D] + j, 4 * sizeof(double),
-                    hipMemcpyDeviceToHost));
+      double *d = nullptr;
+      HCK(hipMalloc(&d, 4 * sizeof(double)));
+      hipLaunchKernelGGL(k_near_pair, dim3(1), dim3(1), 0, c->stream, c->pnl, x, y, c->kappa, c->eps, c->Q,
+                         c->qorder, d);
+      hipError_t e = hipMemcpyAsync(ker4, d, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      hipFree(d);
+      if (e != hipSuccess) { setErr("fbh_near_coeff: %s", hipGetErrorString(e)); return 0; }
       return 1;
     }
   setErr("fbh_near_coeff: panels %d and %d are not near neighbours", x, y);
@@ -4199,6 +4366,7 @@ extern "C" int fbh_stats(fbh_ctx *c, long long *out) {
   if (!c) return 0;
   out[0] = c->n; out[1] = c->nInter; out[2] = c->nM2L; out[3] = c->nmMax; out[4] = c->nmLeaf;
   out[5] = (long long)c->bytes; out[6] = c->nLinks; out[7] = c->nCube;
+  out[8] = c->nStored; out[9] = c->nFlyTiles; out[10] = c->nTiles; out[11] = c->nSpec;
   return 1;
 }
 

@@ -142,3 +142,26 @@ def test_autograd_wrapper(fa, meshdir):
     assert abs(float(E) - e0) <= 1e-12 * abs(e0)
     assert np.max(np.abs(q.grad.numpy() - dq)) <= 1e-12 * np.max(np.abs(dq))
     assert abs(float(k.grad) - dk) <= 1e-12 * abs(dk)
+
+
+@pytest.mark.parametrize("frac", ["0", "1"])
+def test_adjoint_identity_near_modes(fa, meshdir, monkeypatch, frac):
+    """The transpose identity with every nearfield tile computed on the fly (0) and with
+    every tile streaming stored coefficients (1); the default mixes the two."""
+    import torch
+    monkeypatch.setenv("FABIPB_NEAR_STREAM", frac)
+    mesh = fa.Mesh(str(meshdir / "glob4"))
+    P = fa.Problem(mesh, CASES["cases"]["glob4"]["t"], order=FL["p"], sep=FL["S"], eps1=FL["eps1"],
+                   eps2=FL["eps2"], qorder=FL["q"], height=FL["H"])
+    op = P.operator(FL["k"])
+    st = op.stats()
+    assert (st["near_fly_tiles"] == 0) == (frac == "1") and (st["near_stored"] == 0) == (frac == "0")
+    n2 = 2 * P.tree.n
+    g = torch.Generator(device="cpu").manual_seed(6)
+    x = (torch.rand(n2, generator=g, dtype=torch.float64) * 2 - 1).cuda()
+    y = (torch.rand(n2, generator=g, dtype=torch.float64) * 2 - 1).cuda()
+    Ay, ATx = torch.empty_like(x), torch.empty_like(x)
+    op.matvec_dev(y, Ay)
+    op.matvec_t_dev(x, ATx)
+    lhs, rhs = float(torch.dot(ATx, y)), float(torch.dot(x, Ay))
+    assert abs(lhs - rhs) <= 1e-11 * float(ATx.norm() * y.norm())

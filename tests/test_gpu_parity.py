@@ -125,6 +125,33 @@ def test_rhs(fa, meshdir, name):
     assert rel(op.rhs(mesh.pos, mesh.chr), F["rhs"]) <= 1e-13
 
 
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_near_modes_agree(fa, meshdir, monkeypatch, name):
+    """The nearfield launch mixes tiles that stream stored Galerkin coefficients with
+    tiles that evaluate the one-point formula on the fly (q = 1) plus stored
+    corrections for the self / shared-vertex pairs.  All-streamed, all-on-the-fly and
+    the default mix give the same matvec to rounding, in FP64 and in FP32."""
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, name)
+    g = torch.Generator(device="cpu").manual_seed(12)
+    x = (torch.rand(2 * tree.n, generator=g, dtype=torch.float64) * 2 - 1).cuda()
+    ys, y32 = {}, {}
+    for frac in ("0", "1", None):
+        if frac is None:
+            o = op
+        else:
+            monkeypatch.setenv("FABIPB_NEAR_STREAM", frac)
+            o = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+        o.enable_fp32()
+        ys[frac], y32[frac] = torch.empty_like(x), torch.empty_like(x)
+        o.matvec_dev(x, ys[frac])
+        o.matvec32_dev(x, y32[frac])
+    for frac in ("0", "1"):
+        assert rel(ys[frac].cpu().numpy(), ys[None].cpu().numpy()) <= 1e-14
+        assert rel(y32[frac].cpu().numpy(), ys[None].cpu().numpy()) <= 1e-5
+    assert rel(y32[None].cpu().numpy(), ys[None].cpu().numpy()) <= 1e-5
+
+
 @pytest.mark.parametrize("name", harness_cases())
 def test_energy_walk(fa, meshdir, name):
     import torch
