@@ -28,7 +28,12 @@ globule, 8000 charges, depth 5) and the differentiation form (the level-6
 globule with 5000 charges: adjoint gradient dE/dq, dE/dkappa against one
 forward solve).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-large] [--no-grad] [--calib-copy]
+The last block is the capsid-scale shell (2,621,440 panels, 1,000,000 charges,
+order 8, MAC 0.7, depth 6): charge-tree RHS and energy, FP64 and FP32-refined
+solves (skipped with less than 160 GB of free device memory).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-large] [--no-grad] [--no-capsid]
+                       [--calib-copy]
 """
 import argparse
 import importlib.util
@@ -50,6 +55,11 @@ DEPTH, ORDER, MAC, KAPPA, EPS1, EPS2, Q, TOL, RESTART, MAXIT = 4, 6, 0.8, 0.1257
 MESH_LARGE = ["globule", "--level", "7", "--radius", "40", "--merge", "0.05", "--charges", "inside", "--nq", "8000",
               "--sigma", "0.4", "--depth", "1.5", "--seed", "3"]
 DEPTH_LARGE = 5
+# capsid-scale shell: outer R = 250 A and inner R = 210 A level-8 surfaces (0.03 Y_lm noise, l <= 12),
+# 2,621,440 panels, 1,000,000 charges in the wall, Taylor order 8, MAC 0.7, depth 6 (84 panels per leaf)
+MESH_CAPSID = ["shell", "--level", "8", "--radius", "250", "--inner", "210", "--amp", "0.03", "--lmax", "12",
+               "--charges", "wall", "--nq", "1000000", "--sigma", "0.3", "--depth", "1.5", "--seed", "5"]
+DEPTH_CAPSID, ORDER_CAPSID, MAC_CAPSID = 6, 8, 0.7
 # differentiation form: the level-6 globule with 5000 charges
 MESH_GRAD = ["globule", "--level", "6", "--radius", "22", "--charges", "inside", "--nq", "5000", "--sigma", "0.4",
              "--depth", "1.5", "--seed", "4"]
@@ -163,6 +173,63 @@ def precision_large(fa, torch, tmp):
     return out
 
 
+def capsid(fa, torch, tmp):
+    """The capsid-scale shell: charge-tree RHS and energy (2.6e12 panel-charge pairs), FP64 and
+    FP32-refined solves with the diagonal preconditioner."""
+    free, _ = torch.cuda.mem_get_info()
+    if free < 160e9:
+        return {"skipped": f"{free / 1e9:.0f} GB free, the shell needs ~120 GB"}
+    base = os.path.join(tmp, "capsid")
+    subprocess.check_call([os.path.join(ROOT, "cuda-fabipb_amd", "bin", "fbmesh"), MESH_CAPSID[0], base] + MESH_CAPSID[1:],
+                          stdout=subprocess.DEVNULL)
+    t0 = time.perf_counter()
+    mesh = fa.Mesh(base)
+    tree = fa.Tree(mesh, DEPTH_CAPSID, order=ORDER_CAPSID, sep=MAC_CAPSID)
+    t_tree = time.perf_counter() - t0
+    op = fa.Operator(tree, KAPPA, EPS1, EPS2, Q)
+    op.set_precond(3)
+    op.enable_fp32()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0 - t_tree
+    ct = tree.charge_tree()
+    n2 = 2 * tree.n
+    b = torch.empty(n2, dtype=torch.float64, device="cuda")
+    out = {}
+    for prec in ("fp64", "fp32"):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        op.rhs_tree_dev(ct, mesh.pos, mesh.chr, b)
+        xs = torch.zeros_like(b) if prec == "fp32" else b.clone()
+        torch.cuda.synchronize()
+        tr = time.perf_counter()
+        gm = (op.gmres_refined_dev if prec == "fp32" else op.gmres_dev)(b, xs, restart=RESTART, tol=TOL, maxit=200)
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        raw = op.energy_dev(ct, mesh.pos, mesh.chr, xs)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        out[prec] = {"wall_s": te - ts, "rhs_s": tr - ts, "gmres_s": tg - tr, "energy_s": te - tg,
+                     "iterations": gm["iters"], "final_residual": gm["resid"], "energy_raw": raw}
+    x = torch.rand(n2, dtype=torch.float64, device="cuda") * 2 - 1
+    y = torch.empty_like(x)
+    st = op.stats()
+    res = {"panels": tree.n, "charges": int(len(mesh.chr)), "depth": DEPTH_CAPSID, "order": ORDER_CAPSID,
+           "mac": MAC_CAPSID, "near_interactions": st["near_interactions"], "near_stored": st["near_stored"],
+           "m2l_pairs": st["m2l_pairs"], "device_gb": st["device_bytes"] / 1e9, "tree_s": t_tree,
+           "setup_s": t_setup, "charge_tree_order": int(ct["order"][-1]),
+           "stages64_ms": op.profile(x, y, iters=2), "stages32_ms": op.profile(x, y, iters=2, precision="fp32"),
+           "fp64": out["fp64"], "fp32_refined": out["fp32"],
+           "wall_speedup": out["fp64"]["wall_s"] / out["fp32"]["wall_s"],
+           "energy_kcal_per_mol": out["fp64"]["energy_raw"] * fa.ENERGY_SCALE,
+           "energy_rel_err": abs(out["fp32"]["energy_raw"] - out["fp64"]["energy_raw"]) / abs(out["fp64"]["energy_raw"])}
+    print(f"[bench] capsid shell {tree.n} panels, {len(mesh.chr)} charges: fp64 {out['fp64']['wall_s']:.3f} s, "
+          f"fp32-refined {out['fp32']['wall_s']:.3f} s ({res['wall_speedup']:.2f}x), err {res['energy_rel_err']:.2e}",
+          file=sys.stderr, flush=True)
+    del op
+    torch.cuda.empty_cache()
+    return res
+
+
 def differentiation(fa, torch, np, tmp):
     """Adjoint gradient dE/dq (every charge) and dE/dkappa against one forward solve (direct energy)."""
     base = os.path.join(tmp, "glob6q")
@@ -214,6 +281,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-capsid", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU (one MI355X per box); --gpus must be 1", file=sys.stderr)
@@ -332,11 +400,15 @@ def main():
               f"outer={precision['outer']} err={precision['energy_rel_err_vs_fp64']:.2e} "
               f"matvec32={ms32:.3f} ms", file=sys.stderr, flush=True)
 
-    large = grad = None
+    large = grad = shell = None
     if not args.no_solve and not args.no_large:
         large = precision_large(fa, torch, tmp)
     if not args.no_solve and not args.no_grad:
         grad = differentiation(fa, torch, np, tmp)
+    if not args.no_solve and not args.no_capsid:
+        del op
+        torch.cuda.empty_cache()
+        shell = capsid(fa, torch, tmp)
 
     if args.calib_copy:
         calib_copy(torch)
@@ -385,6 +457,7 @@ def main():
         "precision_form": precision,
         "precision_form_large": large,
         "differentiation_form": grad,
+        "capsid_shell": shell,
     }
     print(json.dumps(out), flush=True)
 

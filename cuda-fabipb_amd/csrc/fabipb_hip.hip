@@ -3070,6 +3070,12 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
     if (fm && !strcmp(fm, "stream")) f = 1.0;
     if (fm && !strcmp(fm, "fly")) f = 0.0;
     if (qorder != 1) f = 1.0;
+    // the store is capped at a fifth of the free device memory (the FP32 copy
+    // and the adjoint's transposed store come on top): at capsid scale more of
+    // the nearfield is computed
+    size_t freeB = 0, totalB = 0;
+    if (qorder == 1 && hipMemGetInfo(&freeB, &totalB) == hipSuccess && c->nInter > 0)
+      f = std::min(f, 0.2 * (double)freeB / (32.0 * (double)c->nInter));
   }
   std::vector<long long> tileCoef(tiles.size());
   {

@@ -101,6 +101,7 @@ int main(int argc, char **argv) {
   int order = -1, orderMom = 0, numItr = 100, arnoldiSz = 30, depth = 5, height = 2, qOrd = 1;
   int gpuMode = -1, bench = 0, meshFlag = 1, meshOnly = 0, matvecMode = 0, precond = 3, precondExplicit = 0;
   double sep = 0.8, tol = 1.0e-4, kappa = 0.1257, eps1 = 1.0, eps2 = 80.0, para = 332.0716;
+  int sepExplicit = 0;
   for (int i = 1; i < argc; i++) {
     const char *a = argv[i], *v;
     if (a[0] != '-') {
@@ -109,7 +110,7 @@ int main(int argc, char **argv) {
       continue;
     }
     if (!strcmp(a, "-h")) { usage(argv[0]); return 0; }
-    if ((v = optValue(a, "S"))) sep = optDouble(a, v, 1e-12, 10.0);
+    if ((v = optValue(a, "S"))) { sep = optDouble(a, v, 1e-12, 10.0); sepExplicit = 1; }
     else if ((v = optValue(a, "o"))) tol = optDouble(a, v, 1e-300, 1.0);
     else if ((v = optValue(a, "a"))) arnoldiSz = optInt(a, v, 1, 100000);
     else if ((v = optValue(a, "i"))) numItr = optInt(a, v, 1, 1000000);
@@ -162,13 +163,16 @@ int main(int argc, char **argv) {
   if (meshOnly) { printf("Mesh-only mode: stopping after mesh generation.\n"); return 0; }
   /* preconditioner policy (src/fabipb_options.c:85-88, 198-265): capsid-scale
    * charge counts take the diagonal, high dielectric contrast the block LU */
-  if (!precondExplicit) {
+  {
     const char *ev = getenv("FABIPB_HUGE_CAPSID_ATOMS");
     if (!ev || !*ev) ev = getenv("FABIPB_Q2M_HUGE_CAPSID_ATOMS");
     unsigned long long huge = (ev && *ev) ? strtoull(ev, NULL, 10) : 5000000ULL;
     int hugeCapsid = (unsigned long long)mesh->nChar > huge;
-    int hi = eps1 > 0.0 && eps1 <= 2.0 && eps2 / eps1 >= 40.0;
-    precond = hugeCapsid ? 3 : hi ? 2 : 3;
+    if (!precondExplicit) {
+      int hi = eps1 > 0.0 && eps1 <= 2.0 && eps2 / eps1 >= 40.0;
+      precond = hugeCapsid ? 3 : hi ? 2 : 3;
+    }
+    if (!sepExplicit && hugeCapsid) sep = 1.2;   /* HUGE_CAPSID_SEP_RATIO, src/fabipb_options.c:216-218 */
   }
   double tLoad = now() - tStage;
   if (bench) {

@@ -433,7 +433,7 @@ fb_tree *fb_tree_build(const fb_mesh *m, int depth, int height, int order, int o
   countUp(&T);
 
   fb_tree *t = NEW(fb_tree, 1);
-  t->n = n; t->depth = depth; t->height = height;
+  t->n = n; t->depth = depth; t->height = height; t->nChar = m->nChar;
   t->perm = NEW(int, n);
   int idx = 0, nLeaf = 0;
   for (int c = T.first[depth]; c >= 0; c = T.c[c].next) {
@@ -685,10 +685,25 @@ int fb_should_use_tree_rhs(int nPnls, int nChar, int gpuMode) {
 }
 
 /* src/chargeTree.c:117-129: min(max(ordMom[level], 4), maxOrder) unless overridden */
+static unsigned long long envULL(const char *name, unsigned long long dflt) {
+  const char *e = getenv(name);
+  return (e && *e) ? strtoull(e, NULL, 10) : dflt;
+}
+
+/* src/chargeTree.c:117-129 with the capsid-scale policy of
+ * src/fabipb_options.c:90-97, 220-229: more than 5e6 charges
+ * (FABIPB_HUGE_CAPSID_ATOMS) or panels (FABIPB_HUGE_CAPSID_PANELS) fix the
+ * order at 3; FABIPB_CHARGE_TREE_ORDER overrides both */
 int fb_charge_tree_order(const fb_tree *t, int level) {
   const char *e = getenv("FABIPB_CHARGE_TREE_ORDER");
   int o;
+  const char *ea = getenv("FABIPB_HUGE_CAPSID_ATOMS");
+  if (!ea || !*ea) ea = getenv("FABIPB_Q2M_HUGE_CAPSID_ATOMS");
+  const unsigned long long atoms = (ea && *ea) ? strtoull(ea, NULL, 10) : 5000000ULL;
+  const int capsid = (unsigned long long)t->nChar > atoms ||
+                     (unsigned long long)t->n > envULL("FABIPB_HUGE_CAPSID_PANELS", 5000000ULL);
   if (e && atoi(e) >= 1 && atoi(e) <= 32) o = atoi(e);
+  else if (capsid) o = 3;
   else o = t->ordMom[level] > 4 ? t->ordMom[level] : 4;
   return o < t->maxOrder ? o : t->maxOrder;
 }

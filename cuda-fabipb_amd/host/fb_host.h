@@ -47,6 +47,7 @@ typedef struct {
   int *m2lOff;                    /* nCube+1: pairs of destination cube c */
   int maxLeafPanels;
   int nPrecondBlocks, maxPrecondBlock;
+  int nChar;                      /* charges of the mesh (capsid-scale policy) */
 } fb_tree;
 
 /* charge octree (src/chargeTree.c:228-307): all levels 0..depth, DFS per level */

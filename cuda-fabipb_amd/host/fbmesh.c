@@ -25,6 +25,13 @@
  *            a_lm ~ N(0,1)/l, real orthonormal Y_lm (no Condon-Shortley
  *            phase); vertex normals are the analytic surface normals,
  *            evaluated by central differences of the parametrisation.
+ *   shell    a hollow shell: an outer globule surface of radius R and an
+ *            inner one of radius --inner (both modulated with --amp, --lmax,
+ *            separate coefficients, outer drawn first), each a level-L
+ *            icosphere; the inner surface's faces are reversed and its
+ *            normals point toward the centre, i.e. out of the wall.  Its
+ *            radius function uses recurrences for P_l^m and cos/sin(m phi)
+ *            (the same function, evaluated faster for lmax ~ 12).
  * Charges:
  *   ball     nq charges uniform in the ball |x| <= rq, q ~ U[-1,1]
  *   center   one charge +q0 at the origin
@@ -32,6 +39,9 @@
  *            `depth` from every mesh triangle (exact point-triangle
  *            distance), q ~ N(0, sigma), shifted so the total is the
  *            nearest integer.
+ *   wall     (shell) nq charges uniform in the wall between the two
+ *            surfaces, at least `depth` from every triangle, q ~ N(0, sigma),
+ *            total shifted to the nearest integer.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
@@ -156,7 +166,7 @@ static void icosphere(mesh_t *m, int level) {
 }
 
 /* ---------------- real spherical harmonics ---------------- */
-typedef struct { int lmax; double amp, R; double *a; } shape_t; /* a[l*l + l + m] */
+typedef struct { int lmax; double amp, R; double *a; int fast; double *N; } shape_t; /* a, N: [l*l + l + m] */
 
 /* associated Legendre P_l^m(x) without Condon-Shortley phase, m >= 0 */
 static double plm(int l, int m, double x) {
@@ -190,12 +200,55 @@ static double shape_r(const shape_t *s, const double *u) {
     for (int m = -l; m <= l; m++) sum += s->a[l * l + l + m] * ylm(l, m, u);
   return s->R * (1.0 + s->amp * sum);
 }
+/* normalisations sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) of the real Y_lm, m >= 0 */
+static double *ylm_norms(int lmax) {
+  double *N = calloc((size_t)(lmax + 1) * (lmax + 1), sizeof(double));
+  for (int l = 0; l <= lmax; l++)
+    for (int m = 0; m <= l; m++) {
+      double lf = 1.0;
+      for (int i = l - m + 1; i <= l + m; i++) lf /= i;
+      N[l * l + l + m] = sqrt((2 * l + 1) / (4.0 * M_PI) * lf);
+    }
+  return N;
+}
+
+/* the same r(u) with the Legendre functions by recurrence in l and the
+ * azimuthal factors by recurrence in m: O(lmax^2) per direction */
+static double shape_r_fast(const shape_t *s, const double *u) {
+  if (s->lmax <= 0) return s->R;
+  enum { LM = 32 };
+  const int L = s->lmax < LM - 1 ? s->lmax : LM - 1;
+  const double x = u[2], st = sqrt(fmax(0.0, 1.0 - x * x)), ph = atan2(u[1], u[0]);
+  double cm[LM], sm[LM];
+  const double c1 = cos(ph), s1 = sin(ph);
+  cm[0] = 1.0; sm[0] = 0.0;
+  for (int m = 1; m <= L; m++) { cm[m] = cm[m - 1] * c1 - sm[m - 1] * s1; sm[m] = sm[m - 1] * c1 + cm[m - 1] * s1; }
+  double sum = 0.0, pmm = 1.0, fact = 1.0;
+  for (int m = 0; m <= L; m++) {
+    if (m > 0) { pmm *= fact * st; fact += 2.0; }
+    double pl2 = 0.0, pl1 = pmm;   /* P_{l-2}^m, P_{l-1}^m */
+    for (int l = m; l <= L; l++) {
+      double p;
+      if (l == m) p = pmm;
+      else if (l == m + 1) p = x * (2 * m + 1) * pmm;
+      else p = (x * (2 * l - 1) * pl1 - (l + m - 1) * pl2) / (l - m);
+      if (l > m) { pl2 = pl1; pl1 = p; }
+      if (l >= 1) {
+        const double N = s->N[l * l + l + m];
+        if (m == 0) sum += s->a[l * l + l] * N * p;
+        else sum += sqrt(2.0) * N * p * (s->a[l * l + l + m] * cm[m] + s->a[l * l + l - m] * sm[m]);
+      }
+    }
+  }
+  return s->R * (1.0 + s->amp * sum);
+}
+
 static void normalize3(double *x) {
   double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   x[0] /= n; x[1] /= n; x[2] /= n;
 }
 static void shape_point(const shape_t *s, const double *u, double *p) {
-  double r = shape_r(s, u);
+  double r = s->fast ? shape_r_fast(s, u) : shape_r(s, u);
   p[0] = r * u[0]; p[1] = r * u[1]; p[2] = r * u[2];
 }
 /* outward surface normal at direction u, by central differences of p(u) */
@@ -446,10 +499,138 @@ static FILE *open_out(const char *base, const char *ext) {
   return fp;
 }
 
+static void usage(void);
+
+/* vertices, normals and faces of both surfaces, then charges in the wall */
+static int make_shell(const char *base, int level, double R, double Rin, double amp, int lmax, const char *charges,
+                      int nq, double sigma, double depth, rng_t *rng, uint64_t seed) {
+  if (!(Rin > 0.0 && Rin < R) || strcmp(charges, "wall") != 0 || lmax > 30) usage();
+  shape_t so = {lmax, amp, R, calloc((size_t)(lmax + 1) * (lmax + 1), sizeof(double)), 1, ylm_norms(lmax)};
+  shape_t si = {lmax, amp, Rin, calloc((size_t)(lmax + 1) * (lmax + 1), sizeof(double)), 1, ylm_norms(lmax)};
+  for (int l = 1; l <= lmax; l++)
+    for (int m = -l; m <= l; m++) so.a[l * l + l + m] = rng_normal(rng) / l;
+  for (int l = 1; l <= lmax; l++)
+    for (int m = -l; m <= l; m++) si.a[l * l + l + m] = rng_normal(rng) / l;
+  mesh_t m;
+  icosphere(&m, level);
+  const int nv1 = m.nv, nf1 = m.nf, nv = 2 * nv1, nf = 2 * nf1;
+  double *pos = malloc(sizeof(double) * 3 * (size_t)nv), *nrm = malloc(sizeof(double) * 3 * (size_t)nv);
+  int *fc = malloc(sizeof(int) * 3 * (size_t)nf);
+  for (int i = 0; i < nv1; i++) {
+    const double *u = &m.v[3 * i];
+    shape_point(&so, u, &pos[3 * i]);
+    shape_normal(&so, u, &nrm[3 * i]);
+    shape_point(&si, u, &pos[3 * (nv1 + i)]);
+    shape_normal(&si, u, &nrm[3 * (nv1 + i)]);
+    for (int k = 0; k < 3; k++) nrm[3 * (nv1 + i) + k] = -nrm[3 * (nv1 + i) + k];
+  }
+  for (int f = 0; f < nf1; f++) {
+    for (int k = 0; k < 3; k++) fc[3 * f + k] = m.f[3 * f + k];
+    fc[3 * (nf1 + f)] = nv1 + m.f[3 * f];          /* reversed: the normal points inward */
+    fc[3 * (nf1 + f) + 1] = nv1 + m.f[3 * f + 2];
+    fc[3 * (nf1 + f) + 2] = nv1 + m.f[3 * f + 1];
+  }
+  /* charges: rejection in the bounding cube, radially inside the wall, then
+     the exact point-triangle distance against a uniform grid of centroids */
+  double rmax = 0.0, emax = 0.0;
+  for (int i = 0; i < nv; i++) rmax = fmax(rmax, sqrt(dot3(&pos[3 * i], &pos[3 * i])));
+  double *cen = malloc(sizeof(double) * 3 * (size_t)nf);
+  for (int f = 0; f < nf; f++) {
+    const double *a = &pos[3 * fc[3 * f]], *b = &pos[3 * fc[3 * f + 1]], *c = &pos[3 * fc[3 * f + 2]];
+    for (int k = 0; k < 3; k++) cen[3 * f + k] = (a[k] + b[k] + c[k]) / 3.0;
+    for (int k = 0; k < 3; k++) {
+      const double *v = (k == 0) ? a : (k == 1) ? b : c;
+      double d2 = 0.0;
+      for (int d = 0; d < 3; d++) d2 += (v[d] - cen[3 * f + d]) * (v[d] - cen[3 * f + d]);
+      emax = fmax(emax, sqrt(d2));
+    }
+  }
+  const double reach = depth + emax, reach2 = reach * reach, depth2 = depth * depth;
+  const int G = (int)fmin(256.0, fmax(1.0, floor(2.0 * rmax / reach)));
+  const double cell = 2.0 * rmax / G;
+  const size_t G3 = (size_t)G * G * G;
+  int *gstart = calloc(G3 + 1, sizeof(int)), *gface = malloc(sizeof(int) * (size_t)nf), *gcell = malloc(sizeof(int) * (size_t)nf);
+  for (int f = 0; f < nf; f++) {
+    int ci[3];
+    for (int k = 0; k < 3; k++) {
+      ci[k] = (int)floor((cen[3 * f + k] + rmax) / cell);
+      ci[k] = ci[k] < 0 ? 0 : ci[k] >= G ? G - 1 : ci[k];
+    }
+    gcell[f] = (ci[0] * G + ci[1]) * G + ci[2];
+    gstart[gcell[f] + 1]++;
+  }
+  for (size_t i = 0; i < G3; i++) gstart[i + 1] += gstart[i];
+  {
+    int *fill = malloc(sizeof(int) * G3);
+    memcpy(fill, gstart, sizeof(int) * G3);
+    for (int f = 0; f < nf; f++) gface[fill[gcell[f]]++] = f;
+    free(fill);
+  }
+  double *qx = malloc(sizeof(double) * 3 * (size_t)nq), *qq = malloc(sizeof(double) * (size_t)nq);
+  long tries = 0;
+  for (int j = 0; j < nq;) {
+    if (++tries > 4000000000L) { fprintf(stderr, "fbmesh: charge placement failed\n"); return 1; }
+    double p[3], d[3];
+    for (int k = 0; k < 3; k++) p[k] = rmax * (2.0 * rng_uniform(rng) - 1.0);
+    double rho = sqrt(dot3(p, p));
+    if (rho < 1e-12) continue;
+    for (int k = 0; k < 3; k++) d[k] = p[k] / rho;
+    if (rho > shape_r_fast(&so, d) - depth * 0.5 || rho < shape_r_fast(&si, d) + depth * 0.5) continue;
+    int ok = 1, pc[3];
+    for (int k = 0; k < 3; k++) pc[k] = (int)floor((p[k] + rmax) / cell);
+    for (int i0 = pc[0] - 1; i0 <= pc[0] + 1 && ok; i0++)
+      for (int i1 = pc[1] - 1; i1 <= pc[1] + 1 && ok; i1++)
+        for (int i2 = pc[2] - 1; i2 <= pc[2] + 1 && ok; i2++) {
+          if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= G || i1 >= G || i2 >= G) continue;
+          const size_t cc = ((size_t)i0 * G + i1) * G + i2;
+          for (int q = gstart[cc]; q < gstart[cc + 1] && ok; q++) {
+            const int f = gface[q];
+            double dx = p[0] - cen[3 * f], dy = p[1] - cen[3 * f + 1], dz = p[2] - cen[3 * f + 2];
+            if (dx * dx + dy * dy + dz * dz > reach2) continue;
+            if (pt_tri_dist2(p, &pos[3 * fc[3 * f]], &pos[3 * fc[3 * f + 1]], &pos[3 * fc[3 * f + 2]]) < depth2) ok = 0;
+          }
+        }
+    if (!ok) continue;
+    for (int k = 0; k < 3; k++) qx[3 * j + k] = p[k];
+    qq[j] = sigma * rng_normal(rng);
+    j++;
+  }
+  double sq = 0.0;
+  for (int j = 0; j < nq; j++) sq += qq[j];
+  const double shift = (nearbyint(sq) - sq) / nq;
+  for (int j = 0; j < nq; j++) qq[j] += shift;
+
+  FILE *fp = open_out(base, ".vert");
+  fprintf(fp, "# fbmesh shell level=%d R=%.17g Rin=%.17g amp=%.17g lmax=%d seed=%llu\n", level, R, Rin, amp, lmax,
+          (unsigned long long)seed);
+  fprintf(fp, "#faces  #sphere density probe_r\n");
+  fprintf(fp, "%d %d %.2f %.2f\n", nv, nq, 1.0, 1.4);
+  for (int i = 0; i < nv; i++)
+    fprintf(fp, "%.17g %.17g %.17g %.17g %.17g %.17g 0 %d 2\n", pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
+            nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 1);
+  fclose(fp);
+  fp = open_out(base, ".face");
+  fprintf(fp, "# fbmesh shell level=%d\n", level);
+  fprintf(fp, "#faces  #sphere density probe_r\n");
+  fprintf(fp, "%d %d %.2f %.2f\n", nf, nq, 1.0, 1.4);
+  for (int f = 0; f < nf; f++) fprintf(fp, "%d %d %d 1 1\n", fc[3 * f] + 1, fc[3 * f + 1] + 1, fc[3 * f + 2] + 1);
+  fclose(fp);
+  fp = open_out(base, ".pqr");
+  for (int j = 0; j < nq; j++)
+    fprintf(fp, "ATOM %6d  C   CHG A%4d    %.17g %.17g %.17g %.17g 1.5000\n", j + 1, j % 10000, qx[3 * j],
+            qx[3 * j + 1], qx[3 * j + 2], qq[j]);
+  fprintf(fp, "END\n");
+  fclose(fp);
+  printf("fbmesh: %s vertices=%d faces=%d charges=%d (tries %ld)\n", base, nv, nf, nq, tries);
+  free(pos); free(nrm); free(fc); free(cen); free(gstart); free(gface); free(gcell); free(qx); free(qq);
+  free(so.a); free(si.a); free(so.N); free(si.N);
+  return 0;
+}
+
 static void usage(void) {
   fprintf(stderr,
-          "usage: fbmesh <sphere|globule> <out-base> [--level L] [--radius R]\n"
-          "              [--amp A] [--lmax L] [--charges ball|center|inside]\n"
+          "usage: fbmesh <sphere|globule|shell> <out-base> [--level L] [--radius R] [--inner Rin]\n"
+          "              [--amp A] [--lmax L] [--charges ball|center|inside|wall]\n"
           "              [--nq N] [--rq r] [--q0 q] [--sigma s] [--depth d] [--seed S]\n"
           "              [--merge A]   (collapse faces below area A, square Angstrom)\n");
   exit(2);
@@ -459,7 +640,7 @@ int main(int argc, char **argv) {
   if (argc < 3) usage();
   const char *kind = argv[1], *base = argv[2];
   int level = 3, lmax = 6, nq = 1;
-  double R = 10.0, amp = 0.08, rq = 8.0, q0 = 1.0, sigma = 0.4, depth = 1.5, merge = 0.0;
+  double R = 10.0, amp = 0.08, rq = 8.0, q0 = 1.0, sigma = 0.4, depth = 1.5, merge = 0.0, inner = 0.0;
   const char *charges = "center";
   uint64_t seed = 1;
   for (int i = 3; i < argc; i++) {
@@ -478,6 +659,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(a, "--depth")) depth = atof(v);
     else if (!strcmp(a, "--seed")) seed = strtoull(v, NULL, 10);
     else if (!strcmp(a, "--merge")) merge = atof(v);
+    else if (!strcmp(a, "--inner")) inner = atof(v);
     else usage();
   }
   if (level < 0 || level > 9 || nq < 1) usage();
@@ -485,7 +667,8 @@ int main(int argc, char **argv) {
   rng_t rng;
   rng_seed(&rng, seed);
   shape_t shape;
-  shape.R = R; shape.amp = amp; shape.lmax = 0; shape.a = NULL;
+  shape.R = R; shape.amp = amp; shape.lmax = 0; shape.a = NULL; shape.fast = 0; shape.N = NULL;
+  if (!strcmp(kind, "shell")) return make_shell(base, level, R, inner, amp, lmax, charges, nq, sigma, depth, &rng, seed);
   if (!strcmp(kind, "globule")) {
     shape.lmax = lmax;
     shape.a = calloc((size_t)(lmax + 1) * (lmax + 1), sizeof(double));

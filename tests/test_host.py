@@ -151,3 +151,56 @@ def test_cli_options(built, meshdir):
     assert r.returncode == 1 and "CPU path" in r.stderr
     r = subprocess.run([exe, "-P=4", str(meshdir / "sph3_ball")], stderr=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
     assert r.returncode == 1 and "outside the accepted range" in r.stderr
+
+
+def test_shell_generator(built, tmp_path):
+    """fbmesh shell: two level-L surfaces, the inner one reversed (faces and normals point
+    toward the centre, out of the wall); charges in the wall with an integer total; the
+    fast radius function (recurrences) gives the globule generator's outer surface."""
+    import numpy as np
+    exe = os.path.join(ROOT, "cuda-fabipb_amd", "bin", "fbmesh")
+    sh, gl = str(tmp_path / "sh"), str(tmp_path / "gl")
+    subprocess.check_call([exe, "shell", sh, "--level", "3", "--radius", "250", "--inner", "210", "--amp", "0.03",
+                           "--lmax", "12", "--charges", "wall", "--nq", "3000", "--sigma", "0.3", "--depth", "1.5",
+                           "--seed", "5"], stdout=subprocess.DEVNULL)
+    subprocess.check_call([exe, "globule", gl, "--level", "3", "--radius", "250", "--amp", "0.03", "--lmax", "12",
+                           "--charges", "center", "--seed", "5"], stdout=subprocess.DEVNULL)
+    v = np.loadtxt(sh + ".vert", skiprows=3, usecols=range(6))
+    f = np.loadtxt(sh + ".face", skiprows=3, usecols=range(3), dtype=int) - 1
+    g = np.loadtxt(gl + ".vert", skiprows=3, usecols=range(6))
+    nv, nf = len(v) // 2, len(f) // 2
+    assert len(v) == 2 * len(g) and nf == 1280
+    assert np.max(np.abs(v[:nv, :3] - g[:, :3])) <= 1e-12 * 250 and np.max(np.abs(v[:nv, 3:] - g[:, 3:])) <= 1e-9
+    P, N = v[:, :3], v[:, 3:]
+    a, b, c = P[f[:, 0]], P[f[:, 1]], P[f[:, 2]]
+    s = np.einsum("ij,ij->i", np.cross(b - a, c - a), (a + b + c) / 3)
+    assert np.all(s[:nf] > 0) and np.all(s[nf:] < 0)
+    assert np.all(np.einsum("ij,ij->i", N[:nv], P[:nv]) > 0) and np.all(np.einsum("ij,ij->i", N[nv:], P[nv:]) < 0)
+    q = np.array([l.split()[-5:] for l in open(sh + ".pqr") if l.startswith("ATOM")], float)
+    r = np.linalg.norm(q[:, :3], axis=1)
+    rin, rout = np.linalg.norm(P[nv:], axis=1), np.linalg.norm(P[:nv], axis=1)
+    assert len(q) == 3000 and rin.min() < r.min() and r.max() < rout.max()
+    assert abs(q[:, 3].sum() - round(q[:, 3].sum())) < 1e-9 and 0.25 < q[:, 3].std() < 0.35
+
+
+def test_capsid_charge_tree_order(built, tmp_path, monkeypatch):
+    """src/fabipb_options.c:90-97, 220-229: above FABIPB_HUGE_CAPSID_ATOMS charges (or
+    FABIPB_HUGE_CAPSID_PANELS panels) the charge-tree order is 3; below, max(ordMom, 4)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("fa_host", os.path.join(ROOT, "cuda-fabipb_amd", "__init__.py"))
+    fa = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(fa)
+    base = str(tmp_path / "g")
+    subprocess.check_call([os.path.join(ROOT, "cuda-fabipb_amd", "bin", "fbmesh"), "globule", base, "--level", "3",
+                           "--radius", "15", "--charges", "inside", "--nq", "40", "--seed", "3"], stdout=subprocess.DEVNULL)
+    mesh = fa.Mesh(base)
+    tree = fa.Tree(mesh, 3, order=8)
+    L = fa.host_lib()
+    assert [L.fb_charge_tree_order(tree.h, l) for l in range(4)] == [8, 8, 8, 8]
+    monkeypatch.setenv("FABIPB_HUGE_CAPSID_ATOMS", "10")
+    assert L.fb_charge_tree_order(tree.h, 3) == 3
+    monkeypatch.delenv("FABIPB_HUGE_CAPSID_ATOMS")
+    monkeypatch.setenv("FABIPB_HUGE_CAPSID_PANELS", "100")
+    assert L.fb_charge_tree_order(tree.h, 3) == 3
+    monkeypatch.setenv("FABIPB_CHARGE_TREE_ORDER", "5")
+    assert L.fb_charge_tree_order(tree.h, 3) == 5
