@@ -210,6 +210,19 @@ def capsid(fa, torch, tmp):
         te = time.perf_counter()
         out[prec] = {"wall_s": te - ts, "rhs_s": tr - ts, "gmres_s": tg - tr, "energy_s": te - tg,
                      "iterations": gm["iters"], "final_residual": gm["resid"], "energy_raw": raw}
+    # the adjoint gradient with the direct energy (every panel x charge pair; exact, O(n m))
+    import numpy as np
+    q = np.ascontiguousarray(mesh.chr, dtype=np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_dir = op.energy_direct_dev(mesh.pos, q, xs)
+    torch.cuda.synchronize()
+    t_edir = time.perf_counter() - t0
+    lam = torch.zeros_like(xs)
+    t0 = time.perf_counter()
+    dq, dk, ginfo = op.gradient_dev(mesh.pos, q, xs, lam, restart=RESTART, tol=TOL, maxit=200)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
     x = torch.rand(n2, dtype=torch.float64, device="cuda") * 2 - 1
     y = torch.empty_like(x)
     st = op.stats()
@@ -221,7 +234,11 @@ def capsid(fa, torch, tmp):
            "fp64": out["fp64"], "fp32_refined": out["fp32"],
            "wall_speedup": out["fp64"]["wall_s"] / out["fp32"]["wall_s"],
            "energy_kcal_per_mol": out["fp64"]["energy_raw"] * fa.ENERGY_SCALE,
-           "energy_rel_err": abs(out["fp32"]["energy_raw"] - out["fp64"]["energy_raw"]) / abs(out["fp64"]["energy_raw"])}
+           "energy_rel_err": abs(out["fp32"]["energy_raw"] - out["fp64"]["energy_raw"]) / abs(out["fp64"]["energy_raw"]),
+           "adjoint_direct": {"energy_direct_s": t_edir, "gradient_s": t_grad, "adjoint_iterations": ginfo["iters"],
+                              "direct_vs_tree_energy_rel": abs(e_dir - out["fp32"]["energy_raw"]) / abs(e_dir),
+                              "dEdkappa_kcal_per_mol_A": dk * fa.ENERGY_SCALE,
+                              "dEdq_norm_kcal_per_mol_e": float(np.linalg.norm(dq)) * fa.ENERGY_SCALE}}
     print(f"[bench] capsid shell {tree.n} panels, {len(mesh.chr)} charges: fp64 {out['fp64']['wall_s']:.3f} s, "
           f"fp32-refined {out['fp32']['wall_s']:.3f} s ({res['wall_speedup']:.2f}x), err {res['energy_rel_err']:.2e}",
           file=sys.stderr, flush=True)
