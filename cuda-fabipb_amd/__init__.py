@@ -133,6 +133,10 @@ def hip_lib():
         L.fbh_gradient_dev.restype = C.c_int
         L.fbh_gradient_dev.argtypes = [_vp, C.c_int, _dp, _dp, _vp, _vp, C.c_int, C.c_double, C.c_int,
                                        _dp, _dp, _ip, _dp, _ip]
+        L.fbh_gradient_tree_dev.restype = C.c_int
+        L.fbh_gradient_tree_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _ip,
+                                            _ip, _ip, C.c_double, _vp, _vp, C.c_int, C.c_double, C.c_int, _dp,
+                                            _ip, _dp, _ip]
         L.fbh_profile.restype = C.c_int
         L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
         L.fbh_profile32.restype = C.c_int
@@ -410,6 +414,21 @@ class Operator:
             self.h, len(chr_), _d(pos), _d(chr_), C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()),
             restart, tol, maxit, _d(dq), C.byref(dk), C.byref(it), C.byref(res), C.byref(info)))
         return dq, dk.value, dict(iters=it.value, resid=res.value, info=info.value)
+
+    def gradient_tree_dev(self, ct, pos, chr_, sgm, lam, theta=0.8, restart=30, tol=1e-6, maxit=100):
+        """dE/dq of the panel-tree energy by tree walks (fbh_gradient_tree_dev); lam in/out."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        keep = {k: np.ascontiguousarray(v) for k, v in ct.items() if isinstance(v, np.ndarray)}
+        dq = np.zeros(len(chr_))
+        res = C.c_double(0.0)
+        it, info = C.c_int(0), C.c_int(0)
+        self._ck(hip_lib().fbh_gradient_tree_dev(
+            self.h, len(chr_), _d(pos), _d(chr_), len(keep["level"]), _i(keep["level"]), _d(keep["x"]),
+            _d(keep["eRad"]), _i(keep["kidStart"]), _i(keep["nKids"]), _i(keep["chgStart"]), _i(keep["nChgs"]),
+            _i(keep["chgIdx"]), _i(keep["order"]), theta, C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()),
+            restart, tol, maxit, _d(dq), C.byref(it), C.byref(res), C.byref(info)))
+        return dq, dict(iters=it.value, resid=res.value, info=info.value)
 
     def rhs_dev(self, pos, chr_, b):
         pos = np.ascontiguousarray(pos, dtype=np.float64)

@@ -2317,8 +2317,9 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, 
   y1 += s1;
 }
 
-// Panel-tree energy (src/treecode.c:617-695, 581-615; MODE 0) and the
-// charge-tree right-hand side (src/treecode.c:381-450; MODE 1) with one lane per panel:
+// Panel-tree energy (src/treecode.c:617-695, 581-615; MODE 0), its weight
+// vector dE/dsgm (MODE 2) and the charge-tree right-hand side
+// (src/treecode.c:381-450; MODE 1) with one lane per panel:
 // the 64 panels of a wave (consecutive in leaf order, so close together) walk
 // the charge tree together, depth first with kids in order, carrying the mask
 // of lanes still descending.  A lane's contributions arrive in the order of its
@@ -2376,13 +2377,15 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
         const bool acc = on && eRad[cb] < theta * dist && lev >= height;
         const bool dir = on && !acc && lev == depth;
         const unsigned long long down = __ballot(on && !acc && lev < depth);
-        if (acc) enFar<P, MODE>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps, y0, y1);
+        if (acc)
+          enFar<P, MODE == 1 ? 1 : 0>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps, y0,
+                                      y1);
         if (__ballot(dir) != 0ull) {
           const int s = chgStart[cb], nq = nChgs[cb];
           for (int k = 0; k < nq; k++) {
             const int j = chgIdx[s + k];
             if (dir) {
-              if constexpr (MODE == 0) {
+              if constexpr (MODE != 1) {
                 enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
               } else {
                 rhsDirect(qp, nr, pos + 3 * j, chr[j], y0, y1);
@@ -2411,6 +2414,9 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
   if (live) {
     if constexpr (MODE == 0) {
       part[p] = o0 * sgm[p + n] + o1 * sgm[p];
+    } else if constexpr (MODE == 2) {   // part is the energy's weight vector g[2n]: E = g . [sgm | sgm_n]
+      part[p] = o1;
+      part[p + n] = o0;
     } else {   // part is the right-hand side b[2n], fac = 1/(4 pi eps1) passed as eps
       part[p] = eps * o0;
       part[p + n] = eps * o1;
@@ -2530,6 +2536,247 @@ __global__ void k_scale(const double *__restrict__ in, double *__restrict__ out,
 
 // serial sum of energy partials in panel order (one thread; n is panels)
 
+// ------------------------------------ dE/dq by a walk of the panel FMM tree
+// The charge gradient of the energy at fixed solution u and adjoint lam is the
+// potential at each charge of the panel layers: the energy's kernels
+// (G0 - Gk on sgm_n, eps dGk/dn - dG0/dn on sgm) and the right-hand side's
+// Coulomb kernel transposed (fac/r on lam, fac d(1/r)/dn on lam_n).  The far
+// part comes from the panel cubes' multipole moments (Q2M and M2M of u, and
+// of (lam_n, lam) for the Coulomb layers) through per-cube polynomial tables
+// of the closed-form Cartesian derivatives, as the charge-tree walk does;
+// leaves not accepted are summed directly with the one-point rule of the
+// direct energy and right-hand side.  One lane per charge, a wave walks the
+// FMM tree together from the cubes at `height`.
+
+// leaf moments of the panel layers at their quadrature points (the one-point
+// rule of the direct energy and right-hand side, not the exact panel
+// integrals of Q2M): a point y with weight w_s (single layer) and w_d (double
+// layer, normal n) adds w_s (y-c)^a/a! to M_dpdn[a] and
+// w_d sum_k n_k (y-c)^(a-e_k)/(a-e_k)! to M_pot[a].  Weights: 2A t w w times
+// (sgm_n, sgm) or, with coul, fac (lam, lam_n).  One block per leaf cube.
+__global__ __launch_bounds__(256) void k_pt_moments(const Pnl *__restrict__ pnl, const int *__restrict__ leafStart,
+                                                    const int *__restrict__ leafCount,
+                                                    const double *__restrict__ cubeX, const double *__restrict__ x,
+                                                    int n, int nm, int nmMax, DevTabs T, QTab Q, int q, int coul,
+                                                    double fac, double *__restrict__ mom) {
+  const int L = blockIdx.x;
+  const double c0 = cubeX[3 * L], c1 = cubeX[3 * L + 1], c2 = cubeX[3 * L + 2];
+  for (int a = threadIdx.x; a < nm; a += blockDim.x) {
+    const int4 m = T.mi[a];
+    const int e[3] = {m.x & 0xff, (m.x >> 8) & 0xff, (m.x >> 16) & 0xff};
+    const double ifa = T.ifact3[a];
+    double md = 0.0, mp = 0.0;
+    for (int k = 0; k < leafCount[L]; k++) {
+      const int p = leafStart[L] + k;
+      const Pnl &P = pnl[p];
+      const double a2 = 2.0 * P.area;
+      const double ws = coul ? fac * a2 * x[p] : a2 * x[p + n], wd = coul ? fac * a2 * x[p + n] : a2 * x[p];
+      for (int ix = 0; ix < q; ix++)
+        for (int jx = 0; jx < q; jx++) {
+          const double wq = Q.t[ix] * Q.w[ix] * Q.w[jx];
+          double t[3];
+          t[0] = P.v[0] + Q.t[ix] * (P.a[6] + Q.t[jx] * P.a[0]) - c0;
+          t[1] = P.v[1] + Q.t[ix] * (P.a[7] + Q.t[jx] * P.a[1]) - c1;
+          t[2] = P.v[2] + Q.t[ix] * (P.a[8] + Q.t[jx] * P.a[2]) - c2;
+          double pw[3][10];   // powers t_d^0..t_d^9
+          for (int d = 0; d < 3; d++) {
+            pw[d][0] = 1.0;
+            for (int i = 1; i <= e[d] && i < 10; i++) pw[d][i] = pw[d][i - 1] * t[d];
+          }
+          md += ws * wq * pw[0][e[0]] * pw[1][e[1]] * pw[2][e[2]] * ifa;
+          double nmt = 0.0;   // sum_k n_k a_k t^(a-e_k) / a!
+          if (e[0] > 0) nmt += P.nr[0] * e[0] * pw[0][e[0] - 1] * pw[1][e[1]] * pw[2][e[2]];
+          if (e[1] > 0) nmt += P.nr[1] * e[1] * pw[0][e[0]] * pw[1][e[1] - 1] * pw[2][e[2]];
+          if (e[2] > 0) nmt += P.nr[2] * e[2] * pw[0][e[0]] * pw[1][e[1]] * pw[2][e[2] - 1];
+          mp += wd * wq * nmt * ifa;
+        }
+    }
+    mom[(long long)L * 2 * nmMax + a] = mp;
+    mom[(long long)L * 2 * nmMax + nmMax + a] = md;
+  }
+}
+
+// three W tables per FMM cube, from sgn-folded moments: a = M_dpdn(u), b =
+// M_pot(u), c = M_pot + M_dpdn of the Coulomb layers (one kernel for both)
+__global__ __launch_bounds__(256) void k_pc_poly(const double *__restrict__ mom, const double *__restrict__ momL,
+                                                 int nmMax, const int4 *__restrict__ desc, int nDesc, int size,
+                                                 double *__restrict__ poly) {
+  const int c = blockIdx.x;
+  const double *Mp = mom + (long long)c * 2 * nmMax, *Md = Mp + nmMax;
+  const double *Lp = momL + (long long)c * 2 * nmMax, *Ld = Lp + nmMax;
+  double *out = poly + (long long)c * 3 * size;
+  for (int e = threadIdx.x; e < nDesc; e += blockDim.x) {
+    const int4 ds = desc[e];
+    const int g[3] = {ds.x & 0xff, (ds.x >> 8) & 0xff, (ds.x >> 16) & 0xff};
+    const int j = ds.y, o = ds.w;
+    const double gsgn = ((g[0] + g[1] + g[2]) & 1) ? -1.0 : 1.0;
+    const double igf = 1.0 / (factd(g[0]) * factd(g[1]) * factd(g[2]) * (double)(1 << j));
+    double va = 0.0, vb = 0.0, vc = 0.0;
+    for (int b1 = 0; b1 <= j; b1++)
+      for (int b2 = 0; b2 <= j - b1; b2++) {
+        const int b[3] = {b1, b2, j - b1 - b2};
+        const int ap[3] = {g[0] + 2 * b[0], g[1] + 2 * b[1], g[2] + 2 * b[2]};
+        const double C = gsgn * factd(ap[0]) * factd(ap[1]) * factd(ap[2]) * igf /
+                         (factd(b[0]) * factd(b[1]) * factd(b[2]));
+        const int k = gidx3(ap[0], ap[1], ap[2]);
+        va += Md[k] * C;
+        vb += Mp[k] * C;
+        vc += (Lp[k] + Ld[k]) * C;
+      }
+    out[o] = va;
+    out[size + o] = vb;
+    out[2 * size + o] = vc;
+  }
+}
+
+// the three tables at r = x - x_cube: sum_g r^g sum_j W[g][j] h_(|g|+j), nested Horner;
+// kernels G0 - Gk (a), eps Gk - G0 (b), 1/r = fac4 G0 (c)
+template <int P>
+__device__ __forceinline__ double pcFar(const double *__restrict__ co, double rx, double ry, double rz, double kappa,
+                                        double eps, double fac4) {
+  constexpr int SZ = enPolyW(P);
+  const double rho = sqrt(rx * rx + ry * ry + rz * rz);
+  double g0[P + 2], gk[P + 2];
+  radial(kappa, rho, P, g0, 1, gk, 1);
+  double ha[P + 1], hb[P + 1], hc[P + 1];
+#pragma unroll
+  for (int k = 0; k <= P; k++) {
+    ha[k] = g0[k] - gk[k];
+    hb[k] = eps * gk[k] - g0[k];
+    hc[k] = fac4 * g0[k];
+  }
+  int o = 0;
+  double s = 0.0;
+#pragma unroll
+  for (int a = P; a >= 0; a--) {
+    double qa = 0.0;
+#pragma unroll
+    for (int b = P - a; b >= 0; b--) {
+      double rb = 0.0;
+#pragma unroll
+      for (int c = P - a - b; c >= 0; c--) {
+        const int d = a + b + c;
+        double cw = 0.0;
+#pragma unroll
+        for (int j = 0; j <= (P - d) / 2; j++)
+          cw += ha[d + j] * co[o + j] + hb[d + j] * co[SZ + o + j] + hc[d + j] * co[2 * SZ + o + j];
+        o += (P - d) / 2 + 1;
+        rb = rb * rz + cw;
+      }
+      qa = qa * ry + rb;
+    }
+    s = s * rx + qa;
+  }
+  return s;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_charge_walk(int nChar, const double *__restrict__ pos, int root0, int nRoot,
+                                                     const int *__restrict__ cubeLevel,
+                                                     const double *__restrict__ cubeX,
+                                                     const double *__restrict__ cubeR,
+                                                     const int *__restrict__ kidStart, const int *__restrict__ nKids,
+                                                     const int *__restrict__ leafStart,
+                                                     const int *__restrict__ leafCount, const Pnl *__restrict__ pnl,
+                                                     const double *__restrict__ u, const double *__restrict__ lam,
+                                                     const double *__restrict__ poly, int n, int depth, double theta,
+                                                     double kappa, double eps, double fac, QTab Q, int q,
+                                                     double *__restrict__ out) {
+  __shared__ int stC[4][EN_WSTACK];
+  __shared__ unsigned long long stM[4][EN_WSTACK];
+  constexpr int SZ = enPolyW(P);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = (blockIdx.x * 4 + w) * 64 + lane;
+  const bool live = j < nChar;
+  const unsigned long long all = __ballot(live);
+  if (all == 0ull) return;   // whole wave past the end (uniform)
+  const int jj = live ? j : nChar - 1;
+  const double cx = pos[3 * jj], cy = pos[3 * jj + 1], cz = pos[3 * jj + 2];
+  const double fac4 = 1.0 / kFourPiI;   // 1/r = fac4 * G0 (fac is in the Coulomb moments)
+  double phi = 0.0;
+  int sp = 0;
+  for (int k = nRoot - 1; k >= 0; k--, sp++) {   // the roots, pushed in reverse: visited in order
+    stC[w][sp] = root0 + k;
+    stM[w][sp] = all;
+  }
+  while (sp > 0) {
+    sp--;
+    const int cb = __builtin_amdgcn_readfirstlane(stC[w][sp]);
+    const unsigned long long mraw = stM[w][sp];
+    const unsigned long long m =
+        ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(mraw >> 32)) << 32) |
+        (unsigned)__builtin_amdgcn_readfirstlane((int)(mraw & 0xffffffffu));
+    const bool on = (m >> lane) & 1ull;
+    const double rx = cx - cubeX[3 * cb], ry = cy - cubeX[3 * cb + 1], rz = cz - cubeX[3 * cb + 2];
+    const double dist = sqrt(rx * rx + ry * ry + rz * rz);
+    const int lev = cubeLevel[cb];
+    const bool acc = on && cubeR[cb] < theta * dist;
+    const bool dir = on && !acc && lev == depth;
+    const unsigned long long down = __ballot(on && !acc && lev < depth);
+    if (acc) phi += pcFar<P>(poly + (long long)cb * 3 * SZ, rx, ry, rz, kappa, eps, fac4);
+    if (__ballot(dir) != 0ull) {
+      const int s0 = leafStart[cb], cnt = leafCount[cb];
+      for (int k = 0; k < cnt; k++) {
+        const int p = s0 + k;
+        const Pnl &Pp = pnl[p];
+        const double a2 = 2.0 * Pp.area;
+        const double ws = a2 * u[p], wn = a2 * u[p + n], ls = fac * a2 * lam[p], ln = fac * a2 * lam[p + n];
+        if (dir)
+          for (int ix = 0; ix < q; ix++)
+            for (int jx = 0; jx < q; jx++) {
+              const double wq = Q.t[ix] * Q.w[ix] * Q.w[jx];
+              double x[3];
+              for (int d = 0; d < 3; d++) x[d] = Pp.v[d] + Q.t[ix] * (Pp.a[6 + d] + Q.t[jx] * Pp.a[d]);
+              x[0] -= cx;
+              x[1] -= cy;
+              x[2] -= cz;
+              const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2), ri = 1.0 / rn;
+              const double G0 = kFourPiI * ri, ek = exp(-kappa * rn), Gk = ek * G0;
+              const double coef = (kappa * rn + 1.0) * ek;
+              const double ip = Pp.nr[0] * x[0] + Pp.nr[1] * x[1] + Pp.nr[2] * x[2];
+              const double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+              phi += wq * ((G0 - Gk) * wn + (eps * dGkdn - dG0dn) * ws + ri * ls - ip * ri * ri * ri * ln);
+            }
+      }
+    }
+    if (down != 0ull) {
+      const int nk = nKids[cb], k0 = kidStart[cb];
+      if (sp + nk > EN_WSTACK) {   // stack bound: poison, never fault
+        phi = __builtin_nan("");
+        break;
+      }
+      for (int k = nk - 1; k >= 0; k--) {
+        stC[w][sp] = k0 + k;
+        stM[w][sp] = down;
+        sp++;
+      }
+    }
+  }
+  if (live) out[j] = phi;
+}
+
+// bounding radius of each FMM cube's panels about its centre (vertices)
+__global__ void k_cube_radius(const Pnl *__restrict__ pnl, const int *__restrict__ first,
+                              const int *__restrict__ count, const double *__restrict__ cubeX,
+                              double *__restrict__ R) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double m = 0.0;
+  for (int p = first[c] + threadIdx.x; p < first[c] + count[c]; p += blockDim.x)
+    for (int v = 0; v < 3; v++) {
+      const double dx = pnl[p].v[3 * v] - cubeX[3 * c], dy = pnl[p].v[3 * v + 1] - cubeX[3 * c + 1],
+                   dz = pnl[p].v[3 * v + 2] - cubeX[3 * c + 2];
+      m = fmax(m, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) R[c] = red[0];
+}
+
 // ================================================================ host side
 
 struct DevBuf {
@@ -2612,6 +2859,7 @@ struct fbh_ctx {
   double4 *coefT = nullptr;
   double *adjL = nullptr, *adjM = nullptr;
   double4 *specDk = nullptr;   // d/dkappa corrections of the special pairs
+  double *cubeR = nullptr;     // bounding radius of each FMM cube's panels (the charge walk's MAC)
   rocblas_handle blas = nullptr;
   // dense direct operator (-r=1/2): every panel pair through the nearfield store
   bool direct = false;
@@ -4142,6 +4390,111 @@ extern "C" int fbh_gradient_dev(fbh_ctx *c, int nChar, const double *pos, const 
   return ok;
 }
 
+static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, const double *chr, int nc,
+                          const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                          const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                          const int *ctOrder, double theta, const double *sgm_dev, double *b_dev,
+                          double *energy_raw);
+
+// dE/dq with tree walks instead of direct sums, for charge counts where the
+// direct panel x charge sums cost more than the solve: the adjoint's
+// right-hand side is the panel-tree energy's weight vector (the exact
+// dE/dsgm of fbh_energy_dev), and dE/dq is the potential at the charges of
+// the panel layers by the walk of the panel FMM tree (theta as the energy's).
+extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                                     const int *ctLevel, const double *ctX, const double *ctERad,
+                                     const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                                     const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
+                                     const double *sgm_dev, double *lam_dev, int restrt, double tol, int maxit,
+                                     double *dEdq, int *iters, double *resid, int *info) {
+  if (!c || nChar <= 0 || !pos || !chr || !sgm_dev || !lam_dev || !dEdq || !iters || !resid || !info ||
+      restrt < 1 || maxit < 1 || !(theta > 0.0)) {
+    setErr("fbh_gradient_tree_dev: bad arguments");
+    return 0;
+  }
+  if (!ensureAdjoint(c) || !ensureGmres(c, restrt)) return 0;
+  const int n = c->n, P = c->ordMom[c->height];
+  for (int l = c->height; l <= c->depth; l++)
+    if (c->ordMom[l] != P) { setErr("fbh_gradient_tree_dev: the FMM levels must share one expansion order"); return 0; }
+  if (P < 1 || P > 9) { setErr("fbh_gradient_tree_dev: expansion order %d outside 1..9", P); return 0; }
+  const int root0 = c->levStart[c->height], nRoot = c->levCount[c->height];
+  if (nRoot + 8 * (c->depth - c->height + 1) > EN_WSTACK) {
+    setErr("fbh_gradient_tree_dev: %d cubes at the top FMM level exceed the walk's stack", nRoot);
+    return 0;
+  }
+  if (!c->cubeR) {   // panel range of every cube (leaves, then parents from their kids), then the radii
+    std::vector<int> first(c->nCube, 0), count(c->nCube, 0);
+    for (int l = 0; l < c->nLeaf; l++) { first[l] = c->hLeafStart[l]; count[l] = c->hLeafCount[l]; }
+    for (int lev = c->depth - 1; lev >= c->height; lev--)
+      for (int cb = c->levStart[lev]; cb < c->levStart[lev] + c->levCount[lev]; cb++) {
+        const int k0 = c->hKidStart[cb], nk = c->hNKids[cb];
+        first[cb] = nk > 0 ? first[k0] : 0;
+        for (int k = 0; k < nk; k++) count[cb] += count[k0 + k];
+      }
+    int *dF = nullptr, *dC = nullptr;
+    if (!(c->cubeR = dalloc<double>(c, c->nCube)) || !(dF = dupload(c, first.data(), first.size())) ||
+        !(dC = dupload(c, count.data(), count.size())))
+      return 0;
+    hipLaunchKernelGGL(k_cube_radius, dim3(c->nCube), dim3(256), 0, c->stream, c->pnl, dF, dC, c->cubeX, c->cubeR);
+    HCK(hipGetLastError());
+  }
+  const int SZ = enPolyW(P);
+  std::vector<int4> dsc;
+  {
+    int o = 0;
+    for (int a = P; a >= 0; a--)   // the order pcFar reads them in
+      for (int b = P - a; b >= 0; b--)
+        for (int g3 = P - a - b; g3 >= 0; g3--)
+          for (int j = 0; j <= (P - a - b - g3) / 2; j++) dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 0, o++));
+    if (o != SZ) { setErr("fbh_gradient_tree_dev: table has %d of %d entries", o, SZ); return 0; }
+  }
+  double *g = nullptr, *poly = nullptr, *dp = nullptr, *dc = nullptr, *dq = nullptr;
+  int4 *dDesc = nullptr;
+  int ok = hipMalloc(&g, 2 * (size_t)n * sizeof(double)) == hipSuccess &&
+           hipMalloc(&poly, (size_t)c->nCube * 3 * SZ * sizeof(double)) == hipSuccess &&
+           hipMalloc(&dq, (size_t)nChar * sizeof(double)) == hipSuccess &&
+           hipMalloc(&dDesc, dsc.size() * sizeof(int4)) == hipSuccess &&
+           hipMemcpy(dDesc, dsc.data(), dsc.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) setErr("fbh_gradient_tree_dev: allocation failed");
+  // the adjoint: A^T lam = dE/dsgm of the panel-tree energy
+  if (ok)
+    ok = chargeTreeWalk(c, 2, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
+                        ctChgIdx, ctOrder, theta, nullptr, g, nullptr) &&
+         gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info);
+  if (ok) ok = uploadCharges(nChar, pos, chr, &dp, &dc);
+  if (ok) {   // moments of u and of (lam_n, lam), tables, walk
+    OpBufs<double> B = bufs64(c);
+    const double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
+    hipLaunchKernelGGL(k_pt_moments, dim3(c->nLeaf), dim3(256), 0, c->stream, c->pnl, c->leafStart, c->leafCount,
+                       c->cubeX, sgm_dev, n, NMOM(P), c->nmMax, c->T, c->Q, c->qorder, 0, fac, c->mom);
+    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
+    hipLaunchKernelGGL(k_pt_moments, dim3(c->nLeaf), dim3(256), 0, c->stream, c->pnl, c->leafStart, c->leafCount,
+                       c->cubeX, (const double *)lam_dev, n, NMOM(P), c->nmMax, c->T, c->Q, c->qorder, 1, fac,
+                       c->adjL);
+    B.mom = c->adjL;
+    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
+    hipLaunchKernelGGL(k_pc_poly, dim3(c->nCube), dim3(256), 0, c->stream, c->mom, c->adjL, c->nmMax, dDesc,
+                       (int)dsc.size(), SZ, poly);
+    const int blocks = ((nChar + 63) / 64 + 3) / 4;
+#define CH_WALK(PP)                                                                                                \
+  case PP:                                                                                                         \
+    hipLaunchKernelGGL(k_charge_walk<PP>, dim3(blocks), dim3(256), 0, c->stream, nChar, dp, root0, nRoot,          \
+                       c->cubeLevelD, c->cubeX, c->cubeR, c->kidStart, c->nKids, c->leafStart, c->leafCount, c->pnl, \
+                       sgm_dev, lam_dev, poly, n, c->depth, theta, c->kappa, c->eps, fac, c->Q, c->qorder, dq);     \
+    break;
+    switch (P) { CH_WALK(1) CH_WALK(2) CH_WALK(3) CH_WALK(4) CH_WALK(5) CH_WALK(6) CH_WALK(7) CH_WALK(8) CH_WALK(9) }
+#undef CH_WALK
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(dEdq, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) setErr("fbh_gradient_tree_dev: walk kernels failed");
+  }
+  hipStreamSynchronize(c->stream);
+  for (void *q : {(void *)g, (void *)poly, (void *)dp, (void *)dc, (void *)dq, (void *)dDesc})
+    if (q) hipFree(q);
+  return ok;
+}
+
 extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b) {
   if (!fbh_rhs_dev(c, nChar, pos, chr, c->ys)) return 0;
   HCK(hipMemcpy(b, c->ys, 2 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost));
@@ -4149,13 +4502,14 @@ extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *c
 }
 
 // the charge-tree walks: mode 0 the panel-tree energy of sgm_dev into
-// *energy_raw, mode 1 the tree right-hand side into b_dev
+// *energy_raw, mode 1 the tree right-hand side into b_dev, mode 2 the energy's
+// weight vector (dE/dsgm) into b_dev
 static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, const double *chr, int nc,
                           const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
                           const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
                           const int *ctOrder, double theta, const double *sgm_dev, double *b_dev,
                           double *energy_raw) {
-  if (!c || nChar <= 0 || nc <= 0 || (mode == 0 && (!energy_raw || !sgm_dev)) || (mode == 1 && !b_dev)) {
+  if (!c || nChar <= 0 || nc <= 0 || (mode == 0 && (!energy_raw || !sgm_dev)) || (mode != 0 && !b_dev)) {
     setErr("fbh_energy/fbh_rhs_tree: bad arguments");
     return 0;
   }
@@ -4268,16 +4622,18 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
   case PP:                                                                                                        \
     hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
                        dLevel, dX, dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta,  \
-                       c->kappa, MM ? kFourPiI / c->eps1 : c->eps, c->Q, c->qorder, MM ? b_dev : dPart);           \
+                       c->kappa, MM == 1 ? kFourPiI / c->eps1 : c->eps, c->Q, c->qorder, MM ? b_dev : dPart);      \
     break;
     if (mode == 0) {
       switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
+    } else if (mode == 2) {
+      switch (P) { EN_WAVE(1, 2) EN_WAVE(2, 2) EN_WAVE(3, 2) EN_WAVE(4, 2) EN_WAVE(5, 2) EN_WAVE(6, 2) EN_WAVE(7, 2) EN_WAVE(8, 2) EN_WAVE(9, 2) }
     } else {
       switch (P) { EN_WAVE(1, 1) EN_WAVE(2, 1) EN_WAVE(3, 1) EN_WAVE(4, 1) EN_WAVE(5, 1) EN_WAVE(6, 1) EN_WAVE(7, 1) EN_WAVE(8, 1) EN_WAVE(9, 1) }
     }
 #undef EN_WAVE
-  } else if (mode == 1) {
-    setErr("fbh_rhs_tree: the accepted levels must share one expansion order");
+  } else if (mode != 0) {
+    setErr("fbh_rhs_tree / tree gradient: the accepted levels must share one expansion order");
     return 0;
   } else {
     int P1 = ord[depth + 1];

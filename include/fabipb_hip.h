@@ -221,6 +221,19 @@ int fbh_energy_direct_dev(fbh_ctx *ctx, int nChar, const double *pos, const doub
 int fbh_gradient_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
                      const double *sgm_dev, double *lam_dev, int restart, double tol, int maxit,
                      double *dEdq, double *dEdkappa, int *iters, double *resid, int *info);
+/* The same dE/dq with tree walks in place of the O(n * nChar) direct sums (capsid
+ * scale): the adjoint's right-hand side is the weight vector of the panel-tree
+ * energy (fbh_energy_dev's charge tree and theta), and dE/dq is the potential at
+ * each charge of the panel layers of sgm (energy kernels) and lam (the right-hand
+ * side's Coulomb kernel), from a walk of the panel FMM tree (far cubes by their
+ * multipole moments, theta as given; the rest direct).  Accurate to the walks'
+ * truncation; needs one expansion order over the FMM levels. */
+int fbh_gradient_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                          int nc, const int *ctLevel, const double *ctX, const double *ctERad,
+                          const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                          const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
+                          const double *sgm_dev, double *lam_dev, int restart, double tol, int maxit,
+                          double *dEdq, int *iters, double *resid, int *info);
 
 /* per-stage device timing over `iters` matvecs (HIP events on the context stream):
  * ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole matvec */

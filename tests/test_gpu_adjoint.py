@@ -165,3 +165,26 @@ def test_adjoint_identity_near_modes(fa, meshdir, monkeypatch, frac):
     op.matvec_t_dev(x, ATx)
     lhs, rhs = float(torch.dot(ATx, y)), float(torch.dot(x, Ay))
     assert abs(lhs - rhs) <= 1e-11 * float(ATx.norm() * y.norm())
+
+
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_gradient_tree_against_direct(fa, meshdir, name):
+    """The tree-walk dE/dq (charge-tree energy weights for the adjoint, panel-tree walk to
+    the charges) converges to the exact direct-sum gradient as theta shrinks."""
+    import torch
+    P = problem(fa, meshdir, name)
+    op = P.operator(FL["k"])
+    q0 = P.mesh.chr.astype(np.float64)
+    u, _, _ = P.solve(q0, FL["k"])
+    _, dq, _ = P.gradient(q0, FL["k"], u=u)
+    ct = P.tree.charge_tree()
+    scale = np.max(np.abs(dq)) / fa.ENERGY_SCALE
+    errs = {}
+    for theta in (0.3, 0.8):
+        lam = torch.zeros_like(u)
+        dqt, info = op.gradient_tree_dev(ct, P.mesh.pos, q0, u, lam, theta=theta, tol=1e-11, maxit=400)
+        assert info["info"] == 0
+        errs[theta] = np.max(np.abs(dqt - dq / fa.ENERGY_SCALE)) / scale
+    print(f"\n[{name}] tree dE/dq max rel difference from direct: theta 0.3 {errs[0.3]:.2e}, 0.8 {errs[0.8]:.2e}",
+          flush=True)
+    assert errs[0.3] <= 1e-5 and errs[0.8] <= 2e-2 and errs[0.3] < errs[0.8]
