@@ -285,10 +285,10 @@ class Operator:
             raise FabipbError(_hip_err())
 
     def stats(self):
-        out = (C.c_longlong * 12)()
+        out = (C.c_longlong * 16)()
         self._ck(hip_lib().fbh_stats(self.h, out))
         keys = ["n", "near_interactions", "m2l_pairs", "nmom_max", "nmom_leaf", "device_bytes",
-                "links", "cubes", "near_stored", "near_fly_tiles", "near_tiles", "special_pairs"]
+                "links", "cubes", "near_stored", "near_fly_tiles", "near_tiles", "special_pairs", "m2l_grid_steps"]
         return dict(zip(keys, list(out)))
 
     # host-array calls

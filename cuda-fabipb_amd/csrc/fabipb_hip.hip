@@ -1533,6 +1533,139 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
   }
 }
 
+// M2L on a fixed term grid (src/fmm.c:774-787, src/expan.c:258-290, 471-494):
+// the host lays the (alpha, beta) terms of one pair on K steps x 64 lanes; a
+// lane keeps one alpha for each run of L consecutive steps, so run r's four
+// running sums live in registers (acc[r]) for the whole interaction list of the
+// cube, and no lane stores anything per pair.  At each step the lanes' beta and
+// alpha+beta slots are chosen on the host to spread over the LDS banks; an idle
+// lane reads the zero slot of the moments.  After the list, the waves add their
+// registers into one [run][lane] table in wave order and one thread per alpha
+// sums the cells that hold it (a fixed list), then combines the four sums
+//   t1 = sum c1 a0, t2 = sum c1 ak, t3 = sum c2 a0, t4 = sum c2 ak
+// into the locals (TR: into the adjoint moments), as k_m2l_cached does per item.
+template <typename R, int K, int L, bool TR = false>
+__global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
+                                                  const int *__restrict__ cubeLevel, const int *__restrict__ ordM2L,
+                                                  const int *__restrict__ ordMom, const R *__restrict__ G,
+                                                  const R *__restrict__ mom, R *__restrict__ lec, int nmMax, R eps,
+                                                  DevTabs T, const unsigned *__restrict__ gridTerms,
+                                                  const int *__restrict__ gridOff, const int *__restrict__ redStart,
+                                                  const unsigned short *__restrict__ redCell, int slotZ, int nccP,
+                                                  int nggP) {
+  typedef typename Vec2<R>::type R2;
+  constexpr int NR = K / L;
+  const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
+  const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
+  R2 *__restrict__ cc = reinterpret_cast<R2 *>(dynLds) + w * (nccP + nggP);   // c1 = sgn M_pot, c2 = sgn M_dpdn
+  R2 *__restrict__ gg = cc + nccP;                                            // a0 = D G0, ak = D Gk
+  unsigned pe[K];
+  const unsigned *gt = gridTerms + gridOff[2 * p];
+#pragma unroll
+  for (int k = 0; k < K; k++) pe[k] = gt[k * 64 + lane];
+  if (lane == 0) {
+    R2 z;
+    z.x = z.y = 0;
+    cc[slotZ] = z;
+  }
+  R sg[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) sg[q] = lane + 64 * q < nm ? (R)T.ep[lane + 64 * q].w : (R)0;
+  R acc[NR][4];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
+  R ga[4], gb[4], ma[4], mb[4];
+  auto fetch = [&](int pr) {
+    const int src = m2lSrc[pr];
+    const R *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
+    const R *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = lane + 64 * q;
+      if (i < nm) {
+        ga[q] = g0[i];
+        gb[q] = gk[i];
+        ma[q] = mp[i];
+        mb[q] = md[i];
+      }
+    }
+  };
+  const int b1 = m2lOff[c + 1];
+  int pr = m2lOff[c] + w;
+  if (pr < b1) fetch(pr);
+  for (; pr < b1; pr += nw) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = lane + 64 * q;
+      if (i < nm) {
+        R2 x, g;
+        x.x = sg[q] * ma[q];
+        x.y = sg[q] * mb[q];
+        cc[i] = x;
+        g.x = ga[q];
+        g.y = gb[q];
+        gg[i] = g;
+      }
+    }
+    waveSync();
+    if (pr + nw < b1) fetch(pr + nw);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const R2 x = cc[pe[k] & 0xffffu];
+      const R2 g = gg[pe[k] >> 16];
+      R *a = acc[k / L];
+      a[0] += x.x * g.x;
+      a[1] += x.x * g.y;
+      a[2] += x.y * g.x;
+      a[3] += x.y * g.y;
+    }
+    waveSync();
+  }
+  __syncthreads();
+  R *red = reinterpret_cast<R *>(dynLds);   // [run][lane][4], over the slots
+  for (int ww = 0; ww < nw; ww++) {
+    if (w == ww) {
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          R *d = red + (r * 64 + lane) * 4 + q;
+          *d = ww == 0 ? acc[r][q] : *d + acc[r][q];
+        }
+    }
+    __syncthreads();
+  }
+  const int nmc = NMOM(ordMom[cubeLevel[c]]);
+  const int *rs = redStart + gridOff[2 * p + 1];
+  constexpr int NO = TR ? 2 : 4;
+  for (int a = tid; a < nmMax; a += blockDim.x) {
+    R v[NO];
+#pragma unroll
+    for (int k = 0; k < NO; k++) v[k] = 0;
+    if (a < nm && a < nmc) {
+      R t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+      for (int i = rs[a]; i < rs[a + 1]; i++) {
+        const R *q = red + 4 * redCell[i];
+        t1 += q[0];
+        t2 += q[1];
+        t3 += q[2];
+        t4 += q[3];
+      }
+      if constexpr (TR) {   // x = (sgn b, sgn a): t1 = b*G0, t2 = b*Gk, t3 = a*G0, t4 = a*Gk
+        v[0] = eps * t4 - t3 + t2 - t1;
+        v[1] = t3 - t4 + t1 - t2 / eps;
+      } else {
+        v[0] = t3 - t4;
+        v[1] = eps * t2 - t1;
+        v[2] = t3 - t4 / eps;
+        v[3] = t2 - t1;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NO; k++) lec[((long long)c * NO + k) * nmMax + a] = v[k];
+  }
+}
+
 // src/expan.c:403-452 (transL2LWs) + convL2L (:295-326): one block per child,
 // thread per output multi-index, (beta, alpha+beta) table in LDS
 // NA = 4: the local expansions l1..l4; NA = 2: the transposed M2M of the
@@ -2824,6 +2957,12 @@ struct fbh_ctx {
   size_t m2lLds = 0;
   int m2lWaves = 0;
   int m2lMaxT = 0;   // register-resident term lists (16, 32) or 0
+  // M2L term grid (k_m2l_grid): K steps x 64 lanes per order, runs of L steps
+  int gridK = 0, gridL = 0, gridSlotZ = 0, gridNcc = 0, gridNgg = 0;
+  unsigned *gridTerms = nullptr;
+  int *gridOff = nullptr, *gridRedStart = nullptr;
+  unsigned short *gridRedCell = nullptr;
+  size_t gridLds = 0, gridLds32 = 0;
   int ldsMax = 65536;
   // FP32 operator (precision form): copies of the cached operator in FP32
   bool fp32 = false;
@@ -3126,6 +3265,145 @@ static bool buildM2LPlan(fbh_ctx *c) {
   return true;
 }
 
+// Term grid of k_m2l_grid for order p on K steps x 64 lanes, runs of L steps.
+// The terms of one alpha are cut into cells of L (the last one shorter); the
+// cells, longest first, fill the lanes lane by lane (cell i: run i % R, lane
+// i / R).  Then, step by step, each busy lane takes the next beta of its
+// alpha (lanes whose alpha has fewest terms left choose first) among the ones
+// whose moment slot and tensor slot share no bank with a different slot of the
+// lanes already placed in its 8- and 16-lane group (16-byte slots; else the
+// fewest such clashes).  Idle lanes read the zero moment slot and a tensor slot
+// another lane of the group reads.  Returns false when the cells do not fit.
+static bool planGrid(const fbh_ctx *c, int p, int K, int L, int slotZ, std::vector<unsigned> &terms,
+                     std::vector<int> &redStart, std::vector<unsigned short> &redCell) {
+  const int nm = NMOM(p), R = K / L;
+  std::vector<int> deg(nm);
+  for (int d = 0, g = 0; d <= p; d++)
+    for (int k = 0; k < (d + 1) * (d + 2) / 2; k++) deg[g++] = d;
+  std::vector<std::vector<std::pair<int, int>>> pool(nm);
+  struct Cell { int alpha, cnt; };
+  std::vector<Cell> cells;
+  for (int a = 0; a < nm; a++) {
+    const int cnt = NMOM(p - deg[a]);
+    for (int e = 0; e < cnt; e++) {
+      const int pe = c->hConv[c->hConvStart[a] + e];
+      pool[a].push_back({pe & 0xffff, pe >> 16});
+    }
+    for (int f = 0; f < cnt; f += L) cells.push_back({a, std::min(L, cnt - f)});
+  }
+  if ((int)cells.size() > R * 64) return false;
+  std::stable_sort(cells.begin(), cells.end(), [](const Cell &x, const Cell &y) { return x.cnt > y.cnt; });
+  std::vector<int> place((size_t)R * 64, -1);
+  for (size_t i = 0; i < cells.size(); i++) place[(i % R) * 64 + i / R] = (int)i;
+  std::vector<int> cs(64), gs(64);   // this step's slots per lane (-1: not placed yet)
+  terms.assign((size_t)K * 64, 0);
+  auto clash = [&](const std::vector<int> &v, int l, int s) {
+    int n = 0;
+    for (int G = 8; G <= 16; G *= 2)
+      for (int l2 = l / G * G; l2 < l / G * G + G; l2++)
+        if (l2 != l && v[l2] >= 0 && v[l2] != s && v[l2] % G == s % G) n++;
+    return n;
+  };
+  for (int r = 0; r < R; r++)
+    for (int t = 0; t < L; t++) {
+      std::fill(cs.begin(), cs.end(), -1);
+      std::fill(gs.begin(), gs.end(), -1);
+      std::vector<int> busy;
+      for (int l = 0; l < 64; l++) {
+        const int ci = place[r * 64 + l];
+        if (ci >= 0 && cells[ci].cnt > t) busy.push_back(l);
+      }
+      std::stable_sort(busy.begin(), busy.end(), [&](int x, int y) {
+        return pool[cells[place[r * 64 + x]].alpha].size() < pool[cells[place[r * 64 + y]].alpha].size();
+      });
+      for (int l : busy) {
+        auto &pl = pool[cells[place[r * 64 + l]].alpha];
+        size_t best = 0;
+        int bc = 1 << 30;
+        for (size_t i = 0; i < pl.size() && bc > 0; i++) {
+          const int k = clash(cs, l, pl[i].first) + clash(gs, l, pl[i].second);
+          if (k < bc) { bc = k; best = i; }
+        }
+        cs[l] = pl[best].first;
+        gs[l] = pl[best].second;
+        pl.erase(pl.begin() + best);
+      }
+      for (int l = 0; l < 64; l++)
+        if (cs[l] < 0) {
+          int g = -1;
+          for (int G = 8; G <= 16 && g < 0; G *= 2)
+            for (int l2 = l / G * G; l2 < l / G * G + G && g < 0; l2++)
+              if (gs[l2] >= 0 && cs[l2] != slotZ) g = gs[l2];
+          cs[l] = slotZ;
+          gs[l] = g < 0 ? 0 : g;
+        }
+      for (int l = 0; l < 64; l++) terms[(size_t)(r * L + t) * 64 + l] = (unsigned)cs[l] | ((unsigned)gs[l] << 16);
+    }
+  for (int a = 0; a < nm; a++)
+    if (!pool[a].empty()) return false;
+  redStart.assign(nm + 1, 0);
+  std::vector<std::vector<int>> byAlpha(nm);
+  for (int cell = 0; cell < R * 64; cell++)
+    if (place[cell] >= 0) byAlpha[cells[place[cell]].alpha].push_back(cell);
+  redCell.clear();
+  for (int a = 0; a < nm; a++) {
+    redStart[a] = (int)redCell.size();
+    for (int cell : byAlpha[a]) redCell.push_back((unsigned short)cell);
+  }
+  redStart[nm] = (int)redCell.size();
+  return true;
+}
+
+// The term grid for every order the levels use, on the smallest compiled shape
+// (K, L) that holds the largest of them; none above order 8 (k_m2l_cached then).
+static bool buildM2LGrid(fbh_ctx *c) {
+  int pU = 0;
+  for (int l = c->height; l <= c->depth; l++) pU = std::max(pU, c->ordM2L[l]);
+  static const int shapes[][3] = {{3, 4, 1}, {5, 8, 1}, {6, 16, 2}, {7, 32, 2}, {8, 56, 4}};   // max order, K, L
+  int K = 0, L = 0;
+  for (auto &sh : shapes)
+    if (pU <= sh[0]) { K = sh[1]; L = sh[2]; break; }
+  const char *em = getenv("FABIPB_M2L");   // "items": the item-list kernel only (check path)
+  if (!K || (em && !strcmp(em, "items"))) return true;
+  const int nmU = NMOM(pU);
+  std::vector<unsigned> allTerms;
+  std::vector<int> off(2 * (pU + 1), 0), allStart;
+  std::vector<unsigned short> allCell;
+  std::vector<bool> used(pU + 1, false);
+  for (int l = c->height; l <= c->depth; l++) used[c->ordM2L[l]] = true;
+  for (int p = 0; p <= pU; p++) {
+    if (!used[p]) continue;
+    std::vector<unsigned> t;
+    std::vector<int> rs;
+    std::vector<unsigned short> rc;
+    if (!planGrid(c, p, K, L, nmU, t, rs, rc)) return true;   // does not fit: k_m2l_cached
+    off[2 * p] = (int)allTerms.size();
+    off[2 * p + 1] = (int)allStart.size();
+    for (int &v : rs) v += (int)allCell.size();
+    allTerms.insert(allTerms.end(), t.begin(), t.end());
+    allStart.insert(allStart.end(), rs.begin(), rs.end());
+    allCell.insert(allCell.end(), rc.begin(), rc.end());
+  }
+  const int ncc = (nmU + 2) & ~1, ngg = (nmU + 1) & ~1, nw = 4;
+  auto lds = [&](size_t elem) {
+    return std::max((size_t)nw * (ncc + ngg) * 2 * elem, (size_t)(K / L) * 64 * 4 * elem);
+  };
+  if (lds(sizeof(double)) > (size_t)c->ldsMax) return true;
+  if (!(c->gridTerms = dupload(c, allTerms.data(), allTerms.size())) ||
+      !(c->gridOff = dupload(c, off.data(), off.size())) ||
+      !(c->gridRedStart = dupload(c, allStart.data(), allStart.size())) ||
+      !(c->gridRedCell = dupload(c, allCell.data(), allCell.size())))
+    return false;
+  c->gridK = K;
+  c->gridL = L;
+  c->gridSlotZ = nmU;
+  c->gridNcc = ncc;
+  c->gridNgg = ngg;
+  c->gridLds = lds(sizeof(double));
+  c->gridLds32 = lds(sizeof(float));
+  return true;
+}
+
 static inline int blk64(int n, int cap) {
   int b = ((n + 63) / 64) * 64;
   return std::max(64, std::min(b, cap));
@@ -3357,7 +3635,7 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   }
   // M2L derivative tensors per pair, cached when they fit (1344 B per pair at p=6)
   c->m2lG = nullptr;
-  if (!buildM2LPlan(c)) FAIL();
+  if (!buildM2LPlan(c) || !buildM2LGrid(c)) FAIL();
   if (c->m2lWaves > 0 && c->nM2L > 0 && (double)c->nM2L * 2 * c->nmMax * sizeof(double) <= M2L_CACHE_BYTES) {
     std::vector<int> pd((size_t)c->nM2L);
     for (int cb = 0; cb < nCube; cb++)
@@ -3398,8 +3676,31 @@ static void m2lLaunch(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
                      c->planTerms, c->planIdx, c->planOff);
 }
 
+template <typename R, int K, int L, bool TR = false>
+static void m2lGridLaunch(fbh_ctx *c, const R *G, const R *mom, R *lec, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((k_m2l_grid<R, K, L, TR>), dim3(c->nCube), dim3(256), lds, st, c->m2lOff, c->m2lSrc,
+                     c->cubeLevelD, c->ordM2LD, c->ordMomD, G, mom, lec, c->nmMax, (R)c->eps, c->T, c->gridTerms,
+                     c->gridOff, c->gridRedStart, c->gridRedCell, c->gridSlotZ, c->gridNcc, c->gridNgg);
+}
+
+// k_m2l_grid at the shape the plan was built for; false when there is no grid
+template <typename R, bool TR = false>
+static bool m2lGrid(fbh_ctx *c, const R *G, const R *mom, R *lec, hipStream_t st) {
+  if (!c->gridK || !G) return false;
+  const size_t lds = sizeof(R) == sizeof(double) ? c->gridLds : c->gridLds32;
+  switch (c->gridK) {
+    case 4: m2lGridLaunch<R, 4, 1, TR>(c, G, mom, lec, lds, st); return true;
+    case 8: m2lGridLaunch<R, 8, 1, TR>(c, G, mom, lec, lds, st); return true;
+    case 16: m2lGridLaunch<R, 16, 2, TR>(c, G, mom, lec, lds, st); return true;
+    case 32: m2lGridLaunch<R, 32, 2, TR>(c, G, mom, lec, lds, st); return true;
+    case 56: m2lGridLaunch<R, 56, 4, TR>(c, G, mom, lec, lds, st); return true;
+  }
+  return false;
+}
+
 template <typename R>
 static void m2lPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
+  if (m2lGrid<R>(c, B.G, B.mom, B.lec, st)) return;
   if (B.G) {
     if (c->m2lMaxT == 16) m2lLaunch<R, 16>(c, B, st);
     else if (c->m2lMaxT == 32) m2lLaunch<R, 32>(c, B, st);
@@ -3557,7 +3858,8 @@ static void farPassT(fbh_ctx *c, const double *y, hipStream_t st) {
                        c->cubeLevelD, c->ordM2LD, c->ordMomD, (const double *)c->m2lG, (const double *)c->adjL,
                        c->adjM, c->nmMax, c->eps, c->T, c->planTerms, c->planIdx, c->planOff);
   };
-  if (c->m2lMaxT == 16) launch(k_m2l_cached<double, 16, true>);
+  if (m2lGrid<double, true>(c, c->m2lG, c->adjL, c->adjM, st)) {
+  } else if (c->m2lMaxT == 16) launch(k_m2l_cached<double, 16, true>);
   else if (c->m2lMaxT == 32) launch(k_m2l_cached<double, 32, true>);
   else launch(k_m2l_cached<double, 0, true>);
   for (int lev = c->height; lev < c->depth; lev++) {
@@ -4729,6 +5031,7 @@ extern "C" int fbh_stats(fbh_ctx *c, long long *out) {
   out[0] = c->n; out[1] = c->nInter; out[2] = c->nM2L; out[3] = c->nmMax; out[4] = c->nmLeaf;
   out[5] = (long long)c->bytes; out[6] = c->nLinks; out[7] = c->nCube;
   out[8] = c->nStored; out[9] = c->nFlyTiles; out[10] = c->nTiles; out[11] = c->nSpec;
+  out[12] = c->gridK;
   return 1;
 }
 

@@ -199,7 +199,7 @@ int main(int argc, char **argv) {
   if (!ctx) die("operator setup failed");
   double tSetup = now() - tStage;
   if (bench) {
-    long long st[12];
+    long long st[16];
     fbh_stats(ctx, st);
     printf("Flattened apply layout: leaf-cubes=%d near-pairs=%d\n", t->nLeaf, t->nNear);
     printf("Flattened interaction layout: m2l-pairs=%d dst-groups=%d\n", t->nM2L, t->nM2LGroups);

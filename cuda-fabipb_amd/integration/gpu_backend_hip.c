@@ -102,7 +102,7 @@ static int ensure(ssystem *sys) {
   }
   free(vtx); free(nrm); free(area); free(cx); free(level); free(kidStart); free(nKids); free(parent); free(m2lOff);
   if (!gCtx) return ok ? fail("operator setup") : 0;
-  long long st[12];
+  long long st[16];
   fbh_stats(gCtx, st);
   gNm = (int)st[3];
   gMom = calloc((size_t)nCube * 2 * gNm, sizeof(double));

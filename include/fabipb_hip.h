@@ -241,11 +241,12 @@ int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, dou
 /* the same for the FP32 operator (after fbh_enable_fp32) */
 int fbh_profile32(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
 
-/* sizes (out holds 12): out[0]=n, [1]=nearfield interactions, [2]=M2L pairs,
+/* sizes (out holds 13): out[0]=n, [1]=nearfield interactions, [2]=M2L pairs,
  * [3]=nMom(maxOrder), [4]=nMom(ordMom[depth]), [5]=device bytes held, [6]=M2M/L2L
  * links, [7]=nCube, [8]=interactions with stored coefficients (streamed tiles),
  * [9]=nearfield tiles computed on the fly, [10]=nearfield tiles, [11]=corrected
- * special pairs (0 until the on-the-fly tiles need them) */
+ * special pairs (0 until the on-the-fly tiles need them), [12]=steps of the M2L
+ * term grid (0: the M2L runs from per-lane item lists) */
 int fbh_stats(fbh_ctx *ctx, long long *out);
 
 /* the context's stream (hipStream_t), for callers that order their own work on it */

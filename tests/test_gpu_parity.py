@@ -152,6 +152,32 @@ def test_near_modes_agree(fa, meshdir, monkeypatch, name):
     assert rel(y32[None].cpu().numpy(), ys[None].cpu().numpy()) <= 1e-5
 
 
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_m2l_paths_agree(fa, meshdir, monkeypatch, name):
+    """The M2L on the fixed term grid (registers per run, host-placed LDS slots) and the
+    M2L over per-lane item lists give the same matvec, FP32 matvec and transposed matvec
+    to rounding (they sum the same terms in different orders)."""
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, name)
+    assert op.stats()["m2l_grid_steps"] > 0
+    monkeypatch.setenv("FABIPB_M2L", "items")
+    oi = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    assert oi.stats()["m2l_grid_steps"] == 0
+    g = torch.Generator(device="cpu").manual_seed(13)
+    x = (torch.rand(2 * tree.n, generator=g, dtype=torch.float64) * 2 - 1).cuda()
+    out = {}
+    for key, o in (("grid", op), ("items", oi)):
+        o.enable_fp32()
+        y, y32, yt = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+        o.matvec_dev(x, y)
+        o.matvec32_dev(x, y32)
+        o.matvec_t_dev(x, yt)
+        out[key] = [v.cpu().numpy() for v in (y, y32, yt)]
+    d = [rel(a, b) for a, b in zip(out["grid"], out["items"])]
+    print(f"\n[{name}] grid vs items: fp64 {d[0]:.2e} fp32 {d[1]:.2e} transposed {d[2]:.2e}", flush=True)
+    assert d[0] <= 1e-14 and d[2] <= 1e-14 and d[1] <= 1e-6
+
+
 @pytest.mark.parametrize("name", harness_cases())
 def test_energy_walk(fa, meshdir, name):
     import torch
@@ -231,6 +257,32 @@ def test_orders_against_oracle(fa, meshdir, name, order):
     ref = ora.energy(ct, mesh.pos, mesh.chr, sg)
     print(f"\n[{name} p={order}] energy {e!r} oracle {ref!r}", flush=True)
     assert abs(e - ref) <= 1e-12 * abs(ref)
+
+
+@pytest.mark.parametrize("order,steps", [(3, 4), (5, 8), (7, 32), (8, 56)])
+def test_m2l_grid_shapes(fa, meshdir, order, steps):
+    """Each compiled shape of the M2L term grid (steps x runs): matvec against the oracle
+    on the product's own tree, and the transpose identity of the adjoint operator."""
+    import torch
+    import oracle_binding as O
+    name = "sph3_ball"
+    mesh = fa.Mesh(str(meshdir / name))
+    tree = fa.Tree(mesh, CASES["cases"][name]["t"], order=order, height=FL["H"], sep=FL["S"])
+    op = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    assert op.stats()["m2l_grid_steps"] == steps
+    ora = O.Oracle(tree.oracle_layout(FL["k"], FL["eps1"], FL["eps2"], FL["q"]))
+    x = splitmix_vec(4244, 2 * tree.n)
+    d = rel(op.matvec(x), ora.matvec(x))
+    g = torch.Generator(device="cpu").manual_seed(14)
+    u = (torch.rand(2 * tree.n, generator=g, dtype=torch.float64) * 2 - 1).cuda()
+    v = (torch.rand(2 * tree.n, generator=g, dtype=torch.float64) * 2 - 1).cuda()
+    Av, ATu = torch.empty_like(u), torch.empty_like(u)
+    op.matvec_dev(v, Av)
+    op.matvec_t_dev(u, ATu)
+    ident = abs(float(torch.dot(ATu, v)) - float(torch.dot(u, Av))) / float(ATu.norm() * v.norm())
+    print(f"\n[{name} p={order}] grid {steps} steps: matvec vs oracle {d:.2e}, transpose identity {ident:.2e}",
+          flush=True)
+    assert d <= 1e-13 and ident <= 1e-11
 
 
 @pytest.mark.parametrize("name", ["glob4", "cfg1_sphere"])
