@@ -1555,6 +1555,7 @@ __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff
                                                   int nggP) {
   typedef typename Vec2<R>::type R2;
   constexpr int NR = K / L;
+  constexpr int NQ = K <= 8 ? 1 : K <= 32 ? 2 : 3;   // moments per lane: nMom(5) = 56, nMom(7) = 120, nMom(8) = 165
   const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
   const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
   R2 *__restrict__ cc = reinterpret_cast<R2 *>(dynLds) + w * (nccP + nggP);   // c1 = sgn M_pot, c2 = sgn M_dpdn
@@ -1568,19 +1569,19 @@ __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff
     z.x = z.y = 0;
     cc[slotZ] = z;
   }
-  R sg[4];
+  R sg[NQ];
 #pragma unroll
-  for (int q = 0; q < 4; q++) sg[q] = lane + 64 * q < nm ? (R)T.ep[lane + 64 * q].w : (R)0;
+  for (int q = 0; q < NQ; q++) sg[q] = lane + 64 * q < nm ? (R)T.ep[lane + 64 * q].w : (R)0;
   R acc[NR][4];
 #pragma unroll
   for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
-  R ga[4], gb[4], ma[4], mb[4];
+  R ga[NQ], gb[NQ], ma[NQ], mb[NQ];
   auto fetch = [&](int pr) {
     const int src = m2lSrc[pr];
     const R *g0 = G + (long long)pr * 2 * nmMax, *gk = g0 + nmMax;
     const R *mp = mom + (long long)src * 2 * nmMax, *md = mp + nmMax;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < NQ; q++) {
       const int i = lane + 64 * q;
       if (i < nm) {
         ga[q] = g0[i];
@@ -1595,7 +1596,7 @@ __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff
   if (pr < b1) fetch(pr);
   for (; pr < b1; pr += nw) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < NQ; q++) {
       const int i = lane + 64 * q;
       if (i < nm) {
         R2 x, g;
@@ -3359,7 +3360,7 @@ static bool planGrid(const fbh_ctx *c, int p, int K, int L, int slotZ, std::vect
 static bool buildM2LGrid(fbh_ctx *c) {
   int pU = 0;
   for (int l = c->height; l <= c->depth; l++) pU = std::max(pU, c->ordM2L[l]);
-  static const int shapes[][3] = {{3, 4, 1}, {5, 8, 1}, {6, 16, 2}, {7, 32, 2}, {8, 56, 4}};   // max order, K, L
+  static const int shapes[][3] = {{3, 4, 1}, {5, 8, 1}, {6, 16, 2}, {7, 32, 4}, {8, 64, 8}};   // max order, K, L
   int K = 0, L = 0;
   for (auto &sh : shapes)
     if (pU <= sh[0]) { K = sh[1]; L = sh[2]; break; }
@@ -3692,8 +3693,8 @@ static bool m2lGrid(fbh_ctx *c, const R *G, const R *mom, R *lec, hipStream_t st
     case 4: m2lGridLaunch<R, 4, 1, TR>(c, G, mom, lec, lds, st); return true;
     case 8: m2lGridLaunch<R, 8, 1, TR>(c, G, mom, lec, lds, st); return true;
     case 16: m2lGridLaunch<R, 16, 2, TR>(c, G, mom, lec, lds, st); return true;
-    case 32: m2lGridLaunch<R, 32, 2, TR>(c, G, mom, lec, lds, st); return true;
-    case 56: m2lGridLaunch<R, 56, 4, TR>(c, G, mom, lec, lds, st); return true;
+    case 32: m2lGridLaunch<R, 32, 4, TR>(c, G, mom, lec, lds, st); return true;
+    case 64: m2lGridLaunch<R, 64, 8, TR>(c, G, mom, lec, lds, st); return true;
   }
   return false;
 }

@@ -259,7 +259,7 @@ def test_orders_against_oracle(fa, meshdir, name, order):
     assert abs(e - ref) <= 1e-12 * abs(ref)
 
 
-@pytest.mark.parametrize("order,steps", [(3, 4), (5, 8), (7, 32), (8, 56)])
+@pytest.mark.parametrize("order,steps", [(3, 4), (5, 8), (7, 32), (8, 64)])
 def test_m2l_grid_shapes(fa, meshdir, order, steps):
     """Each compiled shape of the M2L term grid (steps x runs): matvec against the oracle
     on the product's own tree, and the transpose identity of the adjoint operator."""
