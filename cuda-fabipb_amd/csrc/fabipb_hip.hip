@@ -634,10 +634,56 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
 // staged in LDS.  The values differ from the stored coefficients only by the
 // rounding of the formula's operation order.
 #define FLY64_CHUNK 512
+#define FLY64_RB 2   // rows per pass over the staged sources
 // TR: the transposed operator's row x, i.e. KER(y, x) read transposed: with
 // r, n_x, n_y as below, y0 and y3 are symmetric in (x, y) and
 //   y1(y,x) = -(eps coef - 1) dG0dx,  y2(y,x) = -(coef/eps - 1) dG0dy,
 //   out[x] += y3 A sgm_n + y1(y,x) A sgm,  out[x+n] += y2(y,x) A sgm_n + y0 A sgm
+// 1/sqrt(x) for x > 0 (a normal FP64 number): the hardware estimate, then two
+// Newton steps y <- y + y (1 - x y^2) / 2, each of which squares the relative
+// error, so the result is within a few ulp of the correctly rounded value.
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-(x * y), y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+// One pair of the on-the-fly FP64 nearfield (kernelKER4 at the one-point rule,
+// src/kernel.c:109-147): target point/normal X, XN against staged source
+// P = (point, A sigma), N = (normal, A sigma_n).  skip: the panel itself (its
+// weights are zeroed and r2 replaced by 1 so nothing divides by zero).
+template <bool TR>
+__device__ __forceinline__ void fly64Pair(const double4 &X, const double4 &XN, const double4 &P, const double4 &N,
+                                          bool skip, double kappa, double kap2, double eps, double ieps,
+                                          double &s0, double &s1) {
+  const double rx = X.x - P.x, ry = X.y - P.y, rz = X.z - P.z;
+  const double r2 = skip ? 1.0 : rx * rx + ry * ry + rz * rz;
+  const double ri = rsq64(r2), rn = r2 * ri, ri2 = ri * ri;
+  const double e = exp(-kappa * rn);
+  const double ip0 = XN.x * N.x + XN.y * N.y + XN.z * N.z;
+  const double ipX = XN.x * rx + XN.y * ry + XN.z * rz;
+  const double ipY = N.x * rx + N.y * ry + N.z * rz;
+  const double G0 = ri * kFourPiI, Gk = e * G0, coef = (kappa * rn + 1.0) * e;
+  const double dG0dy = ipY * G0 * ri2, dG0dx = ipX * G0 * ri2;
+  const double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) * ri2;
+  const double y0 = G0 - Gk;
+  const double y3 = (coef * ddG0 - kap2 * e * ipX * dG0dy) - ddG0;
+  const double pw = skip ? 0.0 : P.w, nw = skip ? 0.0 : N.w;
+  if constexpr (TR) {
+    const double y1 = -(eps * (coef * dG0dx) - dG0dx);
+    const double y2 = -(coef * dG0dy * ieps - dG0dy);
+    s0 += y3 * nw + y1 * pw;
+    s1 += y2 * nw + y0 * pw;
+  } else {
+    const double y1 = eps * (coef * dG0dy) - dG0dy;
+    const double y2 = coef * dG0dx * ieps - dG0dx;
+    s0 += y0 * nw + y1 * pw;
+    s1 += y2 * nw + y3 * pw;
+  }
+}
+
 template <bool TR>
 __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ tiles,
                                               const int *__restrict__ leafStart, const int *__restrict__ leafCount,
@@ -654,6 +700,12 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
   const int *src = srcList + srcOff[D];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double ieps = 1.0 / eps, kap2 = kappa * kappa;
+  // wave w owns rows w, w + 4, w + 8, w + 12 and takes each staged source once
+  // for FLY64_RB of them at a time, held in registers (rows past the tile are
+  // skipped wave-uniformly)
+  int xi[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) xi[r] = leafStart[D] + r0 + min(w + 4 * r, rows - 1);
   double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
   for (int c0 = 0; c0 < ncol; c0 += FLY64_CHUNK) {
     const int cn = min(FLY64_CHUNK, ncol - c0);
@@ -666,48 +718,31 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
       sY[j] = y;
     }
     __syncthreads();
-    for (int r = 0; r < 4; r++) {
-      const int row = w + 4 * r;
-      if (row >= rows) continue;
-      const int xi = leafStart[D] + r0 + row;
-      const double4 X = PA[xi], XN = PN[xi];
-      double s0 = 0.0, s1 = 0.0;
-      for (int j = lane; j < cn; j += 64) {
-        if (sY[j] == xi) continue;
-        const double4 P = sP[j], N = sN[j];
-        const double rx = X.x - P.x, ry = X.y - P.y, rz = X.z - P.z;
-        const double r2 = rx * rx + ry * ry + rz * rz, rn = sqrt(r2), ri = 1.0 / rn, ri2 = ri * ri;
-        const double e = exp(-kappa * rn);
-        const double ip0 = XN.x * N.x + XN.y * N.y + XN.z * N.z;
-        const double ipX = XN.x * rx + XN.y * ry + XN.z * rz;
-        const double ipY = N.x * rx + N.y * ry + N.z * rz;
-        const double G0 = ri * kFourPiI, Gk = e * G0, coef = (kappa * rn + 1.0) * e;
-        const double dG0dy = ipY * G0 * ri2, dG0dx = ipX * G0 * ri2;
-        const double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) * ri2;
-        const double y0 = G0 - Gk;
-        const double y3 = (coef * ddG0 - kap2 * e * ipX * dG0dy) - ddG0;
-        if constexpr (TR) {
-          const double y1 = -(eps * (coef * dG0dx) - dG0dx);
-          const double y2 = -(coef * dG0dy * ieps - dG0dy);
-          s0 += y3 * N.w + y1 * P.w;
-          s1 += y2 * N.w + y0 * P.w;
-        } else {
-          const double y1 = eps * (coef * dG0dy) - dG0dy;
-          const double y2 = coef * dG0dx * ieps - dG0dx;
-          s0 += y0 * N.w + y1 * P.w;
-          s1 += y2 * N.w + y3 * P.w;
-        }
+#pragma unroll
+    for (int h = 0; h < 4; h += FLY64_RB) {
+      if (w + 4 * h >= rows) break;
+      double4 X[FLY64_RB], XN[FLY64_RB];
+#pragma unroll
+      for (int r = 0; r < FLY64_RB; r++) {
+        X[r] = PA[xi[h + r]];
+        XN[r] = PN[xi[h + r]];
       }
-      a0[r] += s0;
-      a1[r] += s1;
+      for (int j = lane; j < cn; j += 64) {
+        const double4 P = sP[j], N = sN[j];
+        const int y = sY[j];
+#pragma unroll
+        for (int r = 0; r < FLY64_RB; r++)
+          if (w + 4 * (h + r) < rows)
+            fly64Pair<TR>(X[r], XN[r], P, N, y == xi[h + r], kappa, kap2, eps, ieps, a0[h + r], a1[h + r]);
+      }
     }
   }
   for (int r = 0; r < 4; r++) {
     const int row = w + 4 * r;
-    const int xi = leafStart[D] + r0 + min(row, rows - 1);
-    double t0 = PA[xi].w * a0[r], t1 = PA[xi].w * a1[r];
+    const double A = PA[xi[r]].w;
+    double t0 = A * a0[r], t1 = A * a1[r];
     if (row < rows)
-      for (long long e = specOff[xi] + lane; e < specOff[xi + 1]; e += 64) {
+      for (long long e = specOff[xi[r]] + lane; e < specOff[xi[r] + 1]; e += 64) {
         const int y = specCol[e];
         const double4 d = specVal[e];
         t0 += d.x * x[y + n] + d.y * x[y];
@@ -716,8 +751,8 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
     t0 = waveSumT(t0);
     t1 = waveSumT(t1);
     if (row < rows && lane == 0) {
-      out[xi] = t0;
-      out[xi + n] = t1;
+      out[xi[r]] = t0;
+      out[xi[r] + n] = t1;
     }
   }
 }
@@ -903,6 +938,7 @@ __device__ __forceinline__ void flyExp(float t, float e, float &omE, float &cm1)
 }
 
 #define FLY_CHUNK 512
+#define FLY_RB 4   // rows per pass over the staged sources
 // LDS: (p hi, A sigma), (n, A sigma_n), (p lo, source panel index), FLY_CHUNK each
 __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ tiles,
                                               const int *__restrict__ leafStart, const int *__restrict__ leafCount,
@@ -942,18 +978,20 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
       sL[j] = make_float4(l.x, l.y, l.z, __int_as_float(y));
     }
     __syncthreads();
+    // each staged source is read once for FLY_RB rows at a time
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int row = w + 4 * r;
-      if (row < rows) {
-        float s0 = 0.f, s1 = 0.f;
-        for (int j = lane; j < cn; j += 64) {
-          const float4 Lo = sL[j];
-          if (__float_as_int(Lo.w) == xi[r]) continue;
-          const float4 P = sP[j], N = sN[j];
+    for (int h = 0; h < 4; h += FLY_RB) {
+      if (w + 4 * h >= rows) break;
+      for (int j = lane; j < cn; j += 64) {
+        const float4 Lo = sL[j], P = sP[j], N = sN[j];
+        const int y = __float_as_int(Lo.w);
+#pragma unroll
+        for (int r = h; r < h + FLY_RB; r++) {
+          if (w + 4 * r >= rows) continue;
+          const bool skip = y == xi[r];   // the panel itself: weights zeroed, r2 replaced by 1
           const float rx = (XP[r].x - P.x) + (XL[r].x - Lo.x), ry = (XP[r].y - P.y) + (XL[r].y - Lo.y),
                       rz = (XP[r].z - P.z) + (XL[r].z - Lo.z);
-          const float r2 = rx * rx + ry * ry + rz * rz, ri = rsqrtf(r2), ri2 = ri * ri, rr = r2 * ri;
+          const float r2 = skip ? 1.f : rx * rx + ry * ry + rz * rz, ri = rsqrtf(r2), ri2 = ri * ri, rr = r2 * ri;
           const float t = kappa * rr, e = expf(-t);
           float omE, cm1;
           flyExp(t, e, omE, cm1);
@@ -967,11 +1005,10 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
           const float y1 = dG0dy * (eps * coef - 1.f);
           const float y2 = dG0dx * (coef * ieps - 1.f);
           const float y3 = cm1 * ddG0 - kap2 * e * ipX * dG0dy;
-          s0 += y0 * N.w + y1 * P.w;
-          s1 += y2 * N.w + y3 * P.w;
+          const float pw = skip ? 0.f : P.w, nw = skip ? 0.f : N.w;
+          a0[r] += y0 * nw + y1 * pw;
+          a1[r] += y2 * nw + y3 * pw;
         }
-        a0[r] += s0;
-        a1[r] += s1;
       }
     }
   }
@@ -3586,11 +3623,13 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
         if (cubeLevel[m2lSrc[k]] != cubeLevel[cb]) { c->m2lSym = false; break; }
   }
   // Tiles that stream stored coefficients and tiles computed on the fly (q = 1
-  // only), interleaved in tile order so the HBM stream and the arithmetic run
-  // side by side: a tile streams while the streamed share of the interactions
-  // so far stays within the fraction f (FABIPB_NEAR_STREAM; FABIPB_NEAR=stream
-  // or fly for all of one kind).
-  double f = 0.45;
+  // only), interleaved in tile order: a tile streams while the streamed share
+  // of the interactions so far stays within the fraction f (FABIPB_NEAR_STREAM;
+  // FABIPB_NEAR=stream or fly for all of one kind).  The default computes every
+  // tile: the on-the-fly tiles (two target rows per pass over the staged
+  // sources) outrun the 32-byte coefficient stream, and the far chain's
+  // memory-bound passes overlap them better than they overlap the stream.
+  double f = 0.0;
   {
     const char *fs = getenv("FABIPB_NEAR_STREAM"), *fm = getenv("FABIPB_NEAR");
     if (fs) f = std::min(1.0, std::max(0.0, atof(fs)));

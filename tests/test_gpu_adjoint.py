@@ -144,10 +144,10 @@ def test_autograd_wrapper(fa, meshdir):
     assert abs(float(k.grad) - dk) <= 1e-12 * abs(dk)
 
 
-@pytest.mark.parametrize("frac", ["0", "1"])
+@pytest.mark.parametrize("frac", ["0.45", "1"])
 def test_adjoint_identity_near_modes(fa, meshdir, monkeypatch, frac):
-    """The transpose identity with every nearfield tile computed on the fly (0) and with
-    every tile streaming stored coefficients (1); the default mixes the two."""
+    """The transpose identity with 45 % of the nearfield interactions streaming stored
+    coefficients and with every tile streaming (1); the default computes every tile."""
     import torch
     monkeypatch.setenv("FABIPB_NEAR_STREAM", frac)
     mesh = fa.Mesh(str(meshdir / "glob4"))
@@ -155,7 +155,7 @@ def test_adjoint_identity_near_modes(fa, meshdir, monkeypatch, frac):
                    eps2=FL["eps2"], qorder=FL["q"], height=FL["H"])
     op = P.operator(FL["k"])
     st = op.stats()
-    assert (st["near_fly_tiles"] == 0) == (frac == "1") and (st["near_stored"] == 0) == (frac == "0")
+    assert (st["near_fly_tiles"] == 0) == (frac == "1") and st["near_stored"] > 0
     n2 = 2 * P.tree.n
     g = torch.Generator(device="cpu").manual_seed(6)
     x = (torch.rand(n2, generator=g, dtype=torch.float64) * 2 - 1).cuda()
