@@ -4803,7 +4803,27 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
     ok = chargeTreeWalk(c, 2, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
                         ctChgIdx, ctOrder, theta, nullptr, g, nullptr) &&
          gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info);
-  if (ok) ok = uploadCharges(nChar, pos, chr, &dp, &dc);
+  // lanes take the charges in the charge tree's leaf order (ctChgIdx, a
+  // permutation of all charges grouped by leaf), so the 64 charges of a wave
+  // are close together and walk nearly the same cubes; in input order a wave's
+  // charges may lie anywhere and the wave walks the union of their paths
+  std::vector<int> perm(nChar);
+  {
+    std::vector<char> seen(nChar, 0);
+    bool isPerm = ctChgIdx != nullptr;
+    for (int j = 0; j < nChar && isPerm; j++) {
+      const int k = ctChgIdx[j];
+      isPerm = k >= 0 && k < nChar && !seen[k];
+      if (isPerm) seen[k] = 1;
+    }
+    for (int j = 0; j < nChar; j++) perm[j] = isPerm ? ctChgIdx[j] : j;
+  }
+  std::vector<double> posP(3 * (size_t)nChar), chrP(nChar);
+  for (int j = 0; j < nChar; j++) {
+    for (int d = 0; d < 3; d++) posP[3 * (size_t)j + d] = pos[3 * (size_t)perm[j] + d];
+    chrP[j] = chr[perm[j]];
+  }
+  if (ok) ok = uploadCharges(nChar, posP.data(), chrP.data(), &dp, &dc);
   if (ok) {   // moments of u and of (lam_n, lam), tables, walk
     OpBufs<double> B = bufs64(c);
     const double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
@@ -4826,9 +4846,13 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
     break;
     switch (P) { CH_WALK(1) CH_WALK(2) CH_WALK(3) CH_WALK(4) CH_WALK(5) CH_WALK(6) CH_WALK(7) CH_WALK(8) CH_WALK(9) }
 #undef CH_WALK
+    std::vector<double> dqP(nChar);
     ok = hipGetLastError() == hipSuccess &&
-         hipMemcpyAsync(dEdq, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         hipMemcpyAsync(dqP.data(), dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream) ==
+             hipSuccess &&
          hipStreamSynchronize(c->stream) == hipSuccess;
+    if (ok)
+      for (int j = 0; j < nChar; j++) dEdq[perm[j]] = dqP[j];
     if (!ok) setErr("fbh_gradient_tree_dev: walk kernels failed");
   }
   hipStreamSynchronize(c->stream);
