@@ -2199,6 +2199,12 @@ __global__ void k_ct_m2m(int levStart, const int *__restrict__ kidStart, const i
   const int par = levStart + blockIdx.x;
   const int ordOut = ordOf[lev], nOut = NMOM(ordOut), nIn = NMOM(ordOf[lev + 1]);
   double *R = sm, *b = sm + nOut;
+  // the (alpha - k, k) pair table of the outputs, staged once: the inner loop
+  // then reads LDS instead of waiting on global loads term by term
+  int *tS = reinterpret_cast<int *>(b + nOut), *tab = tS + nOut + 1;
+  const int t0 = T.m2mStart[0];
+  for (int a = threadIdx.x; a <= nOut; a += blockDim.x) tS[a] = T.m2mStart[a] - t0;
+  for (int e = threadIdx.x; e < T.m2mStart[nOut] - t0; e += blockDim.x) tab[e] = T.m2m[t0 + e];
   double acc[4] = {0, 0, 0, 0};
   for (int k = 0; k < nKids[par]; k++) {
     const int kid = kidStart[par] + k;
@@ -2215,8 +2221,8 @@ __global__ void k_ct_m2m(int levStart, const int *__restrict__ kidStart, const i
       const int a = threadIdx.x + s * blockDim.x;
       if (a < nOut) {
         double t = 0.0;
-        for (int e = T.m2mStart[a]; e < T.m2mStart[a + 1]; e++) {
-          int pr = T.m2m[e];
+        for (int e = tS[a]; e < tS[a + 1]; e++) {
+          const int pr = tab[e];
           if ((pr >> 16) < nIn) t += R[pr & 0xffff] * b[pr >> 16];
         }
         acc[s] += t;
@@ -2507,11 +2513,16 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
                                                      const int *__restrict__ chgIdx, const double *__restrict__ pos,
                                                      const double *__restrict__ chr, const double *__restrict__ poly,
                                                      int c0, int depth, int height, double theta, double kappa,
-                                                     double eps, QTab Q, int q, double *__restrict__ part) {
+                                                     double eps, QTab Q, int q, int nH, int S,
+                                                     double *__restrict__ part) {
   __shared__ int stC[4][EN_WSTACK];
   __shared__ unsigned long long stM[4][EN_WSTACK];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int p = (blockIdx.x * 4 + w) * 64 + lane;
+  // S waves per 64 panels: slice s walks the clusters c0 + s, c0 + s + S, ... of
+  // level `height` (no cluster above it can be accepted, so every lane reaches
+  // all of them) and writes its own slice of part
+  const int g = blockIdx.x * 4 + w, sl = g % S;
+  const int p = (g / S) * 64 + lane;
   const bool live = p < n;
   const unsigned long long all = __ballot(live);
   if (all == 0ull) return;   // whole wave past the end (uniform)
@@ -2531,9 +2542,11 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
       double qp[3];
       for (int k = 0; k < 3; k++) qp[k] = v0[k] + Q.t[ix] * (e2[k] + Q.t[jx] * e0[k]);
       double y0 = 0.0, y1 = 0.0;
-      stC[w][0] = 0;
-      stM[w][0] = all;
-      int sp = 1;
+      int sp = 0;
+      for (int k = sl + ((nH - 1 - sl) / S) * S; k >= sl; k -= S, sp++) {   // pushed in reverse: visited in order
+        stC[w][sp] = c0 + k;
+        stM[w][sp] = all;
+      }
       while (sp > 0) {
         sp--;
         const int cb = __builtin_amdgcn_readfirstlane(stC[w][sp]);
@@ -2584,15 +2597,24 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
   o1 *= 2.0 * area;
   if (live) {
     if constexpr (MODE == 0) {
-      part[p] = o0 * sgm[p + n] + o1 * sgm[p];
+      part[(long long)sl * n + p] = o0 * sgm[p + n] + o1 * sgm[p];
     } else if constexpr (MODE == 2) {   // part is the energy's weight vector g[2n]: E = g . [sgm | sgm_n]
-      part[p] = o1;
-      part[p + n] = o0;
+      part[(long long)sl * 2 * n + p] = o1;
+      part[(long long)sl * 2 * n + p + n] = o0;
     } else {   // part is the right-hand side b[2n], fac = 1/(4 pi eps1) passed as eps
-      part[p] = eps * o0;
-      part[p + n] = eps * o1;
+      part[(long long)sl * 2 * n + p] = eps * o0;
+      part[(long long)sl * 2 * n + p + n] = eps * o1;
     }
   }
+}
+
+// out[i] = sum over the S slices of a (slice stride m), in slice order
+__global__ void k_sum_slices(const double *__restrict__ a, int S, long long m, double *__restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double v = a[i];
+  for (int s = 1; s < S; s++) v += a[(long long)s * m + i];
+  out[i] = v;
 }
 
 // block partials of sum a[i] over a fixed grid (deterministic order)
@@ -4915,6 +4937,15 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
   for (int l = c->height; l <= depth && uni; l++) uni = ord[l] == P;
   std::vector<int4> dsc;
   const int SZ = uni ? enPolySize(P) : 0, nClus = uni ? nc - c0 : 0;
+  // waves per 64 panels (the wave walk): enough waves to fill the device when
+  // the panels alone do not, at most one per cluster of level `height`, and
+  // few enough start clusters per wave for the walk's stack
+  int S = 1;
+  if (uni) {
+    const int nH = lvCount[c->height], panelWaves = (c->n + 63) / 64;
+    S = std::max(1, std::min(nH, (8192 + panelWaves - 1) / panelWaves));
+    while ((nH + S - 1) / S + 8 * (depth - c->height + 1) > EN_WSTACK) S++;
+  }
   if (uni) {
     int o = 0;
     for (int a = P; a >= 0; a--)   // the order enFar reads them in
@@ -4946,7 +4977,7 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
                iChr = add(chr, nChar * sizeof(double)), iDesc = add(dsc.data(), dsc.size() * sizeof(int4));
   const size_t inBytes = top;
   const size_t iMom = add(nullptr, (size_t)nc * nmMax * sizeof(double)),
-               iPart = add(nullptr, (size_t)c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
+               iPart = add(nullptr, (size_t)S * 2 * c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
                iRed = add(nullptr, RED_BLOCKS * sizeof(double)),
                iPoly = add(nullptr, (size_t)nClus * SZ * sizeof(double));
   if (c->enArenaBytes < top) {
@@ -4977,18 +5008,20 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
   for (int l = depth - 1; l >= c->height; l--) {
     if (lvCount[l] <= 0) continue;
     int nOut = NMOM(ord[l]);
-    hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), 2 * nOut * sizeof(double),
-                       c->stream, lvStart[l], dKid, dNK, dX, dOrd, l, dMom, nmMax, c->T);
+    const size_t lds = 2 * nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut] - c->hM2mStart[0]) * sizeof(int);
+    hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l], dKid,
+                       dNK, dX, dOrd, l, dMom, nmMax, c->T);
   }
   if (uni) {
     hipLaunchKernelGGL(k_ct_poly, dim3(nClus), dim3(256), 0, c->stream, c0, dMom, nmMax, dDesc, (int)dsc.size(), SZ,
                        dPoly);
-    const int blocks = ((c->n + 63) / 64 + 3) / 4;
+    const int blocks = ((c->n + 63) / 64 * S + 3) / 4;
 #define EN_WAVE(PP, MM)                                                                                           \
   case PP:                                                                                                        \
     hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
                        dLevel, dX, dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta,  \
-                       c->kappa, MM == 1 ? kFourPiI / c->eps1 : c->eps, c->Q, c->qorder, MM ? b_dev : dPart);      \
+                       c->kappa, MM == 1 ? kFourPiI / c->eps1 : c->eps, c->Q, c->qorder, lvCount[c->height], S,  \
+                       (MM && S == 1) ? b_dev : dPart);                                                           \
     break;
     if (mode == 0) {
       switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
@@ -4998,6 +5031,9 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
       switch (P) { EN_WAVE(1, 1) EN_WAVE(2, 1) EN_WAVE(3, 1) EN_WAVE(4, 1) EN_WAVE(5, 1) EN_WAVE(6, 1) EN_WAVE(7, 1) EN_WAVE(8, 1) EN_WAVE(9, 1) }
     }
 #undef EN_WAVE
+    if (mode != 0 && S > 1)
+      hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, c->stream, dPart, S,
+                         2 * (long long)c->n, b_dev);
   } else if (mode != 0) {
     setErr("fbh_rhs_tree / tree gradient: the accepted levels must share one expansion order");
     return 0;
@@ -5010,7 +5046,7 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
   }
   hipError_t e = hipSuccess;
   if (mode == 0) {
-    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, c->n, dRed);
+    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, S * c->n, dRed);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
