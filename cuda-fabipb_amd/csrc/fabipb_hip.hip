@@ -650,6 +650,20 @@ __device__ __forceinline__ double rsq64(double x) {
   return fma(0.5 * y, e, y);
 }
 
+// e^x for x <= 0 by the table-driven method of Tang (ACM TOMS 15 (1989) 144-157):
+// x = (256 m + j) ln2/256 + r with |r| <= ln2/512 (ln2/256 split in two so that
+// k ln2/256 is exact), e^x = 2^m 2^(j/256) e^r, e^r from its Taylor polynomial
+// through r^4 (truncation r^5/120 < 4e-17), 2^(j/256) from the table T.  x is
+// clamped at -700, so the result stays a normal number and nothing overflows.
+__device__ __forceinline__ double expNeg(double x, const double *__restrict__ T) {
+  x = fmax(x, -700.0);
+  const double kf = rint(x * 369.3299304675746);                       // 256 / ln2
+  const double r = fma(-kf, -0x1.718432a1b0e26p-43, fma(-kf, 0x1.62e42ff000000p-9, x));
+  const int k = (int)kf;
+  const double q = fma(r, fma(r, fma(r, fma(r, 1.0 / 24.0, 1.0 / 6.0), 0.5), 1.0), 1.0);
+  return ldexp(T[k & 255] * q, k >> 8);
+}
+
 // One pair of the on-the-fly FP64 nearfield (kernelKER4 at the one-point rule,
 // src/kernel.c:109-147): target point/normal X, XN against staged source
 // P = (point, A sigma), N = (normal, A sigma_n).  skip: the panel itself (its
@@ -657,11 +671,11 @@ __device__ __forceinline__ double rsq64(double x) {
 template <bool TR>
 __device__ __forceinline__ void fly64Pair(const double4 &X, const double4 &XN, const double4 &P, const double4 &N,
                                           bool skip, double kappa, double kap2, double eps, double ieps,
-                                          double &s0, double &s1) {
+                                          const double *__restrict__ T, double &s0, double &s1) {
   const double rx = X.x - P.x, ry = X.y - P.y, rz = X.z - P.z;
   const double r2 = skip ? 1.0 : rx * rx + ry * ry + rz * rz;
   const double ri = rsq64(r2), rn = r2 * ri, ri2 = ri * ri;
-  const double e = exp(-kappa * rn);
+  const double e = expNeg(-kappa * rn, T);
   const double ip0 = XN.x * N.x + XN.y * N.y + XN.z * N.z;
   const double ipX = XN.x * rx + XN.y * ry + XN.z * rz;
   const double ipY = N.x * rx + N.y * ry + N.z * rz;
@@ -692,7 +706,8 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
                                               const double4 *__restrict__ PN, const double *__restrict__ x,
                                               const long long *__restrict__ specOff, const int *__restrict__ specCol,
                                               const double4 *__restrict__ specVal, double *__restrict__ out, int n,
-                                              double kappa, double eps, double4 *sP, double4 *sN, int *sY) {
+                                              double kappa, double eps, double4 *sP, double4 *sN, int *sY,
+                                              double *sE) {
   const int2 tl = tiles[t];
   const int D = tl.x, r0 = tl.y;
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
@@ -733,7 +748,7 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
 #pragma unroll
         for (int r = 0; r < FLY64_RB; r++)
           if (w + 4 * (h + r) < rows)
-            fly64Pair<TR>(X[r], XN[r], P, N, y == xi[h + r], kappa, kap2, eps, ieps, a0[h + r], a1[h + r]);
+            fly64Pair<TR>(X[r], XN[r], P, N, y == xi[h + r], kappa, kap2, eps, ieps, sE, a0[h + r], a1[h + r]);
       }
     }
   }
@@ -774,17 +789,26 @@ __global__ __launch_bounds__(256) void k_near_mix64(const int2 *__restrict__ til
                                                     const long long *__restrict__ specOff,
                                                     const int *__restrict__ specCol,
                                                     const double4 *__restrict__ specVal, double *__restrict__ out,
-                                                    int n, double kappa, double eps) {
+                                                    int n, double kappa, double eps, int nTiles,
+                                                    const int *__restrict__ order) {
   __shared__ __align__(16) unsigned char lds[MIX64_LDS];
-  const int t = blockIdx.x;
-  if (tileCoef[t] >= 0) {
-    double *sS = reinterpret_cast<double *>(lds);
-    nearStreamTile<double>(t, tiles, leafStart, leafCount, srcOff, ncolArr, tileCoef, srcList, coef, x, out, n, sS,
-                           sS + NEAR_CHUNK);
-  } else {
-    double4 *sP = reinterpret_cast<double4 *>(lds);
-    nearFly64Tile<TR>(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, x, specOff, specCol, specVal,
-                      out, n, kappa, eps, sP, sP + FLY64_CHUNK, reinterpret_cast<int *>(sP + 2 * FLY64_CHUNK));
+  __shared__ double sE[256];   // 2^(j/256) for expNeg
+  for (int j = threadIdx.x; j < 256; j += blockDim.x) sE[j] = exp2((double)j / 256.0);
+  // tiles in `order`, costliest first (the default grid has one block per tile;
+  // a smaller grid strides over them)
+  for (int k = blockIdx.x; k < nTiles; k += gridDim.x) {
+    const int t = order[k];
+    __syncthreads();
+    if (tileCoef[t] >= 0) {
+      double *sS = reinterpret_cast<double *>(lds);
+      nearStreamTile<double>(t, tiles, leafStart, leafCount, srcOff, ncolArr, tileCoef, srcList, coef, x, out, n, sS,
+                             sS + NEAR_CHUNK);
+    } else {
+      double4 *sP = reinterpret_cast<double4 *>(lds);
+      nearFly64Tile<TR>(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, x, specOff, specCol,
+                        specVal, out, n, kappa, eps, sP, sP + FLY64_CHUNK, reinterpret_cast<int *>(sP + 2 * FLY64_CHUNK),
+                        sE);
+    }
   }
 }
 
@@ -1047,9 +1071,9 @@ __global__ __launch_bounds__(256) void k_near_mix32(const int2 *__restrict__ til
                                                     const long long *__restrict__ specOff,
                                                     const int *__restrict__ specCol,
                                                     const float4 *__restrict__ specVal, float *__restrict__ out, int n,
-                                                    float kappa, float eps) {
+                                                    float kappa, float eps, const int *__restrict__ order) {
   __shared__ __align__(16) unsigned char lds[MIX32_LDS];
-  const int t = blockIdx.x;
+  const int t = order[blockIdx.x];   // costliest tiles first
   if (tileCoef[t] >= 0) {
     float *sS = reinterpret_cast<float *>(lds);
     nearStreamTile<float>(t, tiles, leafStart, leafCount, srcOff, ncolArr, tileCoef, srcList, coef, x, out, n, sS,
@@ -3017,6 +3041,9 @@ struct fbh_ctx {
   size_t m2lLds = 0;
   int m2lWaves = 0;
   int m2lMaxT = 0;   // register-resident term lists (16, 32) or 0
+  // FP64 mixed nearfield launch: tiles costliest first, grid-stride over them
+  int *tileOrder = nullptr;
+  int nearGrid = 0;
   // M2L term grid (k_m2l_grid): K steps x 64 lanes per order, runs of L steps
   int gridK = 0, gridL = 0, gridSlotZ = 0, gridNcc = 0, gridNgg = 0;
   unsigned *gridTerms = nullptr;
@@ -3628,6 +3655,21 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   }
   c->nInter = coefOff[nLeaf];
   c->nTiles = (int)tiles.size();
+  {   // tiles costliest first (rows x columns) for the grid-stride launch
+    std::vector<int> order(tiles.size());
+    std::vector<long long> cost(tiles.size());
+    for (size_t t = 0; t < tiles.size(); t++) {
+      order[t] = (int)t;
+      cost[t] = (long long)std::min(NEAR_ROWS, leafCount[tiles[t].x] - tiles[t].y) * ncol[tiles[t].x];
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    if (!(c->tileOrder = dupload(c, order.data(), order.size()))) FAIL();
+    // blocks of the FP64 nearfield launch (0: one per tile, the fastest here;
+    // resident grids of 2-4 blocks per CU were slower beside the far chain)
+    const char *eg = getenv("FABIPB_NEAR_GRID");
+    const int g = eg ? atoi(eg) : 0;
+    c->nearGrid = g > 0 ? std::min(g, c->nTiles) : c->nTiles;
+  }
   {   // the adjoint operator relies on symmetric near and M2L lists
     auto symmetric = [](int cnt, const int *off, const int *lst) {
       std::vector<std::pair<int, int>> e;
@@ -3821,13 +3863,14 @@ template <typename R>
 static void nearPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
   if (c->nFlyTiles > 0) {
     if constexpr (std::is_same<R, double>::value)
-      hipLaunchKernelGGL(k_near_mix64<false>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
-                         c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->fly64PA, c->fly64PN, x, c->specOff,
-                         c->specCol, c->specVal64, B.nearv, c->n, c->kappa, c->eps);
+      hipLaunchKernelGGL(k_near_mix64<false>, dim3(c->nearGrid), dim3(256), 0, st, c->tiles, c->leafStart,
+                         c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->fly64PA, c->fly64PN, x,
+                         c->specOff, c->specCol, c->specVal64, B.nearv, c->n, c->kappa, c->eps, c->nTiles,
+                         c->tileOrder);
     else
       hipLaunchKernelGGL(k_near_mix32, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
                          c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->flyPA, c->flyPN, c->flyPL, x,
-                         c->specOff, c->specCol, c->specVal, B.nearv, c->n, (float)c->kappa, (float)c->eps);
+                         c->specOff, c->specCol, c->specVal, B.nearv, c->n, (float)c->kappa, (float)c->eps, c->tileOrder);
     return;
   }
   hipLaunchKernelGGL(k_near_apply<R>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
@@ -3943,9 +3986,10 @@ static int matvecTr(fbh_ctx *c, double alpha, const double *x, double beta, doub
   HCK(hipEventRecord(c->evFork, c->stream));
   HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
   if (c->nFlyTiles > 0)
-    hipLaunchKernelGGL(k_near_mix64<true>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
+    hipLaunchKernelGGL(k_near_mix64<true>, dim3(c->nearGrid), dim3(256), 0, c->side, c->tiles, c->leafStart,
                        c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT, c->fly64PA, c->fly64PN, x,
-                       c->specOff, c->specCol, c->specValT64, c->nearv, c->n, c->kappa, c->eps);
+                       c->specOff, c->specCol, c->specValT64, c->nearv, c->n, c->kappa, c->eps, c->nTiles,
+                       c->tileOrder);
   else
     hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
                        c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT, x, c->nearv, c->n);
