@@ -635,19 +635,19 @@ __global__ __launch_bounds__(256) void k_near_apply(const int2 *__restrict__ til
 // rounding of the formula's operation order.
 #define FLY64_CHUNK 512
 #define FLY64_RB 2   // rows per pass over the staged sources
+#define FLY64_OWNSPLIT 1   // the self-pair selects only on the tile's own leaf columns
 // TR: the transposed operator's row x, i.e. KER(y, x) read transposed: with
 // r, n_x, n_y as below, y0 and y3 are symmetric in (x, y) and
 //   y1(y,x) = -(eps coef - 1) dG0dx,  y2(y,x) = -(coef/eps - 1) dG0dy,
 //   out[x] += y3 A sgm_n + y1(y,x) A sgm,  out[x+n] += y2(y,x) A sgm_n + y0 A sgm
-// 1/sqrt(x) for x > 0 (a normal FP64 number): the hardware estimate, then two
-// Newton steps y <- y + y (1 - x y^2) / 2, each of which squares the relative
-// error, so the result is within a few ulp of the correctly rounded value.
+// 1/sqrt(x) for x > 0 (a normal FP64 number): the hardware estimate (relative
+// error 2^-24 measured on gfx950), then one third-order step
+// y <- y (1 + e/2 + 3e^2/8), e = 1 - x y^2, which cubes the error (measured
+// worst case 1.4e-16 over 4M arguments from 1e-4 to 1e4, as two Newton steps).
 __device__ __forceinline__ double rsq64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double e = fma(-(x * y), y, 1.0);
-  y = fma(0.5 * y, e, y);
-  e = fma(-(x * y), y, 1.0);
-  return fma(0.5 * y, e, y);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.0);
+  return fma(y, e * fma(e, 0.375, 0.5), y);
 }
 
 // e^x for x <= 0 by the table-driven method of Tang (ACM TOMS 15 (1989) 144-157):
@@ -668,12 +668,13 @@ __device__ __forceinline__ double expNeg(double x, const double *__restrict__ T)
 // src/kernel.c:109-147): target point/normal X, XN against staged source
 // P = (point, A sigma), N = (normal, A sigma_n).  skip: the panel itself (its
 // weights are zeroed and r2 replaced by 1 so nothing divides by zero).
-template <bool TR>
+// CHK: the source may be the panel itself (the tile's own leaf columns)
+template <bool TR, bool CHK>
 __device__ __forceinline__ void fly64Pair(const double4 &X, const double4 &XN, const double4 &P, const double4 &N,
                                           bool skip, double kappa, double kap2, double eps, double ieps,
                                           const double *__restrict__ T, double &s0, double &s1) {
   const double rx = X.x - P.x, ry = X.y - P.y, rz = X.z - P.z;
-  const double r2 = skip ? 1.0 : rx * rx + ry * ry + rz * rz;
+  const double r2 = CHK && skip ? 1.0 : rx * rx + ry * ry + rz * rz;
   const double ri = rsq64(r2), rn = r2 * ri, ri2 = ri * ri;
   const double e = expNeg(-kappa * rn, T);
   const double ip0 = XN.x * N.x + XN.y * N.y + XN.z * N.z;
@@ -684,7 +685,7 @@ __device__ __forceinline__ void fly64Pair(const double4 &X, const double4 &XN, c
   const double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) * ri2;
   const double y0 = G0 - Gk;
   const double y3 = (coef * ddG0 - kap2 * e * ipX * dG0dy) - ddG0;
-  const double pw = skip ? 0.0 : P.w, nw = skip ? 0.0 : N.w;
+  const double pw = CHK && skip ? 0.0 : P.w, nw = CHK && skip ? 0.0 : N.w;
   if constexpr (TR) {
     const double y1 = -(eps * (coef * dG0dx) - dG0dx);
     const double y2 = -(coef * dG0dy * ieps - dG0dy);
@@ -707,9 +708,10 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
                                               const long long *__restrict__ specOff, const int *__restrict__ specCol,
                                               const double4 *__restrict__ specVal, double *__restrict__ out, int n,
                                               double kappa, double eps, double4 *sP, double4 *sN, int *sY,
-                                              double *sE) {
+                                              double *sE, const int *__restrict__ ownOff) {
   const int2 tl = tiles[t];
   const int D = tl.x, r0 = tl.y;
+  const int oc = ownOff[D], nOwn = oc >= 0 ? leafCount[D] : 0;   // D's own columns: [oc, oc + nOwn)
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
   const int ncol = ncolArr[D];
   const int *src = srcList + srcOff[D];
@@ -744,11 +746,19 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
       }
       for (int j = lane; j < cn; j += 64) {
         const double4 P = sP[j], N = sN[j];
-        const int y = sY[j];
+        if (!FLY64_OWNSPLIT || (unsigned)(c0 + j - oc) < (unsigned)nOwn) {   // D's own columns: skip the panel itself
+          const int y = sY[j];
 #pragma unroll
-        for (int r = 0; r < FLY64_RB; r++)
-          if (w + 4 * (h + r) < rows)
-            fly64Pair<TR>(X[r], XN[r], P, N, y == xi[h + r], kappa, kap2, eps, ieps, sE, a0[h + r], a1[h + r]);
+          for (int r = 0; r < FLY64_RB; r++)
+            if (w + 4 * (h + r) < rows)
+              fly64Pair<TR, true>(X[r], XN[r], P, N, y == xi[h + r], kappa, kap2, eps, ieps, sE, a0[h + r],
+                                  a1[h + r]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < FLY64_RB; r++)
+            if (w + 4 * (h + r) < rows)
+              fly64Pair<TR, false>(X[r], XN[r], P, N, false, kappa, kap2, eps, ieps, sE, a0[h + r], a1[h + r]);
+        }
       }
     }
   }
@@ -790,7 +800,7 @@ __global__ __launch_bounds__(256) void k_near_mix64(const int2 *__restrict__ til
                                                     const int *__restrict__ specCol,
                                                     const double4 *__restrict__ specVal, double *__restrict__ out,
                                                     int n, double kappa, double eps, int nTiles,
-                                                    const int *__restrict__ order) {
+                                                    const int *__restrict__ order, const int *__restrict__ ownOff) {
   __shared__ __align__(16) unsigned char lds[MIX64_LDS];
   __shared__ double sE[256];   // 2^(j/256) for expNeg
   for (int j = threadIdx.x; j < 256; j += blockDim.x) sE[j] = exp2((double)j / 256.0);
@@ -807,7 +817,7 @@ __global__ __launch_bounds__(256) void k_near_mix64(const int2 *__restrict__ til
       double4 *sP = reinterpret_cast<double4 *>(lds);
       nearFly64Tile<TR>(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, x, specOff, specCol,
                         specVal, out, n, kappa, eps, sP, sP + FLY64_CHUNK, reinterpret_cast<int *>(sP + 2 * FLY64_CHUNK),
-                        sE);
+                        sE, ownOff);
     }
   }
 }
@@ -3044,6 +3054,7 @@ struct fbh_ctx {
   // FP64 mixed nearfield launch: tiles costliest first, grid-stride over them
   int *tileOrder = nullptr;
   int nearGrid = 0;
+  int *ownOff = nullptr;   // per leaf: the column of its own first panel in its column list (-1: none)
   // M2L term grid (k_m2l_grid): K steps x 64 lanes per order, runs of L steps
   int gridK = 0, gridL = 0, gridSlotZ = 0, gridNcc = 0, gridNgg = 0;
   unsigned *gridTerms = nullptr;
@@ -3641,10 +3652,12 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   std::vector<int> srcList;
   std::vector<int2> tiles;
   srcOff[0] = 0; coefOff[0] = 0;
+  std::vector<int> ownOff(nLeaf, -1);
   for (int l = 0; l < nLeaf; l++) {
     int nc = 0;
     for (int k = nearOff[l]; k < nearOff[l + 1]; k++) {
       int s = nearSrc[k];
+      if (s == l && ownOff[l] < 0) ownOff[l] = nc;
       for (int j = 0; j < leafCount[s]; j++) srcList.push_back(leafStart[s] + j);
       nc += leafCount[s];
     }
@@ -3663,7 +3676,8 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
       cost[t] = (long long)std::min(NEAR_ROWS, leafCount[tiles[t].x] - tiles[t].y) * ncol[tiles[t].x];
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    if (!(c->tileOrder = dupload(c, order.data(), order.size()))) FAIL();
+    if (!(c->tileOrder = dupload(c, order.data(), order.size())) || !(c->ownOff = dupload(c, ownOff.data(), ownOff.size())))
+      FAIL();
     // blocks of the FP64 nearfield launch (0: one per tile, the fastest here;
     // resident grids of 2-4 blocks per CU were slower beside the far chain)
     const char *eg = getenv("FABIPB_NEAR_GRID");
@@ -3866,7 +3880,7 @@ static void nearPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_
       hipLaunchKernelGGL(k_near_mix64<false>, dim3(c->nearGrid), dim3(256), 0, st, c->tiles, c->leafStart,
                          c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->fly64PA, c->fly64PN, x,
                          c->specOff, c->specCol, c->specVal64, B.nearv, c->n, c->kappa, c->eps, c->nTiles,
-                         c->tileOrder);
+                         c->tileOrder, c->ownOff);
     else
       hipLaunchKernelGGL(k_near_mix32, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
                          c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->flyPA, c->flyPN, c->flyPL, x,
@@ -3989,7 +4003,7 @@ static int matvecTr(fbh_ctx *c, double alpha, const double *x, double beta, doub
     hipLaunchKernelGGL(k_near_mix64<true>, dim3(c->nearGrid), dim3(256), 0, c->side, c->tiles, c->leafStart,
                        c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT, c->fly64PA, c->fly64PN, x,
                        c->specOff, c->specCol, c->specValT64, c->nearv, c->n, c->kappa, c->eps, c->nTiles,
-                       c->tileOrder);
+                       c->tileOrder, c->ownOff);
   else
     hipLaunchKernelGGL(k_near_apply<double>, dim3(c->nTiles), dim3(256), 0, c->side, c->tiles, c->leafStart,
                        c->leafCount, c->srcOff, c->ncol, c->tileCoef, c->srcList, c->coefT, x, c->nearv, c->n);
