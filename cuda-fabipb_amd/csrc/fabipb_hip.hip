@@ -1808,6 +1808,7 @@ __global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *
     R t0 = 0, t1 = 0;
     if (j < c) {
       const R *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
+#pragma unroll 4   // several loads in flight per lane
       for (int a = l8; a < nm; a += 8) {
         t0 += q0[a] * L12[a];
         t1 += q1[a] * L34[a];
