@@ -668,34 +668,41 @@ __device__ __forceinline__ double expNeg(double x, const double *__restrict__ T)
 // src/kernel.c:109-147): target point/normal X, XN against staged source
 // P = (point, A sigma), N = (normal, A sigma_n).  skip: the panel itself (its
 // weights are zeroed and r2 replaced by 1 so nothing divides by zero).
-// CHK: the source may be the panel itself (the tile's own leaf columns)
+// CHK: the source may be the panel itself (the tile's own leaf columns).
+// Every term is linear in G0 = 1/(4 pi r), so the sums are kept with G0
+// replaced by 1/r and the row's total is scaled by 1/(4 pi) once; with
+// a = ipX dG0dy shared by ddG0 and the kappa^2 term, and the accumulations as
+// FMA chains, a pair is 60 FP64/integer VALU instructions.
 template <bool TR, bool CHK>
 __device__ __forceinline__ void fly64Pair(const double4 &X, const double4 &XN, const double4 &P, const double4 &N,
                                           bool skip, double kappa, double kap2, double eps, double ieps,
                                           const double *__restrict__ T, double &s0, double &s1) {
   const double rx = X.x - P.x, ry = X.y - P.y, rz = X.z - P.z;
   const double r2 = CHK && skip ? 1.0 : rx * rx + ry * ry + rz * rz;
-  const double ri = rsq64(r2), rn = r2 * ri, ri2 = ri * ri;
-  const double e = expNeg(-kappa * rn, T);
+  const double ri = rsq64(r2), rn = r2 * ri, ri2 = ri * ri, ri3 = ri * ri2;
+  const double kr = kappa * rn;
+  const double e = expNeg(-kr, T);
   const double ip0 = XN.x * N.x + XN.y * N.y + XN.z * N.z;
   const double ipX = XN.x * rx + XN.y * ry + XN.z * rz;
   const double ipY = N.x * rx + N.y * ry + N.z * rz;
-  const double G0 = ri * kFourPiI, Gk = e * G0, coef = (kappa * rn + 1.0) * e;
-  const double dG0dy = ipY * G0 * ri2, dG0dx = ipX * G0 * ri2;
-  const double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) * ri2;
-  const double y0 = G0 - Gk;
-  const double y3 = (coef * ddG0 - kap2 * e * ipX * dG0dy) - ddG0;
+  // 4 pi times: G0, dG0/dn_y, dG0/dn_x, d2G0/dn_x dn_y
+  const double coef = (kr + 1.0) * e;
+  const double dy = ipY * ri3, dx = ipX * ri3;
+  const double a = ipX * dy;
+  const double ddG = fma(a * ri2, -3.0, ip0 * ri3);
+  const double y0 = fma(-ri, e, ri);
+  const double y3 = fma(coef, ddG, -(kap2 * e) * a) - ddG;
   const double pw = CHK && skip ? 0.0 : P.w, nw = CHK && skip ? 0.0 : N.w;
   if constexpr (TR) {
-    const double y1 = -(eps * (coef * dG0dx) - dG0dx);
-    const double y2 = -(coef * dG0dy * ieps - dG0dy);
-    s0 += y3 * nw + y1 * pw;
-    s1 += y2 * nw + y0 * pw;
+    const double y1 = fma(-eps, coef * dx, dx);
+    const double y2 = fma(-(coef * dy), ieps, dy);
+    s0 = fma(y3, nw, fma(y1, pw, s0));
+    s1 = fma(y2, nw, fma(y0, pw, s1));
   } else {
-    const double y1 = eps * (coef * dG0dy) - dG0dy;
-    const double y2 = coef * dG0dx * ieps - dG0dx;
-    s0 += y0 * nw + y1 * pw;
-    s1 += y2 * nw + y3 * pw;
+    const double y1 = fma(eps, coef * dy, -dy);
+    const double y2 = fma(coef * dx, ieps, -dx);
+    s0 = fma(y0, nw, fma(y1, pw, s0));
+    s1 = fma(y2, nw, fma(y3, pw, s1));
   }
 }
 
@@ -764,7 +771,7 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
   }
   for (int r = 0; r < 4; r++) {
     const int row = w + 4 * r;
-    const double A = PA[xi[r]].w;
+    const double A = PA[xi[r]].w * kFourPiI;
     double t0 = A * a0[r], t1 = A * a1[r];
     if (row < rows)
       for (long long e = specOff[xi[r]] + lane; e < specOff[xi[r] + 1]; e += 64) {
@@ -1032,16 +1039,17 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
           const float ip0 = XN[r].x * N.x + XN[r].y * N.y + XN[r].z * N.z;
           const float ipX = XN[r].x * rx + XN[r].y * ry + XN[r].z * rz;
           const float ipY = N.x * rx + N.y * ry + N.z * rz;
-          const float G0 = ri * k4pi, coef = cm1 + 1.f;
-          const float dG0dy = ipY * G0 * ri2, dG0dx = ipX * G0 * ri2;
-          const float ddG0 = (ip0 * G0 - dG0dy * ipX * 3.f) * ri2;
-          const float y0 = G0 * omE;
-          const float y1 = dG0dy * (eps * coef - 1.f);
-          const float y2 = dG0dx * (coef * ieps - 1.f);
-          const float y3 = cm1 * ddG0 - kap2 * e * ipX * dG0dy;
+          // 4 pi times G0 and its derivatives (the row's total is scaled by 1/(4 pi))
+          const float coef = cm1 + 1.f, ri3 = ri * ri2;
+          const float dy = ipY * ri3, dx = ipX * ri3, a = ipX * dy;
+          const float ddG = fmaf(a * ri2, -3.f, ip0 * ri3);
+          const float y0 = ri * omE;
+          const float y1 = dy * fmaf(eps, coef, -1.f);
+          const float y2 = dx * fmaf(coef, ieps, -1.f);
+          const float y3 = fmaf(cm1, ddG, -(kap2 * e) * a);
           const float pw = skip ? 0.f : P.w, nw = skip ? 0.f : N.w;
-          a0[r] += y0 * nw + y1 * pw;
-          a1[r] += y2 * nw + y3 * pw;
+          a0[r] = fmaf(y0, nw, fmaf(y1, pw, a0[r]));
+          a1[r] = fmaf(y2, nw, fmaf(y3, pw, a1[r]));
         }
       }
     }
@@ -1049,7 +1057,8 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int row = w + 4 * r;
-    float t0 = XP[r].w * a0[r], t1 = XP[r].w * a1[r];
+    const float A = XP[r].w * k4pi;
+    float t0 = A * a0[r], t1 = A * a1[r];
     if (row < rows) {   // the stored corrections of this row
       for (long long e = specOff[xi[r]] + lane; e < specOff[xi[r] + 1]; e += 64) {
         const int y = specCol[e];
