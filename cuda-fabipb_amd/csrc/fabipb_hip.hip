@@ -706,6 +706,28 @@ __device__ __forceinline__ void fly64Pair(const double4 &X, const double4 &XN, c
   }
 }
 
+// Staged columns [j0, j1) against the wave's rows h .. h + FLY64_RB - 1, one
+// column per lane (rows past the tile skipped wave-uniformly).  CHK: the
+// columns may hold the row's own panel.  The own leaf's columns run as a
+// range of their own so that no pass executes both forms under a split
+// lane mask.
+template <bool TR, bool CHK>
+__device__ __forceinline__ void fly64Cols(int h, int j0, int j1, int lane, int w, int rows, const double4 *sP,
+                                          const double4 *sN, const int *sY, const double4 (&X)[FLY64_RB],
+                                          const double4 (&XN)[FLY64_RB], const int (&xi)[4], double kappa,
+                                          double kap2, double eps, double ieps, const double *__restrict__ T,
+                                          double (&a0)[4], double (&a1)[4]) {
+  for (int j = j0 + lane; j < j1; j += 64) {
+    const double4 P = sP[j], N = sN[j];
+    const int y = CHK ? sY[j] : -1;
+#pragma unroll
+    for (int r = 0; r < FLY64_RB; r++)
+      if (w + 4 * (h + r) < rows)
+        fly64Pair<TR, CHK>(X[r], XN[r], P, N, CHK && y == xi[h + r], kappa, kap2, eps, ieps, T, a0[h + r],
+                           a1[h + r]);
+  }
+}
+
 template <bool TR>
 __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ tiles,
                                               const int *__restrict__ leafStart, const int *__restrict__ leafCount,
@@ -751,21 +773,13 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
         X[r] = PA[xi[h + r]];
         XN[r] = PN[xi[h + r]];
       }
-      for (int j = lane; j < cn; j += 64) {
-        const double4 P = sP[j], N = sN[j];
-        if (!FLY64_OWNSPLIT || (unsigned)(c0 + j - oc) < (unsigned)nOwn) {   // D's own columns: skip the panel itself
-          const int y = sY[j];
-#pragma unroll
-          for (int r = 0; r < FLY64_RB; r++)
-            if (w + 4 * (h + r) < rows)
-              fly64Pair<TR, true>(X[r], XN[r], P, N, y == xi[h + r], kappa, kap2, eps, ieps, sE, a0[h + r],
-                                  a1[h + r]);
-        } else {
-#pragma unroll
-          for (int r = 0; r < FLY64_RB; r++)
-            if (w + 4 * (h + r) < rows)
-              fly64Pair<TR, false>(X[r], XN[r], P, N, false, kappa, kap2, eps, ieps, sE, a0[h + r], a1[h + r]);
-        }
+      if (FLY64_OWNSPLIT) {   // D's own columns [lo, hi) of this chunk apart, each range a loop of its own
+        const int lo = min(max(oc - c0, 0), cn), hi = min(max(oc + nOwn - c0, 0), cn);
+        fly64Cols<TR, false>(h, 0, lo, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
+        fly64Cols<TR, true>(h, lo, hi, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
+        fly64Cols<TR, false>(h, hi, cn, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
+      } else {
+        fly64Cols<TR, true>(h, 0, cn, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
       }
     }
   }
