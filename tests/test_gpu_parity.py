@@ -153,6 +153,33 @@ def test_near_modes_agree(fa, meshdir, monkeypatch, name):
     assert rel(y32[None].cpu().numpy(), ys[None].cpu().numpy()) <= 1e-5
 
 
+def test_large_leaves_all_near(fa, meshdir, monkeypatch):
+    """Depth 1: eight leaves of 560-735 panels and no far field, so every pair is near
+    and a leaf's own columns run past the first 512-column chunk the on-the-fly
+    nearfield stages at a time.  The default (every pair on the fly, the own leaf's
+    columns as a range of their own) matches the oracle and the all-streamed path."""
+    import torch
+    import oracle_binding as O
+    mesh = fa.Mesh(str(meshdir / "glob4"))
+    tree = fa.Tree(mesh, 1, order=FL["p"], height=1, sep=FL["S"])
+    assert len(tree.a["m2lSrc"]) == 0 and tree.a["leafCount"].max() > 512
+    op = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    assert op.stats()["near_stored"] == 0
+    ora = O.Oracle(tree.oracle_layout(FL["k"], FL["eps1"], FL["eps2"], FL["q"]))
+    x = splitmix_vec(779, 2 * tree.n)
+    y = op.matvec(x)
+    assert rel(y, ora.matvec(x)) <= 1e-13
+    monkeypatch.setenv("FABIPB_NEAR_STREAM", "1")
+    os_ = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    assert os_.stats()["near_stored"] > 0
+    xd = torch.from_numpy(x).cuda()
+    ya, yb = torch.empty_like(xd), torch.empty_like(xd)
+    op.matvec_dev(xd, ya)
+    os_.matvec_dev(xd, yb)
+    assert rel(ya.cpu().numpy(), yb.cpu().numpy()) <= 1e-14
+    assert rel(ya.cpu().numpy(), y) <= 1e-14
+
+
 @pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
 def test_m2l_paths_agree(fa, meshdir, monkeypatch, name):
     """The M2L on the fixed term grid (registers per run, host-placed LDS slots) and the
