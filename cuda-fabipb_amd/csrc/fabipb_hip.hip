@@ -773,8 +773,11 @@ __device__ __forceinline__ void nearFly64Tile(int t, const int2 *__restrict__ ti
         X[r] = PA[xi[h + r]];
         XN[r] = PN[xi[h + r]];
       }
-      if (FLY64_OWNSPLIT) {   // D's own columns [lo, hi) of this chunk apart, each range a loop of its own
-        const int lo = min(max(oc - c0, 0), cn), hi = min(max(oc + nOwn - c0, 0), cn);
+      if (FLY64_OWNSPLIT) {   // D's own columns [lo, hi) of this chunk apart, each range a loop of its own;
+        // the checked range is extended to whole passes of 64 columns (the check
+        // is exact for any column), so it leaves no partly idle pass behind
+        const int lo = min(max(oc - c0, 0), cn);
+        const int hi = min(lo + ((min(max(oc + nOwn - c0, 0), cn) - lo + 63) & ~63), cn);
         fly64Cols<TR, false>(h, 0, lo, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
         fly64Cols<TR, true>(h, lo, hi, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
         fly64Cols<TR, false>(h, hi, cn, lane, w, rows, sP, sN, sY, X, XN, xi, kappa, kap2, eps, ieps, sE, a0, a1);
