@@ -997,6 +997,45 @@ __device__ __forceinline__ void flyExp(float t, float e, float &omE, float &cm1)
 
 #define FLY_CHUNK 512
 #define FLY_RB 4   // rows per pass over the staged sources
+// Staged columns [j0, j1) against the wave's rows h .. h + FLY_RB - 1, one
+// column per lane (FP32).  CHK: the columns may hold the row's own panel.
+template <bool CHK>
+__device__ __forceinline__ void fly32Cols(int h, int j0, int j1, int lane, int w, int rows, const float4 *sP,
+                                          const float4 *sN, const float4 *sL, const float4 (&XP)[4],
+                                          const float4 (&XN)[4], const float4 (&XL)[4], const int (&xi)[4],
+                                          float kappa, float kap2, float eps, float ieps, float (&a0)[4],
+                                          float (&a1)[4]) {
+  for (int j = j0 + lane; j < j1; j += 64) {
+    const float4 Lo = sL[j], P = sP[j], N = sN[j];
+    const int y = __float_as_int(Lo.w);
+#pragma unroll
+    for (int r = h; r < h + FLY_RB; r++) {
+      if (w + 4 * r >= rows) continue;
+      const bool skip = CHK && y == xi[r];   // the panel itself: weights zeroed, r2 replaced by 1
+      const float rx = (XP[r].x - P.x) + (XL[r].x - Lo.x), ry = (XP[r].y - P.y) + (XL[r].y - Lo.y),
+                  rz = (XP[r].z - P.z) + (XL[r].z - Lo.z);
+      const float r2 = skip ? 1.f : rx * rx + ry * ry + rz * rz, ri = rsqrtf(r2), ri2 = ri * ri, rr = r2 * ri;
+      const float t = kappa * rr, e = expf(-t);
+      float omE, cm1;
+      flyExp(t, e, omE, cm1);
+      const float ip0 = XN[r].x * N.x + XN[r].y * N.y + XN[r].z * N.z;
+      const float ipX = XN[r].x * rx + XN[r].y * ry + XN[r].z * rz;
+      const float ipY = N.x * rx + N.y * ry + N.z * rz;
+      // 4 pi times G0 and its derivatives (the row's total is scaled by 1/(4 pi))
+      const float coef = cm1 + 1.f, ri3 = ri * ri2;
+      const float dy = ipY * ri3, dx = ipX * ri3, a = ipX * dy;
+      const float ddG = fmaf(a * ri2, -3.f, ip0 * ri3);
+      const float y0 = ri * omE;
+      const float y1 = dy * fmaf(eps, coef, -1.f);
+      const float y2 = dx * fmaf(coef, ieps, -1.f);
+      const float y3 = fmaf(cm1, ddG, -(kap2 * e) * a);
+      const float pw = skip ? 0.f : P.w, nw = skip ? 0.f : N.w;
+      a0[r] = fmaf(y0, nw, fmaf(y1, pw, a0[r]));
+      a1[r] = fmaf(y2, nw, fmaf(y3, pw, a1[r]));
+    }
+  }
+}
+
 // LDS: (p hi, A sigma), (n, A sigma_n), (p lo, source panel index), FLY_CHUNK each
 __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ tiles,
                                               const int *__restrict__ leafStart, const int *__restrict__ leafCount,
@@ -1006,9 +1045,10 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
                                               const double *__restrict__ x, const long long *__restrict__ specOff,
                                               const int *__restrict__ specCol, const float4 *__restrict__ specVal,
                                               float *__restrict__ out, int n, float kappa, float eps, float4 *sP,
-                                              float4 *sN, float4 *sL) {
+                                              float4 *sN, float4 *sL, const int *__restrict__ ownOff) {
   const int2 tl = tiles[t];
   const int D = tl.x, r0 = tl.y;
+  const int oc = ownOff[D], nOwn = oc >= 0 ? leafCount[D] : 0;   // D's own columns: [oc, oc + nOwn)
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
   const int ncol = ncolArr[D];
   const int *src = srcList + srcOff[D];
@@ -1040,35 +1080,12 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
 #pragma unroll
     for (int h = 0; h < 4; h += FLY_RB) {
       if (w + 4 * h >= rows) break;
-      for (int j = lane; j < cn; j += 64) {
-        const float4 Lo = sL[j], P = sP[j], N = sN[j];
-        const int y = __float_as_int(Lo.w);
-#pragma unroll
-        for (int r = h; r < h + FLY_RB; r++) {
-          if (w + 4 * r >= rows) continue;
-          const bool skip = y == xi[r];   // the panel itself: weights zeroed, r2 replaced by 1
-          const float rx = (XP[r].x - P.x) + (XL[r].x - Lo.x), ry = (XP[r].y - P.y) + (XL[r].y - Lo.y),
-                      rz = (XP[r].z - P.z) + (XL[r].z - Lo.z);
-          const float r2 = skip ? 1.f : rx * rx + ry * ry + rz * rz, ri = rsqrtf(r2), ri2 = ri * ri, rr = r2 * ri;
-          const float t = kappa * rr, e = expf(-t);
-          float omE, cm1;
-          flyExp(t, e, omE, cm1);
-          const float ip0 = XN[r].x * N.x + XN[r].y * N.y + XN[r].z * N.z;
-          const float ipX = XN[r].x * rx + XN[r].y * ry + XN[r].z * rz;
-          const float ipY = N.x * rx + N.y * ry + N.z * rz;
-          // 4 pi times G0 and its derivatives (the row's total is scaled by 1/(4 pi))
-          const float coef = cm1 + 1.f, ri3 = ri * ri2;
-          const float dy = ipY * ri3, dx = ipX * ri3, a = ipX * dy;
-          const float ddG = fmaf(a * ri2, -3.f, ip0 * ri3);
-          const float y0 = ri * omE;
-          const float y1 = dy * fmaf(eps, coef, -1.f);
-          const float y2 = dx * fmaf(coef, ieps, -1.f);
-          const float y3 = fmaf(cm1, ddG, -(kap2 * e) * a);
-          const float pw = skip ? 0.f : P.w, nw = skip ? 0.f : N.w;
-          a0[r] = fmaf(y0, nw, fmaf(y1, pw, a0[r]));
-          a1[r] = fmaf(y2, nw, fmaf(y3, pw, a1[r]));
-        }
-      }
+      // D's own columns, padded to whole passes of 64, are checked for the panel itself
+      const int lo = min(max(oc - c0, 0), cn);
+      const int hi = min(lo + ((min(max(oc + nOwn - c0, 0), cn) - lo + 63) & ~63), cn);
+      fly32Cols<false>(h, 0, lo, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, kap2, eps, ieps, a0, a1);
+      fly32Cols<true>(h, lo, hi, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, kap2, eps, ieps, a0, a1);
+      fly32Cols<false>(h, hi, cn, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, kap2, eps, ieps, a0, a1);
     }
   }
 #pragma unroll
@@ -1107,7 +1124,8 @@ __global__ __launch_bounds__(256) void k_near_mix32(const int2 *__restrict__ til
                                                     const long long *__restrict__ specOff,
                                                     const int *__restrict__ specCol,
                                                     const float4 *__restrict__ specVal, float *__restrict__ out, int n,
-                                                    float kappa, float eps, const int *__restrict__ order) {
+                                                    float kappa, float eps, const int *__restrict__ order,
+                                                    const int *__restrict__ ownOff) {
   __shared__ __align__(16) unsigned char lds[MIX32_LDS];
   const int t = order[blockIdx.x];   // costliest tiles first
   if (tileCoef[t] >= 0) {
@@ -1117,7 +1135,7 @@ __global__ __launch_bounds__(256) void k_near_mix32(const int2 *__restrict__ til
   } else {
     float4 *sP = reinterpret_cast<float4 *>(lds);
     nearFly32Tile(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, PL, x, specOff, specCol, specVal,
-                  out, n, kappa, eps, sP, sP + FLY_CHUNK, sP + 2 * FLY_CHUNK);
+                  out, n, kappa, eps, sP, sP + FLY_CHUNK, sP + 2 * FLY_CHUNK, ownOff);
   }
 }
 
@@ -3911,7 +3929,8 @@ static void nearPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_
     else
       hipLaunchKernelGGL(k_near_mix32, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
                          c->srcOff, c->ncol, c->tileCoef, c->srcList, B.coef, c->flyPA, c->flyPN, c->flyPL, x,
-                         c->specOff, c->specCol, c->specVal, B.nearv, c->n, (float)c->kappa, (float)c->eps, c->tileOrder);
+                         c->specOff, c->specCol, c->specVal, B.nearv, c->n, (float)c->kappa, (float)c->eps, c->tileOrder,
+                         c->ownOff);
     return;
   }
   hipLaunchKernelGGL(k_near_apply<R>, dim3(c->nTiles), dim3(256), 0, st, c->tiles, c->leafStart, c->leafCount,
