@@ -664,6 +664,14 @@ __device__ __forceinline__ double expNeg(double x, const double *__restrict__ T)
   return ldexp(T[k & 255] * q, k >> 8);
 }
 
+// 2^(j/256), j = 0..255, the table of expNeg: filled once per device at set-up
+// (k_exp2_tab) and copied into LDS by every nearfield block
+static __device__ double gExp2Tab[256];
+__global__ void k_exp2_tab() {
+  const int j = threadIdx.x;
+  if (j < 256) gExp2Tab[j] = exp2((double)j / 256.0);
+}
+
 // One pair of the on-the-fly FP64 nearfield (kernelKER4 at the one-point rule,
 // src/kernel.c:109-147): target point/normal X, XN against staged source
 // P = (point, A sigma), N = (normal, A sigma_n).  skip: the panel itself (its
@@ -827,7 +835,7 @@ __global__ __launch_bounds__(256) void k_near_mix64(const int2 *__restrict__ til
                                                     const int *__restrict__ order, const int *__restrict__ ownOff) {
   __shared__ __align__(16) unsigned char lds[MIX64_LDS];
   __shared__ double sE[256];   // 2^(j/256) for expNeg
-  for (int j = threadIdx.x; j < 256; j += blockDim.x) sE[j] = exp2((double)j / 256.0);
+  for (int j = threadIdx.x; j < 256; j += blockDim.x) sE[j] = gExp2Tab[j];
   // tiles in `order`, costliest first (the default grid has one block per tile;
   // a smaller grid strides over them)
   for (int k = blockIdx.x; k < nTiles; k += gridDim.x) {
@@ -4101,6 +4109,7 @@ static int enableFly64(fbh_ctx *c) {
     return 0;
   hipLaunchKernelGGL(k_fly64_panels, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->pnl, (int)n, c->Q.t[0],
                      c->fly64PA, c->fly64PN);
+  hipLaunchKernelGGL(k_exp2_tab, dim3(1), dim3(256), 0, c->stream);
   hipLaunchKernelGGL(k_spec_vals<false>, dim3(n), dim3(64), 0, c->stream, c->pnl, c->specOff, c->specCol,
                      (float4 *)nullptr, c->specVal64, c->kappa, c->eps, c->Q, c->qorder);
   HCK(hipGetLastError());
