@@ -145,6 +145,13 @@ def hip_lib():
         L.fbh_stats.argtypes = [_vp, C.POINTER(C.c_longlong)]
         L.fbh_stream.restype = _vp
         L.fbh_stream.argtypes = [_vp]
+        L.fbh_set_rhs_factor.restype = C.c_int
+        L.fbh_set_rhs_factor.argtypes = [_vp, C.c_double]
+        L.fbh_pair_coeffs.restype = C.c_int
+        L.fbh_pair_coeffs.argtypes = [_vp, C.c_longlong, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.fbh_panel_pair_coeffs.restype = C.c_int
+        L.fbh_panel_pair_coeffs.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_longlong,
+                                            _ip, _ip, _dp, _dp, _dp, _dp]
         _hip = L
     return _hip
 
@@ -330,6 +337,18 @@ class Operator:
         self._ck(hip_lib().fbh_near_coeff(self.h, int(x), int(y), _d(k)))
         return k
 
+    def set_rhs_factor(self, fac):
+        """Scale of every right-hand side (the fac of gpuSetupRHS); 0 restores 1/(4 pi eps1)."""
+        self._ck(hip_lib().fbh_set_rhs_factor(self.h, float(fac)))
+
+    def pair_coeffs(self, x, y):
+        """KER[4] = panelIA0(X = panel x[p], Y = panel y[p]) of listed pairs: array (nPairs, 4)."""
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        k = np.zeros((4, len(x)))
+        self._ck(hip_lib().fbh_pair_coeffs(self.h, len(x), _i(x), _i(y), _d(k[0]), _d(k[1]), _d(k[2]), _d(k[3])))
+        return k.T.copy()
+
     def rhs(self, pos, chr_):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
@@ -387,11 +406,11 @@ class Operator:
 
     def gmres_t_dev(self, b, x, restart=30, tol=1e-6, maxit=100):
         it, res, info = C.c_int(0), C.c_double(0.0), C.c_int(0)
-        hist = np.zeros(maxit + 2)
+        hist = np.zeros(2 * maxit + 2)   # one entry per iteration and one per restart cycle
         self._ck(hip_lib().fbh_gmres_t_dev(self.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
                                            restart, tol, maxit, C.byref(it), C.byref(res), _d(hist),
                                            C.byref(info)))
-        return dict(iters=it.value, resid=res.value, info=info.value)
+        return dict(iters=it.value, resid=res.value, info=info.value, hist=hist)
 
     def energy_direct_dev(self, pos, chr_, sgm, g=None):
         """Raw direct energy g . sgm (times 2 pi 332.0716 for kcal/mol); g (2n tensor) optional out."""

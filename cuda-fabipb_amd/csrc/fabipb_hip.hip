@@ -421,6 +421,25 @@ __global__ void k_near_pair(const Pnl *__restrict__ pnl, int x, int y, double ka
   for (int i = 0; i < 4; i++) out[i] = k[i];
 }
 
+// Galerkin coefficients of listed panel pairs, one thread per pair:
+// (k0..k3)[p] = panelIA0(X = pnl[xi[p]], Y = pnl[yi[p]]) with the rule its
+// geometric case selects (the assembly hooks of src/precond_fmm.c:223, 496)
+__global__ __launch_bounds__(256) void k_pair_coeffs(const Pnl *__restrict__ pnl, long long nPairs,
+                                                     const int *__restrict__ xi, const int *__restrict__ yi,
+                                                     double kappa, double eps, QTab Q, int q, double *__restrict__ k0,
+                                                     double *__restrict__ k1, double *__restrict__ k2,
+                                                     double *__restrict__ k3) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= nPairs) return;
+  const int x = xi[p], y = yi[p];
+  double k[4];
+  panelIA0(pnl[x], pnl[y], x == y, kappa, eps, Q, q, k);
+  k0[p] = k[0];
+  k1[p] = k[1];
+  k2[p] = k[2];
+  k3[p] = k[3];
+}
+
 // ------------------------------------------------- block preconditioner
 // Leaf-parent blocks (src/precond_fmm.c:147-268, 553-652): block b covers the
 // m panels [start, start+m) of one cube at level depth-1, unknowns
@@ -3069,6 +3088,7 @@ struct fbh_ctx {
   hipStream_t stream = nullptr;
   bool ownStream = false;
   double kappa, eps1, eps2, eps;
+  double rhsFac = 0.0;   // right-hand side factor; 0: 1/(4 pi eps1), src/fabipb.c:755
   int qorder;
   QTab Q;
   int n, depth, height, maxOrder, P, nmMax, nmLeaf;
@@ -4321,6 +4341,89 @@ extern "C" int fbh_near_coeff(fbh_ctx *c, int x, int y, double *ker4) {
   return 0;
 }
 
+// panelIA0 of listed pairs of the panels pnl[0..nPanels) (device), host index
+// and result arrays, in chunks; the device buffers live for the call only
+static int pairCoeffs(const Pnl *pnl, int nPanels, long long nPairs, const int *xi, const int *yi, double kappa,
+                      double eps, const QTab &Q, int q, double *k0, double *k1, double *k2, double *k3,
+                      hipStream_t st) {
+  for (long long p = 0; p < nPairs; p++)
+    if (xi[p] < 0 || xi[p] >= nPanels || yi[p] < 0 || yi[p] >= nPanels) {
+      setErr("pair %lld: panel index outside 0..%d", p, nPanels - 1);
+      return 0;
+    }
+  const long long chunk = std::min<long long>(nPairs, 1 << 22);
+  int *dx = nullptr, *dy = nullptr;
+  double *dk = nullptr;
+  bool ok = hipMalloc(&dx, chunk * sizeof(int)) == hipSuccess && hipMalloc(&dy, chunk * sizeof(int)) == hipSuccess &&
+            hipMalloc(&dk, 4 * chunk * sizeof(double)) == hipSuccess;
+  double *out[4] = {k0, k1, k2, k3};
+  for (long long p0 = 0; ok && p0 < nPairs; p0 += chunk) {
+    const long long m = std::min(chunk, nPairs - p0);
+    ok = hipMemcpyAsync(dx, xi + p0, m * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(dy, yi + p0, m * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess;
+    if (!ok) break;
+    hipLaunchKernelGGL(k_pair_coeffs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, pnl, m, dx, dy, kappa, eps,
+                       Q, q, dk, dk + chunk, dk + 2 * chunk, dk + 3 * chunk);
+    ok = hipGetLastError() == hipSuccess;
+    for (int i = 0; ok && i < 4; i++)
+      ok = hipMemcpyAsync(out[i] + p0, dk + i * chunk, m * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
+  }
+  if (!ok) setErr("pair coefficients: %s", hipGetErrorString(hipGetLastError()));
+  hipStreamSynchronize(st);
+  if (dx) hipFree(dx);
+  if (dy) hipFree(dy);
+  if (dk) hipFree(dk);
+  return ok ? 1 : 0;
+}
+
+extern "C" int fbh_pair_coeffs(fbh_ctx *c, long long nPairs, const int *xIdx, const int *yIdx, double *k0,
+                               double *k1, double *k2, double *k3) {
+  if (!c || nPairs < 0 || (nPairs > 0 && (!xIdx || !yIdx || !k0 || !k1 || !k2 || !k3))) {
+    setErr("fbh_pair_coeffs: bad arguments");
+    return 0;
+  }
+  if (nPairs == 0) return 1;
+  HCK(hipStreamSynchronize(c->stream));
+  return pairCoeffs(c->pnl, c->n, nPairs, xIdx, yIdx, c->kappa, c->eps, c->Q, c->qorder, k0, k1, k2, k3, c->stream);
+}
+
+extern "C" int fbh_panel_pair_coeffs(int nPanels, const double *vtx, const double *nrm, const double *area,
+                                     double kappa, double eps, int qorder, long long nPairs, const int *xLoc,
+                                     const int *yLoc, double *k0, double *k1, double *k2, double *k3) {
+  if (nPanels <= 0 || !vtx || !nrm || !area || qorder < 1 || qorder > 10 || !(eps > 0) || nPairs < 0 ||
+      (nPairs > 0 && (!xLoc || !yLoc || !k0 || !k1 || !k2 || !k3))) {
+    setErr("fbh_panel_pair_coeffs: bad arguments");
+    return 0;
+  }
+  if (nPairs == 0) return 1;
+  if (!fbh_available()) return 0;
+  std::vector<Pnl> hp(nPanels);
+  for (int i = 0; i < nPanels; i++) {
+    const double *v = &vtx[9 * i];
+    for (int k = 0; k < 9; k++) hp[i].v[k] = v[k];
+    for (int j = 0; j < 3; j++) {   // src/input.c:743-747
+      hp[i].a[j] = v[6 + j] - v[3 + j];
+      hp[i].a[3 + j] = v[j] - v[6 + j];
+      hp[i].a[6 + j] = v[3 + j] - v[j];
+    }
+    for (int k = 0; k < 3; k++) hp[i].nr[k] = nrm[3 * i + k];
+    hp[i].area = area[i];
+  }
+  QTab Q;
+  for (int i = 0; i < 10; i++) { Q.t[i] = hT[qorder][i]; Q.w[i] = hW[qorder][i]; }
+  Pnl *dp = nullptr;
+  if (hipMalloc(&dp, nPanels * sizeof(Pnl)) != hipSuccess ||
+      hipMemcpy(dp, hp.data(), nPanels * sizeof(Pnl), hipMemcpyHostToDevice) != hipSuccess) {
+    if (dp) hipFree(dp);
+    setErr("fbh_panel_pair_coeffs: uploading %d panels failed", nPanels);
+    return 0;
+  }
+  const int ok = pairCoeffs(dp, nPanels, nPairs, xLoc, yLoc, kappa, eps, Q, qorder, k0, k1, k2, k3, nullptr);
+  hipFree(dp);
+  return ok;
+}
+
 extern "C" int fbh_psolve_diag_dev(fbh_ctx *c, double *out, const double *in) {
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_psolve, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, in, out, c->area, c->n,
@@ -4642,11 +4745,14 @@ __global__ void k_axpy(double *__restrict__ y, const double *__restrict__ x, int
 
 // Precision form: iterative refinement around GMRES on the FP32 operator.
 // Each correction solves A32 d = r with d0 = 0 to a relative residual of
-// half the outer tolerance measured against ||b||; x += d; the residual that
-// decides convergence is r = b - A64 x in FP64 with the same test as the plain
-// solve, ||M^-1 r|| / ||b|| <= tol.  The first residual (x = x0) is taken with
-// the FP32 operator.  iters counts inner iterations (FP32 products), outer the
-// corrections, mv64 the FP64 products.
+// 0.2 tol measured against ||b||; x += d; convergence is decided on
+// r = b - A64 x in FP64 with the same test as the plain solve,
+// ||M^-1 r|| / ||b|| <= tol.  The first residual (x = x0) is taken with the
+// FP32 operator.  After a small correction (an FP64 residual already at hand
+// and within 100 tol ||b||) the residual is updated as r - A32 d, and a pass
+// of the test on that update is confirmed with b - A64 x before returning.
+// iters counts inner iterations (FP32 products), outer the corrections, mv64
+// the FP64 products.
 extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit,
                                      int maxOuter, int *iters, double *resid, double *hist, int *info, int *outer,
                                      int *mv64) {
@@ -4686,12 +4792,13 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
     *iters += it;
     ++*outer;
     hipLaunchKernelGGL(k_axpy, dim3(g), dim3(256), 0, c->stream, x, c->dx, n2);
+    bool updated = false;
     if (*mv64 > 0 && rr <= 100.0 * tol * braw) {
-      // ro held b - A x in FP64 before this small correction: the update
-      // ro -= A dx needs the operator only to the accuracy of dx itself, and
-      // the FP32 operator's error on it, eps32 * |A dx| ~ eps32 * |ro|, stays
-      // four orders below tol * |b|
+      // ro held b - A x in FP64 before this small correction: ro -= A32 dx,
+      // whose error (the FP32 operator's relative error times |A dx| ~ |ro|)
+      // is small against tol |b|; a pass is confirmed in FP64 below
       if (!applyOp(c, OP32, -1.0, c->dx, 1.0, c->ro)) return 0;
+      updated = true;
     } else {
       HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
       if (!applyOp(c, OP64, -1.0, x, 1.0, c->ro)) return 0;
@@ -4701,12 +4808,30 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
     double rn;
     if (!devNorm(c, c->w, &rn)) return 0;
     *resid = rn / bnrm2;
+    if (rn != rn) { if (hist) hist[nh++] = *resid; *info = 2; return 1; }
+    if (updated && *resid <= tol) {   // the FP64 residual decides
+      HCK(hipMemcpyAsync(c->ro, b, vb, hipMemcpyDeviceToDevice, c->stream));
+      if (!applyOp(c, OP64, -1.0, x, 1.0, c->ro)) return 0;
+      ++*mv64;
+      if (!fbh_psolve_dev(c, c->w, c->ro)) return 0;
+      if (!devNorm(c, c->w, &rn)) return 0;
+      *resid = rn / bnrm2;
+    }
     if (hist) hist[nh++] = *resid;
     if (rn != rn) { *info = 2; return 1; }
     if (*resid <= tol) return 1;
     if (*iters >= maxit || *outer >= maxOuter) { *info = 1; return 1; }
   }
   *resid = 0.0;
+  return 1;
+}
+
+// the factor every right-hand side (direct, tree, and the gradient's R^T) is scaled by
+static double rhsFactor(const fbh_ctx *c) { return c->rhsFac != 0.0 ? c->rhsFac : kFourPiI / c->eps1; }
+
+extern "C" int fbh_set_rhs_factor(fbh_ctx *c, double fac) {
+  if (!c || !(fac == fac)) { setErr("fbh_set_rhs_factor: bad arguments"); return 0; }
+  c->rhsFac = fac;
   return 1;
 }
 
@@ -4717,7 +4842,7 @@ extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const doubl
   HCK(hipMalloc(&dc, (size_t)nChar * sizeof(double)));
   HCK(hipMemcpy(dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
   HCK(hipMemcpy(dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
-  double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
+  const double fac = rhsFactor(c);
   if (c->qorder == 1)
     hipLaunchKernelGGL(k_rhs_q1, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc,
                        b_dev, fac, c->Q.t[0], c->Q.t[0] * c->Q.w[0] * c->Q.w[0]);
@@ -4817,7 +4942,7 @@ extern "C" int fbh_gradient_dev(fbh_ctx *c, int nChar, const double *pos, const 
                                                       nullptr, info);
   }
   if (ok) {   // dE/dq
-    const double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
+    const double fac = rhsFactor(c);
     hipLaunchKernelGGL(k_grad_q, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar, dp,
                        sgm_dev, lam_dev, part, c->kappa, c->eps, fac, c->Q, c->qorder);
     hipLaunchKernelGGL(k_grad_sum, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, part, nChunk, nChar, dq);
@@ -4961,7 +5086,7 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   if (ok) ok = uploadCharges(nChar, posP.data(), chrP.data(), &dp, &dc);
   if (ok) {   // moments of u and of (lam_n, lam), tables, walk
     OpBufs<double> B = bufs64(c);
-    const double fac = kFourPiI / c->eps1;   // src/fabipb.c:755
+    const double fac = rhsFactor(c);
     hipLaunchKernelGGL(k_pt_moments, dim3(c->nLeaf), dim3(256), 0, c->stream, c->pnl, c->leafStart, c->leafCount,
                        c->cubeX, sgm_dev, n, NMOM(P), c->nmMax, c->T, c->Q, c->qorder, 0, fac, c->mom);
     for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
@@ -5133,7 +5258,7 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
   case PP:                                                                                                        \
     hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
                        dLevel, dX, dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta,  \
-                       c->kappa, MM == 1 ? kFourPiI / c->eps1 : c->eps, c->Q, c->qorder, lvCount[c->height], S,  \
+                       c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder, lvCount[c->height], S,  \
                        (MM && S == 1) ? b_dev : dPart);                                                           \
     break;
     if (mode == 0) {

@@ -10,9 +10,16 @@
  * or none), then the panel-tree solvation energy.  Every numeric stage runs on
  * the GPU through include/fabipb_hip.h; the CPU path of the reference (-g=0)
  * is not part of this build, nor is meshing (MSMS/NanoShaper).  -r=1/2 (the
- * dense direct operator) both run it on the device.  Refused options are
- * parsed and validated exactly as the reference does, then refused with a
- * message, so scripts fail loudly rather than run something else.
+ * dense direct operator) both run it on the device.  Every option is parsed
+ * and validated exactly as the reference does; then
+ *   -g=0 (the CPU path), -c=1 and -C=1 (the reference's CPU-against-GPU
+ *   applyFMM and preconditioner debug compares, src/fabipb.c:103-201, which need
+ *   its CPU path) are refused with a message, so scripts fail loudly;
+ *   -R and -d (meshing parameters), -G (the reference CUDA backend's nearfield
+ *   storage strategy) and -Q (whether its Q2M/L2P run on the GPU) have no
+ *   counterpart here (the mesh is read from existing files, there is one
+ *   nearfield path and every stage runs on the device): they are accepted and
+ *   a note on stderr says that they are ignored.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -101,7 +108,9 @@ int main(int argc, char **argv) {
   int order = -1, orderMom = 0, numItr = 100, arnoldiSz = 30, depth = 5, height = 2, qOrd = 1;
   int gpuMode = -1, bench = 0, meshFlag = 1, meshOnly = 0, matvecMode = 0, precond = 3, precondExplicit = 0;
   double sep = 0.8, tol = 1.0e-4, kappa = 0.1257, eps1 = 1.0, eps2 = 80.0, para = 332.0716;
-  int sepExplicit = 0;
+  int sepExplicit = 0, dbgApply = 0, dbgPrecond = 0;
+  const char *ignored[8];
+  int nIgnored = 0;
   for (int i = 1; i < argc; i++) {
     const char *a = argv[i], *v;
     if (a[0] != '-') {
@@ -119,7 +128,7 @@ int main(int argc, char **argv) {
     else if ((v = optValue(a, "q"))) qOrd = optInt(a, v, 1, 10);
     else if ((v = optValue(a, "t"))) depth = optInt(a, v, -1, 64);
     else if ((v = optValue(a, "H"))) height = optInt(a, v, 0, 64);
-    else if ((v = optValue(a, "R"))) (void)optDouble(a, v, 1e-12, 1e6);
+    else if ((v = optValue(a, "R"))) { (void)optDouble(a, v, 1e-12, 1e6); if (nIgnored < 8) ignored[nIgnored++] = a; }
     else if ((v = optValue(a, "eps1")) || (v = optValue(a, "e1"))) eps1 = optDouble(a, v, 1e-12, 1e6);
     else if ((v = optValue(a, "eps2")) || (v = optValue(a, "e2"))) eps2 = optDouble(a, v, 1e-12, 1e6);
     else if ((v = optValue(a, "k"))) kappa = optDouble(a, v, 0.0, 1e6);
@@ -127,13 +136,13 @@ int main(int argc, char **argv) {
     else if ((v = optValue(a, "M"))) meshOnly = optInt(a, v, 0, 1);
     else if ((v = optValue(a, "B"))) bench = optInt(a, v, 0, 1);
     else if ((v = optValue(a, "g"))) gpuMode = optInt(a, v, -1, 1);
-    else if ((v = optValue(a, "c"))) (void)optInt(a, v, 0, 1);
-    else if ((v = optValue(a, "C"))) (void)optInt(a, v, 0, 1);
+    else if ((v = optValue(a, "c"))) dbgApply = optInt(a, v, 0, 1);
+    else if ((v = optValue(a, "C"))) dbgPrecond = optInt(a, v, 0, 1);
     else if ((v = optValue(a, "r"))) matvecMode = optInt(a, v, 0, 2);
-    else if ((v = optValue(a, "Q"))) (void)optInt(a, v, 0, 1);
-    else if ((v = optValue(a, "G"))) (void)optInt(a, v, 0, 2);
+    else if ((v = optValue(a, "Q"))) { (void)optInt(a, v, 0, 1); if (nIgnored < 8) ignored[nIgnored++] = a; }
+    else if ((v = optValue(a, "G"))) { (void)optInt(a, v, 0, 2); if (nIgnored < 8) ignored[nIgnored++] = a; }
     else if ((v = optValue(a, "P"))) { precond = optInt(a, v, -1, 3); precondExplicit = 1; }
-    else if ((v = optValue(a, "d"))) (void)optDouble(a, v, 1e-12, 1e9);
+    else if ((v = optValue(a, "d"))) { (void)optDouble(a, v, 1e-12, 1e9); if (nIgnored < 8) ignored[nIgnored++] = a; }
     else {
       fprintf(stderr, "Error: unknown option '%s'\nRun '%s -h' for the list of supported options.\n", a, argv[0]);
       exit(1);
@@ -143,6 +152,18 @@ int main(int argc, char **argv) {
   if (!panelfile[0]) { fprintf(stderr, "Error: no input given\n"); usage(argv[0]); return 1; }
   if (depth < 1 || height < 0 || height > depth) { printf("Bad FMM level range: height=%d depth=%d\n", height, depth); return 0; }
   if (gpuMode == 0) { fprintf(stderr, "Error: -g=0 (the CPU path) is not part of this build; it runs on the GPU only\n"); return 1; }
+  if (dbgApply || dbgPrecond) {
+    fprintf(stderr, "Error: %s compares the reference's CPU %s with its GPU one; this build has no CPU path "
+            "(its parity checks against the reference are the tests)\n", dbgApply ? "-c=1" : "-C=1",
+            dbgApply ? "applyFMM" : "block preconditioner");
+    return 1;
+  }
+  for (int k = 0; k < nIgnored; k++)
+    fprintf(stderr, "Note: %s ignored: %s\n", ignored[k],
+            ignored[k][1] == 'R' || ignored[k][1] == 'd'
+                ? "a meshing parameter; the mesh is read from the existing .vert/.face files"
+                : ignored[k][1] == 'G' ? "the reference CUDA backend's nearfield strategy; this build has one nearfield path"
+                                       : "Q2M and L2P always run on the device here");
   if (!fbh_available()) die("GPU backend unavailable");
 
   /* inputs: <base>.pqr/.vert/.face; a path without '/' is looked up in test_proteins/ (src/input.c:24-36) */

@@ -15,9 +15,8 @@
  * the reference caches per sys pointer (src/gpu_backend_cuda.cu:342-351).
  * Not re-entrant, like the reference's backend.
  *
- * Entry points without a device implementation here (dense direct operator,
- * rhs-reuse energy, charge-tree RHS, preconditioner builders) return 0, which
- * the reference host treats as "use the CPU path".
+ * The two rhs-reuse entry points (the rhs-reuse energy mode, out of scope
+ * here) return 0, which the reference host treats as "use the CPU path".
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -40,6 +39,17 @@ static double *gMom, *gLec;
 static char gMsg[600];
 
 static int nmom(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+
+/* FABIPB_HIP_TRACE=1: one line on stderr per completed hook call that has a
+ * CPU fallback in the reference, so a run can show which work the device did */
+static void trace(const char *hook, long long count) {
+  static int on = -1;
+  if (on < 0) {
+    const char *e = getenv("FABIPB_HIP_TRACE");
+    on = e && *e && *e != '0';
+  }
+  if (on) fprintf(stderr, "gpu_backend_hip: %s done (%lld)\n", hook, count);
+}
 
 static int fail(const char *what) {
   snprintf(gMsg, sizeof(gMsg), "%s: %s", what, fbh_last_error());
@@ -163,9 +173,10 @@ int gpuL2PApply(ssystem *sys, double alpha, double beta, double *pot) {
 }
 
 int gpuSetupRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
-  (void)fac;   /* fbh_rhs applies 1/(4 pi eps1), the fac of src/fabipb.c:755 */
-  if (qOrder != sys->maxQuadOrder || !ensure(sys)) return 0;
-  return fbh_rhs(gCtx, sys->nChar, sys->pos, sys->chr, sgm) ? 1 : fail("setupRHS");
+  if (qOrder != sys->maxQuadOrder || !ensure(sys) || !fbh_set_rhs_factor(gCtx, fac)) return 0;
+  if (!fbh_rhs(gCtx, sys->nChar, sys->pos, sys->chr, sgm)) return fail("setupRHS");
+  trace("gpuSetupRHS", sys->nPnls);
+  return 1;
 }
 
 /* the reference's charge tree (sys->chgCubeList, src/chargeTree.c:228-307) as flat arrays */
@@ -239,21 +250,22 @@ int gpuPanelChargeTreeEnergy(ssystem *sys, const double *sgm, double *pot) {
   return rc;
 }
 
-/* sgm = the charge-tree right-hand side (setupRHSTree, src/fabipb.c:733-742); fac
- * is 1/(4 pi eps1) there as in fbh_rhs_tree_dev, qOrder the operator's own */
+/* sgm = the charge-tree right-hand side (setupRHSTree, src/fabipb.c:733-742) scaled
+ * by fac; qOrder the operator's own */
 int gpuChargeTreeRHS(ssystem *sys, int qOrder, double fac, double *sgm) {
-  (void)fac;
-  if (!sys->chgCubeList || !ensure(sys) || qOrder != sys->maxQuadOrder) return 0;
+  if (!sys->chgCubeList || !ensure(sys) || qOrder != sys->maxQuadOrder || !fbh_set_rhs_factor(gCtx, fac)) return 0;
   flatCT t;
   if (!flattenCT(sys, &t)) return 0;
   int rc = 0, n2 = 2 * sys->nPnls;
   double *d = fbh_dev_alloc(gCtx, n2);
   if (d && fbh_rhs_tree_dev(gCtx, sys->nChar, sys->pos, sys->chr, t.nc, t.level, t.x, t.eRad, t.kidStart,
                             t.nKids, t.chgStart, t.nChgs, t.chgIdx, t.ord, rhsTreeTheta(), d) &&
-      fbh_copy_d2h(gCtx, sgm, d, n2))
+      fbh_copy_d2h(gCtx, sgm, d, n2)) {
     rc = 1;
-  else
+    trace("gpuChargeTreeRHS", sys->nPnls);
+  } else {
     fail("tree right-hand side");
+  }
   fbh_dev_free(gCtx, d);
   freeCT(&t);
   return rc;
@@ -272,13 +284,47 @@ int gpuRhsReusePanelChargeTreeEnergy(ssystem *sys, const double *rhs, const doub
   (void)sys; (void)rhs; (void)sgm; (void)pot;
   return 0;
 }
+/* the disjoint-pair coefficients of a batch of preconditioner blocks
+ * (src/precond_fmm.c:453-552): k[p] = panelIA0(X = pairSrc[p], Y = pairDst[p]), pnl->idx
+ * indices; the caller keeps the disjoint pairs' values */
 int gpuBuildPrecondPairsBatched(ssystem *sys, const int *pairSrc, const int *pairDst, long long nPairs, double *k0,
                                 double *k1, double *k2, double *k3) {
-  (void)sys; (void)pairSrc; (void)pairDst; (void)nPairs; (void)k0; (void)k1; (void)k2; (void)k3;
-  return 0;
+  if (!sys || !pairSrc || !pairDst || nPairs <= 0 || !k0 || !k1 || !k2 || !k3 || !ensure(sys)) return 0;
+  if (!fbh_pair_coeffs(gCtx, nPairs, pairSrc, pairDst, k0, k1, k2, k3)) return fail("preconditioner pairs");
+  trace("gpuBuildPrecondPairsBatched", nPairs);
+  return 1;
 }
+
+/* one block's disjoint pairs (src/precond_fmm.c:223-240, used at -q=1): KER = panelIA0(X =
+ * panels[srcLocal], Y = panels[dstLocal]) written into the row-major 2m x 2m block as the
+ * reference's own loop writes it (-KER1 | -KER0 over -KER3 | -KER2) */
 int gpuBuildPrecondDisjointBlock(panel **panels, int nPanels, const int *dstLocal, const int *srcLocal, int nPairs,
                                  double *block) {
-  (void)panels; (void)nPanels; (void)dstLocal; (void)srcLocal; (void)nPairs; (void)block;
-  return 0;
+  if (!panels || nPanels <= 0 || !dstLocal || !srcLocal || nPairs <= 0 || !block) return 0;
+  double *vtx = malloc(sizeof(double) * 9 * (size_t)nPanels), *nrm = malloc(sizeof(double) * 3 * (size_t)nPanels);
+  double *area = malloc(sizeof(double) * (size_t)nPanels), *k = malloc(sizeof(double) * 4 * (size_t)nPairs);
+  int ok = vtx && nrm && area && k;
+  for (int i = 0; ok && i < nPanels; i++) {
+    for (int j = 0; j < 9; j++) vtx[9 * i + j] = panels[i]->vtx[j / 3][j % 3];
+    for (int j = 0; j < 3; j++) nrm[3 * i + j] = panels[i]->normal[j];
+    area[i] = panels[i]->area;
+  }
+  if (ok)
+    ok = fbh_panel_pair_coeffs(nPanels, vtx, nrm, area, kappa, epsilon, 1, nPairs, srcLocal, dstLocal, k,
+                               k + nPairs, k + 2 * (size_t)nPairs, k + 3 * (size_t)nPairs);
+  if (ok) {
+    const int hm = nPanels, ms = 2 * nPanels;
+    for (int p = 0; p < nPairs; p++) {
+      const int d = dstLocal[p], s = srcLocal[p];
+      block[(size_t)d * ms + s] = -k[nPairs + p];
+      block[(size_t)d * ms + s + hm] = -k[p];
+      block[(size_t)(d + hm) * ms + s] = -k[3 * (size_t)nPairs + p];
+      block[(size_t)(d + hm) * ms + s + hm] = -k[2 * (size_t)nPairs + p];
+    }
+    trace("gpuBuildPrecondDisjointBlock", nPairs);
+  } else {
+    fail("preconditioner block");
+  }
+  free(vtx); free(nrm); free(area); free(k);
+  return ok;
 }

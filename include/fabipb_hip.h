@@ -133,6 +133,25 @@ int fbh_set_precond(fbh_ctx *ctx, int mode);
 int fbh_psolve_dev(fbh_ctx *ctx, double *out, const double *in);
 
 /*
+ * Galerkin coefficients of listed panel pairs for a host that assembles the
+ * preconditioner blocks itself (setupPreconditioning, src/precond_fmm.c:147-268):
+ * (k0..k3)[p] = panelIA0(X = panel xIdx[p], Y = panel yIdx[p]) (numQuad.c:629-697,
+ * whichever rule the pair's geometric case selects, the context's quadrature
+ * order).  Host arrays, nPairs entries each.
+ * fbh_pair_coeffs: panels of the context (leaf order, the reference's pnl->idx);
+ *   replaces gpuBuildPrecondPairsBatched (gpu_backend.h:38-41; its pairSrc is X,
+ *   pairDst is Y, src/precond_fmm.c:342-349).
+ * fbh_panel_pair_coeffs: panels given here (vtx[9m], nrm[3m], area[m], edges from
+ *   the vertices as src/input.c:743-747), no context; replaces
+ *   gpuBuildPrecondDisjointBlock (gpu_backend.h:42-44; srcLocal is X, dstLocal Y).
+ */
+int fbh_pair_coeffs(fbh_ctx *ctx, long long nPairs, const int *xIdx, const int *yIdx,
+                    double *k0, double *k1, double *k2, double *k3);
+int fbh_panel_pair_coeffs(int nPanels, const double *vtx, const double *nrm, const double *area,
+                          double kappa, double eps, int qorder, long long nPairs, const int *xLoc,
+                          const int *yLoc, double *k0, double *k1, double *k2, double *k3);
+
+/*
  * Restarted left-preconditioned GMRES with the preconditioner of
  * fbh_set_precond, all vectors in HBM (src/gmres.c:153-373, modified Gram-Schmidt :527-621;
  * convergence on ||M^-1 r|| / ||b||).  b, x are device pointers; x holds the
@@ -164,9 +183,15 @@ int fbh_gmres_refined_dev(fbh_ctx *ctx, const double *b, double *x, int restart,
                           int *info, int *outer, int *mv64);
 
 /* direct right-hand side on host arrays: b[2n] = fac * sum_q ... (setupRHS direct loop,
- * src/fabipb.c:789-798; gpuSetupRHS, gpu_backend.h:21), fac = 1/(4 pi eps1) */
+ * src/fabipb.c:789-798; gpuSetupRHS, gpu_backend.h:21), fac = 1/(4 pi eps1) unless
+ * fbh_set_rhs_factor set another */
 int fbh_rhs(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b);
 int fbh_rhs_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b_dev);
+/* the factor of every right-hand side of the context (direct, charge-tree, and the
+ * R^T of the gradients): the fac argument of gpuSetupRHS / gpuChargeTreeRHS
+ * (gpu_backend.h:21, 36), which src/fabipb.c:755 passes as 1/(4 pi eps1).
+ * 0 restores that default. */
+int fbh_set_rhs_factor(fbh_ctx *ctx, double fac);
 
 /*
  * Panel-tree solvation energy (raw, before the 2*pi*332.0716 scale) of the

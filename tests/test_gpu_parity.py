@@ -72,6 +72,24 @@ def test_near_coefficients_every_case(fa, meshdir, name):
 
 
 @pytest.mark.parametrize("name", harness_cases())
+def test_pair_coeffs_every_case(fa, meshdir, name):
+    """fbh_pair_coeffs (the preconditioner-assembly entry) on the recorded panelIA0
+    pairs of every geometric case, in one batch, and with both index orders: the
+    reference's values (bar as above) and bit-identical to the nearfield's own."""
+    mesh, tree, op = case_op(fa, meshdir, name)
+    F = fixture(name)
+    idx = F["ia0_idx"].reshape(-1, 3)
+    val = F["ia0_val"].reshape(-1, 4)
+    k = op.pair_coeffs(idx[:, 0], idx[:, 1])
+    for (x, y, c), v, got in zip(idx, val, k):
+        assert np.max(np.abs(got - v)) <= 1e-12 * np.max(np.abs(v)), (x, y, c)
+        assert np.array_equal(got, op.near_coeff(x, y))
+    kt = op.pair_coeffs(idx[:, 1], idx[:, 0])   # X and Y exchanged: y0, y3 symmetric for disjoint pairs
+    dis = idx[:, 2] == 0
+    assert np.allclose(kt[dis][:, [0, 3]], k[dis][:, [0, 3]], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("name", harness_cases())
 def test_near_apply(fa, meshdir, name):
     mesh, tree, op = case_op(fa, meshdir, name)
     F = fixture(name)
@@ -399,6 +417,11 @@ def test_matvec_linearity_full_size(fa, meshdir):
     assert np.array_equal(op.matvec(x), op.matvec(x))
 
 
+# the reference host re-executes itself unless all five are set (src/fabipb_options.c:134-163)
+THREAD_ENV = dict(OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1", VECLIB_MAXIMUM_THREADS="1",
+                  BLIS_NUM_THREADS="1")
+
+
 @pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
 def test_reference_host_with_hip_backend(meshdir, name):
     """Drop-in: the reference's own host code (oracle/_ref/fabipb_ref_hip, built with
@@ -412,9 +435,8 @@ def test_reference_host_with_hip_backend(meshdir, name):
         pytest.skip("drop-in binary not built (needs the reference sources)")
     F = fixture(name)
     cli = F["cli"]
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
-               VECLIB_MAXIMUM_THREADS="1", BLIS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
-               FABIPB_ENERGY_MODE="compare", FABIPB_NEARFIELD_APPLY_THREADS="8", FABIPB_SETUP_THREADS="8")
+    env = dict(os.environ, **THREAD_ENV, FABIPB_REUSE_MESH="1", FABIPB_ENERGY_MODE="compare",
+               FABIPB_NEARFIELD_APPLY_THREADS="8", FABIPB_SETUP_THREADS="8")
     r = subprocess.run([exe, "-g=1", "-c=1", "-B=1"] + cli["flags"] + [str(meshdir / name)], env=env,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
     print(r.stdout[-1500:], flush=True)
@@ -440,8 +462,7 @@ def test_reference_host_direct_on_gpu(meshdir, name):
         pytest.skip("drop-in binary not built (needs the reference sources)")
     cli = fixture(name)["cli"]
     flags = [f if f != "-r=2" else "-r=1" for f in cli["flags"]]
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
-               FABIPB_ENERGY_MODE="compare")
+    env = dict(os.environ, **THREAD_ENV, FABIPB_REUSE_MESH="1", FABIPB_ENERGY_MODE="compare")
     r = subprocess.run([exe, "-g=1", "-B=1"] + flags + [str(meshdir / name)], env=env,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
     print(r.stdout[-1200:], flush=True)
@@ -463,12 +484,41 @@ def test_reference_host_tree_rhs_on_gpu(meshdir):
     if not os.path.exists(exe):
         pytest.skip("drop-in binary not built (needs the reference sources)")
     cli = fixture(name)["cli"]
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", FABIPB_REUSE_MESH="1",
-               FABIPB_ENERGY_MODE="compare", FABIPB_FORCE_TREE_RHS="1")
+    env = dict(os.environ, **THREAD_ENV, FABIPB_REUSE_MESH="1", FABIPB_ENERGY_MODE="compare",
+               FABIPB_FORCE_TREE_RHS="1", FABIPB_HIP_TRACE="1")
     r = subprocess.run([exe, "-g=1", "-B=1"] + cli["flags"] + [str(meshdir / name)], env=env,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
     print(r.stdout[-1200:], flush=True)
     assert r.returncode == 0 and "mode=tree" in r.stdout
+    assert "gpu_backend_hip: gpuChargeTreeRHS done" in r.stdout
+    m = re.search(r"GMRES status: info=(\d+) iterations=(\d+)", r.stdout)
+    assert int(m[1]) == cli["info"] and abs(int(m[2]) - cli["iterations"]) <= 1
+    raw = float(re.search(r"panel-tree raw=(\S+)", r.stdout)[1])
+    assert abs(raw - cli["panel_tree_raw"]) <= 1e-9 * abs(cli["panel_tree_raw"])
+
+
+@pytest.mark.parametrize("name,batched", [("sph3_ball_p2", False), ("glob4_p2", False), ("glob4_p2", True)])
+def test_reference_host_precond_on_gpu(meshdir, name, batched):
+    """Drop-in, block preconditioner: the reference host with -P=2 -g=1 assembles its
+    leaf-parent blocks from gpuBuildPrecondDisjointBlock (one call per block) or, with
+    FABIPB_PRECOND_BATCH_MIN_BLOCKS=0, from gpuBuildPrecondPairsBatched (the binding's
+    fbh_panel_pair_coeffs / fbh_pair_coeffs); its solve must reproduce the reference's
+    own -P=2 CPU solve."""
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle/_ref/fabipb_ref_hip")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in binary not built (needs the reference sources)")
+    cli = fixture(name)["cli"]
+    env = dict(os.environ, **THREAD_ENV, FABIPB_REUSE_MESH="1", FABIPB_ENERGY_MODE="compare", FABIPB_HIP_TRACE="1")
+    if batched:
+        env["FABIPB_PRECOND_BATCH_MIN_BLOCKS"] = "0"
+    r = subprocess.run([exe, "-g=1", "-B=1"] + cli["flags"] + [str(meshdir / name)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0
+    hook = "gpuBuildPrecondPairsBatched" if batched else "gpuBuildPrecondDisjointBlock"
+    assert f"gpu_backend_hip: {hook} done" in r.stdout
     m = re.search(r"GMRES status: info=(\d+) iterations=(\d+)", r.stdout)
     assert int(m[1]) == cli["info"] and abs(int(m[2]) - cli["iterations"]) <= 1
     raw = float(re.search(r"panel-tree raw=(\S+)", r.stdout)[1])
