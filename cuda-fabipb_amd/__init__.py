@@ -124,6 +124,16 @@ def hip_lib():
         L.fbh_rhs_tree_dev.restype = C.c_int
         L.fbh_rhs_tree_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip,
                                        _ip, _ip, _ip, _ip, C.c_double, _vp]
+        L.fbh_set_charges.restype = C.c_int
+        L.fbh_set_charges.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _ip, _ip, _ip]
+        L.fbh_rhs_res_dev.restype = C.c_int
+        L.fbh_rhs_res_dev.argtypes = [_vp, _vp]
+        L.fbh_rhs32_res_dev.restype = C.c_int
+        L.fbh_rhs32_res_dev.argtypes = [_vp, _vp]
+        L.fbh_rhs_tree_res_dev.restype = C.c_int
+        L.fbh_rhs_tree_res_dev.argtypes = [_vp, C.c_double, _vp]
+        L.fbh_energy_res_dev.restype = C.c_int
+        L.fbh_energy_res_dev.argtypes = [_vp, C.c_double, _vp, _dp]
         L.fbh_matvec_t_dev.restype = C.c_int
         L.fbh_matvec_t_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp]
         L.fbh_gmres_t_dev.restype = C.c_int
@@ -141,6 +151,10 @@ def hip_lib():
         L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
         L.fbh_profile32.restype = C.c_int
         L.fbh_profile32.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
+        L.fbh_near_timing.restype = C.c_int
+        L.fbh_near_timing.argtypes = [_vp, C.c_int]
+        L.fbh_near_timing_read.restype = C.c_int
+        L.fbh_near_timing_read.argtypes = [_vp, C.POINTER(C.c_longlong), _dp]
         L.fbh_stats.restype = C.c_int
         L.fbh_stats.argtypes = [_vp, C.POINTER(C.c_longlong)]
         L.fbh_stream.restype = _vp
@@ -486,6 +500,45 @@ class Operator:
             _i(keep["chgStart"]), _i(keep["nChgs"]), _i(keep["chgIdx"]), _i(keep["order"]),
             theta, C.c_void_p(b.data_ptr())))
 
+    def set_charges(self, ct, pos, chr_):
+        """Stages the charges and their charge tree on the device once (fbh_set_charges);
+        rhs_res_dev, rhs_tree_res_dev and energy_res_dev then read them there."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        k = {q: np.ascontiguousarray(v) for q, v in ct.items() if isinstance(v, np.ndarray)}
+        self._ck(hip_lib().fbh_set_charges(
+            self.h, len(chr_), _d(pos), _d(chr_), len(k["level"]), _i(k["level"]), _d(k["x"]), _d(k["eRad"]),
+            _i(k["kidStart"]), _i(k["nKids"]), _i(k["chgStart"]), _i(k["nChgs"]), _i(k["chgIdx"]), _i(k["order"])))
+
+    def rhs_res_dev(self, b):
+        """Direct right-hand side of the resident charges into the device tensor b."""
+        self._ck(hip_lib().fbh_rhs_res_dev(self.h, C.c_void_p(b.data_ptr())))
+
+    def rhs32_res_dev(self, b):
+        """The precision form's direct right-hand side (FP32 kernel) of the resident charges."""
+        self._ck(hip_lib().fbh_rhs32_res_dev(self.h, C.c_void_p(b.data_ptr())))
+
+    def rhs_tree_res_dev(self, b, theta=0.8):
+        """Charge-tree right-hand side of the resident charges into b."""
+        self._ck(hip_lib().fbh_rhs_tree_res_dev(self.h, theta, C.c_void_p(b.data_ptr())))
+
+    def energy_res_dev(self, sigma, theta=0.8):
+        """Raw panel-tree energy of the resident charges for the density sigma."""
+        e = C.c_double(0.0)
+        self._ck(hip_lib().fbh_energy_res_dev(self.h, theta, C.c_void_p(sigma.data_ptr()), C.byref(e)))
+        return e.value
+
+    def near_timing(self, on):
+        """Starts (on=True, clearing) or stops event timing of every nearfield launch."""
+        self._ck(hip_lib().fbh_near_timing(self.h, int(bool(on))))
+
+    def near_timing_read(self):
+        """{"fp64": (launches, total ms), "fp32": (...)} recorded since near_timing(True)."""
+        cnt = (C.c_longlong * 2)()
+        ms = np.zeros(2)
+        self._ck(hip_lib().fbh_near_timing_read(self.h, cnt, _d(ms)))
+        return {"fp64": (int(cnt[0]), float(ms[0])), "fp32": (int(cnt[1]), float(ms[1]))}
+
     def profile(self, x, y, iters=10, precision="fp64"):
         ms = np.zeros(8)
         f = hip_lib().fbh_profile32 if precision == "fp32" else hip_lib().fbh_profile
@@ -533,13 +586,14 @@ class Problem:
         """Solvation energy in kcal/mol."""
         return self.solve(q, kappa)[1] * ENERGY_SCALE
 
-    def gradient(self, q, kappa, u=None):
-        """(E, dE/dq, dE/dkappa) in kcal/mol units; u: a solution already at hand."""
+    def gradient(self, q, kappa, u=None, e=None):
+        """(E, dE/dq, dE/dkappa) in kcal/mol units; u: a solution already at hand and e
+        its raw energy (recomputed from u when not given)."""
         torch = _torch()
         q = np.ascontiguousarray(q, dtype=np.float64)
         if u is None:
             u, e, _ = self.solve(q, kappa)
-        else:
+        elif e is None:
             e = self.operator(kappa).energy_direct_dev(self.mesh.pos, q, u)
         lam = torch.zeros_like(u)
         dq, dk, info = self.operator(kappa).gradient_dev(self.mesh.pos, q, u, lam, restart=self.restart,
@@ -574,7 +628,8 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
           precond=3, direct=False, rhs="auto"):
     """One solve (src/fabipb.c:1067-1210): returns energies, iterations and times.
     precision "fp64" is the plain form (GMRES on the FP64 operator); "fp32" is the
-    precision form (refinement around GMRES on the FP32 operator, FP64 residuals).
+    precision form (the direct right-hand side's kernel in FP32, refinement around
+    GMRES on the FP32 operator, FP64 residuals, the energy in FP64).
     precond is the -P mode (3 diagonal; 0, 1, 2 block LU); direct the -r=1/2 dense
     operator in place of the FMM; rhs "direct", "tree" or "auto" (the reference's
     pair-count policy, FABIPB_FORCE_TREE_RHS).  x0 "rhs" starts GMRES from b (the
@@ -599,12 +654,15 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
     n2 = 2 * tree.n
     b = torch.empty(n2, dtype=torch.float64, device="cuda")
     ct = tree.charge_tree()
+    op.set_charges(ct, mesh.pos, mesh.chr)   # staged once for the right-hand side and the energy
     if rhs == "auto":   # src/fabipb_options.c:174-183
         rhs = "tree" if host_lib().fb_should_use_tree_rhs(tree.n, len(mesh.chr), 1) else "direct"
     if rhs == "tree":
-        op.rhs_tree_dev(ct, mesh.pos, mesh.chr, b)
+        op.rhs_tree_res_dev(b)
+    elif precision == "fp32":
+        op.rhs32_res_dev(b)
     else:
-        op.rhs_dev(mesh.pos, mesh.chr, b)
+        op.rhs_res_dev(b)
     if x0 == "auto":
         x0 = "zero" if precision == "fp32" else "rhs"
     x = b.clone() if x0 == "rhs" else torch.zeros_like(b)
@@ -616,7 +674,7 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
         g = op.gmres_dev(b, x, restart=restart, tol=tol, maxit=maxit)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
-    raw = op.energy_dev(ct, mesh.pos, mesh.chr, x)
+    raw = op.energy_res_dev(x)
     torch.cuda.synchronize()
     t5 = time.perf_counter()
     return dict(energy=raw * ENERGY_SCALE, energy_raw=raw, iters=g["iters"], resid=g["resid"],

@@ -1008,29 +1008,43 @@ __global__ __launch_bounds__(64) void k_spec_vals(const Pnl *__restrict__ pnl, c
   }
 }
 
-// (kappa r + 1) e^{-kappa r} - 1 and 1 - e^{-kappa r} without cancellation
-// for small kappa r (Taylor series through t^9), directly otherwise
-__device__ __forceinline__ void flyExp(float t, float e, float &omE, float &cm1) {
-  if (t < 0.5f) {
-    omE = t * (1.f + t * (-1.f / 2 + t * (1.f / 6 + t * (-1.f / 24 + t * (1.f / 120 + t * (-1.f / 720 +
-          t * (1.f / 5040 + t * (-1.f / 40320 + t * (1.f / 362880)))))))));
-    cm1 = t * t * (-1.f / 2 + t * (1.f / 3 + t * (-1.f / 8 + t * (1.f / 30 + t * (-1.f / 144 + t * (1.f / 840 +
-          t * (-1.f / 5760 + t * (1.f / 45360))))))));
-  } else {
-    omE = 1.f - e;
-    cm1 = (t + 1.f) * e - 1.f;
-  }
-}
-
 #define FLY_CHUNK 512
 #define FLY_RB 4   // rows per pass over the staged sources
+// One FP32 pair (kernelKER4 at the one-point rule, src/kernel.c:109-147, with
+// G0 = 1/(4 pi r) replaced by 1/r; the row's total is scaled by 1/(4 pi) once):
+// r from the hi + lo point pairs, 1/r from the hardware reciprocal square root
+// (r2 is never denormal: the closest distinct quadrature points are 0.1 A apart,
+// the panel itself has r2 set to 1), e^{-kappa r} from the hardware 2^x with
+// kappa log2(e) folded into one constant nk = -kappa / ln 2.  1 - e and
+// (1 + kappa r) e - 1 are formed directly: their absolute error (~1e-7) stays
+// inside the FP32 operator's error budget, and the pairs closest together (a
+// shared vertex or edge, or the panel itself) carry stored corrections taken
+// against this same FP32 formula.  48 VALU + 2 transcendental instructions.
+// Returns the four kernels y0..y3 (times 4 pi, unweighted) of the pair.
+__device__ __forceinline__ float4 fly32Ker(const float4 &XP, const float4 &XN, const float4 &XL, const float4 &P,
+                                           const float4 &N, const float4 &Lo, bool skip, float kappa, float nk,
+                                           float kap2, float eps, float ieps) {
+  const float rx = (XP.x - P.x) + (XL.x - Lo.x), ry = (XP.y - P.y) + (XL.y - Lo.y), rz = (XP.z - P.z) + (XL.z - Lo.z);
+  const float r2 = skip ? 1.f : fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+  const float ri = __builtin_amdgcn_rsqf(r2), rr = r2 * ri;
+  const float e = __builtin_amdgcn_exp2f(rr * nk), t = kappa * rr;
+  const float ip0 = XN.x * N.x + XN.y * N.y + XN.z * N.z;
+  const float ipX = XN.x * rx + XN.y * ry + XN.z * rz;
+  const float ipY = N.x * rx + N.y * ry + N.z * rz;
+  const float coef = fmaf(t, e, e), cm1 = coef - 1.f, ri2 = ri * ri, ri3 = ri * ri2;
+  const float dy = ipY * ri3, dx = ipX * ri3, a = ipX * dy;
+  const float ddG = fmaf(a * ri2, -3.f, ip0 * ri3);
+  return make_float4(ri * (1.f - e), dy * fmaf(eps, coef, -1.f), dx * fmaf(coef, ieps, -1.f),
+                     fmaf(cm1, ddG, -(kap2 * e) * a));
+}
+
 // Staged columns [j0, j1) against the wave's rows h .. h + FLY_RB - 1, one
 // column per lane (FP32).  CHK: the columns may hold the row's own panel.
 template <bool CHK>
 __device__ __forceinline__ void fly32Cols(int h, int j0, int j1, int lane, int w, int rows, const float4 *sP,
                                           const float4 *sN, const float4 *sL, const float4 (&XP)[4],
                                           const float4 (&XN)[4], const float4 (&XL)[4], const int (&xi)[4],
-                                          float kappa, float kap2, float eps, float ieps, float (&a0)[4],
+                                          float kappa, float nk, float kap2, float eps, float ieps, float (&a0)[4],
                                           float (&a1)[4]) {
   for (int j = j0 + lane; j < j1; j += 64) {
     const float4 Lo = sL[j], P = sP[j], N = sN[j];
@@ -1039,27 +1053,43 @@ __device__ __forceinline__ void fly32Cols(int h, int j0, int j1, int lane, int w
     for (int r = h; r < h + FLY_RB; r++) {
       if (w + 4 * r >= rows) continue;
       const bool skip = CHK && y == xi[r];   // the panel itself: weights zeroed, r2 replaced by 1
-      const float rx = (XP[r].x - P.x) + (XL[r].x - Lo.x), ry = (XP[r].y - P.y) + (XL[r].y - Lo.y),
-                  rz = (XP[r].z - P.z) + (XL[r].z - Lo.z);
-      const float r2 = skip ? 1.f : rx * rx + ry * ry + rz * rz, ri = rsqrtf(r2), ri2 = ri * ri, rr = r2 * ri;
-      const float t = kappa * rr, e = expf(-t);
-      float omE, cm1;
-      flyExp(t, e, omE, cm1);
-      const float ip0 = XN[r].x * N.x + XN[r].y * N.y + XN[r].z * N.z;
-      const float ipX = XN[r].x * rx + XN[r].y * ry + XN[r].z * rz;
-      const float ipY = N.x * rx + N.y * ry + N.z * rz;
-      // 4 pi times G0 and its derivatives (the row's total is scaled by 1/(4 pi))
-      const float coef = cm1 + 1.f, ri3 = ri * ri2;
-      const float dy = ipY * ri3, dx = ipX * ri3, a = ipX * dy;
-      const float ddG = fmaf(a * ri2, -3.f, ip0 * ri3);
-      const float y0 = ri * omE;
-      const float y1 = dy * fmaf(eps, coef, -1.f);
-      const float y2 = dx * fmaf(coef, ieps, -1.f);
-      const float y3 = fmaf(cm1, ddG, -(kap2 * e) * a);
+      const float4 k = fly32Ker(XP[r], XN[r], XL[r], P, N, Lo, skip, kappa, nk, kap2, eps, ieps);
       const float pw = skip ? 0.f : P.w, nw = skip ? 0.f : N.w;
-      a0[r] = fmaf(y0, nw, fmaf(y1, pw, a0[r]));
-      a1[r] = fmaf(y2, nw, fmaf(y3, pw, a1[r]));
+      a0[r] = fmaf(k.x, nw, fmaf(k.y, pw, a0[r]));
+      a1[r] = fmaf(k.z, nw, fmaf(k.w, pw, a1[r]));
     }
+  }
+}
+
+// FP32 corrections of the listed pairs: the Galerkin rule (FP64, panelIA0)
+// minus what an FP32 on-the-fly tile adds for the pair (fly32Ker on the same
+// FP32 panel data, times the tile's own scale A_x / (4 pi) and A_y), so a
+// corrected pair carries the rule's value up to the rounding of the stored
+// difference.  One block per row.
+__global__ __launch_bounds__(64) void k_spec_vals32(const Pnl *__restrict__ pnl, const float4 *__restrict__ PA,
+                                                    const float4 *__restrict__ PN, const float4 *__restrict__ PL,
+                                                    const long long *__restrict__ specOff,
+                                                    const int *__restrict__ specCol, float4 *__restrict__ v32,
+                                                    double kappa, double eps, QTab Q, int q) {
+  const int x = blockIdx.x;
+  const Pnl X = pnl[x];
+  const float4 XP = PA[x], XN = PN[x], XL = PL[x];
+  const float kf = (float)kappa, ef = (float)eps, nk = -kf * (float)M_LOG2E, kap2 = kf * kf, ieps = 1.f / ef;
+  const double sx = (double)(XP.w * (float)kFourPiI);
+  for (long long e = specOff[x] + threadIdx.x; e < specOff[x + 1]; e += 64) {
+    const int y = specCol[e];
+    double k[4], f[4] = {0, 0, 0, 0};
+    panelIA0(X, pnl[y], x == y, kappa, eps, Q, q, k, false);
+    if (x != y) {
+      const float4 P = PA[y];
+      const float4 kk = fly32Ker(XP, XN, XL, P, PN[y], PL[y], false, kf, nk, kap2, ef, ieps);
+      const double s = sx * (double)P.w;
+      f[0] = kk.x * s;
+      f[1] = kk.y * s;
+      f[2] = kk.z * s;
+      f[3] = kk.w * s;
+    }
+    v32[e] = make_float4((float)(k[0] - f[0]), (float)(k[1] - f[1]), (float)(k[2] - f[2]), (float)(k[3] - f[3]));
   }
 }
 
@@ -1080,7 +1110,7 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
   const int ncol = ncolArr[D];
   const int *src = srcList + srcOff[D];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float k4pi = (float)kFourPiI, kap2 = kappa * kappa, ieps = 1.f / eps;
+  const float k4pi = (float)kFourPiI, kap2 = kappa * kappa, ieps = 1.f / eps, nk = -kappa * (float)M_LOG2E;
   float4 XP[4], XN[4], XL[4];
   int xi[4];
 #pragma unroll
@@ -1110,9 +1140,9 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
       // D's own columns, padded to whole passes of 64, are checked for the panel itself
       const int lo = min(max(oc - c0, 0), cn);
       const int hi = min(lo + ((min(max(oc + nOwn - c0, 0), cn) - lo + 63) & ~63), cn);
-      fly32Cols<false>(h, 0, lo, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, kap2, eps, ieps, a0, a1);
-      fly32Cols<true>(h, lo, hi, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, kap2, eps, ieps, a0, a1);
-      fly32Cols<false>(h, hi, cn, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, kap2, eps, ieps, a0, a1);
+      fly32Cols<false>(h, 0, lo, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, nk, kap2, eps, ieps, a0, a1);
+      fly32Cols<true>(h, lo, hi, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, nk, kap2, eps, ieps, a0, a1);
+      fly32Cols<false>(h, hi, cn, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, nk, kap2, eps, ieps, a0, a1);
     }
   }
 #pragma unroll
@@ -1896,6 +1926,93 @@ __global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *
   }
 }
 
+// The downward pass of the matvec in one launch: L2L down each leaf's
+// ancestor chain (levels height..depth) and L2P at its panels.  One block per
+// leaf recomputes the accumulated locals of its ancestors with k_l2l's
+// arithmetic in k_l2l's order (child = child's M2L locals + sum over the
+// (beta, alpha+beta) table of t^beta/beta! parent[alpha+beta]), so the leaf's
+// total is the one the per-level launches give, then evaluates it as k_l2p.
+// The ancestors' accumulated locals are recomputed by every leaf below them
+// (about 4 nMom^2/3 FMAs per link, a few microseconds in all) instead of
+// being written back and read by the next level's launch: the chain of
+// per-level launches, each waiting for the previous one, is what the far-field
+// pass spends its time on beside the nearfield.  lec is only read.
+template <typename R>
+__global__ __launch_bounds__(256) void k_l2lp(const R *__restrict__ Q0, const R *__restrict__ Q1,
+                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                              const int *__restrict__ parent, const int *__restrict__ cubeLevel,
+                                              const int *__restrict__ ordMom, const double *__restrict__ cubeX,
+                                              const R *__restrict__ lec, R *__restrict__ far, int n, int nm,
+                                              int nmMax, int nmChain, int height, int depth, DevTabs T) {
+  R *sm = reinterpret_cast<R *>(dynLds);
+  R *cur = sm, *nxt = sm + 4 * nmChain, *Rt = sm + 8 * nmChain;   // cur/nxt [4][nmChain]
+  int *cst = (int *)(sm + 9 * nmChain), *tab = cst + nmChain + 1;
+  const int L = blockIdx.x;
+  const int nT = T.convStart[nmChain];
+  for (int i = threadIdx.x; i <= nmChain; i += blockDim.x) cst[i] = T.convStart[i];
+  for (int i = threadIdx.x; i < nT; i += blockDim.x) tab[i] = T.conv[i];
+  int anc[16];   // anc[l - height] = the ancestor of leaf L at level l (l = height..depth, depth - height < 15)
+  anc[depth - height] = L;
+  for (int l = depth; l > height; l--) anc[l - 1 - height] = parent[anc[l - height]];
+  {
+    const int top = anc[0], nTop = NMOM(ordMom[cubeLevel[top]]);
+    for (int a = threadIdx.x; a < nTop; a += blockDim.x)
+      for (int k = 0; k < 4; k++) cur[k * nmChain + a] = lec[((long long)top * 4 + k) * nmMax + a];
+  }
+  __syncthreads();
+  for (int l = height + 1; l <= depth; l++) {
+    const int kid = anc[l - height], par = anc[l - 1 - height];
+    const int ordIn = ordMom[cubeLevel[par]], nIn = NMOM(ordIn), nOut = NMOM(ordMom[cubeLevel[kid]]);
+    double tr[3];
+    for (int d = 0; d < 3; d++) tr[d] = cubeX[3 * kid + d] - cubeX[3 * par + d];
+    for (int a = threadIdx.x; a < nIn; a += blockDim.x) Rt[a] = (R)conVect1(tr, T.mi[a], T.ifact3[a]);
+    __syncthreads();
+    for (int a = threadIdx.x; a < nOut; a += blockDim.x) {
+      const int cnt = NMOM(ordIn - deg_of(T.mi[a]));
+      const int *cv = tab + cst[a];
+      R t[4] = {};
+      for (int e = 0; e < cnt; e++) {
+        int pe = cv[e], j = pe & 0xffff, kk = pe >> 16;
+        R rv = Rt[j];
+        for (int k = 0; k < 4; k++) t[k] += rv * cur[k * nmChain + kk];
+      }
+      for (int k = 0; k < 4; k++) nxt[k * nmChain + a] = lec[((long long)kid * 4 + k) * nmMax + a] + t[k];
+    }
+    __syncthreads();
+    R *sw = cur;
+    cur = nxt;
+    nxt = sw;
+  }
+  // L2P (k_l2p): far[p] = Q2M0^T (l1 + l2), far[p+n] = Q2M1^T (l3 + l4)
+  R *L12 = nxt, *L34 = nxt + nm;
+  for (int a = threadIdx.x; a < nm; a += blockDim.x) {
+    L12[a] = cur[a] + cur[nmChain + a];
+    L34[a] = cur[2 * nmChain + a] + cur[3 * nmChain + a];
+  }
+  __syncthreads();
+  const int g = threadIdx.x >> 3, l8 = threadIdx.x & 7, s = leafStart[L], c = leafCount[L];
+  for (int j0 = 0; j0 < c; j0 += 32) {   // uniform trip count: every lane takes part in the shuffles
+    const int j = j0 + g;
+    R t0 = 0, t1 = 0;
+    if (j < c) {
+      const R *q0 = Q0 + (long long)(s + j) * nm, *q1 = Q1 + (long long)(s + j) * nm;
+#pragma unroll 4   // several loads in flight per lane
+      for (int a = l8; a < nm; a += 8) {
+        t0 += q0[a] * L12[a];
+        t1 += q1[a] * L34[a];
+      }
+    }
+    for (int o = 4; o > 0; o >>= 1) {
+      t0 += __shfl_xor(t0, o, 64);
+      t1 += __shfl_xor(t1, o, 64);
+    }
+    if (j < c && l8 == 0) {
+      far[s + j] = t0;
+      far[s + j + n] = t1;
+    }
+  }
+}
+
 // y = alpha*(D x - far - near) + beta*y (src/fabipb.c:1271-1279)
 template <typename R>
 __global__ void k_combine(const double *__restrict__ area, const double *__restrict__ x, const R *__restrict__ far,
@@ -1996,6 +2113,53 @@ __global__ __launch_bounds__(256) void k_rhs_q1(const Pnl *__restrict__ pnl, int
       b0 += c.w * ri;
       b1 -= c.w * ip * (ri * ri * ri);
     }
+  }
+  if (i < n) {
+    b[i] = b0 * W;
+    b[i + n] = b1 * W;
+  }
+}
+
+// Precision form: the direct right-hand side at q = 1 with r = x - x_j formed
+// in FP64 (so r keeps FP32 relative accuracy whatever the coordinates' size)
+// and the kernel in FP32: 1/r from the hardware reciprocal square root (r2 is
+// not denormal: the charges sit at least 1.5 A inside the surface, and 0
+// gives inf as the FP64 kernel does), sums in FP32 over each staged block of
+// 256 charges and in FP64 across blocks.
+__global__ __launch_bounds__(256) void k_rhs32_q1(const Pnl *__restrict__ pnl, int n, int nChar,
+                                                  const double *__restrict__ pos, const double *__restrict__ chr,
+                                                  double *__restrict__ b, double fac, double t, double ww) {
+  __shared__ double4 sp[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xq[3] = {0, 0, 0}, W = 0.0;
+  float nr[3] = {0, 0, 0};
+  if (i < n) {
+    const Pnl &P = pnl[i];
+    for (int k = 0; k < 3; k++) {
+      xq[k] = P.v[k] + t * (P.a[6 + k] + t * P.a[k]);
+      nr[k] = (float)P.nr[k];
+    }
+    W = ww * 2.0 * P.area * fac;
+  }
+  double b0 = 0.0, b1 = 0.0;
+  for (int j0 = 0; j0 < nChar; j0 += 256) {
+    __syncthreads();
+    const int j = j0 + threadIdx.x;
+    if (j < nChar) sp[threadIdx.x] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], chr[j]);
+    __syncthreads();
+    const int cnt = min(256, nChar - j0);
+    float s0 = 0.f, s1 = 0.f;
+#pragma unroll 4
+    for (int jj = 0; jj < cnt; jj++) {
+      const double4 c = sp[jj];
+      const float rx = (float)(xq[0] - c.x), ry = (float)(xq[1] - c.y), rz = (float)(xq[2] - c.z);
+      const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz)), ri = __builtin_amdgcn_rsqf(r2);
+      const float ip = nr[0] * rx + nr[1] * ry + nr[2] * rz, q = (float)c.w;
+      s0 = fmaf(q, ri, s0);
+      s1 = fmaf(-q * ip, ri * ri * ri, s1);
+    }
+    b0 += s0;
+    b1 += s1;
   }
   if (i < n) {
     b[i] = b0 * W;
@@ -3083,6 +3247,22 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// a charge set staged on the device (stageCharges): inputs and walk scratch in one arena
+struct ChargeSet {
+  void *arena = nullptr;
+  size_t arenaBytes = 0;
+  std::vector<unsigned char> stage;
+  bool ready = false;
+  int nChar = 0, nc = 0, c0 = -1, P = 0, S = 1, SZ = 0, nClus = 0, nDesc = 0;
+  bool uni = false;
+  std::vector<int> lvStart, lvCount, ord;
+  int *dLevel = nullptr, *dKid = nullptr, *dNK = nullptr, *dCS = nullptr, *dNC = nullptr, *dCI = nullptr,
+      *dOrd = nullptr;
+  double *dX = nullptr, *dR = nullptr, *dPos = nullptr, *dChr = nullptr, *dMom = nullptr, *dPart = nullptr,
+         *dOut = nullptr, *dRed = nullptr, *dPoly = nullptr;
+  const int4 *dDesc = nullptr;
+};
+
 struct fbh_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
@@ -3130,6 +3310,7 @@ struct fbh_ctx {
   int *ownOff = nullptr;   // per leaf: the column of its own first panel in its column list (-1: none)
   // M2L term grid (k_m2l_grid): K steps x 64 lanes per order, runs of L steps
   int gridK = 0, gridL = 0, gridSlotZ = 0, gridNcc = 0, gridNgg = 0;
+  int nmChain = 0;   // k_l2lp: nMom of the largest order on the levels height..depth (0: per-level launches)
   unsigned *gridTerms = nullptr;
   int *gridOff = nullptr, *gridRedStart = nullptr;
   unsigned short *gridRedCell = nullptr;
@@ -3152,9 +3333,8 @@ struct fbh_ctx {
   bool fly64 = false;
   double4 *fly64PA = nullptr, *fly64PN = nullptr, *specVal64 = nullptr, *specValT64 = nullptr;
   double *ro = nullptr, *dx = nullptr;   // refinement residual and correction
-  void *enArena = nullptr;                // energy: charge tree, moments and scratch
-  size_t enArenaBytes = 0;
-  std::vector<unsigned char> enStage;
+  ChargeSet csCall;   // charge set of the calls that pass host charge arrays
+  ChargeSet csRes;    // resident charge set (fbh_set_charges)
   // preconditioner: 3 diagonal, 2 leaf-parent block LU (explicit inverses)
   int precMode = 3;
   std::vector<int> hKidStart, hNKids, hLeafStart, hLeafCount;
@@ -3179,6 +3359,12 @@ struct fbh_ctx {
   // nearfield on a second stream, overlapped with the far-field chain
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
+  // nearfield launch timing (fbh_near_timing): an event pair around every
+  // nearfield launch on the side stream, read back after the timed region
+  bool ntOn = false;
+  int ntUsed = 0;
+  std::vector<hipEvent_t> ntEv;
+  std::vector<unsigned char> ntKind;   // 0 FP64, 1 FP32
   // GMRES workspace
   int gmRestart = 0;
   double *V = nullptr, *w = nullptr, *r = nullptr, *part = nullptr, *hcol = nullptr, *ydev = nullptr, *scal = nullptr;
@@ -3235,7 +3421,8 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   if (c->ydev) hipFree(c->ydev);
   if (c->scal) hipFree(c->scal);
   if (c->ro) hipFree(c->ro);
-  if (c->enArena) hipFree(c->enArena);
+  for (ChargeSet *cs : {&c->csCall, &c->csRes})
+    if (cs->arena) hipFree(cs->arena);
   for (void *q : {(void *)c->pbStart, (void *)c->pbM, (void *)c->pbOff, (void *)c->pbTiles, (void *)c->pbInv,
                   (void *)c->pbTmp, (void *)c->pbInvT})
     if (q) hipFree(q);
@@ -3248,6 +3435,7 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
     hipStreamSynchronize(c->side);
     hipStreamDestroy(c->side);
   }
+  for (hipEvent_t e : c->ntEv) hipEventDestroy(e);
   if (c->evFork) hipEventDestroy(c->evFork);
   if (c->evJoin) hipEventDestroy(c->evJoin);
   if (c->ownStream && c->stream) hipStreamDestroy(c->stream);
@@ -3581,6 +3769,7 @@ static inline int blk64(int n, int cap) {
 }
 
 static int enableFly64(fbh_ctx *c);
+static size_t l2lpLds(const fbh_ctx *c, size_t elem);
 
 extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorder, int n, const double *vtx,
                                const double *nrm, const double *area, int depth, int height, const int *ordMom,
@@ -3665,6 +3854,17 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
       c->ldsMax = std::min(lds, 65536);
   }
   if (!buildTables(c)) FAIL();
+  {   // the fused downward pass when the chain fits its ancestor array and its LDS
+    int oc = 0;
+    for (int l = height; l <= depth; l++) oc = std::max(oc, ordMom[l]);
+    c->nmChain = NMOM(oc);
+    if (depth - height >= 15 || depth < height || c->nmChain > c->nmMax ||
+        l2lpLds(c, sizeof(double)) > (size_t)c->ldsMax)
+      c->nmChain = 0;
+    const char *el = getenv("FABIPB_L2L");   // "levels": the per-level launches (check path)
+    if (el && strcmp(el, "levels") == 0)
+      c->nmChain = 0;
+  }
 
   // panels
   std::vector<Pnl> hp(n);
@@ -3936,12 +4136,30 @@ static void l2pPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
                      c->leafStart, c->leafCount, B.lec, B.far, c->n, c->nmLeaf, c->nmMax);
 }
 
-// far field into B.far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
+// the fused downward pass (k_l2lp): its LDS, and whether the tree allows it
+static size_t l2lpLds(const fbh_ctx *c, size_t elem) {
+  return 9 * (size_t)c->nmChain * elem + (size_t)(c->nmChain + 1 + c->hConvStart[c->nmChain]) * sizeof(int);
+}
 template <typename R>
-static void farPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
+static void l2lpPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
+  hipLaunchKernelGGL(k_l2lp<R>, dim3(c->nLeaf), dim3(256), l2lpLds(c, sizeof(R)), st, B.Q0, B.Q1, c->leafStart,
+                     c->leafCount, c->parent, c->cubeLevelD, c->ordMomD, c->cubeX, B.lec, B.far, c->n, c->nmLeaf,
+                     c->nmMax, c->nmChain, c->height, c->depth, c->T);
+}
+
+// far field into B.far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P.
+// fused: the downward pass as one launch (k_l2lp) that leaves the ancestors'
+// locals as M2L wrote them; otherwise level by level, every cube's locals
+// accumulated in place as src/fmm.c:787-806 leaves them (the stage API)
+template <typename R>
+static void farPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st, bool fused = true) {
   q2mPass(c, B, x, st);
   for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, st);
   m2lPass(c, B, st);
+  if (fused && c->nmChain > 0) {
+    l2lpPass(c, B, st);
+    return;
+  }
   for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, B, lev, st);
   l2pPass(c, B, st);
 }
@@ -3987,7 +4205,13 @@ static int matvecT(fbh_ctx *c, const OpBufs<R> &B, double alpha, const double *x
   }
   HCK(hipEventRecord(c->evFork, c->stream));
   HCK(hipStreamWaitEvent(c->side, c->evFork, 0));
+  const bool timed = c->ntOn && 2 * (size_t)c->ntUsed + 1 < c->ntEv.size();
+  if (timed) HCK(hipEventRecord(c->ntEv[2 * c->ntUsed], c->side));
   nearPass(c, B, x, c->side);
+  if (timed) {
+    HCK(hipEventRecord(c->ntEv[2 * c->ntUsed + 1], c->side));
+    c->ntKind[c->ntUsed++] = sizeof(R) == 4;
+  }
   HCK(hipEventRecord(c->evJoin, c->side));
   farPass(c, B, x, c->stream);
   HCK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
@@ -4141,18 +4365,18 @@ static int enableFly64(fbh_ctx *c) {
 extern "C" int fbh_enable_fp32(fbh_ctx *c) {
   if (!c) { setErr("fbh_enable_fp32: null context"); return 0; }
   if (c->fp32) return 1;
-  if (!c->m2lG) { setErr("fbh_enable_fp32: the FP32 operator needs the cached M2L tensors"); return 0; }
+  if (!c->m2lG && c->nM2L > 0) { setErr("fbh_enable_fp32: the FP32 operator needs the cached M2L tensors"); return 0; }
   const size_t n = c->n, nq = n * c->nmLeaf, ne = (size_t)c->nCube * c->nmMax, ng = (size_t)c->nM2L * 2 * c->nmMax;
   if (!(c->Q0f = dalloc<float>(c, nq)) || !(c->Q1f = dalloc<float>(c, nq)) || !(c->momf = dalloc<float>(c, 2 * ne)) ||
       !(c->lecf = dalloc<float>(c, 4 * ne)) || !(c->farf = dalloc<float>(c, 2 * n)) ||
-      !(c->nearf = dalloc<float>(c, 2 * n)) || !(c->m2lGf = dalloc<float>(c, ng)))
+      !(c->nearf = dalloc<float>(c, 2 * n)) || (c->m2lG && !(c->m2lGf = dalloc<float>(c, ng))))
     return 0;
   auto cvt = [&](const double *in, float *out, size_t cnt) {
     hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, c->stream, in, out, (long long)cnt);
   };
   cvt(c->Q0, c->Q0f, nq);
   cvt(c->Q1, c->Q1f, nq);
-  cvt(c->m2lG, c->m2lGf, ng);
+  if (c->m2lG) cvt(c->m2lG, c->m2lGf, ng);   // no far field (no M2L pairs): the locals stay zero
   // the streamed tiles read FP32 copies of their stored coefficients, the
   // others compute in FP32 on the fly
   c->fly = c->nFlyTiles > 0;
@@ -4167,8 +4391,8 @@ extern "C" int fbh_enable_fp32(fbh_ctx *c) {
       return 0;
     hipLaunchKernelGGL(k_fly_panels, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->pnl, (int)n, c->Q.t[0],
                        c->flyPA, c->flyPN, c->flyPL);
-    hipLaunchKernelGGL(k_spec_vals<false>, dim3(n), dim3(64), 0, c->stream, c->pnl, c->specOff, c->specCol,
-                       c->specVal, (double4 *)nullptr, c->kappa, c->eps, c->Q, c->qorder);
+    hipLaunchKernelGGL(k_spec_vals32, dim3(n), dim3(64), 0, c->stream, c->pnl, c->flyPA, c->flyPN, c->flyPL,
+                       c->specOff, c->specCol, c->specVal, c->kappa, c->eps, c->Q, c->qorder);
   }
   HCK(hipMemsetAsync(c->momf, 0, 2 * ne * sizeof(float), c->stream));
   HCK(hipMemsetAsync(c->lecf, 0, 4 * ne * sizeof(float), c->stream));
@@ -4201,7 +4425,7 @@ static int stageApply(fbh_ctx *c, double alpha, const double *sgm, double beta, 
   size_t b = 2 * (size_t)c->n * sizeof(double);
   HCK(hipMemcpyAsync(c->xs, sgm, b, hipMemcpyHostToDevice, c->stream));
   HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
-  if (useFar) farPass(c, bufs64(c), c->xs, c->stream);
+  if (useFar) farPass(c, bufs64(c), c->xs, c->stream, false);
   if (useNear) nearPass(c, bufs64(c), c->xs, c->stream);
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
@@ -4835,25 +5059,48 @@ extern "C" int fbh_set_rhs_factor(fbh_ctx *c, double fac) {
   return 1;
 }
 
-extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b_dev) {
-  if (!c || nChar <= 0 || !pos || !chr || !b_dev) { setErr("fbh_rhs: bad arguments"); return 0; }
-  double *dp = nullptr, *dc = nullptr;
-  HCK(hipMalloc(&dp, 3 * (size_t)nChar * sizeof(double)));
-  HCK(hipMalloc(&dc, (size_t)nChar * sizeof(double)));
-  HCK(hipMemcpy(dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
-  HCK(hipMemcpy(dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice));
+// device copies of the charges (freed by the caller)
+static int uploadCharges(int nChar, const double *pos, const double *chr, double **dp, double **dc) {
+  *dp = *dc = nullptr;
+  if (hipMalloc(dp, 3 * (size_t)nChar * sizeof(double)) != hipSuccess ||
+      hipMalloc(dc, (size_t)nChar * sizeof(double)) != hipSuccess ||
+      hipMemcpy(*dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(*dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    if (*dp) hipFree(*dp);
+    if (*dc) hipFree(*dc);
+    setErr("uploading %d charges failed", nChar);
+    return 0;
+  }
+  return 1;
+}
+
+// the direct right-hand side (src/fabipb.c:789-798) from device copies of the
+// charges; fp32: the precision form's kernel (q = 1)
+static int rhsDirect(fbh_ctx *c, int nChar, const double *dp, const double *dc, double *b_dev, bool fp32 = false) {
   const double fac = rhsFactor(c);
-  if (c->qorder == 1)
+  if (fp32 && c->qorder == 1)
+    hipLaunchKernelGGL(k_rhs32_q1, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc,
+                       b_dev, fac, c->Q.t[0], c->Q.t[0] * c->Q.w[0] * c->Q.w[0]);
+  else if (c->qorder == 1)
     hipLaunchKernelGGL(k_rhs_q1, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc,
                        b_dev, fac, c->Q.t[0], c->Q.t[0] * c->Q.w[0] * c->Q.w[0]);
   else
     hipLaunchKernelGGL(k_rhs, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->pnl, c->n, nChar, dp, dc, b_dev,
                        fac, c->Q, c->qorder);
-  hipError_t e = hipStreamSynchronize(c->stream);
-  hipFree(dp);
-  hipFree(dc);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) { setErr("k_rhs failed: %s", hipGetErrorString(e)); return 0; }
   return 1;
+}
+
+extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b_dev) {
+  if (!c || nChar <= 0 || !pos || !chr || !b_dev) { setErr("fbh_rhs: bad arguments"); return 0; }
+  double *dp = nullptr, *dc = nullptr;
+  if (!uploadCharges(nChar, pos, chr, &dp, &dc)) return 0;
+  const int ok = rhsDirect(c, nChar, dp, dc, b_dev);
+  hipFree(dp);
+  hipFree(dc);
+  return ok;
 }
 
 // ------------------------------------------------ differentiation form
@@ -4872,20 +5119,6 @@ static int devSum(fbh_ctx *c, const double *a, int m, double *out) {
   return 1;
 }
 
-// device copies of the charges (freed by the caller)
-static int uploadCharges(int nChar, const double *pos, const double *chr, double **dp, double **dc) {
-  *dp = *dc = nullptr;
-  if (hipMalloc(dp, 3 * (size_t)nChar * sizeof(double)) != hipSuccess ||
-      hipMalloc(dc, (size_t)nChar * sizeof(double)) != hipSuccess ||
-      hipMemcpy(*dp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(*dc, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-    if (*dp) hipFree(*dp);
-    if (*dc) hipFree(*dc);
-    setErr("uploading %d charges failed", nChar);
-    return 0;
-  }
-  return 1;
-}
 
 extern "C" int fbh_energy_direct_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr,
                                      const double *sgm_dev, double *g_dev, double *energy_raw) {
@@ -5130,17 +5363,24 @@ extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *c
 // the charge-tree walks: mode 0 the panel-tree energy of sgm_dev into
 // *energy_raw, mode 1 the tree right-hand side into b_dev, mode 2 the energy's
 // weight vector (dE/dsgm) into b_dev
-static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, const double *chr, int nc,
-                          const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
-                          const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
-                          const int *ctOrder, double theta, const double *sgm_dev, double *b_dev,
-                          double *energy_raw) {
-  if (!c || nChar <= 0 || nc <= 0 || (mode == 0 && (!energy_raw || !sgm_dev)) || (mode != 0 && !b_dev)) {
-    setErr("fbh_energy/fbh_rhs_tree: bad arguments");
+// A charge set on the device: the charges, their charge tree (the arrays of
+// src/chargeTree.c:228-307), the walk's coefficient descriptors and its
+// scratch (moments, per-cluster polynomial tables, slice partials), in one
+// arena.  stageCharges validates every index the walks dereference and copies
+// the inputs in one transfer; walkCharges runs the moments and one walk on
+// what is staged, so a set staged once (fbh_set_charges) serves any number
+// of right-hand sides and energies without host traffic.
+static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, const double *chr, int nc,
+                        const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                        const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                        const int *ctOrder) {
+  s.ready = false;
+  if (!c || nChar <= 0 || nc <= 0 || !pos || !chr || !ctLevel || !ctX || !ctERad || !ctKidStart || !ctNKids ||
+      !ctChgStart || !ctNChgs || !ctChgIdx || !ctOrder) {
+    setErr("charge set: bad arguments");
     return 0;
   }
   const int depth = c->depth;
-  // validate the tree on the host: every index the walk dereferences
   std::vector<int> lvStart(depth + 2, -1), lvCount(depth + 1, 0);
   int nLeafChg = 0, maxOrd = 0;
   for (int i = 0; i < nc; i++) {
@@ -5197,8 +5437,8 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
             dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 1, o));
     if (o != SZ) { setErr("fbh_energy: coefficient table has %d of %d entries", o, SZ); return 0; }
   }
-  // inputs staged in one host block and copied once into a device arena kept
-  // by the context; scratch follows the inputs in the same arena
+  // inputs staged in one host block and copied once into the set's arena;
+  // scratch follows the inputs in the same arena
   struct Seg { const void *src; size_t bytes; size_t off; };
   std::vector<Seg> seg;
   size_t top = 0;
@@ -5218,48 +5458,88 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
                iPart = add(nullptr, (size_t)S * 2 * c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
                iRed = add(nullptr, RED_BLOCKS * sizeof(double)),
                iPoly = add(nullptr, (size_t)nClus * SZ * sizeof(double));
-  if (c->enArenaBytes < top) {
-    if (c->enArena) hipFree(c->enArena);
-    c->enArena = nullptr;
-    c->enArenaBytes = 0;
-    if (hipMalloc(&c->enArena, top) != hipSuccess) { setErr("fbh_energy: device arena of %zu bytes", top); return 0; }
-    c->enArenaBytes = top;
+  if (s.arenaBytes < top) {
+    if (s.arena) {
+      HCK(hipStreamSynchronize(c->stream));   // a walk on the old arena may still run
+      hipFree(s.arena);
+    }
+    s.arena = nullptr;
+    s.arenaBytes = 0;
+    if (hipMalloc(&s.arena, top) != hipSuccess) { setErr("fbh_energy: device arena of %zu bytes", top); return 0; }
+    s.arenaBytes = top;
   }
-  c->enStage.resize(inBytes);
+  s.stage.resize(inBytes);
   for (const Seg &g : seg)
-    if (g.src && g.bytes) memcpy(c->enStage.data() + g.off, g.src, g.bytes);
-  unsigned char *A = (unsigned char *)c->enArena;
-  HCK(hipMemcpyAsync(A, c->enStage.data(), inBytes, hipMemcpyHostToDevice, c->stream));
-  int *dLevel = (int *)(A + seg[iLevel].off), *dKid = (int *)(A + seg[iKid].off), *dNK = (int *)(A + seg[iNK].off),
-      *dCS = (int *)(A + seg[iCS].off), *dNC = (int *)(A + seg[iNC].off), *dCI = (int *)(A + seg[iCI].off),
-      *dOrd = (int *)(A + seg[iOrd].off);
-  double *dX = (double *)(A + seg[iX].off), *dR = (double *)(A + seg[iR].off), *dPos = (double *)(A + seg[iPos].off),
-         *dChr = (double *)(A + seg[iChr].off), *dMom = (double *)(A + seg[iMom].off),
-         *dPart = (double *)(A + seg[iPart].off), *dOut = (double *)(A + seg[iOut].off),
-         *dRed = (double *)(A + seg[iRed].off), *dPoly = (double *)(A + seg[iPoly].off);
-  const int4 *dDesc = (const int4 *)(A + seg[iDesc].off);
+    if (g.src && g.bytes) memcpy(s.stage.data() + g.off, g.src, g.bytes);
+  unsigned char *A = (unsigned char *)s.arena;
+  HCK(hipMemcpyAsync(A, s.stage.data(), inBytes, hipMemcpyHostToDevice, c->stream));
+  s.dLevel = (int *)(A + seg[iLevel].off);
+  s.dKid = (int *)(A + seg[iKid].off);
+  s.dNK = (int *)(A + seg[iNK].off);
+  s.dCS = (int *)(A + seg[iCS].off);
+  s.dNC = (int *)(A + seg[iNC].off);
+  s.dCI = (int *)(A + seg[iCI].off);
+  s.dOrd = (int *)(A + seg[iOrd].off);
+  s.dX = (double *)(A + seg[iX].off);
+  s.dR = (double *)(A + seg[iR].off);
+  s.dPos = (double *)(A + seg[iPos].off);
+  s.dChr = (double *)(A + seg[iChr].off);
+  s.dMom = (double *)(A + seg[iMom].off);
+  s.dPart = (double *)(A + seg[iPart].off);
+  s.dOut = (double *)(A + seg[iOut].off);
+  s.dRed = (double *)(A + seg[iRed].off);
+  s.dPoly = (double *)(A + seg[iPoly].off);
+  s.dDesc = (const int4 *)(A + seg[iDesc].off);
+  s.nChar = nChar;
+  s.nc = nc;
+  s.lvStart = lvStart;
+  s.lvCount = lvCount;
+  s.ord = ord;
+  s.c0 = c0;
+  s.P = P;
+  s.uni = uni;
+  s.S = S;
+  s.SZ = SZ;
+  s.nClus = nClus;
+  s.nDesc = (int)dsc.size();
+  s.ready = true;
+  return 1;
+}
 
+// the charge moments and one walk of a staged set: mode 0 the panel-tree
+// energy (src/treecode.c:927-1017) into *energy_raw, 1 the charge-tree
+// right-hand side (src/fabipb.c:633-742) into b_dev, 2 the energy's weights
+// dE/dsgm into b_dev
+static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, const double *sgm_dev, double *b_dev,
+                       double *energy_raw) {
+  if (!s.ready) { setErr("fbh_energy/fbh_rhs_tree: no charge set staged"); return 0; }
+  if ((mode == 0 && (!energy_raw || !sgm_dev)) || (mode != 0 && !b_dev)) {
+    setErr("fbh_energy/fbh_rhs_tree: bad arguments");
+    return 0;
+  }
+  const int depth = c->depth, nmMax = c->nmMax, P = s.P, S = s.S;
+  const std::vector<int> &lvStart = s.lvStart, &lvCount = s.lvCount, &ord = s.ord;
   // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416)
   if (lvCount[depth] > 0)
     hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
-                       lvStart[depth], dCS, dNC, dCI, dX, dPos, dChr, ord[depth], dMom, nmMax, c->T);
+                       lvStart[depth], s.dCS, s.dNC, s.dCI, s.dX, s.dPos, s.dChr, ord[depth], s.dMom, nmMax, c->T);
   for (int l = depth - 1; l >= c->height; l--) {
     if (lvCount[l] <= 0) continue;
     int nOut = NMOM(ord[l]);
     const size_t lds = 2 * nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut] - c->hM2mStart[0]) * sizeof(int);
-    hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l], dKid,
-                       dNK, dX, dOrd, l, dMom, nmMax, c->T);
+    hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l],
+                       s.dKid, s.dNK, s.dX, s.dOrd, l, s.dMom, nmMax, c->T);
   }
-  if (uni) {
-    hipLaunchKernelGGL(k_ct_poly, dim3(nClus), dim3(256), 0, c->stream, c0, dMom, nmMax, dDesc, (int)dsc.size(), SZ,
-                       dPoly);
+  if (s.uni) {
+    hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, c->stream, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc, s.SZ,
+                       s.dPoly);
     const int blocks = ((c->n + 63) / 64 * S + 3) / 4;
 #define EN_WAVE(PP, MM)                                                                                           \
   case PP:                                                                                                        \
     hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
-                       dLevel, dX, dR, dKid, dNK, dCS, dNC, dCI, dPos, dChr, dPoly, c0, depth, c->height, theta,  \
-                       c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder, lvCount[c->height], S,  \
-                       (MM && S == 1) ? b_dev : dPart);                                                           \
+                       s.dLevel, s.dX, s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr, s.dPoly, s.c0,     \
+                       depth, c->height, theta, c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder,       \
+                       lvCount[c->height], S, (MM && S == 1) ? b_dev : s.dPart);                                  \
     break;
     if (mode == 0) {
       switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
@@ -5270,7 +5550,7 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
     }
 #undef EN_WAVE
     if (mode != 0 && S > 1)
-      hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, c->stream, dPart, S,
+      hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, c->stream, s.dPart, S,
                          2 * (long long)c->n, b_dev);
   } else if (mode != 0) {
     setErr("fbh_rhs_tree / tree gradient: the accepted levels must share one expansion order");
@@ -5278,22 +5558,36 @@ static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, co
   } else {
     int P1 = ord[depth + 1];
     size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
-    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, dLevel, dX, dR, dKid,
-                       dNK, dCS, dNC, dCI, dPos, dChr, dOrd, dMom, nmMax, depth, c->height, theta, c->kappa, c->eps,
-                       c->Q, c->qorder, c->T, dPart);
+    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, s.dLevel, s.dX, s.dR,
+                       s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr, s.dOrd, s.dMom, nmMax, depth, c->height,
+                       theta, c->kappa, c->eps, c->Q, c->qorder, c->T, s.dPart);
   }
   hipError_t e = hipSuccess;
   if (mode == 0) {
-    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, dPart, S * c->n, dRed);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, dRed, RED_BLOCKS, dOut);
+    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, s.dPart, S * c->n, s.dRed);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, s.dRed, RED_BLOCKS, s.dOut);
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, s.dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   } else {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) { setErr("energy kernels failed: %s", hipGetErrorString(e)); return 0; }
   return 1;
+}
+
+static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, const double *chr, int nc,
+                          const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                          const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                          const int *ctOrder, double theta, const double *sgm_dev, double *b_dev,
+                          double *energy_raw) {
+  if (!c || (mode == 0 && (!energy_raw || !sgm_dev)) || (mode != 0 && !b_dev)) {
+    setErr("fbh_energy/fbh_rhs_tree: bad arguments");
+    return 0;
+  }
+  return stageCharges(c, c->csCall, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart,
+                      ctNChgs, ctChgIdx, ctOrder) &&
+         walkCharges(c, c->csCall, mode, theta, sgm_dev, b_dev, energy_raw);
 }
 
 extern "C" int fbh_energy_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
@@ -5312,6 +5606,39 @@ extern "C" int fbh_rhs_tree_dev(fbh_ctx *c, int nChar, const double *pos, const 
                         ctChgIdx, ctOrder, theta, nullptr, b_dev, nullptr);
 }
 
+// resident charge set: staged once, then right-hand sides and energies with no host traffic
+extern "C" int fbh_set_charges(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                               const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
+                               const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
+                               const int *ctOrder) {
+  if (!stageCharges(c, c->csRes, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart,
+                    ctNChgs, ctChgIdx, ctOrder))
+    return 0;
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+extern "C" int fbh_rhs_res_dev(fbh_ctx *c, double *b_dev) {
+  if (!c || !b_dev) { setErr("fbh_rhs_res_dev: bad arguments"); return 0; }
+  if (!c->csRes.ready) { setErr("fbh_rhs_res_dev: no resident charge set (fbh_set_charges)"); return 0; }
+  return rhsDirect(c, c->csRes.nChar, c->csRes.dPos, c->csRes.dChr, b_dev);
+}
+
+extern "C" int fbh_rhs32_res_dev(fbh_ctx *c, double *b_dev) {
+  if (!c || !b_dev) { setErr("fbh_rhs32_res_dev: bad arguments"); return 0; }
+  if (!c->csRes.ready) { setErr("fbh_rhs32_res_dev: no resident charge set (fbh_set_charges)"); return 0; }
+  return rhsDirect(c, c->csRes.nChar, c->csRes.dPos, c->csRes.dChr, b_dev, true);
+}
+
+extern "C" int fbh_rhs_tree_res_dev(fbh_ctx *c, double theta, double *b_dev) {
+  if (!c || !b_dev) { setErr("fbh_rhs_tree_res_dev: bad arguments"); return 0; }
+  return walkCharges(c, c->csRes, 1, theta, nullptr, b_dev, nullptr);
+}
+
+extern "C" int fbh_energy_res_dev(fbh_ctx *c, double theta, const double *sgm_dev, double *energy_raw) {
+  if (!c || !sgm_dev || !energy_raw) { setErr("fbh_energy_res_dev: bad arguments"); return 0; }
+  return walkCharges(c, c->csRes, 0, theta, sgm_dev, nullptr, energy_raw);
+}
 // per-stage device times of one operator application, stages run back to back
 // on one stream (near, Q2M, M2M, M2L, L2L, L2P, combine, total)
 template <typename R>
@@ -5362,6 +5689,34 @@ extern "C" int fbh_profile32(fbh_ctx *c, const double *x, double *y, int iters, 
   if (!c || !x || !y || !ms || iters < 1 || c->direct) { setErr("fbh_profile32: bad arguments"); return 0; }
   if (!c->fp32) { setErr("fbh_profile32: call fbh_enable_fp32 first"); return 0; }
   return profileT(c, bufs32(c), x, y, iters, ms);
+}
+
+#define NT_MAX 2048   // nearfield launches timed per fbh_near_timing(ctx, 1)
+extern "C" int fbh_near_timing(fbh_ctx *c, int on) {
+  if (!c) { setErr("fbh_near_timing: null context"); return 0; }
+  if (on && c->ntEv.empty()) {
+    c->ntEv.resize(2 * NT_MAX, nullptr);
+    c->ntKind.resize(NT_MAX, 0);
+    for (auto &e : c->ntEv) HCK(hipEventCreate(&e));
+  }
+  c->ntOn = on != 0;
+  if (on) c->ntUsed = 0;
+  return 1;
+}
+
+extern "C" int fbh_near_timing_read(fbh_ctx *c, long long *count, double *ms) {
+  if (!c || !count || !ms) { setErr("fbh_near_timing_read: bad arguments"); return 0; }
+  count[0] = count[1] = 0;
+  ms[0] = ms[1] = 0.0;
+  if (c->ntUsed == 0) return 1;
+  HCK(hipEventSynchronize(c->ntEv[2 * c->ntUsed - 1]));
+  for (int k = 0; k < c->ntUsed; k++) {
+    float t = 0.f;
+    HCK(hipEventElapsedTime(&t, c->ntEv[2 * k], c->ntEv[2 * k + 1]));
+    count[c->ntKind[k]]++;
+    ms[c->ntKind[k]] += t;
+  }
+  return 1;
 }
 
 extern "C" int fbh_stats(fbh_ctx *c, long long *out) {

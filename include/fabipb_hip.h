@@ -221,6 +221,27 @@ int fbh_rhs_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *c
                      const int *ctNChgs, const int *ctChgIdx, const int *ctOrder,
                      double theta, double *b_dev);
 
+/*
+ * Resident charge set: the charges and their charge tree (the arrays of
+ * fbh_energy_dev) validated and copied to the device once, kept by the context
+ * until the next fbh_set_charges or fbh_destroy.  The three calls after it are
+ * fbh_rhs_dev, fbh_rhs_tree_dev and fbh_energy_dev on that set with no host
+ * traffic: in the reference the charges live in host memory for the whole
+ * solve (setupRHS / setupRHSTree, src/fabipb.c:633-800, and
+ * applyPanelChargeTreeEnergy, src/treecode.c:927-1017, read them in place).
+ */
+int fbh_set_charges(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                    int nc, const int *ctLevel, const double *ctX, const double *ctERad,
+                    const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                    const int *ctNChgs, const int *ctChgIdx, const int *ctOrder);
+int fbh_rhs_res_dev(fbh_ctx *ctx, double *b_dev);
+/* the precision form's direct right-hand side of the resident set (q = 1): r in
+ * FP64, the kernel in FP32, sums in FP32 per 256 charges and FP64 across them;
+ * other quadrature orders take the FP64 kernel */
+int fbh_rhs32_res_dev(fbh_ctx *ctx, double *b_dev);
+int fbh_rhs_tree_res_dev(fbh_ctx *ctx, double theta, double *b_dev);
+int fbh_energy_res_dev(fbh_ctx *ctx, double theta, const double *sgm_dev, double *energy_raw);
+
 /* Differentiation form (no counterpart in the reference, which has no
  * adjoint; it differentiates what MtVmain, setupRHS and the panel-tree energy
  * compute: src/fabipb.c:1237-1287, 744-800, src/treecode.c:581-695).
@@ -265,6 +286,14 @@ int fbh_gradient_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const doub
 int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
 /* the same for the FP32 operator (after fbh_enable_fp32) */
 int fbh_profile32(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
+
+/* Nearfield launch timing inside ordinary work (no counterpart in the
+ * reference; it serves measurement): on = 1 clears and starts recording an
+ * event pair around each nearfield launch of every product that follows (up to
+ * 2048 launches), on = 0 stops.  fbh_near_timing_read returns the launches
+ * recorded since the start and their summed device time, [0] FP64, [1] FP32. */
+int fbh_near_timing(fbh_ctx *ctx, int on);
+int fbh_near_timing_read(fbh_ctx *ctx, long long *count, double *ms);
 
 /* sizes (out holds 13): out[0]=n, [1]=nearfield interactions, [2]=M2L pairs,
  * [3]=nMom(maxOrder), [4]=nMom(ordMom[depth]), [5]=device bytes held, [6]=M2M/L2L

@@ -196,6 +196,11 @@ def test_large_leaves_all_near(fa, meshdir, monkeypatch):
     os_.matvec_dev(xd, yb)
     assert rel(ya.cpu().numpy(), yb.cpu().numpy()) <= 1e-14
     assert rel(ya.cpu().numpy(), y) <= 1e-14
+    # the FP32 on-the-fly nearfield over the same long own-column ranges
+    op.enable_fp32()
+    y32 = torch.empty_like(xd)
+    op.matvec32_dev(xd, y32)
+    assert rel(y32.cpu().numpy(), y) <= 1e-5
 
 
 @pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
@@ -224,6 +229,25 @@ def test_m2l_paths_agree(fa, meshdir, monkeypatch, name):
     assert d[0] <= 1e-14 and d[2] <= 1e-14 and d[1] <= 1e-6
 
 
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4", "cfg2_globule"])
+def test_fused_downward_pass(fa, meshdir, monkeypatch, name):
+    """The matvec's downward pass in one launch (L2L down each leaf's ancestor chain
+    plus L2P, k_l2lp) against the per-level L2L launches and L2P: the same arithmetic
+    in the same order, so the FP64 and FP32 matvecs agree bit for bit."""
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, name)
+    monkeypatch.setenv("FABIPB_L2L", "levels")
+    ol = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
+    x = torch.from_numpy(splitmix_vec(5150, 2 * tree.n)).cuda()
+    for o in (op, ol):
+        o.enable_fp32()
+    for f, fl in ((op.matvec_dev, ol.matvec_dev), (op.matvec32_dev, ol.matvec32_dev)):
+        y, yl = torch.empty_like(x), torch.empty_like(x)
+        f(x, y)
+        fl(x, yl)
+        assert torch.equal(y, yl), rel(y.cpu().numpy(), yl.cpu().numpy())
+
+
 @pytest.mark.parametrize("name", harness_cases())
 def test_energy_walk(fa, meshdir, name):
     import torch
@@ -233,6 +257,28 @@ def test_energy_walk(fa, meshdir, name):
     s = torch.tensor(F["en_sigma"], dtype=torch.float64, device="cuda")
     e = op.energy_dev(ct, mesh.pos, mesh.chr, s)
     assert abs(e - F["en_raw"][0]) <= 1e-12 * abs(F["en_raw"][0])
+
+
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_resident_charges(fa, meshdir, name):
+    """The resident charge set (fbh_set_charges) gives what the per-call entries give
+    from host arrays, bit for bit (same kernels on the same data), on repeated calls and
+    after it is replaced by another charge set."""
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, name)
+    ct = tree.charge_tree()
+    s = torch.from_numpy(splitmix_vec(4242, 2 * tree.n)).cuda()
+    b1, b2 = torch.empty_like(s), torch.empty_like(s)
+    for q in (mesh.chr, -0.5 * mesh.chr[::-1].copy()):
+        op.set_charges(ct, mesh.pos, q)
+        for _ in range(2):
+            op.rhs_dev(mesh.pos, q, b1)
+            op.rhs_res_dev(b2)
+            assert torch.equal(b1, b2)
+            op.rhs_tree_dev(ct, mesh.pos, q, b1)
+            op.rhs_tree_res_dev(b2)
+            assert torch.equal(b1, b2)
+            assert op.energy_dev(ct, mesh.pos, q, s) == op.energy_res_dev(s)
 
 
 @pytest.mark.parametrize("name", [c for c in cli_cases() if c != "cfg2_globule"])
@@ -346,11 +392,12 @@ def test_fp32_operator(fa, meshdir, name):
         assert rel(y32, y64) <= 1e-5
 
 
-@pytest.mark.parametrize("name", ["glob4", "cfg1_sphere"])
+@pytest.mark.parametrize("name", ["glob4", "cfg1_sphere", "cfg2_globule"])
 def test_fp32_refined_solve(fa, meshdir, name):
     """Precision form: FP64-refined solve on the FP32 operator reaches the plain solve's
     residual test in FP64 and its energy within 1e-6 relative of the discrete solution
-    (an FP64 solve at tol 1e-10); the plain solve's own distance from it is printed beside."""
+    (an FP64 solve at tol 1e-10) and of the reference's own recorded 17-digit panel-tree
+    energy (its CPU solve at tol 1e-6); the plain solve's own distance is printed beside."""
     mesh, tree, op = case_op(fa, meshdir, name)
     kw = dict(kappa=FL["k"], eps1=FL["eps1"], eps2=FL["eps2"], qorder=FL["q"],
               restart=FL["a"], maxit=400, height=FL["H"], op=op, tree=tree, mesh=mesh)
@@ -364,6 +411,10 @@ def test_fp32_refined_solve(fa, meshdir, name):
           flush=True)
     assert r32["info"] == 0 and r32["resid"] <= FL["o"]
     assert err <= 1e-6
+    cli = fixture(name)["cli"]
+    err_ref = abs(r32["energy_raw"] - cli["panel_tree_raw"]) / abs(cli["panel_tree_raw"])
+    print(f"[{name}] fp32-refined energy against the reference's: {err_ref:.3e}", flush=True)
+    assert err_ref <= 1e-6
 
 
 @pytest.mark.parametrize("name", ["sph3_ball_tr", "glob4_tr"])
