@@ -118,6 +118,8 @@ def hip_lib():
         L.fbh_rhs.argtypes = [_vp, C.c_int, _dp, _dp, _dp]
         L.fbh_rhs_dev.restype = C.c_int
         L.fbh_rhs_dev.argtypes = [_vp, C.c_int, _dp, _dp, _vp]
+        L.fbh_rhs_dev_noalloc.restype = C.c_int
+        L.fbh_rhs_dev_noalloc.argtypes = [_vp, C.c_int, _dp, _dp, _vp]
         L.fbh_energy_dev.restype = C.c_int
         L.fbh_energy_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip,
                                      _ip, _ip, _ip, _ip, C.c_double, _vp, _dp]
@@ -468,6 +470,12 @@ class Operator:
         chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
         self._ck(hip_lib().fbh_rhs_dev(self.h, len(chr_), _d(pos), _d(chr_),
                                        C.c_void_p(b.data_ptr())))
+
+    def rhs_dev_noalloc(self, pos, chr_, b):
+        """rhs_dev with the charges staged in the library's static device array (no allocation)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        self._ck(hip_lib().fbh_rhs_dev_noalloc(self.h, len(chr_), _d(pos), _d(chr_), C.c_void_p(b.data_ptr())))
 
     def gmres_dev(self, b, x, restart=30, tol=1e-6, maxit=100):
         it, res, info = C.c_int(0), C.c_double(0.0), C.c_int(0)

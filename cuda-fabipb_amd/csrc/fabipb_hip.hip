@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -3770,6 +3771,7 @@ static inline int blk64(int n, int cap) {
 
 static int enableFly64(fbh_ctx *c);
 static size_t l2lpLds(const fbh_ctx *c, size_t elem);
+static int resolveScratch(int dev);
 
 extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorder, int n, const double *vtx,
                                const double *nrm, const double *area, int depth, int height, const int *ordMom,
@@ -3820,7 +3822,11 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
       if (nearSrc[k] < 0 || nearSrc[k] >= nLeaf) { setErr("fbh_create: near source %d out of range", k); return nullptr; }
   }
 
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) { setErr("fbh_create: no current device"); return nullptr; }
+  if (!resolveScratch(dev)) return nullptr;   // the no-allocation staging array of this device
   fbh_ctx *c = new fbh_ctx();
+  c->dev = dev;
   c->kappa = kappa; c->eps1 = eps1; c->eps2 = eps2; c->eps = eps2 / eps1; c->qorder = qorder;
   for (int i = 0; i < 10; i++) { c->Q.t[i] = hT[qorder][i]; c->Q.w[i] = hW[qorder][i]; }
   c->n = n; c->depth = depth; c->height = height; c->nCube = nCube; c->nLeaf = nLeaf;
@@ -5074,6 +5080,31 @@ static int uploadCharges(int nChar, const double *pos, const double *chr, double
   return 1;
 }
 
+// No-allocation staging for fbh_rhs_dev_noalloc: a file-scope device array
+// of FBH_NOALLOC_MAX_CHARGES charges (x, y, z, then the charges; 8 MiB), one
+// copy per GPU held until the process ends.  On the host `scratch` names a
+// stand-in, not the device copy, so host code never passes it to a kernel or a
+// copy: fbh_create resolves each device's address once with
+// hipGetSymbolAddress (that device current) and keeps it in gScratchDev.
+static __device__ __align__(256) double scratch[4 * FBH_NOALLOC_MAX_CHARGES];
+#define FBH_MAX_DEVICES 64
+static std::mutex gScratchMu;
+static double *gScratchDev[FBH_MAX_DEVICES];
+
+static int resolveScratch(int dev) {
+  if (dev < 0 || dev >= FBH_MAX_DEVICES) { setErr("device %d beyond the staging table", dev); return 0; }
+  std::lock_guard<std::mutex> lock(gScratchMu);
+  if (gScratchDev[dev]) return 1;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    setErr("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    return 0;
+  }
+  gScratchDev[dev] = (double *)p;
+  return 1;
+}
+
 // the direct right-hand side (src/fabipb.c:789-798) from device copies of the
 // charges; fp32: the precision form's kernel (q = 1)
 static int rhsDirect(fbh_ctx *c, int nChar, const double *dp, const double *dc, double *b_dev, bool fp32 = false) {
@@ -5101,6 +5132,22 @@ extern "C" int fbh_rhs_dev(fbh_ctx *c, int nChar, const double *pos, const doubl
   hipFree(dp);
   hipFree(dc);
   return ok;
+}
+
+// fbh_rhs_dev with the charges staged in the static array instead of device
+// memory allocated per call: no hipMalloc / hipFree.  Calls that overlap on one
+// GPU share its array and race.
+extern "C" int fbh_rhs_dev_noalloc(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b_dev) {
+  if (!c || nChar <= 0 || !pos || !chr || !b_dev) { setErr("fbh_rhs_dev_noalloc: bad arguments"); return 0; }
+  if (nChar > FBH_NOALLOC_MAX_CHARGES) {
+    setErr("fbh_rhs_dev_noalloc: %d charges exceed the static staging array's %d", nChar, FBH_NOALLOC_MAX_CHARGES);
+    return 0;
+  }
+  double *sp = c->dev >= 0 && c->dev < FBH_MAX_DEVICES ? gScratchDev[c->dev] : nullptr;
+  if (!sp) { setErr("fbh_rhs_dev_noalloc: staging array of device %d not resolved", c->dev); return 0; }
+  HCK(hipMemcpyAsync(sp, pos, 3 * (size_t)nChar * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HCK(hipMemcpyAsync(sp + 3 * (size_t)nChar, chr, (size_t)nChar * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return rhsDirect(c, nChar, sp, sp + 3 * (size_t)nChar, b_dev);
 }
 
 // ------------------------------------------------ differentiation form

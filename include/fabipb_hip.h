@@ -187,6 +187,14 @@ int fbh_gmres_refined_dev(fbh_ctx *ctx, const double *b, double *x, int restart,
  * fbh_set_rhs_factor set another */
 int fbh_rhs(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b);
 int fbh_rhs_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b_dev);
+/* fbh_rhs_dev without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The charges are staged in a static device
+ * array of FBH_NOALLOC_MAX_CHARGES charges (8 MiB, one per GPU, held until the
+ * process ends); larger sets are refused.  Calls that overlap on one GPU share
+ * that array and race. */
+#define FBH_NOALLOC_MAX_CHARGES 262144
+int fbh_rhs_dev_noalloc(fbh_ctx *ctx, int nChar, const double *pos, const double *chr, double *b_dev);
 /* the factor of every right-hand side of the context (direct, charge-tree, and the
  * R^T of the gradients): the fac argument of gpuSetupRHS / gpuChargeTreeRHS
  * (gpu_backend.h:21, 36), which src/fabipb.c:755 passes as 1/(4 pi eps1).
