@@ -289,8 +289,9 @@ int fbh_gradient_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const doub
                           const double *sgm_dev, double *lam_dev, int restart, double tol, int maxit,
                           double *dEdq, int *iters, double *resid, int *info);
 
-/* per-stage device timing over `iters` matvecs (HIP events on the context stream):
- * ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole matvec */
+/* per-stage device timing over `iters` matvecs (HIP events on the context stream,
+ * stages back to back): ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole
+ * matvec */
 int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
 /* the same for the FP32 operator (after fbh_enable_fp32) */
 int fbh_profile32(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);

@@ -229,25 +229,6 @@ def test_m2l_paths_agree(fa, meshdir, monkeypatch, name):
     assert d[0] <= 1e-14 and d[2] <= 1e-14 and d[1] <= 1e-6
 
 
-@pytest.mark.parametrize("name", ["sph3_ball", "glob4", "cfg2_globule"])
-def test_fused_downward_pass(fa, meshdir, monkeypatch, name):
-    """The matvec's downward pass in one launch (L2L down each leaf's ancestor chain
-    plus L2P, k_l2lp) against the per-level L2L launches and L2P: the same arithmetic
-    in the same order, so the FP64 and FP32 matvecs agree bit for bit."""
-    import torch
-    mesh, tree, op = case_op(fa, meshdir, name)
-    monkeypatch.setenv("FABIPB_L2L", "levels")
-    ol = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
-    x = torch.from_numpy(splitmix_vec(5150, 2 * tree.n)).cuda()
-    for o in (op, ol):
-        o.enable_fp32()
-    for f, fl in ((op.matvec_dev, ol.matvec_dev), (op.matvec32_dev, ol.matvec32_dev)):
-        y, yl = torch.empty_like(x), torch.empty_like(x)
-        f(x, y)
-        fl(x, yl)
-        assert torch.equal(y, yl), rel(y.cpu().numpy(), yl.cpu().numpy())
-
-
 @pytest.mark.parametrize("name", harness_cases())
 def test_energy_walk(fa, meshdir, name):
     import torch
