@@ -39,7 +39,7 @@ one forward solve) and the capsid-scale shell (2,621,440 panels, 1,000,000
 charges, order 8, MAC 0.7, depth 6; skipped with less than 64 GB free).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-globule] [--no-grad]
-                       [--no-capsid]
+                       [--no-capsid] [--calib-copy]
 """
 import argparse
 import hashlib
@@ -249,6 +249,16 @@ def timed_solves(torch, S, op, precision, steps, warmup):
     dt = time.perf_counter() - t0
     op.near_timing(False)
     return dt, op.near_timing_read()
+
+
+def calib_copy(torch, mib=1024):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    a = torch.ones(mib * 131072, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written per launch", file=sys.stderr, flush=True)
 
 
 def matvec_rate(torch, f, x, y, steps, warmup):
@@ -497,6 +507,7 @@ def main():
     ap.add_argument("--no-globule", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-capsid", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="a 1 GiB device copy for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU (one MI355X per box); --gpus must be 1", file=sys.stderr)
@@ -511,6 +522,8 @@ def main():
     glob = None if args.no_globule else globule(fa, torch, args, tmp)
     grad = None if args.no_grad else differentiation(fa, torch, np, tmp)
     shell = None if args.no_capsid else capsid(fa, torch, tmp)
+    if args.calib_copy:
+        calib_copy(torch)
     cpu = None if args.no_cpu else cpu_baseline(h["base"], DEPTH_LARGE, h["panels"])
     if cpu:
         cpu["gpu_fp64_matvec_per_s"] = h["matvec"]["fp64_per_s"]
