@@ -147,8 +147,11 @@ def hip_lib():
                                        _dp, _dp, _ip, _dp, _ip]
         L.fbh_gradient_tree_dev.restype = C.c_int
         L.fbh_gradient_tree_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _ip,
-                                            _ip, _ip, C.c_double, _vp, _vp, C.c_int, C.c_double, C.c_int, _dp,
-                                            _ip, _dp, _ip]
+                                            _ip, _ip, C.c_double, C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int,
+                                            _dp, _dp, _ip, _dp, _ip]
+        L.fbh_tree_adjoint_dev.restype = C.c_int
+        L.fbh_tree_adjoint_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _ip,
+                                           _ip, _ip, C.c_double, C.c_int, _vp, _dp]
         L.fbh_profile.restype = C.c_int
         L.fbh_profile.argtypes = [_vp, _vp, _vp, C.c_int, _dp]
         L.fbh_profile32.restype = C.c_int
@@ -450,20 +453,36 @@ class Operator:
             restart, tol, maxit, _d(dq), C.byref(dk), C.byref(it), C.byref(res), C.byref(info)))
         return dq, dk.value, dict(iters=it.value, resid=res.value, info=info.value)
 
-    def gradient_tree_dev(self, ct, pos, chr_, sgm, lam, theta=0.8, restart=30, tol=1e-6, maxit=100):
-        """dE/dq of the panel-tree energy by tree walks (fbh_gradient_tree_dev); lam in/out."""
+    def gradient_tree_dev(self, ct, pos, chr_, sgm, lam, theta=0.8, rhs_tree=False, restart=30, tol=1e-6,
+                          maxit=100):
+        """Exact gradient of the raw panel-tree energy (fbh_gradient_tree_dev): (dE/dq [nChar],
+        dE/dkappa, adjoint solve info); rhs_tree: the solve's right-hand side was the
+        charge-tree one; lam in/out."""
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
-        keep = {k: np.ascontiguousarray(v) for k, v in ct.items() if isinstance(v, np.ndarray)}
+        k = {q: np.ascontiguousarray(v) for q, v in ct.items() if isinstance(v, np.ndarray)}
         dq = np.zeros(len(chr_))
-        res = C.c_double(0.0)
+        dk, res = C.c_double(0.0), C.c_double(0.0)
         it, info = C.c_int(0), C.c_int(0)
         self._ck(hip_lib().fbh_gradient_tree_dev(
-            self.h, len(chr_), _d(pos), _d(chr_), len(keep["level"]), _i(keep["level"]), _d(keep["x"]),
-            _d(keep["eRad"]), _i(keep["kidStart"]), _i(keep["nKids"]), _i(keep["chgStart"]), _i(keep["nChgs"]),
-            _i(keep["chgIdx"]), _i(keep["order"]), theta, C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()),
-            restart, tol, maxit, _d(dq), C.byref(it), C.byref(res), C.byref(info)))
-        return dq, dict(iters=it.value, resid=res.value, info=info.value)
+            self.h, len(chr_), _d(pos), _d(chr_), len(k["level"]), _i(k["level"]), _d(k["x"]), _d(k["eRad"]),
+            _i(k["kidStart"]), _i(k["nKids"]), _i(k["chgStart"]), _i(k["nChgs"]), _i(k["chgIdx"]), _i(k["order"]),
+            theta, int(bool(rhs_tree)), C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()), restart, tol, maxit,
+            _d(dq), C.byref(dk), C.byref(it), C.byref(res), C.byref(info)))
+        return dq, dk.value, dict(iters=it.value, resid=res.value, info=info.value)
+
+    def tree_adjoint_dev(self, ct, pos, chr_, w, kind=0, theta=0.8):
+        """d/dq at fixed densities of the panel-tree energy of w (kind 0) or of
+        w . (charge-tree right-hand side) (kind 1), by the transposed walk."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
+        k = {q: np.ascontiguousarray(v) for q, v in ct.items() if isinstance(v, np.ndarray)}
+        dq = np.zeros(len(chr_))
+        self._ck(hip_lib().fbh_tree_adjoint_dev(
+            self.h, len(chr_), _d(pos), _d(chr_), len(k["level"]), _i(k["level"]), _d(k["x"]), _d(k["eRad"]),
+            _i(k["kidStart"]), _i(k["nKids"]), _i(k["chgStart"]), _i(k["nChgs"]), _i(k["chgIdx"]), _i(k["order"]),
+            theta, int(kind), C.c_void_p(w.data_ptr()), _d(dq)))
+        return dq
 
     def rhs_dev(self, pos, chr_, b):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
@@ -556,17 +575,25 @@ class Operator:
 
 class Problem:
     """Differentiation form: a fixed mesh and tree; E(q, kappa) through the direct
-    right-hand side, GMRES on the FP64 operator and the direct energy
-    (Operator.energy_direct_dev), and its adjoint gradient (Operator.gradient_dev).
-    Operators are built per kappa and kept (the nearfield coefficients and M2L
-    tensors depend on kappa)."""
+    right-hand side (or the charge-tree one, rhs="tree"), GMRES on the FP64 operator
+    and the panel-tree energy the plain solve reports (energy="tree", theta 0.8:
+    src/treecode.c:927-1017), or the direct energy (energy="direct": every charge
+    taken directly), and the exact adjoint gradient of that energy
+    (Operator.gradient_tree_dev or Operator.gradient_dev).  Operators are built per
+    kappa and kept (the nearfield corrections and M2L tensors depend on kappa)."""
 
     def __init__(self, mesh, depth, order=6, sep=0.8, eps1=1.0, eps2=80.0, qorder=1, height=2, tol=1e-6,
-                 restart=30, maxit=100, precond=3):
+                 restart=30, maxit=100, precond=3, energy="tree", rhs="direct", theta=0.8):
+        if energy not in ("tree", "direct") or rhs not in ("direct", "tree"):
+            raise ValueError(f"energy must be tree or direct and rhs direct or tree, not {energy!r}, {rhs!r}")
+        if energy == "direct" and rhs == "tree":
+            raise ValueError("the direct energy's gradient takes the direct right-hand side")
         self.mesh = mesh
         self.tree = Tree(mesh, depth, order=order, height=height, sep=sep)
+        self.ct = self.tree.charge_tree()
         self.eps1, self.eps2, self.qorder = eps1, eps2, qorder
         self.tol, self.restart, self.maxit, self.precond = tol, restart, maxit, precond
+        self.energy_kind, self.rhs_kind, self.theta = energy, rhs, theta
         self._ops = {}
 
     def operator(self, kappa):
@@ -578,17 +605,25 @@ class Problem:
         return self._ops[kappa]
 
     def solve(self, q, kappa):
-        """Solution u (device tensor) and the raw direct energy for charges q."""
+        """Solution u (device tensor) and the raw energy for charges q."""
         torch = _torch()
         op = self.operator(kappa)
         q = np.ascontiguousarray(q, dtype=np.float64)
         b = torch.empty(2 * self.tree.n, dtype=torch.float64, device="cuda")
-        op.rhs_dev(self.mesh.pos, q, b)
+        op.set_charges(self.ct, self.mesh.pos, q)
+        if self.rhs_kind == "tree":
+            op.rhs_tree_res_dev(b, theta=self.theta)
+        else:
+            op.rhs_res_dev(b)
         x = b.clone()
         g = op.gmres_dev(b, x, restart=self.restart, tol=self.tol, maxit=self.maxit)
         if g["info"] != 0:
             raise RuntimeError(f"forward GMRES did not converge: {g}")
-        return x, op.energy_direct_dev(self.mesh.pos, q, x), g
+        if self.energy_kind == "tree":
+            e = op.energy_res_dev(x, theta=self.theta)
+        else:
+            e = op.energy_direct_dev(self.mesh.pos, q, x)
+        return x, e, g
 
     def energy(self, q, kappa):
         """Solvation energy in kcal/mol."""
@@ -599,13 +634,21 @@ class Problem:
         its raw energy (recomputed from u when not given)."""
         torch = _torch()
         q = np.ascontiguousarray(q, dtype=np.float64)
+        op = self.operator(kappa)
         if u is None:
             u, e, _ = self.solve(q, kappa)
         elif e is None:
-            e = self.operator(kappa).energy_direct_dev(self.mesh.pos, q, u)
+            op.set_charges(self.ct, self.mesh.pos, q)
+            e = (op.energy_res_dev(u, theta=self.theta) if self.energy_kind == "tree"
+                 else op.energy_direct_dev(self.mesh.pos, q, u))
         lam = torch.zeros_like(u)
-        dq, dk, info = self.operator(kappa).gradient_dev(self.mesh.pos, q, u, lam, restart=self.restart,
-                                                         tol=self.tol, maxit=self.maxit)
+        if self.energy_kind == "tree":
+            dq, dk, info = op.gradient_tree_dev(self.ct, self.mesh.pos, q, u, lam, theta=self.theta,
+                                                rhs_tree=self.rhs_kind == "tree", restart=self.restart,
+                                                tol=self.tol, maxit=self.maxit)
+        else:
+            dq, dk, info = op.gradient_dev(self.mesh.pos, q, u, lam, restart=self.restart, tol=self.tol,
+                                           maxit=self.maxit)
         if info["info"] != 0:
             raise RuntimeError(f"adjoint GMRES did not converge: {info}")
         return e * ENERGY_SCALE, dq * ENERGY_SCALE, dk * ENERGY_SCALE

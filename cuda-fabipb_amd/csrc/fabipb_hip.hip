@@ -35,6 +35,7 @@
 //   GMRES helpers: fixed-grid deterministic dot products, MGS update,
 //                normalisation, diagonal preconditioner, x += V y
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -46,6 +47,7 @@
 #include <cstring>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/fabipb_hip.h"
@@ -2166,6 +2168,8 @@ __global__ __launch_bounds__(256) void k_energy_g(const Pnl *__restrict__ pnl, i
 // the adjoint solution lam.  Block (charge tile, panel chunk); partial sums
 // part[chunk][j] are added in chunk order by k_grad_sum.
 #define GRAD_PCH 2048
+// GP false: the right-hand side's term R^T lam alone (u is not read)
+template <bool GP = true>
 __global__ __launch_bounds__(256) void k_grad_q(const Pnl *__restrict__ pnl, int n, int nChar,
                                                 const double *__restrict__ pos, const double *__restrict__ u,
                                                 const double *__restrict__ lam, double *__restrict__ part,
@@ -2190,8 +2194,8 @@ __global__ __launch_bounds__(256) void k_grad_q(const Pnl *__restrict__ pnl, int
         v[9 + k] = P.nr[k];
       }
       const double a2 = 2.0 * P.area;
-      v[12] = a2 * u[pi];
-      v[13] = a2 * u[pi + n];
+      v[12] = GP ? a2 * u[pi] : 0.0;
+      v[13] = GP ? a2 * u[pi + n] : 0.0;
       v[14] = fac * a2 * lam[pi];
       v[15] = fac * a2 * lam[pi + n];
     }
@@ -2205,23 +2209,28 @@ __global__ __launch_bounds__(256) void k_grad_q(const Pnl *__restrict__ pnl, int
           double x[3];
           for (int k = 0; k < 3; k++) x[k] = v[k] + Q.t[ix] * (v[3 + k] + Q.t[jx] * v[6 + k]) - c[k];
           const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2), ri = 1.0 / rn;
-          const double G0 = kFourPiI * ri, ek = exp(-kappa * rn), Gk = ek * G0;
-          const double coef = (kappa * rn + 1.0) * ek;
           const double ip = v[9] * x[0] + v[10] * x[1] + v[11] * x[2];
-          const double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
-          acc += wq * ((G0 - Gk) * v[13] + (eps * dGkdn - dG0dn) * v[12] + ri * v[14] - ip * ri * ri * ri * v[15]);
+          if constexpr (GP) {
+            const double G0 = kFourPiI * ri, ek = exp(-kappa * rn), Gk = ek * G0;
+            const double coef = (kappa * rn + 1.0) * ek;
+            const double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+            acc += wq * ((G0 - Gk) * v[13] + (eps * dGkdn - dG0dn) * v[12] + ri * v[14] - ip * ri * ri * ri * v[15]);
+          } else {
+            acc += wq * (ri * v[14] - ip * ri * ri * ri * v[15]);
+          }
         }
     }
   }
   if (j < nChar) part[(long long)blockIdx.y * nChar + j] = acc;
 }
 
-__global__ void k_grad_sum(const double *__restrict__ part, int nChunk, int nChar, double *__restrict__ out) {
+__global__ void k_grad_sum(const double *__restrict__ part, int nChunk, int nChar, double *__restrict__ out,
+                           int add = 0) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nChar) return;
   double v = 0.0;
   for (int k = 0; k < nChunk; k++) v += part[(long long)k * nChar + j];
-  out[j] = v;
+  out[j] = add ? out[j] + v : v;
 }
 
 // near part of lam^T (dK/dkappa) u at q = 1: every source of the row's near
@@ -2540,6 +2549,19 @@ __device__ inline void enDirect(const double *qp, const double *nr, const double
   y1 += qj * (eps * dGkdn - dG0dn);
 }
 
+// d/dkappa of enDirect: d(G0 - Gk)/dkappa = e^{-kappa r}/(4 pi),
+// d(eps dGk/dn - dG0/dn)/dkappa = eps dG0/dn (-kappa r^2 e^{-kappa r})
+__device__ inline void enDirectDk(const double *qp, const double *nr, const double *pj, double qj, double kappa,
+                                  double eps, double &y0, double &y1) {
+  double x[3] = {qp[0] - pj[0], qp[1] - pj[1], qp[2] - pj[2]};
+  double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2);
+  double ek = exp(-kappa * rn), G0 = kFourPiI / rn;
+  double ip = nr[0] * x[0] + nr[1] * x[1] + nr[2] * x[2];
+  double dG0dn = -ip * G0 / r2;
+  y0 += qj * (kFourPiI * ek);
+  y1 += qj * (eps * dG0dn * (-kappa * r2 * ek));
+}
+
 // one charge of an unaccepted leaf in the tree right-hand side
 // (src/treecode.c:398-411): y0 += q / r, y1 += -q (n.r) / r^3
 __device__ inline void rhsDirect(const double *qp, const double *nr, const double *pj, double qj, double &y0,
@@ -2645,6 +2667,16 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, 
       h[k] = g0[k] - gk[k];
       u[k] = eps * gk[k] - g0[k];
     }
+  } else if constexpr (MODE == 3) {   // d/dkappa of the energy's kernels: d f_k / dkappa = kappa f_(k-1),
+    double g0[P + 2], gk[P + 2];      // d f_0 / dkappa = -e^{-kappa r} / (4 pi) for f = Gk; G0 is free of kappa
+    radial(kappa, rho, P + 1, g0, 1, gk, 1);
+    h[0] = rho * gk[0];
+    u[0] = -eps * rho * gk[0];
+#pragma unroll
+    for (int k = 1; k <= P + 1; k++) {
+      h[k] = -kappa * gk[k - 1];
+      u[k] = eps * kappa * gk[k - 1];
+    }
   } else {
     const double nr2 = -1.0 / (rho * rho);
     h[0] = 1.0 / rho;
@@ -2702,8 +2734,9 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, 
 }
 
 // Panel-tree energy (src/treecode.c:617-695, 581-615; MODE 0), its weight
-// vector dE/dsgm (MODE 2) and the charge-tree right-hand side
-// (src/treecode.c:381-450; MODE 1) with one lane per panel:
+// vector dE/dsgm (MODE 2), its kappa derivative at fixed sgm (MODE 3) and the
+// charge-tree right-hand side (src/treecode.c:381-450; MODE 1) with one lane
+// per panel:
 // the 64 panels of a wave (consecutive in leaf order, so close together) walk
 // the charge tree together, depth first with kids in order, carrying the mask
 // of lanes still descending.  A lane's contributions arrive in the order of its
@@ -2769,14 +2802,16 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
         const bool dir = on && !acc && lev == depth;
         const unsigned long long down = __ballot(on && !acc && lev < depth);
         if (acc)
-          enFar<P, MODE == 1 ? 1 : 0>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps, y0,
-                                      y1);
+          enFar<P, MODE == 2 ? 0 : MODE>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps,
+                                         y0, y1);
         if (__ballot(dir) != 0ull) {
           const int s = chgStart[cb], nq = nChgs[cb];
           for (int k = 0; k < nq; k++) {
             const int j = chgIdx[s + k];
             if (dir) {
-              if constexpr (MODE != 1) {
+              if constexpr (MODE == 3) {
+                enDirectDk(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
+              } else if constexpr (MODE != 1) {
                 enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
               } else {
                 rhsDirect(qp, nr, pos + 3 * j, chr[j], y0, y1);
@@ -2803,7 +2838,7 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
   o0 *= 2.0 * area;
   o1 *= 2.0 * area;
   if (live) {
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 || MODE == 3) {   // 3: dE/dkappa at fixed sgm
       part[(long long)sl * n + p] = o0 * sgm[p + n] + o1 * sgm[p];
     } else if constexpr (MODE == 2) {   // part is the energy's weight vector g[2n]: E = g . [sgm | sgm_n]
       part[(long long)sl * 2 * n + p] = o1;
@@ -2812,6 +2847,451 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
       part[(long long)sl * 2 * n + p] = eps * o0;
       part[(long long)sl * 2 * n + p + n] = eps * o1;
     }
+  }
+}
+
+// ------------------------- transposed panel-tree walk (exact dE/dq of the tree energy)
+//
+// The panel-tree energy (and the charge-tree right-hand side) is linear in
+// the charges at fixed densities: E = sum_j q_j phi_j.  A far cluster A with
+// moments M_a = sum_{j in A} q_j (x_j - c_A)^a / a! enters a panel point's sum
+// as sum_a M_a K_a(r), r = x - c_A, with
+//   K_a = (-1)^|a| [ wh d^a h(r) + wu d^a (n . grad u)(r) ],
+// h and u the two radial kernels (G0 - Gk and eps Gk - G0 for the energy,
+// 1/r twice for the right-hand side) and wh, wu the point's weights
+// (2A t w w times sgm_n, sgm; or fac 2A t w w times lam, lam_n).  The closed
+// form of the Cartesian derivatives of a radial function (k_energy_wave) gives
+// each d^a f, factorised axis by axis (adjFar).  So phi_j = sum over the clusters A above
+// and at j's leaf of sum_a Lam_A[a] (x_j - c_A)^a / a!, with
+// Lam_A[a] = sum over the points accepting A of K_a, plus the direct terms of
+// the points that reached j's leaf without accepting it.  k_adj_walk walks
+// exactly as k_energy_wave (same acceptance test, same order) and stores each
+// accepting wave's K (summed over its lanes in a fixed order) and each direct
+// wave's per-charge sums as events keyed by the cluster.  A stable sort by
+// cluster then lets one block per cluster add its events in the walk's order:
+// no atomics, and the same sums in the same order on every run.  (Adding the
+// events straight into Lam_A with atomics serialises on the top clusters,
+// which every wave accepts.)
+
+// k! (a loop, not a recursion: with constant k it folds to a constant)
+__host__ __device__ constexpr double cfactc(int k) {
+  double f = 1.0;
+  for (int i = 2; i <= k; i++) f *= i;
+  return f;
+}
+__host__ __device__ constexpr int gidx3c(int i1, int i2, int i3) {
+  return (i1 + i2 + i3 > 0 ? (i1 + i2 + i3) * (i1 + i2 + i3 + 1) * (i1 + i2 + i3 + 2) / 6 : 0) +
+         i1 * (i1 + i2 + i3 + 1) - i1 * (i1 - 1) / 2 + i2;
+}
+#define ADJ_LAYER(P) (((P) + 1) * ((P) + 2) / 2)
+// a! / ((a - 2b)! b! 2^b): the one-axis factor of the closed form
+__host__ __device__ constexpr double hcoef(int a, int b) {
+  return cfactc(a) / (cfactc(a - 2 * b) * cfactc(b) * (double)(1 << b));
+}
+template <int A, int B>
+struct HCoef {
+  static constexpr double v = hcoef(A, B);
+};
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>): a loop
+// unrolled by construction, its index a constant expression in the body
+template <class F, int... I>
+__device__ __forceinline__ void sforImpl(F &f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void sfor(F &&f) {
+  if constexpr (N > 0) sforImpl(f, std::make_integer_sequence<int, N>{});
+}
+
+// One accepting event: every lane forms K_a of its point (zero weights off the
+// event) for all |a| <= P.  With n . grad commuting with d^a,
+//   K_a = (-1)^|a| [ d^a (wh h) + sum_i n_i d^(a+e_i) (wu u) ],
+// and the closed form factorises axis by axis: with F_n[a3] = sum_b3 c(a3,b3)
+// z^(a3-2b3) f_(n+a3-b3), E_m[a2,a3] = sum_b2 c(a2,b2) y^(a2-2b2) F_(m+a2-b2)[a3]
+// and d^a f = sum_b1 c(a1,b1) x^(a1-2b1) E_(a1-b1)[a2,a3] (c = hcoef), so each
+// slab a3 needs two F rows (h with the z-neighbour of u folded in, and u) and
+// each (a2, a3) two E rows (Q: h plus the y- and z-neighbours of u; Eu: u for
+// the x-neighbour).  Every loop is unrolled at compile time (sfor) and every
+// coefficient is a constant.  A slab's (P - a3 + 1)(P - a3 + 2)/2 values (a2,
+// then a1, ascending) pass through LDS (one row of 64 lanes per value, rows 65
+// apart); the 64 lanes are summed in a fixed order (ng groups of 64 / ng
+// lanes, ng the largest power of two with ng m <= 64, one lane per group and
+// value, then the groups' sums combined by lane exchange) into out, slab
+// after slab.
+template <int P, int KIND>
+__device__ __forceinline__ void adjFar(double rx, double ry, double rz, double nx, double ny, double nz, double wh,
+                                       double wu, double kappa, double eps, double *__restrict__ out, double *sl,
+                                       int lane) {
+  const double rho = sqrt(rx * rx + ry * ry + rz * rz);
+  double fh[P + 1], fu[P + 2];   // the radial sequences of wh h and wu u
+  if constexpr (KIND == 0) {
+    double g0[P + 2], gk[P + 2];
+    radial(kappa, rho, P + 1, g0, 1, gk, 1);
+    sfor<P + 2>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      if constexpr (k <= P) fh[k] = wh * (g0[k] - gk[k]);
+      fu[k] = wu * (eps * gk[k] - g0[k]);
+    });
+  } else {
+    const double nr2 = -1.0 / (rho * rho);
+    double c = 1.0 / rho;
+    sfor<P + 2>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      if constexpr (k <= P) fh[k] = wh * c;
+      fu[k] = wu * c;
+      c = (2 * k + 1) * nr2 * c;
+    });
+  }
+  double px[P + 2], py[P + 2], pz[P + 2];
+  px[0] = py[0] = pz[0] = 1.0;
+  sfor<P + 1>([&](auto K) {
+    constexpr int k = decltype(K)::value + 1;
+    px[k] = px[k - 1] * rx;
+    py[k] = py[k - 1] * ry;
+    pz[k] = pz[k - 1] * rz;
+  });
+  int off = 0;
+  sfor<P + 1>([&](auto A3) {
+    constexpr int a3 = decltype(A3)::value;
+    double FhZ[P + 1 - a3], Fu[P + 2 - a3];
+    sfor<P + 1 - a3>([&](auto N) {
+      constexpr int n = decltype(N)::value;
+      double s = 0.0, t = 0.0;
+      sfor<a3 / 2 + 1>([&](auto B) {
+        constexpr int b = decltype(B)::value;
+        s = fma(HCoef<a3, b>::v * pz[a3 - 2 * b], fh[n + a3 - b], s);
+      });
+      sfor<(a3 + 1) / 2 + 1>([&](auto B) {
+        constexpr int b = decltype(B)::value;
+        t = fma(HCoef<a3 + 1, b>::v * pz[a3 + 1 - 2 * b], fu[n + a3 + 1 - b], t);
+      });
+      FhZ[n] = fma(nz, t, s);
+    });
+    sfor<P + 2 - a3>([&](auto N) {
+      constexpr int n = decltype(N)::value;
+      double s = 0.0;
+      sfor<a3 / 2 + 1>([&](auto B) {
+        constexpr int b = decltype(B)::value;
+        s = fma(HCoef<a3, b>::v * pz[a3 - 2 * b], fu[n + a3 - b], s);
+      });
+      Fu[n] = s;
+    });
+    sfor<P + 1 - a3>([&](auto A2) {
+      constexpr int a2 = decltype(A2)::value;
+      constexpr int k0 = a2 * (P - a3 + 1) - a2 * (a2 - 1) / 2;   // the slab index of (0, a2)
+      double Q[P + 1 - a2 - a3], Eu[P + 2 - a2 - a3];
+      sfor<P + 1 - a2 - a3>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        double s = 0.0, t = 0.0;
+        sfor<a2 / 2 + 1>([&](auto B) {
+          constexpr int b = decltype(B)::value;
+          s = fma(HCoef<a2, b>::v * py[a2 - 2 * b], FhZ[m + a2 - b], s);
+        });
+        sfor<(a2 + 1) / 2 + 1>([&](auto B) {
+          constexpr int b = decltype(B)::value;
+          t = fma(HCoef<a2 + 1, b>::v * py[a2 + 1 - 2 * b], Fu[m + a2 + 1 - b], t);
+        });
+        Q[m] = fma(ny, t, s);
+      });
+      sfor<P + 2 - a2 - a3>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        double s = 0.0;
+        sfor<a2 / 2 + 1>([&](auto B) {
+          constexpr int b = decltype(B)::value;
+          s = fma(HCoef<a2, b>::v * py[a2 - 2 * b], Fu[m + a2 - b], s);
+        });
+        Eu[m] = s;
+      });
+      sfor<P + 1 - a2 - a3>([&](auto A1) {
+        constexpr int a1 = decltype(A1)::value;
+        double s = 0.0, t = 0.0;
+        sfor<a1 / 2 + 1>([&](auto B) {
+          constexpr int b = decltype(B)::value;
+          s = fma(HCoef<a1, b>::v * px[a1 - 2 * b], Q[a1 - b], s);
+        });
+        sfor<(a1 + 1) / 2 + 1>([&](auto B) {
+          constexpr int b = decltype(B)::value;
+          t = fma(HCoef<a1 + 1, b>::v * px[a1 + 1 - 2 * b], Eu[a1 + 1 - b], t);
+        });
+        const double v = fma(nx, t, s);
+        sl[(k0 + a1) * 65 + lane] = ((a1 + a2 + a3) & 1) ? -v : v;
+      });
+    });
+    waveSync();
+    constexpr int m = (P - a3 + 1) * (P - a3 + 2) / 2;
+    constexpr int W = m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
+    const int a = lane & (W - 1), r0 = lane & ~(W - 1);   // this lane's value and first lane to add
+    double s = 0.0;
+    if (a < m) {
+      const double *row = sl + a * 65 + r0;
+#pragma unroll 8
+      for (int l = 0; l < W; l++) s += row[l];
+    }
+    sfor<6>([&](auto O) {
+      constexpr int o = 1 << decltype(O)::value;
+      if constexpr (o >= W) s += __shfl_xor(s, o, 64);
+    });
+    if (lane < m) out[off + lane] = s;
+    off += m;
+    waveSync();
+  });
+}
+
+// the multi-index of value t of an event (slabs a3 = 0..p, then a2, a1
+// ascending) as its graded index (gidx3c)
+__device__ inline int slabGidx(int p, int t) {
+  for (int a3 = 0; a3 <= p; a3++)
+    for (int a2 = 0; a2 <= p - a3; a2++) {
+      const int len = p - a3 - a2 + 1;
+      if (t < len) return gidx3c(t, a2, a3);
+      t -= len;
+    }
+  return 0;
+}
+
+// one direct charge: the kernels' values at x = point - charge (k0: G0 - Gk or
+// 1/r; k1: eps dGk/dn - dG0/dn or -n.x/r^3, n the point's normal)
+template <int KIND>
+__device__ __forceinline__ double adjDirect(const double *qp, const double *nr, const double *pj, double wh,
+                                            double wu, double kappa, double eps) {
+  const double x[3] = {qp[0] - pj[0], qp[1] - pj[1], qp[2] - pj[2]};
+  const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  const double ip = nr[0] * x[0] + nr[1] * x[1] + nr[2] * x[2];
+  if constexpr (KIND == 0) {
+    const double rn = sqrt(r2), G0 = kFourPiI / rn, ek = exp(-kappa * rn), Gk = ek * G0;
+    const double coef = (kappa * rn + 1.0) * ek, dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
+    return wh * (G0 - Gk) + wu * (eps * dGkdn - dG0dn);
+  } else {
+    const double ri = 1.0 / sqrt(r2), r3i = ri / r2;
+    return wh * ri + wu * (-ip * r3i);
+  }
+}
+
+// The events of a transposed walk: an accepting event (a wave's lanes that
+// accept one cluster at one quadrature point) carries the NMOM(P) sums over
+// those lanes of K_a; a direct event (the lanes that reach a leaf without
+// accepting it) carries one sum per charge of the leaf.  Per block of the walk:
+// cnt (3 values: accepting events, direct events, direct values) in the count
+// pass; base (3 values per block and one past the last, prefix sums of cnt)
+// places its events in the emit pass, minus the batch's first offsets bA, bD, bV.
+struct AdjEvents {
+  long long *cnt;
+  const long long *base;
+  long long bA, bD, bV;
+  int *accKey, *accIdx, *dirKey, *dirIdx, *dirOff, *err;
+  double *accVals, *dirVals;
+};
+
+// The transposed walk (see above): KIND 0 the panel-tree energy with weights
+// 2A t w w (sgm_n, sgm); KIND 1 the charge-tree right-hand side with weights
+// fac 2A t w w (lam, lam_n).  Walks exactly as k_energy_wave (same acceptance
+// test, same order); EMIT false counts the events of each block, EMIT true
+// writes them (keys = the cluster, values in the walk's order), blocks g0 ...
+template <int P, int KIND, bool EMIT>
+__global__ __launch_bounds__(64) void k_adj_walk(const Pnl *__restrict__ pnl, int n, const double *__restrict__ w,
+                                                 const int *__restrict__ level, const double *__restrict__ cx,
+                                                 const double *__restrict__ eRad,
+                                                 const int *__restrict__ kidStart, const int *__restrict__ nKids,
+                                                 const int *__restrict__ chgStart, const int *__restrict__ nChgs,
+                                                 const int *__restrict__ chgIdx, const double *__restrict__ pos,
+                                                 int c0, int depth, int height, double theta, double kappa,
+                                                 double eps, double fac, QTab Q, int q, int nH, int S, int g0,
+                                                 AdjEvents ev) {
+  __shared__ int stC[EN_WSTACK];
+  __shared__ unsigned long long stM[EN_WSTACK];
+  __shared__ double sl[EMIT ? 65 * ADJ_LAYER(P) : 1];   // value-major rows, padded against bank conflicts
+  __shared__ double pt[8 * 64];   // per lane: the quadrature point, its normal and its two weights
+  const int lane = threadIdx.x;
+  const int g = g0 + blockIdx.x, sl_ = g % S;
+  const int p = (g / S) * 64 + lane;
+  const bool live = p < n;
+  const unsigned long long all = __ballot(live);
+  long long nA = 0, nD = 0, nV = 0;   // wave-uniform event counts
+  long long eA = 0, eD = 0, eV = 0, endA = 0, endD = 0, endV = 0;
+  if constexpr (EMIT) {
+    eA = ev.base[3 * g] - ev.bA;
+    eD = ev.base[3 * g + 1] - ev.bD;
+    eV = ev.base[3 * g + 2] - ev.bV;
+    endA = ev.base[3 * g + 3] - ev.bA;
+    endD = ev.base[3 * g + 4] - ev.bD;
+    endV = ev.base[3 * g + 5] - ev.bV;
+  }
+  bool bad = false;
+  if (all != 0ull) {
+    const int pp = live ? p : n - 1;
+    for (int ix = 0; ix < q && !bad; ix++)
+      for (int jx = 0; jx < q && !bad; jx++) {
+        // this quadrature point of each lane, staged in LDS (the walk reads it
+        // back per event, so it holds no registers across the walk)
+        {
+          double qp[3];
+          for (int k = 0; k < 3; k++) qp[k] = pnl[pp].v[k] + Q.t[ix] * (pnl[pp].a[6 + k] + Q.t[jx] * pnl[pp].a[k]);
+          // the point weights: energy (sgm_n, sgm), right-hand side fac (lam, lam_n)
+          const double wq = 2.0 * pnl[pp].area * Q.t[ix] * Q.w[ix] * Q.w[jx];
+          double wa = 0.0, wb = 0.0;
+          if constexpr (EMIT) {
+            wa = live ? (KIND == 0 ? w[pp + n] : fac * w[pp]) : 0.0;
+            wb = live ? (KIND == 0 ? w[pp] : fac * w[pp + n]) : 0.0;
+          }
+          waveSync();
+          for (int k = 0; k < 3; k++) {
+            pt[k * 64 + lane] = qp[k];
+            pt[(3 + k) * 64 + lane] = pnl[pp].nr[k];
+          }
+          pt[6 * 64 + lane] = wq * wa;
+          pt[7 * 64 + lane] = wq * wb;
+          waveSync();
+        }
+        int sp = 0;
+        for (int k = sl_ + ((nH - 1 - sl_) / S) * S; k >= sl_; k -= S, sp++) {   // pushed in reverse: visited in order
+          stC[sp] = c0 + k;
+          stM[sp] = all;
+        }
+        while (sp > 0) {
+          sp--;
+          const int cb = __builtin_amdgcn_readfirstlane(stC[sp]);
+          const unsigned long long mraw = stM[sp];
+          const unsigned long long m =
+              ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(mraw >> 32)) << 32) |
+              (unsigned)__builtin_amdgcn_readfirstlane((int)(mraw & 0xffffffffu));
+          const bool on = (m >> lane) & 1ull;
+          const double rx = pt[lane] - cx[3 * cb], ry = pt[64 + lane] - cx[3 * cb + 1],
+                       rz = pt[128 + lane] - cx[3 * cb + 2];
+          const double dist = sqrt(rx * rx + ry * ry + rz * rz);
+          const int lev = level[cb];
+          const bool acc = on && eRad[cb] < theta * dist && lev >= height;
+          const bool dir = on && !acc && lev == depth;
+          const unsigned long long down = __ballot(on && !acc && lev < depth);
+          if (__ballot(acc) != 0ull) {   // every lane takes part; those off the event add zeros
+            if constexpr (EMIT) {
+              if (eA + nA >= endA) { bad = true; break; }
+              const double ox = acc ? rx : 1.0, oy = acc ? ry : 0.0, oz = acc ? rz : 0.0;
+              adjFar<P, KIND>(ox, oy, oz, pt[192 + lane], pt[256 + lane], pt[320 + lane], acc ? pt[384 + lane] : 0.0,
+                              acc ? pt[448 + lane] : 0.0, kappa, eps, ev.accVals + (eA + nA) * NMOM(P), sl, lane);
+              if (lane == 0) {
+                ev.accKey[eA + nA] = cb;
+                ev.accIdx[eA + nA] = (int)(eA + nA);
+              }
+            }
+            nA++;
+          }
+          if (__ballot(dir) != 0ull) {
+            const int s = chgStart[cb], nq = nChgs[cb];
+            if constexpr (EMIT) {
+              if (eD + nD >= endD || eV + nV + nq > endV) { bad = true; break; }
+              const double qp[3] = {pt[lane], pt[64 + lane], pt[128 + lane]};
+              const double nr[3] = {pt[192 + lane], pt[256 + lane], pt[320 + lane]};
+              const double wh = pt[384 + lane], wu = pt[448 + lane];
+              for (int k = 0; k < nq; k++) {
+                const int j = chgIdx[s + k];
+                double v = dir ? adjDirect<KIND>(qp, nr, pos + 3 * j, wh, wu, kappa, eps) : 0.0;
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) ev.dirVals[eV + nV + k] = v;
+              }
+              if (lane == 0) {
+                ev.dirKey[eD + nD] = cb;
+                ev.dirIdx[eD + nD] = (int)(eD + nD);
+                ev.dirOff[eD + nD] = (int)(eV + nV);
+              }
+            }
+            nD++;
+            nV += nq;
+          }
+          if (down != 0ull) {
+            const int nk = nKids[cb], k0 = kidStart[cb];
+            if (sp + nk > EN_WSTACK) { bad = true; break; }   // stack bound (checked on the host): stop, never fault
+            for (int k = nk - 1; k >= 0; k--) {
+              stC[sp] = k0 + k;
+              stM[sp] = down;
+              sp++;
+            }
+          }
+        }
+      }
+  }
+  if constexpr (EMIT) {
+    if (!bad && (eA + nA != endA || eD + nD != endD || eV + nV != endV)) bad = true;
+  } else if (lane == 0) {
+    ev.cnt[3 * g] = nA;
+    ev.cnt[3 * g + 1] = nD;
+    ev.cnt[3 * g + 2] = nV;
+  }
+  if (bad && lane == 0) ev.err[0] = 1;
+}
+
+// the run bounds of sorted keys: s0[k] .. s1[k] (the caller zeroes both)
+__global__ void k_seg_bounds(const int *__restrict__ key, int n, int *__restrict__ s0, int *__restrict__ s1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = key[i];
+  if (i == 0 || key[i - 1] != k) s0[k] = i;
+  if (i == n - 1 || key[i + 1] != k) s1[k] = i + 1;
+}
+
+__global__ void k_fill_int(int *__restrict__ y, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+// Lam_A += the sums of cluster A's accepting events, in event order (one block
+// per cluster; ns strided partial sums of each multi-index, then added in
+// stride order)
+__global__ __launch_bounds__(256) void k_adj_red_acc(const int *__restrict__ s0, const int *__restrict__ s1,
+                                                     const int *__restrict__ idx, const double *__restrict__ vals,
+                                                     int p, int nmMax, double *__restrict__ lam) {
+  __shared__ double part[256];
+  const int nm = NMOM(p);
+  const int A = blockIdx.x, i0 = s0[A], i1 = s1[A];
+  if (i1 <= i0) return;
+  const int ns = 256 / nm, t = threadIdx.x, a = t % nm, st = t / nm;
+  double v = 0.0;
+  if (st < ns)
+    for (int i = i0 + st; i < i1; i += ns) v += vals[(long long)idx[i] * nm + a];
+  part[t] = v;
+  __syncthreads();
+  if (t < nm) {
+    double s = part[t];
+    for (int k = 1; k < ns; k++) s += part[k * nm + t];
+    lam[(long long)A * nmMax + slabGidx(p, t)] += s;
+  }
+}
+
+// phi_j += the sums of leaf A's direct events for its charges, in event order
+__global__ __launch_bounds__(64) void k_adj_red_dir(int levStart, const int *__restrict__ s0,
+                                                    const int *__restrict__ s1, const int *__restrict__ idx,
+                                                    const int *__restrict__ off, const double *__restrict__ vals,
+                                                    const int *__restrict__ chgStart, const int *__restrict__ nChgs,
+                                                    const int *__restrict__ chgIdx, double *__restrict__ phi) {
+  const int A = levStart + blockIdx.x, i0 = s0[A], i1 = s1[A];
+  if (i1 <= i0) return;
+  for (int k = threadIdx.x; k < nChgs[A]; k += blockDim.x) {
+    double v = 0.0;
+    for (int i = i0; i < i1; i++) v += vals[off[idx[i]] + k];
+    phi[chgIdx[chgStart[A] + k]] += v;
+  }
+}
+
+// dE/dq_j = phi_j + sum over the clusters A from j's leaf up to level height of
+// sum_a Lam_A[a] (x_j - c_A)^a / a! (the moments' monomials); one thread per
+// charge of a leaf, in the leaf's charge order
+__global__ void k_adj_eval(int levStart, const int *__restrict__ chgStart, const int *__restrict__ nChgs,
+                           const int *__restrict__ chgIdx, const int *__restrict__ parent,
+                           const int *__restrict__ level, const double *__restrict__ cx,
+                           const double *__restrict__ pos, const double *__restrict__ lam, const double *phi,
+                           int height, int P, int nmMax, DevTabs T, double *__restrict__ out) {
+  const int leaf = levStart + blockIdx.x, nm = NMOM(P);
+  for (int k = threadIdx.x; k < nChgs[leaf]; k += blockDim.x) {
+    const int j = chgIdx[chgStart[leaf] + k];
+    double v = phi[j];
+    for (int A = leaf; A >= 0 && level[A] >= height; A = parent[A]) {
+      const double tr[3] = {pos[3 * j] - cx[3 * A], pos[3 * j + 1] - cx[3 * A + 1], pos[3 * j + 2] - cx[3 * A + 2]};
+      const double *L = lam + (long long)A * nmMax;
+      double s = 0.0;
+      for (int a = 0; a < nm; a++) s = fma(L[a], conVect1(tr, T.mi[a], T.ifact3[a]), s);
+      v += s;
+    }
+    out[j] = v;
   }
 }
 
@@ -2934,249 +3414,6 @@ __global__ void k_scale(const double *__restrict__ in, double *__restrict__ out,
   if (i < n) out[i] = in[i] * s;
 }
 
-// serial sum of energy partials in panel order (one thread; n is panels)
-
-// ------------------------------------ dE/dq by a walk of the panel FMM tree
-// The charge gradient of the energy at fixed solution u and adjoint lam is the
-// potential at each charge of the panel layers: the energy's kernels
-// (G0 - Gk on sgm_n, eps dGk/dn - dG0/dn on sgm) and the right-hand side's
-// Coulomb kernel transposed (fac/r on lam, fac d(1/r)/dn on lam_n).  The far
-// part comes from the panel cubes' multipole moments (Q2M and M2M of u, and
-// of (lam_n, lam) for the Coulomb layers) through per-cube polynomial tables
-// of the closed-form Cartesian derivatives, as the charge-tree walk does;
-// leaves not accepted are summed directly with the one-point rule of the
-// direct energy and right-hand side.  One lane per charge, a wave walks the
-// FMM tree together from the cubes at `height`.
-
-// leaf moments of the panel layers at their quadrature points (the one-point
-// rule of the direct energy and right-hand side, not the exact panel
-// integrals of Q2M): a point y with weight w_s (single layer) and w_d (double
-// layer, normal n) adds w_s (y-c)^a/a! to M_dpdn[a] and
-// w_d sum_k n_k (y-c)^(a-e_k)/(a-e_k)! to M_pot[a].  Weights: 2A t w w times
-// (sgm_n, sgm) or, with coul, fac (lam, lam_n).  One block per leaf cube.
-__global__ __launch_bounds__(256) void k_pt_moments(const Pnl *__restrict__ pnl, const int *__restrict__ leafStart,
-                                                    const int *__restrict__ leafCount,
-                                                    const double *__restrict__ cubeX, const double *__restrict__ x,
-                                                    int n, int nm, int nmMax, DevTabs T, QTab Q, int q, int coul,
-                                                    double fac, double *__restrict__ mom) {
-  const int L = blockIdx.x;
-  const double c0 = cubeX[3 * L], c1 = cubeX[3 * L + 1], c2 = cubeX[3 * L + 2];
-  for (int a = threadIdx.x; a < nm; a += blockDim.x) {
-    const int4 m = T.mi[a];
-    const int e[3] = {m.x & 0xff, (m.x >> 8) & 0xff, (m.x >> 16) & 0xff};
-    const double ifa = T.ifact3[a];
-    double md = 0.0, mp = 0.0;
-    for (int k = 0; k < leafCount[L]; k++) {
-      const int p = leafStart[L] + k;
-      const Pnl &P = pnl[p];
-      const double a2 = 2.0 * P.area;
-      const double ws = coul ? fac * a2 * x[p] : a2 * x[p + n], wd = coul ? fac * a2 * x[p + n] : a2 * x[p];
-      for (int ix = 0; ix < q; ix++)
-        for (int jx = 0; jx < q; jx++) {
-          const double wq = Q.t[ix] * Q.w[ix] * Q.w[jx];
-          double t[3];
-          t[0] = P.v[0] + Q.t[ix] * (P.a[6] + Q.t[jx] * P.a[0]) - c0;
-          t[1] = P.v[1] + Q.t[ix] * (P.a[7] + Q.t[jx] * P.a[1]) - c1;
-          t[2] = P.v[2] + Q.t[ix] * (P.a[8] + Q.t[jx] * P.a[2]) - c2;
-          double pw[3][10];   // powers t_d^0..t_d^9
-          for (int d = 0; d < 3; d++) {
-            pw[d][0] = 1.0;
-            for (int i = 1; i <= e[d] && i < 10; i++) pw[d][i] = pw[d][i - 1] * t[d];
-          }
-          md += ws * wq * pw[0][e[0]] * pw[1][e[1]] * pw[2][e[2]] * ifa;
-          double nmt = 0.0;   // sum_k n_k a_k t^(a-e_k) / a!
-          if (e[0] > 0) nmt += P.nr[0] * e[0] * pw[0][e[0] - 1] * pw[1][e[1]] * pw[2][e[2]];
-          if (e[1] > 0) nmt += P.nr[1] * e[1] * pw[0][e[0]] * pw[1][e[1] - 1] * pw[2][e[2]];
-          if (e[2] > 0) nmt += P.nr[2] * e[2] * pw[0][e[0]] * pw[1][e[1]] * pw[2][e[2] - 1];
-          mp += wd * wq * nmt * ifa;
-        }
-    }
-    mom[(long long)L * 2 * nmMax + a] = mp;
-    mom[(long long)L * 2 * nmMax + nmMax + a] = md;
-  }
-}
-
-// three W tables per FMM cube, from sgn-folded moments: a = M_dpdn(u), b =
-// M_pot(u), c = M_pot + M_dpdn of the Coulomb layers (one kernel for both)
-__global__ __launch_bounds__(256) void k_pc_poly(const double *__restrict__ mom, const double *__restrict__ momL,
-                                                 int nmMax, const int4 *__restrict__ desc, int nDesc, int size,
-                                                 double *__restrict__ poly) {
-  const int c = blockIdx.x;
-  const double *Mp = mom + (long long)c * 2 * nmMax, *Md = Mp + nmMax;
-  const double *Lp = momL + (long long)c * 2 * nmMax, *Ld = Lp + nmMax;
-  double *out = poly + (long long)c * 3 * size;
-  for (int e = threadIdx.x; e < nDesc; e += blockDim.x) {
-    const int4 ds = desc[e];
-    const int g[3] = {ds.x & 0xff, (ds.x >> 8) & 0xff, (ds.x >> 16) & 0xff};
-    const int j = ds.y, o = ds.w;
-    const double gsgn = ((g[0] + g[1] + g[2]) & 1) ? -1.0 : 1.0;
-    const double igf = 1.0 / (factd(g[0]) * factd(g[1]) * factd(g[2]) * (double)(1 << j));
-    double va = 0.0, vb = 0.0, vc = 0.0;
-    for (int b1 = 0; b1 <= j; b1++)
-      for (int b2 = 0; b2 <= j - b1; b2++) {
-        const int b[3] = {b1, b2, j - b1 - b2};
-        const int ap[3] = {g[0] + 2 * b[0], g[1] + 2 * b[1], g[2] + 2 * b[2]};
-        const double C = gsgn * factd(ap[0]) * factd(ap[1]) * factd(ap[2]) * igf /
-                         (factd(b[0]) * factd(b[1]) * factd(b[2]));
-        const int k = gidx3(ap[0], ap[1], ap[2]);
-        va += Md[k] * C;
-        vb += Mp[k] * C;
-        vc += (Lp[k] + Ld[k]) * C;
-      }
-    out[o] = va;
-    out[size + o] = vb;
-    out[2 * size + o] = vc;
-  }
-}
-
-// the three tables at r = x - x_cube: sum_g r^g sum_j W[g][j] h_(|g|+j), nested Horner;
-// kernels G0 - Gk (a), eps Gk - G0 (b), 1/r = fac4 G0 (c)
-template <int P>
-__device__ __forceinline__ double pcFar(const double *__restrict__ co, double rx, double ry, double rz, double kappa,
-                                        double eps, double fac4) {
-  constexpr int SZ = enPolyW(P);
-  const double rho = sqrt(rx * rx + ry * ry + rz * rz);
-  double g0[P + 2], gk[P + 2];
-  radial(kappa, rho, P, g0, 1, gk, 1);
-  double ha[P + 1], hb[P + 1], hc[P + 1];
-#pragma unroll
-  for (int k = 0; k <= P; k++) {
-    ha[k] = g0[k] - gk[k];
-    hb[k] = eps * gk[k] - g0[k];
-    hc[k] = fac4 * g0[k];
-  }
-  int o = 0;
-  double s = 0.0;
-#pragma unroll
-  for (int a = P; a >= 0; a--) {
-    double qa = 0.0;
-#pragma unroll
-    for (int b = P - a; b >= 0; b--) {
-      double rb = 0.0;
-#pragma unroll
-      for (int c = P - a - b; c >= 0; c--) {
-        const int d = a + b + c;
-        double cw = 0.0;
-#pragma unroll
-        for (int j = 0; j <= (P - d) / 2; j++)
-          cw += ha[d + j] * co[o + j] + hb[d + j] * co[SZ + o + j] + hc[d + j] * co[2 * SZ + o + j];
-        o += (P - d) / 2 + 1;
-        rb = rb * rz + cw;
-      }
-      qa = qa * ry + rb;
-    }
-    s = s * rx + qa;
-  }
-  return s;
-}
-
-template <int P>
-__global__ __launch_bounds__(256) void k_charge_walk(int nChar, const double *__restrict__ pos, int root0, int nRoot,
-                                                     const int *__restrict__ cubeLevel,
-                                                     const double *__restrict__ cubeX,
-                                                     const double *__restrict__ cubeR,
-                                                     const int *__restrict__ kidStart, const int *__restrict__ nKids,
-                                                     const int *__restrict__ leafStart,
-                                                     const int *__restrict__ leafCount, const Pnl *__restrict__ pnl,
-                                                     const double *__restrict__ u, const double *__restrict__ lam,
-                                                     const double *__restrict__ poly, int n, int depth, double theta,
-                                                     double kappa, double eps, double fac, QTab Q, int q,
-                                                     double *__restrict__ out) {
-  __shared__ int stC[4][EN_WSTACK];
-  __shared__ unsigned long long stM[4][EN_WSTACK];
-  constexpr int SZ = enPolyW(P);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int j = (blockIdx.x * 4 + w) * 64 + lane;
-  const bool live = j < nChar;
-  const unsigned long long all = __ballot(live);
-  if (all == 0ull) return;   // whole wave past the end (uniform)
-  const int jj = live ? j : nChar - 1;
-  const double cx = pos[3 * jj], cy = pos[3 * jj + 1], cz = pos[3 * jj + 2];
-  const double fac4 = 1.0 / kFourPiI;   // 1/r = fac4 * G0 (fac is in the Coulomb moments)
-  double phi = 0.0;
-  int sp = 0;
-  for (int k = nRoot - 1; k >= 0; k--, sp++) {   // the roots, pushed in reverse: visited in order
-    stC[w][sp] = root0 + k;
-    stM[w][sp] = all;
-  }
-  while (sp > 0) {
-    sp--;
-    const int cb = __builtin_amdgcn_readfirstlane(stC[w][sp]);
-    const unsigned long long mraw = stM[w][sp];
-    const unsigned long long m =
-        ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(mraw >> 32)) << 32) |
-        (unsigned)__builtin_amdgcn_readfirstlane((int)(mraw & 0xffffffffu));
-    const bool on = (m >> lane) & 1ull;
-    const double rx = cx - cubeX[3 * cb], ry = cy - cubeX[3 * cb + 1], rz = cz - cubeX[3 * cb + 2];
-    const double dist = sqrt(rx * rx + ry * ry + rz * rz);
-    const int lev = cubeLevel[cb];
-    const bool acc = on && cubeR[cb] < theta * dist;
-    const bool dir = on && !acc && lev == depth;
-    const unsigned long long down = __ballot(on && !acc && lev < depth);
-    if (acc) phi += pcFar<P>(poly + (long long)cb * 3 * SZ, rx, ry, rz, kappa, eps, fac4);
-    if (__ballot(dir) != 0ull) {
-      const int s0 = leafStart[cb], cnt = leafCount[cb];
-      for (int k = 0; k < cnt; k++) {
-        const int p = s0 + k;
-        const Pnl &Pp = pnl[p];
-        const double a2 = 2.0 * Pp.area;
-        const double ws = a2 * u[p], wn = a2 * u[p + n], ls = fac * a2 * lam[p], ln = fac * a2 * lam[p + n];
-        if (dir)
-          for (int ix = 0; ix < q; ix++)
-            for (int jx = 0; jx < q; jx++) {
-              const double wq = Q.t[ix] * Q.w[ix] * Q.w[jx];
-              double x[3];
-              for (int d = 0; d < 3; d++) x[d] = Pp.v[d] + Q.t[ix] * (Pp.a[6 + d] + Q.t[jx] * Pp.a[d]);
-              x[0] -= cx;
-              x[1] -= cy;
-              x[2] -= cz;
-              const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], rn = sqrt(r2), ri = 1.0 / rn;
-              const double G0 = kFourPiI * ri, ek = exp(-kappa * rn), Gk = ek * G0;
-              const double coef = (kappa * rn + 1.0) * ek;
-              const double ip = Pp.nr[0] * x[0] + Pp.nr[1] * x[1] + Pp.nr[2] * x[2];
-              const double dG0dn = -ip * G0 / r2, dGkdn = coef * dG0dn;
-              phi += wq * ((G0 - Gk) * wn + (eps * dGkdn - dG0dn) * ws + ri * ls - ip * ri * ri * ri * ln);
-            }
-      }
-    }
-    if (down != 0ull) {
-      const int nk = nKids[cb], k0 = kidStart[cb];
-      if (sp + nk > EN_WSTACK) {   // stack bound: poison, never fault
-        phi = __builtin_nan("");
-        break;
-      }
-      for (int k = nk - 1; k >= 0; k--) {
-        stC[w][sp] = k0 + k;
-        stM[w][sp] = down;
-        sp++;
-      }
-    }
-  }
-  if (live) out[j] = phi;
-}
-
-// bounding radius of each FMM cube's panels about its centre (vertices)
-__global__ void k_cube_radius(const Pnl *__restrict__ pnl, const int *__restrict__ first,
-                              const int *__restrict__ count, const double *__restrict__ cubeX,
-                              double *__restrict__ R) {
-  __shared__ double red[256];
-  const int c = blockIdx.x;
-  double m = 0.0;
-  for (int p = first[c] + threadIdx.x; p < first[c] + count[c]; p += blockDim.x)
-    for (int v = 0; v < 3; v++) {
-      const double dx = pnl[p].v[3 * v] - cubeX[3 * c], dy = pnl[p].v[3 * v + 1] - cubeX[3 * c + 1],
-                   dz = pnl[p].v[3 * v + 2] - cubeX[3 * c + 2];
-      m = fmax(m, sqrt(dx * dx + dy * dy + dz * dz));
-    }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) R[c] = red[0];
-}
-
 // ================================================================ host side
 
 struct DevBuf {
@@ -3194,7 +3431,7 @@ struct ChargeSet {
   bool uni = false;
   std::vector<int> lvStart, lvCount, ord;
   int *dLevel = nullptr, *dKid = nullptr, *dNK = nullptr, *dCS = nullptr, *dNC = nullptr, *dCI = nullptr,
-      *dOrd = nullptr;
+      *dOrd = nullptr, *dPar = nullptr;
   double *dX = nullptr, *dR = nullptr, *dPos = nullptr, *dChr = nullptr, *dMom = nullptr, *dPart = nullptr,
          *dOut = nullptr, *dRed = nullptr, *dPoly = nullptr;
   const int4 *dDesc = nullptr;
@@ -3286,7 +3523,6 @@ struct fbh_ctx {
   double4 *coefT = nullptr;
   double *adjL = nullptr, *adjM = nullptr;
   double4 *specDk = nullptr;   // d/dkappa corrections of the special pairs
-  double *cubeR = nullptr;     // bounding radius of each FMM cube's panels (the charge walk's MAC)
   rocblas_handle blas = nullptr;
   // dense direct operator (-r=1/2): every panel pair through the nearfield store
   bool direct = false;
@@ -5203,6 +5439,9 @@ extern "C" int fbh_energy_direct_dev(fbh_ctx *c, int nChar, const double *pos, c
   return ok;
 }
 
+static int kappaOperatorTerms(fbh_ctx *c, const double *sgm_dev, const double *lam_dev, double *part, double *eNear,
+                              double *eFar);
+
 // dE/dq and dE/dkappa of E_raw = g(q, kappa) . u with A(kappa) u = b(q):
 //   A^T lam = g,  dE/dq = G^T u + R^T lam,  dE/dkappa = (dg/dkappa) . u + lam^T (dK/dkappa) u
 // (b = R q does not depend on kappa, A = D - K with D free of kappa)
@@ -5233,15 +5472,29 @@ extern "C" int fbh_gradient_dev(fbh_ctx *c, int nChar, const double *pos, const 
   }
   if (ok) {   // dE/dq
     const double fac = rhsFactor(c);
-    hipLaunchKernelGGL(k_grad_q, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar, dp,
+    hipLaunchKernelGGL(k_grad_q<true>, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar, dp,
                        sgm_dev, lam_dev, part, c->kappa, c->eps, fac, c->Q, c->qorder);
-    hipLaunchKernelGGL(k_grad_sum, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, part, nChunk, nChar, dq);
+    hipLaunchKernelGGL(k_grad_sum, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, part, nChunk, nChar, dq, 0);
     ok = hipGetLastError() == hipSuccess &&
          hipMemcpyAsync(dEdq, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
          hipStreamSynchronize(c->stream) == hipSuccess;
     if (!ok) setErr("fbh_gradient_dev: charge gradient kernels failed");
   }
   if (ok) ok = devDot(c, gk, sgm_dev, n2, &eK);
+  if (ok) ok = kappaOperatorTerms(c, sgm_dev, lam_dev, part, &eNear, &eFar);
+  *dEdkappa = eK + eNear + eFar;
+  hipStreamSynchronize(c->stream);
+  for (double *q : {dp, dc, g, gk, part, dq})
+    if (q) hipFree(q);
+  return ok;
+}
+
+// lam^T (dK/dkappa) u, near and far parts (D is free of kappa); part: a device
+// buffer of max(n, nCube) doubles
+static int kappaOperatorTerms(fbh_ctx *c, const double *sgm_dev, const double *lam_dev, double *part, double *eNear,
+                              double *eFar) {
+  const int n = c->n;
+  int ok = 1;
   if (ok) {   // near part of lam^T dK/dkappa u
     const char *er = getenv("FABIPB_DK_RULES");   // "1": the full rules for every pair (check path)
     if (c->qorder == 1 && !(er && er[0] == '1')) {
@@ -5259,7 +5512,7 @@ extern "C" int fbh_gradient_dev(fbh_ctx *c, int nChar, const double *pos, const 
       hipLaunchKernelGGL(k_near_dk_rules, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
                          c->srcList, sgm_dev, lam_dev, part, n, c->kappa, c->eps, c->Q, c->qorder);
     }
-    ok = ok && hipGetLastError() == hipSuccess && devSum(c, part, n, &eNear);
+    ok = ok && hipGetLastError() == hipSuccess && devSum(c, part, n, eNear);
   }
   if (ok) {   // far part: moments of u, adjoint locals of lam, the kappa-differentiated M2L between them
     OpBufs<double> B = bufs64(c);
@@ -5271,144 +5524,12 @@ extern "C" int fbh_gradient_dev(fbh_ctx *c, int nChar, const double *pos, const 
     const size_t lds = (2 * (size_t)derivScratch(c->maxOrder) + 6 * (size_t)NMOM(c->maxOrder)) * sizeof(double);
     hipLaunchKernelGGL(k_m2l_dk, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
                        c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->adjL, part, c->nmMax, c->kappa, c->eps, c->T);
-    ok = hipGetLastError() == hipSuccess && devSum(c, part, c->nCube, &eFar);
+    ok = hipGetLastError() == hipSuccess && devSum(c, part, c->nCube, eFar);
     if (!ok) setErr("fbh_gradient_dev: kappa-derivative kernels failed");
   }
-  *dEdkappa = eK + eNear + eFar;
-  hipStreamSynchronize(c->stream);
-  for (double *q : {dp, dc, g, gk, part, dq})
-    if (q) hipFree(q);
   return ok;
 }
 
-static int chargeTreeWalk(fbh_ctx *c, int mode, int nChar, const double *pos, const double *chr, int nc,
-                          const int *ctLevel, const double *ctX, const double *ctERad, const int *ctKidStart,
-                          const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
-                          const int *ctOrder, double theta, const double *sgm_dev, double *b_dev,
-                          double *energy_raw);
-
-// dE/dq with tree walks instead of direct sums, for charge counts where the
-// direct panel x charge sums cost more than the solve: the adjoint's
-// right-hand side is the panel-tree energy's weight vector (the exact
-// dE/dsgm of fbh_energy_dev), and dE/dq is the potential at the charges of
-// the panel layers by the walk of the panel FMM tree (theta as the energy's).
-extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
-                                     const int *ctLevel, const double *ctX, const double *ctERad,
-                                     const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
-                                     const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
-                                     const double *sgm_dev, double *lam_dev, int restrt, double tol, int maxit,
-                                     double *dEdq, int *iters, double *resid, int *info) {
-  if (!c || nChar <= 0 || !pos || !chr || !sgm_dev || !lam_dev || !dEdq || !iters || !resid || !info ||
-      restrt < 1 || maxit < 1 || !(theta > 0.0)) {
-    setErr("fbh_gradient_tree_dev: bad arguments");
-    return 0;
-  }
-  if (!ensureAdjoint(c) || !ensureGmres(c, restrt)) return 0;
-  const int n = c->n, P = c->ordMom[c->height];
-  for (int l = c->height; l <= c->depth; l++)
-    if (c->ordMom[l] != P) { setErr("fbh_gradient_tree_dev: the FMM levels must share one expansion order"); return 0; }
-  if (P < 1 || P > 9) { setErr("fbh_gradient_tree_dev: expansion order %d outside 1..9", P); return 0; }
-  const int root0 = c->levStart[c->height], nRoot = c->levCount[c->height];
-  if (nRoot + 8 * (c->depth - c->height + 1) > EN_WSTACK) {
-    setErr("fbh_gradient_tree_dev: %d cubes at the top FMM level exceed the walk's stack", nRoot);
-    return 0;
-  }
-  if (!c->cubeR) {   // panel range of every cube (leaves, then parents from their kids), then the radii
-    std::vector<int> first(c->nCube, 0), count(c->nCube, 0);
-    for (int l = 0; l < c->nLeaf; l++) { first[l] = c->hLeafStart[l]; count[l] = c->hLeafCount[l]; }
-    for (int lev = c->depth - 1; lev >= c->height; lev--)
-      for (int cb = c->levStart[lev]; cb < c->levStart[lev] + c->levCount[lev]; cb++) {
-        const int k0 = c->hKidStart[cb], nk = c->hNKids[cb];
-        first[cb] = nk > 0 ? first[k0] : 0;
-        for (int k = 0; k < nk; k++) count[cb] += count[k0 + k];
-      }
-    int *dF = nullptr, *dC = nullptr;
-    if (!(c->cubeR = dalloc<double>(c, c->nCube)) || !(dF = dupload(c, first.data(), first.size())) ||
-        !(dC = dupload(c, count.data(), count.size())))
-      return 0;
-    hipLaunchKernelGGL(k_cube_radius, dim3(c->nCube), dim3(256), 0, c->stream, c->pnl, dF, dC, c->cubeX, c->cubeR);
-    HCK(hipGetLastError());
-  }
-  const int SZ = enPolyW(P);
-  std::vector<int4> dsc;
-  {
-    int o = 0;
-    for (int a = P; a >= 0; a--)   // the order pcFar reads them in
-      for (int b = P - a; b >= 0; b--)
-        for (int g3 = P - a - b; g3 >= 0; g3--)
-          for (int j = 0; j <= (P - a - b - g3) / 2; j++) dsc.push_back(make_int4(a | b << 8 | g3 << 16, j, 0, o++));
-    if (o != SZ) { setErr("fbh_gradient_tree_dev: table has %d of %d entries", o, SZ); return 0; }
-  }
-  double *g = nullptr, *poly = nullptr, *dp = nullptr, *dc = nullptr, *dq = nullptr;
-  int4 *dDesc = nullptr;
-  int ok = hipMalloc(&g, 2 * (size_t)n * sizeof(double)) == hipSuccess &&
-           hipMalloc(&poly, (size_t)c->nCube * 3 * SZ * sizeof(double)) == hipSuccess &&
-           hipMalloc(&dq, (size_t)nChar * sizeof(double)) == hipSuccess &&
-           hipMalloc(&dDesc, dsc.size() * sizeof(int4)) == hipSuccess &&
-           hipMemcpy(dDesc, dsc.data(), dsc.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess;
-  if (!ok) setErr("fbh_gradient_tree_dev: allocation failed");
-  // the adjoint: A^T lam = dE/dsgm of the panel-tree energy
-  if (ok)
-    ok = chargeTreeWalk(c, 2, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
-                        ctChgIdx, ctOrder, theta, nullptr, g, nullptr) &&
-         gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info);
-  // lanes take the charges in the charge tree's leaf order (ctChgIdx, a
-  // permutation of all charges grouped by leaf), so the 64 charges of a wave
-  // are close together and walk nearly the same cubes; in input order a wave's
-  // charges may lie anywhere and the wave walks the union of their paths
-  std::vector<int> perm(nChar);
-  {
-    std::vector<char> seen(nChar, 0);
-    bool isPerm = ctChgIdx != nullptr;
-    for (int j = 0; j < nChar && isPerm; j++) {
-      const int k = ctChgIdx[j];
-      isPerm = k >= 0 && k < nChar && !seen[k];
-      if (isPerm) seen[k] = 1;
-    }
-    for (int j = 0; j < nChar; j++) perm[j] = isPerm ? ctChgIdx[j] : j;
-  }
-  std::vector<double> posP(3 * (size_t)nChar), chrP(nChar);
-  for (int j = 0; j < nChar; j++) {
-    for (int d = 0; d < 3; d++) posP[3 * (size_t)j + d] = pos[3 * (size_t)perm[j] + d];
-    chrP[j] = chr[perm[j]];
-  }
-  if (ok) ok = uploadCharges(nChar, posP.data(), chrP.data(), &dp, &dc);
-  if (ok) {   // moments of u and of (lam_n, lam), tables, walk
-    OpBufs<double> B = bufs64(c);
-    const double fac = rhsFactor(c);
-    hipLaunchKernelGGL(k_pt_moments, dim3(c->nLeaf), dim3(256), 0, c->stream, c->pnl, c->leafStart, c->leafCount,
-                       c->cubeX, sgm_dev, n, NMOM(P), c->nmMax, c->T, c->Q, c->qorder, 0, fac, c->mom);
-    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
-    hipLaunchKernelGGL(k_pt_moments, dim3(c->nLeaf), dim3(256), 0, c->stream, c->pnl, c->leafStart, c->leafCount,
-                       c->cubeX, (const double *)lam_dev, n, NMOM(P), c->nmMax, c->T, c->Q, c->qorder, 1, fac,
-                       c->adjL);
-    B.mom = c->adjL;
-    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
-    hipLaunchKernelGGL(k_pc_poly, dim3(c->nCube), dim3(256), 0, c->stream, c->mom, c->adjL, c->nmMax, dDesc,
-                       (int)dsc.size(), SZ, poly);
-    const int blocks = ((nChar + 63) / 64 + 3) / 4;
-#define CH_WALK(PP)                                                                                                \
-  case PP:                                                                                                         \
-    hipLaunchKernelGGL(k_charge_walk<PP>, dim3(blocks), dim3(256), 0, c->stream, nChar, dp, root0, nRoot,          \
-                       c->cubeLevelD, c->cubeX, c->cubeR, c->kidStart, c->nKids, c->leafStart, c->leafCount, c->pnl, \
-                       sgm_dev, lam_dev, poly, n, c->depth, theta, c->kappa, c->eps, fac, c->Q, c->qorder, dq);     \
-    break;
-    switch (P) { CH_WALK(1) CH_WALK(2) CH_WALK(3) CH_WALK(4) CH_WALK(5) CH_WALK(6) CH_WALK(7) CH_WALK(8) CH_WALK(9) }
-#undef CH_WALK
-    std::vector<double> dqP(nChar);
-    ok = hipGetLastError() == hipSuccess &&
-         hipMemcpyAsync(dqP.data(), dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream) ==
-             hipSuccess &&
-         hipStreamSynchronize(c->stream) == hipSuccess;
-    if (ok)
-      for (int j = 0; j < nChar; j++) dEdq[perm[j]] = dqP[j];
-    if (!ok) setErr("fbh_gradient_tree_dev: walk kernels failed");
-  }
-  hipStreamSynchronize(c->stream);
-  for (void *q : {(void *)g, (void *)poly, (void *)dp, (void *)dc, (void *)dq, (void *)dDesc})
-    if (q) hipFree(q);
-  return ok;
-}
 
 extern "C" int fbh_rhs(fbh_ctx *c, int nChar, const double *pos, const double *chr, double *b) {
   if (!fbh_rhs_dev(c, nChar, pos, chr, c->ys)) return 0;
@@ -5463,6 +5584,10 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
     if (ord[l] < 0 || ord[l] > c->maxOrder) { setErr("fbh_energy: charge-tree order %d beyond %d", ord[l], c->maxOrder); return 0; }
   }
   ord[depth + 1] = maxOrd + 1;
+  std::vector<int> par(nc, -1);   // each cluster's parent (the transposed walk climbs from the leaves)
+  for (int i = 0; i < nc; i++)
+    if (ctLevel[i] < depth)
+      for (int k = 0; k < ctNKids[i]; k++) par[ctKidStart[i] + k] = i;
   const int nmMax = c->nmMax;
   // one expansion order over the levels the walk can accept: the wave walk
   // with per-cluster polynomial tables; otherwise the wave-per-panel walk
@@ -5508,7 +5633,8 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
                iNC = add(ctNChgs, nc * sizeof(int)), iCI = add(ctChgIdx, nChar * sizeof(int)),
                iOrd = add(ord.data(), ord.size() * sizeof(int)), iX = add(ctX, 3 * (size_t)nc * sizeof(double)),
                iR = add(ctERad, nc * sizeof(double)), iPos = add(pos, 3 * (size_t)nChar * sizeof(double)),
-               iChr = add(chr, nChar * sizeof(double)), iDesc = add(dsc.data(), dsc.size() * sizeof(int4));
+               iChr = add(chr, nChar * sizeof(double)), iDesc = add(dsc.data(), dsc.size() * sizeof(int4)),
+               iPar = add(par.data(), nc * sizeof(int));
   const size_t inBytes = top;
   const size_t iMom = add(nullptr, (size_t)nc * nmMax * sizeof(double)),
                iPart = add(nullptr, (size_t)S * 2 * c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
@@ -5536,6 +5662,7 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
   s.dNC = (int *)(A + seg[iNC].off);
   s.dCI = (int *)(A + seg[iCI].off);
   s.dOrd = (int *)(A + seg[iOrd].off);
+  s.dPar = (int *)(A + seg[iPar].off);
   s.dX = (double *)(A + seg[iX].off);
   s.dR = (double *)(A + seg[iR].off);
   s.dPos = (double *)(A + seg[iPos].off);
@@ -5565,11 +5692,13 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
 // the charge moments and one walk of a staged set: mode 0 the panel-tree
 // energy (src/treecode.c:927-1017) into *energy_raw, 1 the charge-tree
 // right-hand side (src/fabipb.c:633-742) into b_dev, 2 the energy's weights
-// dE/dsgm into b_dev
+// dE/dsgm into b_dev, 3 the energy's kappa derivative at fixed sgm into
+// *energy_raw
 static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, const double *sgm_dev, double *b_dev,
                        double *energy_raw) {
   if (!s.ready) { setErr("fbh_energy/fbh_rhs_tree: no charge set staged"); return 0; }
-  if ((mode == 0 && (!energy_raw || !sgm_dev)) || (mode != 0 && !b_dev)) {
+  const bool scalar = mode == 0 || mode == 3;
+  if ((scalar && (!energy_raw || !sgm_dev)) || (!scalar && !b_dev)) {
     setErr("fbh_energy/fbh_rhs_tree: bad arguments");
     return 0;
   }
@@ -5599,13 +5728,15 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
     break;
     if (mode == 0) {
       switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
+    } else if (mode == 3) {
+      switch (P) { EN_WAVE(1, 3) EN_WAVE(2, 3) EN_WAVE(3, 3) EN_WAVE(4, 3) EN_WAVE(5, 3) EN_WAVE(6, 3) EN_WAVE(7, 3) EN_WAVE(8, 3) EN_WAVE(9, 3) }
     } else if (mode == 2) {
       switch (P) { EN_WAVE(1, 2) EN_WAVE(2, 2) EN_WAVE(3, 2) EN_WAVE(4, 2) EN_WAVE(5, 2) EN_WAVE(6, 2) EN_WAVE(7, 2) EN_WAVE(8, 2) EN_WAVE(9, 2) }
     } else {
       switch (P) { EN_WAVE(1, 1) EN_WAVE(2, 1) EN_WAVE(3, 1) EN_WAVE(4, 1) EN_WAVE(5, 1) EN_WAVE(6, 1) EN_WAVE(7, 1) EN_WAVE(8, 1) EN_WAVE(9, 1) }
     }
 #undef EN_WAVE
-    if (mode != 0 && S > 1)
+    if ((mode == 1 || mode == 2) && S > 1)
       hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, c->stream, s.dPart, S,
                          2 * (long long)c->n, b_dev);
   } else if (mode != 0) {
@@ -5619,7 +5750,7 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
                        theta, c->kappa, c->eps, c->Q, c->qorder, c->T, s.dPart);
   }
   hipError_t e = hipSuccess;
-  if (mode == 0) {
+  if (mode == 0 || mode == 3) {
     hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, s.dPart, S * c->n, s.dRed);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, s.dRed, RED_BLOCKS, s.dOut);
     e = hipGetLastError();
@@ -5694,6 +5825,245 @@ extern "C" int fbh_rhs_tree_res_dev(fbh_ctx *c, double theta, double *b_dev) {
 extern "C" int fbh_energy_res_dev(fbh_ctx *c, double theta, const double *sgm_dev, double *energy_raw) {
   if (!c || !sgm_dev || !energy_raw) { setErr("fbh_energy_res_dev: bad arguments"); return 0; }
   return walkCharges(c, c->csRes, 0, theta, sgm_dev, nullptr, energy_raw);
+}
+
+// ------------------------------------------ exact charge gradients of the tree forms
+// device buffers of one tree-gradient call, freed together
+struct GradBufs {
+  std::vector<void *> p;
+  ~GradBufs() {
+    for (void *q : p)
+      if (q) hipFree(q);
+  }
+  template <class T>
+  T *get(size_t count) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return (T *)q;
+  }
+};
+
+// event storage of one batch of the transposed walk
+#define ADJ_BATCH_BYTES (2ll << 30)
+
+// The transposed walk of a staged set (k_adj_walk): kind 0 adds d/dq of the
+// panel-tree energy at fixed sgm (w = sgm), kind 1 d/dq of lam . b_tree, the
+// charge-tree right-hand side transposed (w = lam), into lamBuf (nc x nmMax,
+// the accepted clusters' local sums) and phi (the charges' direct terms).
+// A count pass sizes each block's events; the blocks then run in batches of at
+// most ADJ_BATCH_BYTES of events: emit, sort by cluster, add per cluster.
+static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const double *w, double *lamBuf,
+                   double *phi) {
+  if (!s.uni) { setErr("tree gradient: the accepted levels must share one expansion order"); return 0; }
+  // slices: each wave's walk is split over S waves (one per share of the
+  // clusters at level height), enough for ~16k waves in flight
+  const int nWave = (c->n + 63) / 64, nH = s.lvCount[c->height];
+  const int S = std::max(s.S, std::min(nH, (16384 + nWave - 1) / nWave));
+  const int blocks = nWave * S, nm = NMOM(s.P), nc = s.nc;
+  const double fac = rhsFactor(c);
+  GradBufs B;
+  AdjEvents ev{};
+  ev.cnt = B.get<long long>(3 * (size_t)blocks + 3);
+  ev.err = B.get<int>(1);
+  if (!ev.cnt || !ev.err) { setErr("tree gradient: allocation failed"); return 0; }
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, ev.err, 1, 0);
+#define ADJ_WALK(PP, KK, EE, G, G0)                                                                                \
+  case PP:                                                                                                         \
+    hipLaunchKernelGGL((k_adj_walk<PP, KK, EE>), dim3(G), dim3(64), 0, c->stream, c->pnl, c->n, w, s.dLevel, s.dX,  \
+                       s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.c0, c->depth, c->height, theta, c->kappa, \
+                       c->eps, fac, c->Q, c->qorder, nH, S, G0, ev);                                                \
+    break;
+#define ADJ_SWITCH(KK, EE, G, G0)                                                                                  \
+  switch (s.P) {                                                                                                   \
+    ADJ_WALK(1, KK, EE, G, G0) ADJ_WALK(2, KK, EE, G, G0) ADJ_WALK(3, KK, EE, G, G0) ADJ_WALK(4, KK, EE, G, G0)    \
+    ADJ_WALK(5, KK, EE, G, G0) ADJ_WALK(6, KK, EE, G, G0) ADJ_WALK(7, KK, EE, G, G0) ADJ_WALK(8, KK, EE, G, G0)    \
+    ADJ_WALK(9, KK, EE, G, G0)                                                                                     \
+  }
+  if (kind == 0) {
+    ADJ_SWITCH(0, false, blocks, 0)
+  } else {
+    ADJ_SWITCH(1, false, blocks, 0)
+  }
+  HCK(hipGetLastError());
+  std::vector<long long> cnt(3 * (size_t)blocks), base(3 * (size_t)blocks + 3, 0);
+  HCK(hipMemcpyAsync(cnt.data(), ev.cnt, cnt.size() * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  for (int g = 0; g < blocks; g++)
+    for (int k = 0; k < 3; k++) base[3 * (g + 1) + k] = base[3 * g + k] + cnt[3 * g + k];
+  // batches of whole blocks: accepting events nm values and four ints, direct events three ints,
+  // direct values one double
+  auto bytesOf = [&](int g0, int g1) {
+    const long long a = base[3 * g1] - base[3 * g0], d = base[3 * g1 + 1] - base[3 * g0 + 1],
+                    v = base[3 * g1 + 2] - base[3 * g0 + 2];
+    return a * (8ll * nm + 16) + d * 16 + v * 8;
+  };
+  std::vector<int> cut{0};
+  while (cut.back() < blocks) {
+    int g = cut.back() + 1;
+    while (g < blocks && bytesOf(cut.back(), g + 1) <= ADJ_BATCH_BYTES) g++;
+    cut.push_back(g);
+  }
+  long long mA = 1, mD = 1, mV = 1;
+  for (size_t i = 0; i + 1 < cut.size(); i++) {
+    mA = std::max(mA, base[3 * cut[i + 1]] - base[3 * cut[i]]);
+    mD = std::max(mD, base[3 * cut[i + 1] + 1] - base[3 * cut[i] + 1]);
+    mV = std::max(mV, base[3 * cut[i + 1] + 2] - base[3 * cut[i] + 2]);
+  }
+  if (mA > INT32_MAX || mD > INT32_MAX || mV > INT32_MAX) { setErr("tree gradient: too many events in one block"); return 0; }
+  long long *dBase = B.get<long long>(base.size());
+  ev.accKey = B.get<int>(mA);
+  ev.accIdx = B.get<int>(mA);
+  ev.accVals = B.get<double>((size_t)mA * nm);
+  ev.dirKey = B.get<int>(mD);
+  ev.dirIdx = B.get<int>(mD);
+  ev.dirOff = B.get<int>(mD);
+  ev.dirVals = B.get<double>(mV);
+  int *keyS = B.get<int>(std::max(mA, mD)), *idxS = B.get<int>(std::max(mA, mD)), *s0 = B.get<int>(nc),
+      *s1 = B.get<int>(nc);
+  int bits = 1;
+  while ((1ll << bits) < nc) bits++;
+  size_t tmpA = 0, tmpD = 0;
+  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, ev.accKey, keyS, ev.accIdx, idxS, (int)mA, 0, bits, c->stream));
+  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpD, ev.dirKey, keyS, ev.dirIdx, idxS, (int)mD, 0, bits, c->stream));
+  size_t tmpBytes = std::max(tmpA, tmpD);
+  void *tmp = B.get<unsigned char>(tmpBytes);
+  if (!dBase || !ev.accKey || !ev.accIdx || !ev.accVals || !ev.dirKey || !ev.dirIdx || !ev.dirOff || !ev.dirVals ||
+      !keyS || !idxS || !s0 || !s1 || !tmp) {
+    setErr("tree gradient: event buffers (%lld accepting, %lld direct) allocation failed", mA, mD);
+    return 0;
+  }
+  HCK(hipMemcpyAsync(dBase, base.data(), base.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  ev.base = dBase;
+  const int depth = c->depth;
+  for (size_t i = 0; i + 1 < cut.size(); i++) {
+    const int g0 = cut[i], g1 = cut[i + 1];
+    ev.bA = base[3 * g0];
+    ev.bD = base[3 * g0 + 1];
+    ev.bV = base[3 * g0 + 2];
+    const int nA = (int)(base[3 * g1] - ev.bA), nD = (int)(base[3 * g1 + 1] - ev.bD);
+    if (kind == 0) {
+      ADJ_SWITCH(0, true, g1 - g0, g0)
+    } else {
+      ADJ_SWITCH(1, true, g1 - g0, g0)
+    }
+    HCK(hipGetLastError());
+    if (nA > 0) {
+      size_t tb = tmpBytes;
+      HCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ev.accKey, keyS, ev.accIdx, idxS, nA, 0, bits, c->stream));
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s0, nc, 0);
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s1, nc, 0);
+      hipLaunchKernelGGL(k_seg_bounds, dim3((nA + 255) / 256), dim3(256), 0, c->stream, keyS, nA, s0, s1);
+      hipLaunchKernelGGL(k_adj_red_acc, dim3(nc), dim3(256), 0, c->stream, s0, s1, idxS, ev.accVals, s.P, c->nmMax,
+                         lamBuf);
+    }
+    if (nD > 0 && s.lvCount[depth] > 0) {
+      size_t tb = tmpBytes;
+      HCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ev.dirKey, keyS, ev.dirIdx, idxS, nD, 0, bits, c->stream));
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s0, nc, 0);
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s1, nc, 0);
+      hipLaunchKernelGGL(k_seg_bounds, dim3((nD + 255) / 256), dim3(256), 0, c->stream, keyS, nD, s0, s1);
+      hipLaunchKernelGGL(k_adj_red_dir, dim3(s.lvCount[depth]), dim3(64), 0, c->stream, s.lvStart[depth], s0, s1,
+                         idxS, ev.dirOff, ev.dirVals, s.dCS, s.dNC, s.dCI, phi);
+    }
+    HCK(hipGetLastError());
+  }
+#undef ADJ_SWITCH
+#undef ADJ_WALK
+  int err = 0;
+  HCK(hipMemcpyAsync(&err, ev.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));   // the buffers are freed on return
+  if (err) { setErr("tree gradient: the transposed walk's passes disagree or its stack overflowed"); return 0; }
+  return 1;
+}
+
+// dq (device, nChar) = phi + the clusters' local sums at the charges
+static int adjEval(fbh_ctx *c, const ChargeSet &s, const double *lamBuf, const double *phi, double *dq) {
+  const int depth = c->depth;
+  if (s.lvCount[depth] > 0)
+    hipLaunchKernelGGL(k_adj_eval, dim3(s.lvCount[depth]), dim3(64), 0, c->stream, s.lvStart[depth], s.dCS, s.dNC,
+                       s.dCI, s.dPar, s.dLevel, s.dX, s.dPos, lamBuf, phi, c->height, s.P, c->nmMax, c->T, dq);
+  HCK(hipGetLastError());
+  return 1;
+}
+
+extern "C" int fbh_tree_adjoint_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                                    const int *ctLevel, const double *ctX, const double *ctERad,
+                                    const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                                    const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
+                                    int kind, const double *w_dev, double *dq_host) {
+  if (!c || !w_dev || !dq_host || (kind != 0 && kind != 1)) { setErr("fbh_tree_adjoint_dev: bad arguments"); return 0; }
+  ChargeSet &s = c->csCall;
+  if (!stageCharges(c, s, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
+                    ctChgIdx, ctOrder))
+    return 0;
+  GradBufs B;
+  double *lamBuf = B.get<double>((size_t)nc * c->nmMax), *phi = B.get<double>(nChar), *dq = B.get<double>(nChar);
+  if (!lamBuf || !phi || !dq) { setErr("fbh_tree_adjoint_dev: allocation failed"); return 0; }
+  hipLaunchKernelGGL(k_fill, dim3((nc * c->nmMax + 255) / 256), dim3(256), 0, c->stream, lamBuf, nc * c->nmMax, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, phi, nChar, 0.0);
+  if (!adjWalk(c, s, kind, theta, w_dev, lamBuf, phi) || !adjEval(c, s, lamBuf, phi, dq)) return 0;
+  HCK(hipMemcpyAsync(dq_host, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 1;
+}
+
+// The exact gradient of the panel-tree energy the plain solve reports
+// (src/treecode.c:581-695, 927-1017): E_raw = W(q, kappa) . u with A u = b(q),
+// b the direct right-hand side (rhsTree 0) or the charge-tree one (1).
+//   A^T lam = W q            (the energy's weights: the walk's mode 2)
+//   dE/dq = (transposed energy walk of u) + R^T lam   (R direct sums, or the
+//           transposed charge-tree walk of lam)
+//   dE/dkappa = dE/dkappa at fixed u (the walk's mode 3) + lam^T (dK/dkappa) u
+extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, const double *chr, int nc,
+                                     const int *ctLevel, const double *ctX, const double *ctERad,
+                                     const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                                     const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
+                                     int rhsTree, const double *sgm_dev, double *lam_dev, int restrt, double tol,
+                                     int maxit, double *dEdq, double *dEdkappa, int *iters, double *resid,
+                                     int *info) {
+  if (!c || !sgm_dev || !lam_dev || !dEdq || !dEdkappa || !iters || !resid || !info || restrt < 1 || maxit < 1 ||
+      !(theta > 0.0)) {
+    setErr("fbh_gradient_tree_dev: bad arguments");
+    return 0;
+  }
+  if (!ensureAdjoint(c) || !ensureGmres(c, restrt)) return 0;
+  ChargeSet &s = c->csCall;
+  if (!stageCharges(c, s, nChar, pos, chr, nc, ctLevel, ctX, ctERad, ctKidStart, ctNKids, ctChgStart, ctNChgs,
+                    ctChgIdx, ctOrder))
+    return 0;
+  const int n = c->n, nChunk = (n + GRAD_PCH - 1) / GRAD_PCH;
+  GradBufs B;
+  const size_t partN = std::max<size_t>(rhsTree ? 0 : (size_t)nChunk * nChar, std::max(n, c->nCube));
+  double *g = B.get<double>(2 * (size_t)n), *lamBuf = B.get<double>((size_t)nc * c->nmMax),
+         *phi = B.get<double>(nChar), *dq = B.get<double>(nChar), *part = B.get<double>(partN);
+  if (!g || !lamBuf || !phi || !dq || !part) { setErr("fbh_gradient_tree_dev: allocation failed"); return 0; }
+  // the adjoint: A^T lam = dE/dsgm
+  if (!walkCharges(c, s, 2, theta, nullptr, g, nullptr) ||
+      !gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info))
+    return 0;
+  // dE/dq
+  hipLaunchKernelGGL(k_fill, dim3((nc * c->nmMax + 255) / 256), dim3(256), 0, c->stream, lamBuf, nc * c->nmMax, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, phi, nChar, 0.0);
+  if (!adjWalk(c, s, 0, theta, sgm_dev, lamBuf, phi)) return 0;
+  if (rhsTree && !adjWalk(c, s, 1, theta, lam_dev, lamBuf, phi)) return 0;
+  if (!adjEval(c, s, lamBuf, phi, dq)) return 0;
+  if (!rhsTree) {   // R^T lam by the direct sums
+    hipLaunchKernelGGL(k_grad_q<false>, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar,
+                       s.dPos, (const double *)nullptr, lam_dev, part, c->kappa, c->eps, rhsFactor(c), c->Q,
+                       c->qorder);
+    hipLaunchKernelGGL(k_grad_sum, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, part, nChunk, nChar, dq, 1);
+  }
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(dEdq, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  // dE/dkappa
+  double eK = 0.0, eNear = 0.0, eFar = 0.0;
+  if (!walkCharges(c, s, 3, theta, sgm_dev, nullptr, &eK) ||
+      !kappaOperatorTerms(c, sgm_dev, lam_dev, part, &eNear, &eFar))
+    return 0;
+  *dEdkappa = eK + eNear + eFar;
+  return 1;
 }
 // per-stage device times of one operator application, stages run back to back
 // on one stream (near, Q2M, M2M, M2L, L2L, L2P, combine, total)

@@ -275,19 +275,32 @@ int fbh_energy_direct_dev(fbh_ctx *ctx, int nChar, const double *pos, const doub
 int fbh_gradient_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
                      const double *sgm_dev, double *lam_dev, int restart, double tol, int maxit,
                      double *dEdq, double *dEdkappa, int *iters, double *resid, int *info);
-/* The same dE/dq with tree walks in place of the O(n * nChar) direct sums (capsid
- * scale): the adjoint's right-hand side is the weight vector of the panel-tree
- * energy (fbh_energy_dev's charge tree and theta), and dE/dq is the potential at
- * each charge of the panel layers of sgm (energy kernels) and lam (the right-hand
- * side's Coulomb kernel), from a walk of the panel FMM tree (far cubes by their
- * multipole moments, theta as given; the rest direct).  Accurate to the walks'
- * truncation; needs one expansion order over the FMM levels. */
+/* The exact gradient of the panel-tree energy the plain solve reports
+ * (applyPanelChargeTreeEnergy, src/treecode.c:927-1017, with fbh_energy_dev's
+ * charge tree and theta): E_raw = W(q, kappa) . u with A u = b(q), the
+ * right-hand side direct (rhsTree 0) or the charge-tree one (rhsTree 1, theta
+ * as the energy's).  The adjoint's right-hand side is W q (the walk's weights);
+ * dE/dq comes from the transposed walk (fbh_tree_adjoint_dev of u, plus R^T lam
+ * by direct sums or by the transposed right-hand-side walk of lam); dE/dkappa
+ * from the walk's kappa derivative at fixed u plus lam^T (dK/dkappa) u.  Needs
+ * one expansion order over the levels the walk can accept. */
 int fbh_gradient_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
                           int nc, const int *ctLevel, const double *ctX, const double *ctERad,
                           const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
                           const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
-                          const double *sgm_dev, double *lam_dev, int restart, double tol, int maxit,
-                          double *dEdq, int *iters, double *resid, int *info);
+                          int rhsTree, const double *sgm_dev, double *lam_dev, int restart, double tol,
+                          int maxit, double *dEdq, double *dEdkappa, int *iters, double *resid, int *info);
+/* The transposed walk alone: dq_host[j] = d/dq_j at fixed densities of the
+ * panel-tree energy of w_dev (kind 0, w = sgm) or of lam . (charge-tree
+ * right-hand side) (kind 1, w = lam).  Exact for the walk's own truncation:
+ * each far cluster contributes through its moments' monomials at the charge,
+ * each unaccepted leaf directly.  Its sums over panels use atomic adds, so
+ * repeated calls agree to rounding, not bit for bit. */
+int fbh_tree_adjoint_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
+                         int nc, const int *ctLevel, const double *ctX, const double *ctERad,
+                         const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
+                         const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
+                         int kind, const double *w_dev, double *dq_host);
 
 /* per-stage device timing over `iters` matvecs (HIP events on the context stream,
  * stages back to back): ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole

@@ -210,3 +210,22 @@ def test_capsid_shell_properties(fa, tmp_path):
     op.matvec_dev(x, Ax)
     op.matvec_dev(0.5 * x - 2.0 * y, Az)
     assert float((Az - (0.5 * Ax - 2.0 * Ay)).norm() / Az.norm()) <= 1e-13
+
+    # the exact tree gradient's transposed walks at full size, against forward walks of unit charges
+    q = np.ascontiguousarray(mesh.chr, dtype=np.float64)
+    u = res["fp64"][0]
+    for kind in (0, 1):
+        dq = op.tree_adjoint_dev(ct, mesh.pos, q, u, kind=kind)
+        worst = 0.0
+        for j in np.random.default_rng(8 + kind).choice(len(q), size=2, replace=False):
+            e = np.zeros_like(q)
+            e[j] = 1.0
+            if kind == 0:
+                ref = op.energy_dev(ct, mesh.pos, e, u)
+            else:
+                op.rhs_tree_dev(ct, mesh.pos, e, bd)
+                ref = float(torch.dot(u, bd))
+            worst = max(worst, abs(dq[j] - ref))
+        worst /= np.max(np.abs(dq))
+        print(f"[capsid] transposed walk (kind {kind}) vs unit charges: {worst:.2e} of max |dq|", flush=True)
+        assert worst <= 1e-11
