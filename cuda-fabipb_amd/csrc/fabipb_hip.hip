@@ -1553,7 +1553,7 @@ __global__ __launch_bounds__(64) void k_m2l(const int *__restrict__ m2lOff, cons
 __global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairDst, const int *__restrict__ m2lSrc,
                                                    const int *__restrict__ cubeLevel, const int *__restrict__ ordM2L,
                                                    const double *__restrict__ cubeX, double *__restrict__ G,
-                                                   int nmMax, double kappa, DevTabs T) {
+                                                   int nmMax, double kappa, DevTabs T, bool dk = false) {
   extern __shared__ double sm[];
   const long long pr = blockIdx.x;
   const int c = pairDst[pr], src = m2lSrc[pr];
@@ -1561,7 +1561,7 @@ __global__ __launch_bounds__(64) void k_m2l_derivs(const int *__restrict__ pairD
   double r[3];
   for (int d = 0; d < 3; d++) r[d] = cubeX[3 * c + d] - cubeX[3 * src + d];
   double *g0 = G + pr * 2 * nmMax, *gk = g0 + nmMax;
-  derivs_block(kappa, p, r, sm, sm + dsz, g0, gk, T.mi, threadIdx.x, 64);
+  derivs_block(kappa, p, r, sm, sm + dsz, g0, gk, T.mi, threadIdx.x, 64, dk);   // dk: g0 = dGk/dkappa
 }
 
 // M2L with cached derivative tensors (src/fmm.c:774-787, src/expan.c:258-290,
@@ -1722,7 +1722,9 @@ __global__ __launch_bounds__(256) void k_m2l_cached(const int *__restrict__ m2lO
 // sums the cells that hold it (a fixed list), then combines the four sums
 //   t1 = sum c1 a0, t2 = sum c1 ak, t3 = sum c2 a0, t4 = sum c2 ak
 // into the locals (TR: into the adjoint moments), as k_m2l_cached does per item.
-template <typename R, int K, int L, bool TR = false>
+// DK: G holds (dGk/dkappa, Gk) and the output is the raw pair (t1, t3), the
+// moments against dGk/dkappa (the kappa derivative of the far operator).
+template <typename R, int K, int L, bool TR = false, bool DK = false>
 __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff, const int *__restrict__ m2lSrc,
                                                   const int *__restrict__ cubeLevel, const int *__restrict__ ordM2L,
                                                   const int *__restrict__ ordMom, const R *__restrict__ G,
@@ -1816,7 +1818,7 @@ __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff
   }
   const int nmc = NMOM(ordMom[cubeLevel[c]]);
   const int *rs = redStart + gridOff[2 * p + 1];
-  constexpr int NO = TR ? 2 : 4;
+  constexpr int NO = TR || DK ? 2 : 4;
   for (int a = tid; a < nmMax; a += blockDim.x) {
     R v[NO];
 #pragma unroll
@@ -1830,7 +1832,10 @@ __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff
         t3 += q[2];
         t4 += q[3];
       }
-      if constexpr (TR) {   // x = (sgn b, sgn a): t1 = b*G0, t2 = b*Gk, t3 = a*G0, t4 = a*Gk
+      if constexpr (DK) {
+        v[0] = t1;
+        v[1] = t3;
+      } else if constexpr (TR) {   // x = (sgn b, sgn a): t1 = b*G0, t2 = b*Gk, t3 = a*G0, t4 = a*Gk
         v[0] = eps * t4 - t3 + t2 - t1;
         v[1] = t3 - t4 + t1 - t2 / eps;
       } else {
@@ -2224,6 +2229,44 @@ __global__ __launch_bounds__(256) void k_grad_q(const Pnl *__restrict__ pnl, int
   if (j < nChar) part[(long long)blockIdx.y * nChar + j] = acc;
 }
 
+// (R^T lam)_j at q = 1 (the transposed k_rhs_q1): block (charge tile, panel
+// chunk); each panel's point, normal and two weights fac 2A t w w (lam,
+// lam_n) staged once per chunk; 1/r from rsqrt; part[chunk][j] as k_grad_q
+__global__ __launch_bounds__(256) void k_gradR_q1(const Pnl *__restrict__ pnl, int n, int nChar,
+                                                  const double *__restrict__ pos, const double *__restrict__ lam,
+                                                  double *__restrict__ part, double fac, double t, double ww) {
+  __shared__ double4 sp[256], sn[256];   // (point, fac W lam), (normal, fac W lam_n)
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int p0 = blockIdx.y * GRAD_PCH, p1 = min(n, p0 + GRAD_PCH);
+  double c[3] = {0, 0, 0};
+  if (j < nChar)
+    for (int k = 0; k < 3; k++) c[k] = pos[3 * j + k];
+  double acc = 0.0;
+  for (int b0 = p0; b0 < p1; b0 += 256) {
+    __syncthreads();
+    const int pi = b0 + threadIdx.x;
+    if (pi < p1) {
+      const Pnl &P = pnl[pi];
+      double x[3];
+      for (int k = 0; k < 3; k++) x[k] = P.v[k] + t * (P.a[6 + k] + t * P.a[k]);
+      const double W = ww * 2.0 * P.area * fac;
+      sp[threadIdx.x] = make_double4(x[0], x[1], x[2], W * lam[pi]);
+      sn[threadIdx.x] = make_double4(P.nr[0], P.nr[1], P.nr[2], W * lam[pi + n]);
+    }
+    __syncthreads();
+    const int cnt = min(256, p1 - b0);
+#pragma unroll 4
+    for (int jj = 0; jj < cnt; jj++) {
+      const double4 a = sp[jj], m = sn[jj];
+      const double rx = a.x - c[0], ry = a.y - c[1], rz = a.z - c[2];
+      const double r2 = rx * rx + ry * ry + rz * rz, ri = rsqrt(r2);
+      const double ip = m.x * rx + m.y * ry + m.z * rz;
+      acc += a.w * ri - m.w * ip * (ri * ri * ri);
+    }
+  }
+  if (j < nChar) part[(long long)blockIdx.y * nChar + j] = acc;
+}
+
 __global__ void k_grad_sum(const double *__restrict__ part, int nChunk, int nChar, double *__restrict__ out,
                            int add = 0) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2381,6 +2424,22 @@ __global__ __launch_bounds__(64) void k_m2l_dk(const int *__restrict__ m2lOff, c
       acc += A1[a] * t2 - A2[a] * t4;
     }
     __syncthreads();
+  }
+  acc = waveSumT(acc);
+  if (tid == 0) part[c] = acc;
+}
+
+// the same sum from k_m2l_grid<DK>'s (t1, t3) per cube: part[c] =
+// sum_a (eps a + b)[a] t1[a] - (a + b / eps)[a] t3[a], (b, a) the adjoint locals
+__global__ __launch_bounds__(64) void k_dk_far_dot(const int *__restrict__ cubeLevel, const int *__restrict__ ordMom,
+                                                   const double *__restrict__ adjL, const double *__restrict__ t13,
+                                                   double *__restrict__ part, int nmMax, double eps) {
+  const int c = blockIdx.x, tid = threadIdx.x;
+  const int nmc = NMOM(ordMom[cubeLevel[c]]);
+  double acc = 0.0;
+  for (int i = tid; i < nmc; i += 64) {
+    const double b = adjL[(long long)c * 2 * nmMax + i], a = adjL[(long long)c * 2 * nmMax + nmMax + i];
+    acc += (eps * a + b) * t13[(long long)c * 2 * nmMax + i] - (a + b / eps) * t13[(long long)c * 2 * nmMax + nmMax + i];
   }
   acc = waveSumT(acc);
   if (tid == 0) part[c] = acc;
@@ -2653,90 +2712,133 @@ __global__ __launch_bounds__(256) void k_ct_poly(int c0, const double *__restric
 }
 
 // MODE 0: the energy's kernels G0 - Gk and eps Gk - G0 (src/treecode.c:581-615);
-// MODE 1: the right-hand side's Coulomb kernel 1/r (:283-379), both halves
+// MODE 1: the right-hand side's Coulomb kernel 1/r (:283-379), both halves;
+// MODE 3: d/dkappa of MODE 0's kernels; MODE 4: MODE 0 into (y0, y1) and
+// MODE 3 into (z0, z1) from one pass over the coefficients
 template <int P, int MODE>
-__device__ __forceinline__ void enFar(const double *__restrict__ co, double rx, double ry, double rz, double nx,
-                                      double ny, double nz, double kappa, double eps, double &y0, double &y1) {
+__device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx, double ry, double rz, double nx,
+                                      double ny, double nz, double kappa, double eps, double &y0, double &y1,
+                                      double &z0, double &z1) {
+  constexpr int NS = MODE == 4 ? 2 : 1;   // radial sequence pairs evaluated together
   const double rho = sqrt(rx * rx + ry * ry + rz * rz);
-  double h[P + 2], u[P + 2];
-  if constexpr (MODE == 0) {
+  double h[NS][P + 2], u[NS][P + 2];
+  if constexpr (MODE == 1) {
+    const double nr2 = -1.0 / (rho * rho);
+    h[0][0] = 1.0 / rho;
+#pragma unroll
+    for (int k = 0; k <= P; k++) h[0][k + 1] = (2 * k + 1) * nr2 * h[0][k];
+#pragma unroll
+    for (int k = 0; k <= P + 1; k++) u[0][k] = h[0][k];
+  } else {
     double g0[P + 2], gk[P + 2];
     radial(kappa, rho, P + 1, g0, 1, gk, 1);
+    if constexpr (MODE == 0 || MODE == 4) {
 #pragma unroll
-    for (int k = 0; k <= P + 1; k++) {
-      h[k] = g0[k] - gk[k];
-      u[k] = eps * gk[k] - g0[k];
+      for (int k = 0; k <= P + 1; k++) {
+        h[0][k] = g0[k] - gk[k];
+        u[0][k] = eps * gk[k] - g0[k];
+      }
     }
-  } else if constexpr (MODE == 3) {   // d/dkappa of the energy's kernels: d f_k / dkappa = kappa f_(k-1),
-    double g0[P + 2], gk[P + 2];      // d f_0 / dkappa = -e^{-kappa r} / (4 pi) for f = Gk; G0 is free of kappa
-    radial(kappa, rho, P + 1, g0, 1, gk, 1);
-    h[0] = rho * gk[0];
-    u[0] = -eps * rho * gk[0];
+    if constexpr (MODE == 3 || MODE == 4) {   // d f_k / dkappa = kappa f_(k-1) (d/dkappa commutes with
+      constexpr int t = NS - 1;               // (1/r) d/dr), d f_0 / dkappa = -e^{-kappa r} / (4 pi) for
+      h[t][0] = rho * gk[0];                  // f = Gk; G0 is free of kappa
+      u[t][0] = -eps * rho * gk[0];
 #pragma unroll
-    for (int k = 1; k <= P + 1; k++) {
-      h[k] = -kappa * gk[k - 1];
-      u[k] = eps * kappa * gk[k - 1];
+      for (int k = 1; k <= P + 1; k++) {
+        h[t][k] = -kappa * gk[k - 1];
+        u[t][k] = eps * kappa * gk[k - 1];
+      }
     }
-  } else {
-    const double nr2 = -1.0 / (rho * rho);
-    h[0] = 1.0 / rho;
-#pragma unroll
-    for (int k = 0; k <= P; k++) h[k + 1] = (2 * k + 1) * nr2 * h[k];
-#pragma unroll
-    for (int k = 0; k <= P + 1; k++) u[k] = h[k];
   }
   // nested Horner in x (outer), y, z (inner); a coefficient of r^g is the
-  // j-sum over the radial values of degree |g| + j
+  // j-sum over the radial values of degree |g| + j.  The coefficients are
+  // wave-uniform and come in through scalar loads, one (a, b) chunk at a time:
+  // after each chunk an empty asm takes the table pointer and the running sums,
+  // so the next chunk's loads cannot be issued before this chunk is consumed
+  // (hoisted all at once, they overflow the scalar registers and are spilled).
+  using CD = const __attribute__((address_space(4))) double;
+  CD *co = (CD *)co0;
   int o = 0;
-  double s0 = 0.0;
+  double s0[NS], s1[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) s0[t] = s1[t] = 0.0;
 #pragma unroll
   for (int a = P; a >= 0; a--) {
-    double qa = 0.0;
+    double qa[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++) qa[t] = 0.0;
 #pragma unroll
     for (int b = P - a; b >= 0; b--) {
-      double rb = 0.0;
+      double rb[NS];
+#pragma unroll
+      for (int t = 0; t < NS; t++) rb[t] = 0.0;
 #pragma unroll
       for (int c = P - a - b; c >= 0; c--) {
         const int d = a + b + c;
-        double cw = 0.0;
 #pragma unroll
-        for (int j = 0; j <= (P - d) / 2; j++) cw += h[d + j] * co[o + j];
+        for (int t = 0; t < NS; t++) {
+          double cw = 0.0;
+#pragma unroll
+          for (int j = 0; j <= (P - d) / 2; j++) cw += h[t][d + j] * co[o + j];
+          rb[t] = rb[t] * rz + cw;
+        }
         o += (P - d) / 2 + 1;
-        rb = rb * rz + cw;
       }
-      qa = qa * ry + rb;
+#pragma unroll
+      for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
+      if constexpr (NS == 1)
+        asm volatile("" : "+s"(co), "+v"(qa[0]));
+      else
+        asm volatile("" : "+s"(co), "+v"(qa[0]), "+v"(qa[1]));
     }
-    s0 = s0 * rx + qa;
+#pragma unroll
+    for (int t = 0; t < NS; t++) s0[t] = s0[t] * rx + qa[t];
   }
-  double s1 = 0.0;
 #pragma unroll
   for (int a = P + 1; a >= 0; a--) {
-    double qa = 0.0;
+    double qa[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++) qa[t] = 0.0;
 #pragma unroll
     for (int b = P + 1 - a; b >= 0; b--) {
-      double rb = 0.0;
+      double rb[NS];
+#pragma unroll
+      for (int t = 0; t < NS; t++) rb[t] = 0.0;
 #pragma unroll
       for (int c = P + 1 - a - b; c >= 0; c--) {
         const int d = a + b + c;
-        double cv = 0.0;
 #pragma unroll
-        for (int j = 0; j <= (P + 1 - d) / 2; j++)
-          cv += u[d + j] * (nx * co[o + 3 * j] + ny * co[o + 3 * j + 1] + nz * co[o + 3 * j + 2]);
+        for (int t = 0; t < NS; t++) {
+          double cv = 0.0;
+#pragma unroll
+          for (int j = 0; j <= (P + 1 - d) / 2; j++)
+            cv += u[t][d + j] * (nx * co[o + 3 * j] + ny * co[o + 3 * j + 1] + nz * co[o + 3 * j + 2]);
+          rb[t] = rb[t] * rz + cv;
+        }
         o += 3 * ((P + 1 - d) / 2 + 1);
-        rb = rb * rz + cv;
       }
-      qa = qa * ry + rb;
+#pragma unroll
+      for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
+      if constexpr (NS == 1)
+        asm volatile("" : "+s"(co), "+v"(qa[0]));
+      else
+        asm volatile("" : "+s"(co), "+v"(qa[0]), "+v"(qa[1]));
     }
-    s1 = s1 * rx + qa;
+#pragma unroll
+    for (int t = 0; t < NS; t++) s1[t] = s1[t] * rx + qa[t];
   }
-  y0 += s0;
-  y1 += s1;
+  y0 += s0[0];
+  y1 += s1[0];
+  if constexpr (NS == 2) {
+    z0 += s0[1];
+    z1 += s1[1];
+  }
 }
 
 // Panel-tree energy (src/treecode.c:617-695, 581-615; MODE 0), its weight
-// vector dE/dsgm (MODE 2), its kappa derivative at fixed sgm (MODE 3) and the
-// charge-tree right-hand side (src/treecode.c:381-450; MODE 1) with one lane
-// per panel:
+// vector dE/dsgm (MODE 2), its kappa derivative at fixed sgm (MODE 3; MODE 4:
+// MODE 2 into part and MODE 3 into part2 in one walk) and the charge-tree
+// right-hand side (src/treecode.c:381-450; MODE 1) with one lane per panel:
 // the 64 panels of a wave (consecutive in leaf order, so close together) walk
 // the charge tree together, depth first with kids in order, carrying the mask
 // of lanes still descending.  A lane's contributions arrive in the order of its
@@ -2754,7 +2856,7 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
                                                      const double *__restrict__ chr, const double *__restrict__ poly,
                                                      int c0, int depth, int height, double theta, double kappa,
                                                      double eps, QTab Q, int q, int nH, int S,
-                                                     double *__restrict__ part) {
+                                                     double *__restrict__ part, double *__restrict__ part2) {
   __shared__ int stC[4][EN_WSTACK];
   __shared__ unsigned long long stM[4][EN_WSTACK];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2776,12 +2878,12 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
   }
   const double area = pnl[pp].area;
   constexpr int SZ = enPolySize(P);
-  double o0 = 0.0, o1 = 0.0;
+  double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
   for (int ix = 0; ix < q; ix++)
     for (int jx = 0; jx < q; jx++) {
       double qp[3];
       for (int k = 0; k < 3; k++) qp[k] = v0[k] + Q.t[ix] * (e2[k] + Q.t[jx] * e0[k]);
-      double y0 = 0.0, y1 = 0.0;
+      double y0 = 0.0, y1 = 0.0, z0 = 0.0, z1 = 0.0;
       int sp = 0;
       for (int k = sl + ((nH - 1 - sl) / S) * S; k >= sl; k -= S, sp++) {   // pushed in reverse: visited in order
         stC[w][sp] = c0 + k;
@@ -2803,7 +2905,7 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
         const unsigned long long down = __ballot(on && !acc && lev < depth);
         if (acc)
           enFar<P, MODE == 2 ? 0 : MODE>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps,
-                                         y0, y1);
+                                         y0, y1, z0, z1);
         if (__ballot(dir) != 0ull) {
           const int s = chgStart[cb], nq = nChgs[cb];
           for (int k = 0; k < nq; k++) {
@@ -2811,6 +2913,9 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
             if (dir) {
               if constexpr (MODE == 3) {
                 enDirectDk(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
+              } else if constexpr (MODE == 4) {
+                enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
+                enDirectDk(qp, nr, pos + 3 * j, chr[j], kappa, eps, z0, z1);
               } else if constexpr (MODE != 1) {
                 enDirect(qp, nr, pos + 3 * j, chr[j], kappa, eps, y0, y1);
               } else {
@@ -2822,7 +2927,7 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
         if (down != 0ull) {
           const int nk = nKids[cb], k0 = kidStart[cb];
           if (sp + nk > EN_WSTACK) {   // stack bound: poison, never fault
-            y0 = y1 = __builtin_nan("");
+            y0 = y1 = z0 = z1 = __builtin_nan("");
             break;
           }
           for (int k = nk - 1; k >= 0; k--) {   // pushed in reverse: kids visited in order
@@ -2834,15 +2939,22 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
       }
       o0 += y0 * Q.t[ix] * Q.w[ix] * Q.w[jx];
       o1 += y1 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+      if constexpr (MODE == 4) {
+        o2 += z0 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+        o3 += z1 * Q.t[ix] * Q.w[ix] * Q.w[jx];
+      }
     }
   o0 *= 2.0 * area;
   o1 *= 2.0 * area;
+  o2 *= 2.0 * area;
+  o3 *= 2.0 * area;
   if (live) {
     if constexpr (MODE == 0 || MODE == 3) {   // 3: dE/dkappa at fixed sgm
       part[(long long)sl * n + p] = o0 * sgm[p + n] + o1 * sgm[p];
-    } else if constexpr (MODE == 2) {   // part is the energy's weight vector g[2n]: E = g . [sgm | sgm_n]
+    } else if constexpr (MODE == 2 || MODE == 4) {   // part is the energy's weight vector g[2n]: E = g . [sgm | sgm_n]
       part[(long long)sl * 2 * n + p] = o1;
       part[(long long)sl * 2 * n + p + n] = o0;
+      if constexpr (MODE == 4) part2[(long long)sl * n + p] = o2 * sgm[p + n] + o3 * sgm[p];   // and MODE 3's terms
     } else {   // part is the right-hand side b[2n], fac = 1/(4 pi eps1) passed as eps
       part[(long long)sl * 2 * n + p] = eps * o0;
       part[(long long)sl * 2 * n + p + n] = eps * o1;
@@ -3465,6 +3577,7 @@ struct fbh_ctx {
   int nTiles;
   long long nLinks, nM2L;
   double *m2lG;   // per pair D^a G0, D^a Gk (2*nmMax)
+  double *m2lGdk = nullptr;   // per pair D^a dGk/dkappa, D^a Gk (the gradient's kappa term; built on first use)
   int *m2lDst;
   DevTabs T;    // contraction table: ep = (a+e1, a+e2, a+e3, sign)
   DevTabs Tm;   // moment table: ep = (a-e1, a-e2, a-e3)
@@ -4237,24 +4350,24 @@ static void m2lLaunch(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
                      c->planTerms, c->planIdx, c->planOff);
 }
 
-template <typename R, int K, int L, bool TR = false>
+template <typename R, int K, int L, bool TR = false, bool DK = false>
 static void m2lGridLaunch(fbh_ctx *c, const R *G, const R *mom, R *lec, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_m2l_grid<R, K, L, TR>), dim3(c->nCube), dim3(256), lds, st, c->m2lOff, c->m2lSrc,
+  hipLaunchKernelGGL((k_m2l_grid<R, K, L, TR, DK>), dim3(c->nCube), dim3(256), lds, st, c->m2lOff, c->m2lSrc,
                      c->cubeLevelD, c->ordM2LD, c->ordMomD, G, mom, lec, c->nmMax, (R)c->eps, c->T, c->gridTerms,
                      c->gridOff, c->gridRedStart, c->gridRedCell, c->gridSlotZ, c->gridNcc, c->gridNgg);
 }
 
 // k_m2l_grid at the shape the plan was built for; false when there is no grid
-template <typename R, bool TR = false>
+template <typename R, bool TR = false, bool DK = false>
 static bool m2lGrid(fbh_ctx *c, const R *G, const R *mom, R *lec, hipStream_t st) {
   if (!c->gridK || !G) return false;
   const size_t lds = sizeof(R) == sizeof(double) ? c->gridLds : c->gridLds32;
   switch (c->gridK) {
-    case 4: m2lGridLaunch<R, 4, 1, TR>(c, G, mom, lec, lds, st); return true;
-    case 8: m2lGridLaunch<R, 8, 1, TR>(c, G, mom, lec, lds, st); return true;
-    case 16: m2lGridLaunch<R, 16, 2, TR>(c, G, mom, lec, lds, st); return true;
-    case 32: m2lGridLaunch<R, 32, 4, TR>(c, G, mom, lec, lds, st); return true;
-    case 64: m2lGridLaunch<R, 64, 8, TR>(c, G, mom, lec, lds, st); return true;
+    case 4: m2lGridLaunch<R, 4, 1, TR, DK>(c, G, mom, lec, lds, st); return true;
+    case 8: m2lGridLaunch<R, 8, 1, TR, DK>(c, G, mom, lec, lds, st); return true;
+    case 16: m2lGridLaunch<R, 16, 2, TR, DK>(c, G, mom, lec, lds, st); return true;
+    case 32: m2lGridLaunch<R, 32, 4, TR, DK>(c, G, mom, lec, lds, st); return true;
+    case 64: m2lGridLaunch<R, 64, 8, TR, DK>(c, G, mom, lec, lds, st); return true;
   }
   return false;
 }
@@ -5521,9 +5634,20 @@ static int kappaOperatorTerms(fbh_ctx *c, const double *sgm_dev, const double *l
     launchQ2M<double>(c, c->Q0, c->Q1, lam_dev, c->adjL, 1, c->stream);
     B.mom = c->adjL;
     for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
-    const size_t lds = (2 * (size_t)derivScratch(c->maxOrder) + 6 * (size_t)NMOM(c->maxOrder)) * sizeof(double);
-    hipLaunchKernelGGL(k_m2l_dk, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
-                       c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->adjL, part, c->nmMax, c->kappa, c->eps, c->T);
+    if (c->gridK && c->m2lG && !c->m2lGdk) {   // (dGk/dkappa, Gk) per pair, cached as the M2L tensors are
+      if (!(c->m2lGdk = dalloc<double>(c, (size_t)c->nM2L * 2 * c->nmMax))) return 0;
+      hipLaunchKernelGGL(k_m2l_derivs, dim3((unsigned)c->nM2L), dim3(64),
+                         2 * (size_t)derivScratch(c->maxOrder) * sizeof(double), c->stream, c->m2lDst, c->m2lSrc,
+                         c->cubeLevelD, c->ordM2LD, c->cubeX, c->m2lGdk, c->nmMax, c->kappa, c->T, true);
+    }
+    if (c->m2lGdk && m2lGrid<double, false, true>(c, c->m2lGdk, c->mom, c->lec, c->stream)) {
+      hipLaunchKernelGGL(k_dk_far_dot, dim3(c->nCube), dim3(64), 0, c->stream, c->cubeLevelD, c->ordMomD, c->adjL,
+                         c->lec, part, c->nmMax, c->eps);
+    } else {
+      const size_t lds = (2 * (size_t)derivScratch(c->maxOrder) + 6 * (size_t)NMOM(c->maxOrder)) * sizeof(double);
+      hipLaunchKernelGGL(k_m2l_dk, dim3(c->nCube), dim3(64), lds, c->stream, c->m2lOff, c->m2lSrc, c->cubeLevelD,
+                         c->ordM2LD, c->ordMomD, c->cubeX, c->mom, c->adjL, part, c->nmMax, c->kappa, c->eps, c->T);
+    }
     ok = hipGetLastError() == hipSuccess && devSum(c, part, c->nCube, eFar);
     if (!ok) setErr("fbh_gradient_dev: kappa-derivative kernels failed");
   }
@@ -5637,7 +5761,7 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
                iPar = add(par.data(), nc * sizeof(int));
   const size_t inBytes = top;
   const size_t iMom = add(nullptr, (size_t)nc * nmMax * sizeof(double)),
-               iPart = add(nullptr, (size_t)S * 2 * c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
+               iPart = add(nullptr, (size_t)S * 3 * c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
                iRed = add(nullptr, RED_BLOCKS * sizeof(double)),
                iPoly = add(nullptr, (size_t)nClus * SZ * sizeof(double));
   if (s.arenaBytes < top) {
@@ -5697,8 +5821,8 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
 static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, const double *sgm_dev, double *b_dev,
                        double *energy_raw) {
   if (!s.ready) { setErr("fbh_energy/fbh_rhs_tree: no charge set staged"); return 0; }
-  const bool scalar = mode == 0 || mode == 3;
-  if ((scalar && (!energy_raw || !sgm_dev)) || (!scalar && !b_dev)) {
+  const bool scalar = mode == 0 || mode == 3 || mode == 4;   // 4: the weights into b_dev as well
+  if ((scalar && (!energy_raw || !sgm_dev)) || (!(mode == 0 || mode == 3) && !b_dev)) {
     setErr("fbh_energy/fbh_rhs_tree: bad arguments");
     return 0;
   }
@@ -5724,7 +5848,7 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
     hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
                        s.dLevel, s.dX, s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr, s.dPoly, s.c0,     \
                        depth, c->height, theta, c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder,       \
-                       lvCount[c->height], S, (MM && S == 1) ? b_dev : s.dPart);                                  \
+                       lvCount[c->height], S, (MM && S == 1) ? b_dev : s.dPart, s.dPart + 2 * (size_t)S * c->n);  \
     break;
     if (mode == 0) {
       switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
@@ -5732,11 +5856,13 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
       switch (P) { EN_WAVE(1, 3) EN_WAVE(2, 3) EN_WAVE(3, 3) EN_WAVE(4, 3) EN_WAVE(5, 3) EN_WAVE(6, 3) EN_WAVE(7, 3) EN_WAVE(8, 3) EN_WAVE(9, 3) }
     } else if (mode == 2) {
       switch (P) { EN_WAVE(1, 2) EN_WAVE(2, 2) EN_WAVE(3, 2) EN_WAVE(4, 2) EN_WAVE(5, 2) EN_WAVE(6, 2) EN_WAVE(7, 2) EN_WAVE(8, 2) EN_WAVE(9, 2) }
+    } else if (mode == 4) {
+      switch (P) { EN_WAVE(1, 4) EN_WAVE(2, 4) EN_WAVE(3, 4) EN_WAVE(4, 4) EN_WAVE(5, 4) EN_WAVE(6, 4) EN_WAVE(7, 4) EN_WAVE(8, 4) EN_WAVE(9, 4) }
     } else {
       switch (P) { EN_WAVE(1, 1) EN_WAVE(2, 1) EN_WAVE(3, 1) EN_WAVE(4, 1) EN_WAVE(5, 1) EN_WAVE(6, 1) EN_WAVE(7, 1) EN_WAVE(8, 1) EN_WAVE(9, 1) }
     }
 #undef EN_WAVE
-    if ((mode == 1 || mode == 2) && S > 1)
+    if ((mode == 1 || mode == 2 || mode == 4) && S > 1)
       hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, c->stream, s.dPart, S,
                          2 * (long long)c->n, b_dev);
   } else if (mode != 0) {
@@ -5750,8 +5876,9 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
                        theta, c->kappa, c->eps, c->Q, c->qorder, c->T, s.dPart);
   }
   hipError_t e = hipSuccess;
-  if (mode == 0 || mode == 3) {
-    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, s.dPart, S * c->n, s.dRed);
+  if (mode == 0 || mode == 3 || mode == 4) {
+    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream,
+                       mode == 4 ? s.dPart + 2 * (size_t)S * c->n : s.dPart, S * c->n, s.dRed);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, s.dRed, RED_BLOCKS, s.dOut);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, s.dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
@@ -6038,8 +6165,10 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   double *g = B.get<double>(2 * (size_t)n), *lamBuf = B.get<double>((size_t)nc * c->nmMax),
          *phi = B.get<double>(nChar), *dq = B.get<double>(nChar), *part = B.get<double>(partN);
   if (!g || !lamBuf || !phi || !dq || !part) { setErr("fbh_gradient_tree_dev: allocation failed"); return 0; }
-  // the adjoint: A^T lam = dE/dsgm
-  if (!walkCharges(c, s, 2, theta, nullptr, g, nullptr) ||
+  // the adjoint: A^T lam = dE/dsgm (the walk's mode 2), with dE/dkappa at
+  // fixed sgm (mode 3) from the same walk
+  double eK = 0.0;
+  if (!walkCharges(c, s, 4, theta, sgm_dev, g, &eK) ||
       !gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info))
     return 0;
   // dE/dq
@@ -6049,19 +6178,21 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   if (rhsTree && !adjWalk(c, s, 1, theta, lam_dev, lamBuf, phi)) return 0;
   if (!adjEval(c, s, lamBuf, phi, dq)) return 0;
   if (!rhsTree) {   // R^T lam by the direct sums
-    hipLaunchKernelGGL(k_grad_q<false>, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar,
-                       s.dPos, (const double *)nullptr, lam_dev, part, c->kappa, c->eps, rhsFactor(c), c->Q,
-                       c->qorder);
+    if (c->qorder == 1)
+      hipLaunchKernelGGL(k_gradR_q1, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n, nChar,
+                         s.dPos, lam_dev, part, rhsFactor(c), c->Q.t[0], c->Q.t[0] * c->Q.w[0] * c->Q.w[0]);
+    else
+      hipLaunchKernelGGL(k_grad_q<false>, dim3((nChar + 255) / 256, nChunk), dim3(256), 0, c->stream, c->pnl, n,
+                         nChar, s.dPos, (const double *)nullptr, lam_dev, part, c->kappa, c->eps, rhsFactor(c), c->Q,
+                         c->qorder);
     hipLaunchKernelGGL(k_grad_sum, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, part, nChunk, nChar, dq, 1);
   }
   HCK(hipGetLastError());
   HCK(hipMemcpyAsync(dEdq, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
   // dE/dkappa
-  double eK = 0.0, eNear = 0.0, eFar = 0.0;
-  if (!walkCharges(c, s, 3, theta, sgm_dev, nullptr, &eK) ||
-      !kappaOperatorTerms(c, sgm_dev, lam_dev, part, &eNear, &eFar))
-    return 0;
+  double eNear = 0.0, eFar = 0.0;
+  if (!kappaOperatorTerms(c, sgm_dev, lam_dev, part, &eNear, &eFar)) return 0;
   *dEdkappa = eK + eNear + eFar;
   return 1;
 }
