@@ -417,8 +417,8 @@ def capsid(fa, torch, tmp):
     import numpy as np
     q = np.ascontiguousarray(mesh.chr, dtype=np.float64)
     # the exact gradient of the reported (panel-tree) energy, charge-tree right-hand side
-    lam_t = torch.zeros_like(xs)
     for rep in range(2):   # the first call builds the transposed operator's stores
+        lam_t = torch.zeros_like(xs)   # each call solves the adjoint from zero
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         dqt, dkt, tinfo = op.gradient_tree_dev(S.ct, mesh.pos, q, xs, lam_t, theta=0.8, rhs_tree=True,
