@@ -2711,6 +2711,9 @@ __global__ __launch_bounds__(256) void k_ct_poly(int c0, const double *__restric
   }
 }
 
+#ifndef EN_CHUNK
+#define EN_CHUNK 32   // doubles of walk coefficients per scalar-load chunk (0: no chunking)
+#endif
 // MODE 0: the energy's kernels G0 - Gk and eps Gk - G0 (src/treecode.c:581-615);
 // MODE 1: the right-hand side's Coulomb kernel 1/r (:283-379), both halves;
 // MODE 3: d/dkappa of MODE 0's kernels; MODE 4: MODE 0 into (y0, y1) and
@@ -2752,13 +2755,14 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
   }
   // nested Horner in x (outer), y, z (inner); a coefficient of r^g is the
   // j-sum over the radial values of degree |g| + j.  The coefficients are
-  // wave-uniform and come in through scalar loads, one (a, b) chunk at a time:
-  // after each chunk an empty asm takes the table pointer and the running sums,
-  // so the next chunk's loads cannot be issued before this chunk is consumed
-  // (hoisted all at once, they overflow the scalar registers and are spilled).
+  // wave-uniform and come in through scalar loads in chunks of (a, b) rows of
+  // at least EN_CHUNK doubles: after a chunk an empty asm takes the table
+  // pointer and the running sums, so the next chunk's loads cannot be issued
+  // before this chunk is consumed (hoisted all at once, they overflow the
+  // scalar registers and are spilled).
   using CD = const __attribute__((address_space(4))) double;
   CD *co = (CD *)co0;
-  int o = 0;
+  int o = 0, since = 0;   // since: doubles read since the last chunk boundary
   double s0[NS], s1[NS];
 #pragma unroll
   for (int t = 0; t < NS; t++) s0[t] = s1[t] = 0.0;
@@ -2783,13 +2787,17 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
           rb[t] = rb[t] * rz + cw;
         }
         o += (P - d) / 2 + 1;
+        since += (P - d) / 2 + 1;
       }
 #pragma unroll
       for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
-      if constexpr (NS == 1)
-        asm volatile("" : "+s"(co), "+v"(qa[0]));
-      else
-        asm volatile("" : "+s"(co), "+v"(qa[0]), "+v"(qa[1]));
+      if (EN_CHUNK > 0 && since >= EN_CHUNK) {   // folds at compile time
+        if constexpr (NS == 1)
+          asm volatile("" : "+s"(co), "+v"(qa[0]));
+        else
+          asm volatile("" : "+s"(co), "+v"(qa[0]), "+v"(qa[1]));
+        since = 0;
+      }
     }
 #pragma unroll
     for (int t = 0; t < NS; t++) s0[t] = s0[t] * rx + qa[t];
@@ -2816,13 +2824,17 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
           rb[t] = rb[t] * rz + cv;
         }
         o += 3 * ((P + 1 - d) / 2 + 1);
+        since += 3 * ((P + 1 - d) / 2 + 1);
       }
 #pragma unroll
       for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
-      if constexpr (NS == 1)
-        asm volatile("" : "+s"(co), "+v"(qa[0]));
-      else
-        asm volatile("" : "+s"(co), "+v"(qa[0]), "+v"(qa[1]));
+      if (EN_CHUNK > 0 && since >= EN_CHUNK) {   // folds at compile time
+        if constexpr (NS == 1)
+          asm volatile("" : "+s"(co), "+v"(qa[0]));
+        else
+          asm volatile("" : "+s"(co), "+v"(qa[0]), "+v"(qa[1]));
+        since = 0;
+      }
     }
 #pragma unroll
     for (int t = 0; t < NS; t++) s1[t] = s1[t] * rx + qa[t];
@@ -2846,8 +2858,14 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
 // unaccepted leaves are summed directly.  Cluster data are wave-uniform, so the
 // coefficient tables come in through the scalar cache.
 #define EN_WSTACK 256
+#ifndef EN_WPB
+#define EN_WPB 4   // waves per block of the wave walk
+#endif
+#ifndef EN_TARGET_WAVES
+#define EN_TARGET_WAVES 8192   // slices are added until the walk has this many waves
+#endif
 template <int P, int MODE>
-__global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
+__global__ __launch_bounds__(64 * EN_WPB) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
                                                      const int *__restrict__ level, const double *__restrict__ cx,
                                                      const double *__restrict__ eRad,
                                                      const int *__restrict__ kidStart, const int *__restrict__ nKids,
@@ -2857,14 +2875,20 @@ __global__ __launch_bounds__(256) void k_energy_wave(const Pnl *__restrict__ pnl
                                                      int c0, int depth, int height, double theta, double kappa,
                                                      double eps, QTab Q, int q, int nH, int S,
                                                      double *__restrict__ part, double *__restrict__ part2) {
-  __shared__ int stC[4][EN_WSTACK];
-  __shared__ unsigned long long stM[4][EN_WSTACK];
+  __shared__ int stC[EN_WPB][EN_WSTACK];
+  __shared__ unsigned long long stM[EN_WPB][EN_WSTACK];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // S waves per 64 panels: slice s walks the clusters c0 + s, c0 + s + S, ... of
   // level `height` (no cluster above it can be accepted, so every lane reaches
-  // all of them) and writes its own slice of part
-  const int g = blockIdx.x * 4 + w, sl = g % S;
-  const int p = (g / S) * 64 + lane;
+  // all of them) and writes its own slice of part.  The EN_WPB waves of a block
+  // take one slice for EN_WPB consecutive panel groups: they walk nearly the
+  // same clusters in the same order, so the coefficient tables one wave brings
+  // into the CU's scalar cache serve the others.  Blocks are numbered so that
+  // each XCD (blocks b, b + 8, ... in dispatch order) gets a contiguous range.
+  const int nb = gridDim.x, xb = blockIdx.x, per = nb / 8, rem = nb % 8, xcd = xb % 8;
+  const int b = xcd < rem ? xcd * (per + 1) + xb / 8 : rem * (per + 1) + (xcd - rem) * per + xb / 8;
+  const int sl = b % S;
+  const int p = ((b / S) * EN_WPB + w) * 64 + lane;
   const bool live = p < n;
   const unsigned long long all = __ballot(live);
   if (all == 0ull) return;   // whole wave past the end (uniform)
@@ -5726,7 +5750,7 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
   int S = 1;
   if (uni) {
     const int nH = lvCount[c->height], panelWaves = (c->n + 63) / 64;
-    S = std::max(1, std::min(nH, (8192 + panelWaves - 1) / panelWaves));
+    S = std::max(1, std::min(nH, (EN_TARGET_WAVES + panelWaves - 1) / panelWaves));
     while ((nH + S - 1) / S + 8 * (depth - c->height + 1) > EN_WSTACK) S++;
   }
   if (uni) {
@@ -5842,10 +5866,11 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
   if (s.uni) {
     hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, c->stream, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc, s.SZ,
                        s.dPoly);
-    const int blocks = ((c->n + 63) / 64 * S + 3) / 4;
+    const int blocks = ((c->n + 64 * EN_WPB - 1) / (64 * EN_WPB)) * S;
 #define EN_WAVE(PP, MM)                                                                                           \
   case PP:                                                                                                        \
-    hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(256), 0, c->stream, c->pnl, c->n, sgm_dev,      \
+    hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(64 * EN_WPB), 0, c->stream, c->pnl, c->n,       \
+                       sgm_dev,                                                                                   \
                        s.dLevel, s.dX, s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr, s.dPoly, s.c0,     \
                        depth, c->height, theta, c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder,       \
                        lvCount[c->height], S, (MM && S == 1) ? b_dev : s.dPart, s.dPart + 2 * (size_t)S * c->n);  \
