@@ -52,6 +52,9 @@ def host_lib():
         L.fb_host_error.restype = C.c_char_p
         L.fb_tree_build.restype = _vp
         L.fb_tree_build.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.fb_tree_build_leaves.restype = _vp
+        L.fb_tree_build_leaves.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _ip, _dp, _dp,
+                                           _dp, _ip, C.c_int, _vp, _ip, _dp]
         L.fb_tree_free.argtypes = [_vp]
         L.fb_ctree_build.restype = _vp
         L.fb_ctree_build.argtypes = [_vp, C.c_int]
@@ -73,6 +76,9 @@ def hip_lib():
         L = _load("libfabipb_hip.so")
         L.fbh_available.restype = C.c_int
         L.fbh_last_error.restype = C.c_char_p
+        L.fbh_tree_panels.restype = C.c_int
+        L.fbh_tree_panels.argtypes = [C.c_int, _dp, _dp, _dp, _ip, C.c_int, _ip, _dp, _dp, _dp, _ip, _ip, _vp,
+                                      _ip, _dp]
         L.fbh_create.restype = _vp
         L.fbh_create.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int,
                                  C.c_int, _dp, _dp, _dp,
@@ -222,9 +228,27 @@ TREE_KEYS = ["ordMom", "ordM2L", "perm", "vtx", "nrm", "area", "face", "levStart
 class Tree:
     """Octree, interaction lists and flattened layouts (src/gkSetup.c, src/fmm.c:234-524)."""
 
-    def __init__(self, mesh, depth, order=6, height=2, order_mom=0, sep=0.8):
+    def __init__(self, mesh, depth, order=6, height=2, order_mom=0, sep=0.8, device=False):
+        """device: the panel pass (leaf of every panel, leaf order, leaf boxes) on the GPU
+        (fbh_tree_panels), the cubes and lists above the leaves on the host; the same arrays."""
         L = host_lib()
-        self.h = L.fb_tree_build(mesh.h, depth, height, order, order_mom, sep)
+        if device:
+            H = hip_lib()
+            vtx, nrm = _arr(L.fb_mesh_array, mesh.h, "vtx"), _arr(L.fb_mesh_array, mesh.h, "nrm")
+            area, face = _arr(L.fb_mesh_array, mesh.h, "area"), _arr(L.fb_mesh_array, mesh.h, "face")
+            n = len(area)
+            perm, faceR, cnt = np.empty(n, np.int32), np.empty(n, np.int32), np.empty(n, np.int32)
+            vtxR, nrmR, areaR = np.empty(9 * n), np.empty(3 * n), np.empty(n)
+            key, box, nleaf = np.empty(n, np.uint32), np.empty(6 * n), C.c_int(0)
+            if not H.fbh_tree_panels(n, _d(vtx), _d(nrm), _d(area), _i(face), depth, _i(perm), _d(vtxR), _d(nrmR),
+                                     _d(areaR), _i(faceR), C.byref(nleaf), key.ctypes.data_as(_vp), _i(cnt),
+                                     _d(box)):
+                raise FabipbError(H.fbh_last_error().decode())
+            self.h = L.fb_tree_build_leaves(mesh.h, depth, height, order, order_mom, sep, _i(perm), _d(vtxR),
+                                            _d(nrmR), _d(areaR), _i(faceR), nleaf.value, key.ctypes.data_as(_vp),
+                                            _i(cnt), _d(box))
+        else:
+            self.h = L.fb_tree_build(mesh.h, depth, height, order, order_mom, sep)
         if not self.h:
             raise FabipbError(L.fb_host_error().decode())
         self.depth, self.height = depth, height

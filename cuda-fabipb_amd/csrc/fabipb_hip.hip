@@ -2446,6 +2446,118 @@ __global__ __launch_bounds__(64) void k_dk_far_dot(const int *__restrict__ cubeL
 }
 #pragma clang fp contract(off)
 
+// ------------------------------------------------------------ panel tree, the panel pass
+// (src/gkSetup.c:90-183 bounding box and goDown, :264-301 leaf boxes).  No
+// contraction here: the comparisons against the cube midplanes must see the
+// same doubles as the host's.
+
+// per block: min (0..2) and max (3..5) of the panels' vertex coordinates
+__global__ __launch_bounds__(256) void k_tree_bbox(const double *__restrict__ vtx, int n, double *__restrict__ part) {
+  __shared__ double sm[6][256];
+  double lo[3], up[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = __builtin_inf();
+    up[k] = -__builtin_inf();
+  }
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) {
+        const double v = vtx[9 * (long long)p + 3 * j + k];
+        lo[k] = v < lo[k] ? v : lo[k];
+        up[k] = v > up[k] ? v : up[k];
+      }
+  for (int k = 0; k < 3; k++) {
+    sm[k][threadIdx.x] = lo[k];
+    sm[3 + k][threadIdx.x] = up[k];
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 3; k++) {
+        const double a = sm[k][threadIdx.x + o], b = sm[3 + k][threadIdx.x + o];
+        sm[k][threadIdx.x] = a < sm[k][threadIdx.x] ? a : sm[k][threadIdx.x];
+        sm[3 + k][threadIdx.x] = b > sm[3 + k][threadIdx.x] ? b : sm[3 + k][threadIdx.x];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+struct TreeSides {
+  double s[11];   // cube side per level 0..depth (depth <= 10)
+};
+
+// each panel's leaf: its centroid against the midplanes of the cube it is in,
+// level by level (strict >), the octant digits ix + 2 iy + 4 iz concatenated
+// coarsest first; idx = the panel
+__global__ __launch_bounds__(256) void k_tree_keys(const double *__restrict__ vtx, int n, double minx, double miny,
+                                                   double minz, TreeSides side, int depth,
+                                                   unsigned *__restrict__ key, int *__restrict__ idx) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const double *v = vtx + 9 * (long long)p;
+  double c[3];
+  for (int k = 0; k < 3; k++) c[k] = 0.33333333333333333333333333 * (v[k] + v[3 + k] + v[6 + k]);
+  int i = 0, j = 0, l = 0;
+  unsigned kk = 0;
+  for (int lev = 0; lev < depth; lev++) {
+    const int ix = c[0] > minx + (i + 0.5) * side.s[lev] ? 1 : 0;
+    const int iy = c[1] > miny + (j + 0.5) * side.s[lev] ? 1 : 0;
+    const int iz = c[2] > minz + (l + 0.5) * side.s[lev] ? 1 : 0;
+    kk = (kk << 3) | (unsigned)(ix + 2 * iy + 4 * iz);
+    i = 2 * i + ix;
+    j = 2 * j + iy;
+    l = 2 * l + iz;
+  }
+  key[p] = kk;
+  idx[p] = p;
+}
+
+// the panel arrays in leaf order
+__global__ __launch_bounds__(256) void k_tree_gather(const int *__restrict__ perm, int n, const double *__restrict__ vtx,
+                                                     const double *__restrict__ nrm, const double *__restrict__ area,
+                                                     const int *__restrict__ face, double *__restrict__ vtxR,
+                                                     double *__restrict__ nrmR, double *__restrict__ areaR,
+                                                     int *__restrict__ faceR) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const long long p = perm[i];
+  for (int k = 0; k < 9; k++) vtxR[9 * (long long)i + k] = vtx[9 * p + k];
+  for (int k = 0; k < 3; k++) nrmR[3 * (long long)i + k] = nrm[3 * p + k];
+  areaR[i] = area[p];
+  faceR[i] = face[p];
+}
+
+// per leaf (a run of equal keys): lo and up of its panels' vertices
+__global__ __launch_bounds__(64) void k_tree_leafbox(const int *__restrict__ start, const int *__restrict__ count,
+                                                     const double *__restrict__ vtxR, double *__restrict__ box) {
+  __shared__ double sm[6][64];
+  const int L = blockIdx.x, s0 = start[L], s1 = s0 + count[L];
+  double lo[3], up[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = __builtin_inf();
+    up[k] = -__builtin_inf();
+  }
+  for (int i = s0 + threadIdx.x; i < s1; i += 64)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) {
+        const double v = vtxR[9 * (long long)i + 3 * j + k];
+        lo[k] = v < lo[k] ? v : lo[k];
+        up[k] = v > up[k] ? v : up[k];
+      }
+  for (int k = 0; k < 3; k++) {
+    sm[k][threadIdx.x] = lo[k];
+    sm[3 + k][threadIdx.x] = up[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double r = sm[k][0];
+    for (int t = 1; t < 64; t++) r = k < 3 ? (sm[k][t] < r ? sm[k][t] : r) : (sm[k][t] > r ? sm[k][t] : r);
+    box[6 * (long long)L + k] = r;
+  }
+}
+
 // ------------------------------------------------------------ charge tree
 
 // leaf moments of the charges (src/chargeTree.c:313-337): block per leaf
@@ -3705,6 +3817,111 @@ static T *dupload(fbh_ctx *c, const T *h, size_t count) {
     return nullptr;
   }
   return d;
+}
+
+// The panel pass of the FMM tree on the device (src/gkSetup.c:90-301): each
+// panel's leaf by the host's own comparisons, a stable sort by leaf (so the
+// panels of a leaf keep their input order), the panel arrays in that order,
+// and each leaf's digit key, panel count and vertex box.  The rest of the tree
+// (cubes, boxes above the leaves, lists, layouts) is fb_tree_build_leaves.
+// Host arrays in and out; the leaf arrays hold up to n entries.
+extern "C" int fbh_tree_panels(int n, const double *vtx, const double *nrm, const double *area, const int *face,
+                               int depth, int *perm, double *vtxR, double *nrmR, double *areaR, int *faceR,
+                               int *nLeaf, unsigned *leafKey, int *leafCount, double *leafBox) {
+  if (n <= 0 || depth < 1 || depth > 10 || !vtx || !nrm || !area || !face || !perm || !vtxR || !nrmR || !areaR ||
+      !faceR || !nLeaf || !leafKey || !leafCount || !leafBox) {
+    setErr("fbh_tree_panels: bad arguments (n=%d depth=%d)", n, depth);
+    return 0;
+  }
+  hipStream_t st = nullptr;
+  HCK(hipStreamCreate(&st));
+  struct Guard {
+    hipStream_t st;
+    std::vector<void *> p;
+    ~Guard() {
+      for (void *q : p) hipFree(q);
+      hipStreamDestroy(st);
+    }
+  } G{st, {}};
+  auto get = [&](size_t bytes) -> void * {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+    G.p.push_back(q);
+    return q;
+  };
+  const size_t N = (size_t)n;
+  const int nb = std::min(1024, (n + 255) / 256);
+  double *dV = (double *)get(9 * N * 8), *dN = (double *)get(3 * N * 8), *dA = (double *)get(N * 8),
+         *dVR = (double *)get(9 * N * 8), *dNR = (double *)get(3 * N * 8), *dAR = (double *)get(N * 8),
+         *dPart = (double *)get(6 * (size_t)nb * 8), *dBox = (double *)get(6 * N * 8);
+  int *dF = (int *)get(N * 4), *dFR = (int *)get(N * 4), *dIdx = (int *)get(N * 4), *dPerm = (int *)get(N * 4),
+      *dCnt = (int *)get(N * 4), *dStart = (int *)get(N * 4), *dNRun = (int *)get(4);
+  unsigned *dKey = (unsigned *)get(N * 4), *dKeyS = (unsigned *)get(N * 4), *dUniq = (unsigned *)get(N * 4);
+  if (!dV || !dN || !dA || !dVR || !dNR || !dAR || !dPart || !dBox || !dF || !dFR || !dIdx || !dPerm || !dCnt ||
+      !dStart || !dNRun || !dKey || !dKeyS || !dUniq) {
+    setErr("fbh_tree_panels: device allocation failed (%d panels)", n);
+    return 0;
+  }
+  HCK(hipMemcpyAsync(dV, vtx, 9 * N * 8, hipMemcpyHostToDevice, st));
+  HCK(hipMemcpyAsync(dN, nrm, 3 * N * 8, hipMemcpyHostToDevice, st));
+  HCK(hipMemcpyAsync(dA, area, N * 8, hipMemcpyHostToDevice, st));
+  HCK(hipMemcpyAsync(dF, face, N * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_tree_bbox, dim3(nb), dim3(256), 0, st, dV, n, dPart);
+  std::vector<double> part(6 * (size_t)nb);
+  HCK(hipMemcpyAsync(part.data(), dPart, part.size() * 8, hipMemcpyDeviceToHost, st));
+  HCK(hipStreamSynchronize(st));
+  double lo[3], up[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = part[k];
+    up[k] = part[3 + k];
+  }
+  for (int b = 1; b < nb; b++)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = part[6 * b + k] < lo[k] ? part[6 * b + k] : lo[k];
+      up[k] = part[6 * b + 3 + k] > up[k] ? part[6 * b + 3 + k] : up[k];
+    }
+  // side lengths as the host builds them (src/gkSetup.c:90-128)
+  TreeSides side;
+  double len = (up[0] - lo[0]) > (up[1] - lo[1]) ? (up[0] - lo[0]) : (up[1] - lo[1]);
+  len = (up[2] - lo[2]) > len ? (up[2] - lo[2]) : len;
+  for (int lev = 0; lev <= depth; lev++) {
+    side.s[lev] = len;
+    len *= 0.5;
+  }
+  hipLaunchKernelGGL(k_tree_keys, dim3((n + 255) / 256), dim3(256), 0, st, dV, n, lo[0], lo[1], lo[2], side, depth,
+                     dKey, dIdx);
+  size_t tb = 0, tb2 = 0, tb3 = 0;
+  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dKey, dKeyS, dIdx, dPerm, n, 0, 3 * depth, st));
+  HCK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, dKeyS, dUniq, dCnt, dNRun, n, st));
+  HCK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, dCnt, dStart, n, st));
+  void *tmp = get(std::max(tb, std::max(tb2, tb3)));
+  if (!tmp) { setErr("fbh_tree_panels: sort storage allocation failed"); return 0; }
+  tb = std::max(tb, std::max(tb2, tb3));
+  size_t t1 = tb;
+  HCK(hipcub::DeviceRadixSort::SortPairs(tmp, t1, dKey, dKeyS, dIdx, dPerm, n, 0, 3 * depth, st));
+  hipLaunchKernelGGL(k_tree_gather, dim3((n + 255) / 256), dim3(256), 0, st, dPerm, n, dV, dN, dA, dF, dVR, dNR, dAR,
+                     dFR);
+  t1 = tb;
+  HCK(hipcub::DeviceRunLengthEncode::Encode(tmp, t1, dKeyS, dUniq, dCnt, dNRun, n, st));
+  int nRun = 0;
+  HCK(hipMemcpyAsync(&nRun, dNRun, 4, hipMemcpyDeviceToHost, st));
+  HCK(hipStreamSynchronize(st));
+  if (nRun < 1 || nRun > n) { setErr("fbh_tree_panels: %d leaves", nRun); return 0; }
+  t1 = tb;
+  HCK(hipcub::DeviceScan::ExclusiveSum(tmp, t1, dCnt, dStart, nRun, st));
+  hipLaunchKernelGGL(k_tree_leafbox, dim3(nRun), dim3(64), 0, st, dStart, dCnt, dVR, dBox);
+  HCK(hipGetLastError());
+  HCK(hipMemcpyAsync(perm, dPerm, N * 4, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(vtxR, dVR, 9 * N * 8, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(nrmR, dNR, 3 * N * 8, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(areaR, dAR, N * 8, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(faceR, dFR, N * 4, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(leafKey, dUniq, (size_t)nRun * 4, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(leafCount, dCnt, (size_t)nRun * 4, hipMemcpyDeviceToHost, st));
+  HCK(hipMemcpyAsync(leafBox, dBox, 6 * (size_t)nRun * 8, hipMemcpyDeviceToHost, st));
+  HCK(hipStreamSynchronize(st));
+  *nLeaf = nRun;
+  return 1;
 }
 
 extern "C" int fbh_available(void) {

@@ -384,6 +384,8 @@ void fb_tree_free(fb_tree *t) {
   free(t);
 }
 
+static fb_tree *finishTree(otree_t T, fb_tree *t, int depth, int height, int order, int orderMom, double eta0);
+
 /* src/gkSetup.c:446-508 (gkInit) and src/fmm.c:234-524 (setupFMM layouts) */
 fb_tree *fb_tree_build(const fb_mesh *m, int depth, int height, int order, int orderMom, double eta0) {
   if (depth < 1 || height < 0 || height > depth || order == 0) {
@@ -463,6 +465,83 @@ fb_tree *fb_tree_build(const fb_mesh *m, int depth, int height, int order, int o
       i0 += C->nPnls;
     }
   }
+  /* panels in idx order */
+  t->vtx = NEW(double, 9 * (size_t)n); t->nrm = NEW(double, 3 * (size_t)n);
+  t->area = NEW(double, n); t->face = NEW(int, n);
+  for (int i = 0; i < n; i++) {
+    int p = t->perm[i];
+    memcpy(&t->vtx[9 * i], &m->vtx[9 * p], 9 * sizeof(double));
+    memcpy(&t->nrm[3 * i], &m->nrm[3 * p], 3 * sizeof(double));
+    t->area[i] = m->area[p];
+    t->face[i] = m->face[p];
+  }
+  return finishTree(T, t, depth, height, order, orderMom, eta0);
+}
+
+/* The same tree from its leaves, as a device pass finds them (fbh_tree_panels):
+ * panels sorted by leaf (the octant digits of each level, coarsest first: the
+ * leaves' depth-first order) and, within a leaf, in input order (perm), the
+ * panel arrays in that order, and per leaf its digit key, panel count and
+ * vertex box (lo, up).  Cubes, links, boxes and lists follow as in
+ * fb_tree_build. */
+fb_tree *fb_tree_build_leaves(const fb_mesh *m, int depth, int height, int order, int orderMom, double eta0,
+                              const int *perm, const double *vtx, const double *nrm, const double *area,
+                              const int *face, int nLeaf, const unsigned *leafKey, const int *leafCount,
+                              const double *leafBox) {
+  if (depth < 1 || depth > 10 || height < 0 || height > depth || order == 0 || nLeaf < 1) {
+    setErr("bad tree parameters depth=%d height=%d order=%d leaves=%d", depth, height, order, nLeaf);
+    return NULL;
+  }
+  int n = m->n;
+  otree_t T;
+  memset(&T, 0, sizeof(T));
+  T.maxLev = depth;
+  T.side = NEW(double, depth + 1);
+  newCube(&T, 0, 0, 0, 0, -1);
+  long long total = 0;
+  for (int L = 0; L < nLeaf; L++) {
+    if (L > 0 && leafKey[L] <= leafKey[L - 1]) { setErr("leaf keys not increasing at %d", L); otreeFree(&T); return NULL; }
+    int cb = 0;
+    for (int lev = 1; lev <= depth; lev++) {
+      int q = (int)((leafKey[L] >> (3 * (depth - lev))) & 7u);
+      if (T.c[cb].kids[q] < 0) {
+        cube_t *c = &T.c[cb];
+        int ci = 2 * c->i + (q & 1), cj = 2 * c->j + ((q >> 1) & 1), ck = 2 * c->k + ((q >> 2) & 1);
+        int kid = newCube(&T, lev, ci, cj, ck, cb);
+        T.c[cb].kids[q] = kid;   /* T.c may have moved */
+      }
+      cb = T.c[cb].kids[q];
+    }
+    cube_t *C = &T.c[cb];
+    C->nPnls = leafCount[L];
+    total += leafCount[L];
+    for (int k = 0; k < 3; k++) { C->lo[k] = leafBox[6 * L + k]; C->up[k] = leafBox[6 * L + 3 + k]; }
+    for (int k = 0; k < 3; k++) C->x[k] = 0.5 * (C->lo[k] + C->up[k]);
+    C->eRad = 0.5 * sqrt(SQR(C->up[0] - C->lo[0]) + SQR(C->up[1] - C->lo[1]) + SQR(C->up[2] - C->lo[2]));
+  }
+  if (total != n) { setErr("leaf counts add to %lld, not %d panels", total, n); otreeFree(&T); return NULL; }
+  T.first = NEW(int, depth + 1);
+  T.last = NEW(int, depth + 1);
+  for (int l = 0; l <= depth; l++) T.first[l] = T.last[l] = -1;
+  linkCubes(&T, 0);
+  countUp(&T);
+  fb_tree *t = NEW(fb_tree, 1);
+  t->n = n; t->depth = depth; t->height = height; t->nChar = m->nChar;
+  t->perm = NEW(int, n);
+  t->vtx = NEW(double, 9 * (size_t)n); t->nrm = NEW(double, 3 * (size_t)n);
+  t->area = NEW(double, n); t->face = NEW(int, n);
+  memcpy(t->perm, perm, (size_t)n * sizeof(int));
+  memcpy(t->vtx, vtx, 9 * (size_t)n * sizeof(double));
+  memcpy(t->nrm, nrm, 3 * (size_t)n * sizeof(double));
+  memcpy(t->area, area, (size_t)n * sizeof(double));
+  memcpy(t->face, face, (size_t)n * sizeof(int));
+  return finishTree(T, t, depth, height, order, orderMom, eta0);
+}
+
+/* boxes above the leaves, lists, orders and the flat layouts; frees T */
+static fb_tree *finishTree(otree_t T, fb_tree *t, int depth, int height, int order, int orderMom, double eta0) {
+  int idx, nLeaf = 0;
+  for (int c = T.first[depth]; c >= 0; c = T.c[c].next) nLeaf++;
   parentBoxes(&T, height);
   neighbours(&T, height, depth, eta0);
 
@@ -480,17 +559,6 @@ fb_tree *fb_tree_build(const fb_mesh *m, int depth, int height, int order, int o
       int o = orderMom >= 0 ? t->ordM2L[l] + orderMom : t->maxOrder;
       t->ordMom[l] = o < t->maxOrder ? o : t->maxOrder;
     }
-  }
-
-  /* panels in idx order */
-  t->vtx = NEW(double, 9 * (size_t)n); t->nrm = NEW(double, 3 * (size_t)n);
-  t->area = NEW(double, n); t->face = NEW(int, n);
-  for (int i = 0; i < n; i++) {
-    int p = t->perm[i];
-    memcpy(&t->vtx[9 * i], &m->vtx[9 * p], 9 * sizeof(double));
-    memcpy(&t->nrm[3 * i], &m->nrm[3 * p], 3 * sizeof(double));
-    t->area[i] = m->area[p];
-    t->face[i] = m->face[p];
   }
 
   /* FMM cube layout, src/fmm.c:295-324 */

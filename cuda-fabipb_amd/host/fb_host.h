@@ -67,6 +67,12 @@ const char *fb_host_error(void);
 
 /* tree: order > 0 fixed order, order < 0 variable order (gkSetup.c:42-81) */
 fb_tree *fb_tree_build(const fb_mesh *m, int depth, int height, int order, int orderMom, double sepRatio);
+/* the same tree from the leaves of a device panel pass (fbh_tree_panels): perm and the panel arrays in
+ * leaf order, and per leaf (nLeaf, increasing keys) its octant-digit key, panel count and box (lo, up) */
+fb_tree *fb_tree_build_leaves(const fb_mesh *m, int depth, int height, int order, int orderMom, double sepRatio,
+                              const int *perm, const double *vtx, const double *nrm, const double *area,
+                              const int *face, int nLeaf, const unsigned *leafKey, const int *leafCount,
+                              const double *leafBox);
 void fb_tree_free(fb_tree *t);
 
 fb_ctree *fb_ctree_build(const fb_mesh *m, int depth);

@@ -29,6 +29,22 @@ int fbh_available(void);
 const char *fbh_last_error(void);
 
 /*
+ * The panel pass of the FMM tree on the device, in place of the host's
+ * bounding box, goDown insertion and leaf boxes (src/gkSetup.c:90-183,
+ * 264-301): each panel's leaf by the same midplane comparisons (strict >), the
+ * panels stably sorted by leaf (depth-first leaf order, input order within a
+ * leaf) and gathered in that order, and per leaf its octant-digit key
+ * (coarsest digit first, 3 bits per level), panel count and vertex box
+ * (lo[3], up[3]).  Host arrays: vtx[9n], nrm[3n], area[n], face[n] in; perm[n]
+ * (input panel of each position), vtxR/nrmR/areaR/faceR out; leafKey,
+ * leafCount[n], leafBox[6n] hold *nLeaf entries.  depth 1..10.  The cubes and
+ * lists above the leaves: fb_tree_build_leaves (cuda-fabipb_amd/host/fb_host.h).
+ */
+int fbh_tree_panels(int n, const double *vtx, const double *nrm, const double *area, const int *face, int depth,
+                    int *perm, double *vtxR, double *nrmR, double *areaR, int *faceR, int *nLeaf,
+                    unsigned *leafKey, int *leafCount, double *leafBox);
+
+/*
  * Build the device-resident operator for one mesh and tree.  Replaces the
  * per-sys-pointer caches built lazily by gpuNearfieldApply / gpuM2LApply /
  * gpuQ2MApply (src/gpu_backend_cuda.cu:328-798): geometry, layouts, leaf

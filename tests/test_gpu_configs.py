@@ -13,8 +13,10 @@
 - Capsid-scale shell, 2,621,440 panels with 1,000,000 charges (order 8, MAC 0.7,
   depth 6), where no reference run fits: size-independent properties -- the
   FP32-refined energy within 1e-6 of the FP64 one, the transpose identity of the
-  whole operator, linearity of the matvec, and the charge-tree right-hand side
-  against the direct sum at the walk's truncation level.
+  whole operator, linearity of the matvec, the charge-tree right-hand side
+  against the direct sum at the walk's truncation level, the device-built tree
+  against the host's, and the exact gradient's transposed walks against forward
+  walks of unit charges.
 """
 import os
 import subprocess
@@ -161,7 +163,15 @@ def test_capsid_shell_properties(fa, tmp_path):
     subprocess.check_call([os.path.join(ROOT, "cuda-fabipb_amd/bin/fbmesh"), CAPSID[0], base] + CAPSID[1:],
                           stdout=subprocess.DEVNULL)
     mesh = fa.Mesh(base)
+    t0 = time.perf_counter()
     tree = fa.Tree(mesh, 6, order=8, sep=0.7)
+    t1 = time.perf_counter()
+    dtree = fa.Tree(mesh, 6, order=8, sep=0.7, device=True)
+    t2 = time.perf_counter()
+    print(f"[capsid] tree: host {t1 - t0:.3f} s, device panel pass {t2 - t1:.3f} s", flush=True)
+    for k in fa.TREE_KEYS:
+        assert np.array_equal(dtree.a[k], tree.a[k]), k
+    del dtree
     assert tree.n == 2621440 and len(mesh.chr) == 1000000
     op = fa.Operator(tree, FL["k"], FL["eps1"], FL["eps2"], FL["q"])
     op.set_precond(3)
