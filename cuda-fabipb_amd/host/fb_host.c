@@ -89,6 +89,66 @@ void fb_mesh_free(fb_mesh *m) {
   free(m->vtx); free(m->nrm); free(m->area); free(m->face); free(m->pos); free(m->chr); free(m);
 }
 
+/* The rest of fp from its current position, NUL-terminated; *len bytes. */
+static char *slurp(FILE *fp, size_t *len) {
+  long at = ftell(fp);
+  if (at < 0 || fseek(fp, 0, SEEK_END) != 0) return NULL;
+  long end = ftell(fp);
+  if (end < at || fseek(fp, at, SEEK_SET) != 0) return NULL;
+  size_t n = (size_t)(end - at);
+  char *b = malloc(n + 1);
+  if (!b) return NULL;
+  if (fread(b, 1, n, fp) != n) { free(b); return NULL; }
+  b[n] = 0;
+  *len = n;
+  return b;
+}
+
+/* starts of the first `want` lines of b (want + 1 entries: the last is the end
+ * of line want - 1, past its '\n'); 0 if b has fewer lines */
+static int lineStarts(const char *b, size_t len, int want, size_t *st) {
+  size_t at = 0;
+  for (int i = 0; i < want; i++) {
+    if (at >= len) return 0;
+    st[i] = at;
+    const char *nl = memchr(b + at, '\n', len - at);
+    at = nl ? (size_t)(nl - b) + 1 : len;
+  }
+  st[want] = at;
+  return 1;
+}
+
+/* the whitespace-separated tokens of b[s, e): doubles first (nd of them, each
+ * consumed whole by strtod), then plain integers (ni: [+-]digits, by strtol);
+ * 1 if the line is exactly that, which is where fscanf's token stream and this
+ * per-line reading agree */
+static int lineTokens(const char *b, size_t s, size_t e, int nd, double *d, int ni, long *iv) {
+  const char *p = b + s, *end = b + e;
+  for (int k = 0; k < nd + ni; k++) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n')) p++;
+    const char *t = p;
+    while (p < end && !(*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n')) p++;
+    if (p == t) return 0;
+    char tok[128];
+    if ((size_t)(p - t) >= sizeof(tok)) return 0;
+    memcpy(tok, t, (size_t)(p - t));
+    tok[p - t] = 0;
+    char *q;
+    if (k < nd) {
+      d[k] = strtod(tok, &q);
+    } else {
+      const char *c = tok + (tok[0] == '+' || tok[0] == '-');
+      if (!*c) return 0;
+      for (; *c; c++)
+        if (*c < '0' || *c > '9') return 0;
+      iv[k - nd] = strtol(tok, &q, 10);
+    }
+    if (*q) return 0;
+  }
+  while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n')) p++;
+  return p == end;
+}
+
 /* src/input.c:320-821 (loadPanel) with the mesh taken from existing files */
 fb_mesh *fb_mesh_load(const char *base, int useGeom) {
   char path[4096], line[512];
@@ -101,17 +161,62 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
   int cap = 1024;
   m->pos = NEW(double, 3 * cap);
   m->chr = NEW(double, cap);
-  while (fgets(line, sizeof(line), fp)) {
-    if (pqrLine(line, v) != 1) continue;
-    if (m->nChar == cap) {
-      cap *= 2;
-      m->pos = realloc(m->pos, sizeof(double) * 3 * cap);
-      m->chr = realloc(m->chr, sizeof(double) * cap);
+  /* every line parsed on its own, in parallel, when no line is long enough
+   * for fgets to split it; the records kept in file order either way */
+  size_t blen = 0, nl = 0, scap = 0;
+  size_t *st = NULL;
+  char *buf = slurp(fp, &blen);
+  int par = buf != NULL;
+  for (size_t at = 0; par && at < blen;) {   /* line starts; no line longer than fgets reads at once */
+    const char *e = memchr(buf + at, '\n', blen - at);
+    size_t next = e ? (size_t)(e - buf) + 1 : blen;
+    if (next - at > sizeof(line) - 1) par = 0;
+    if (nl == scap) {
+      scap = scap ? 2 * scap : 4096;
+      st = realloc(st, sizeof(size_t) * (scap + 1));
     }
-    m->pos[3 * m->nChar] = v[0]; m->pos[3 * m->nChar + 1] = v[1]; m->pos[3 * m->nChar + 2] = v[2];
-    m->chr[m->nChar] = v[3];
-    m->nChar++;
+    st[nl++] = at;
+    at = next;
   }
+  if (par) {
+    st[nl] = blen;
+    signed char *ok = NEW(signed char, nl + 1);
+    double *vv = NEW(double, 5 * (nl + 1));
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)nl; i++) {
+      char l2[512];
+      size_t w = st[i + 1] - st[i];
+      memcpy(l2, buf + st[i], w);
+      l2[w] = 0;
+      ok[i] = (signed char)pqrLine(l2, &vv[5 * i]);
+    }
+    for (size_t i = 0; i < nl; i++) {
+      if (ok[i] != 1) continue;
+      if (m->nChar == cap) {
+        cap *= 2;
+        m->pos = realloc(m->pos, sizeof(double) * 3 * cap);
+        m->chr = realloc(m->chr, sizeof(double) * cap);
+      }
+      m->pos[3 * m->nChar] = vv[5 * i]; m->pos[3 * m->nChar + 1] = vv[5 * i + 1]; m->pos[3 * m->nChar + 2] = vv[5 * i + 2];
+      m->chr[m->nChar] = vv[5 * i + 3];
+      m->nChar++;
+    }
+    free(ok); free(vv);
+  } else {
+    rewind(fp);
+    while (fgets(line, sizeof(line), fp)) {
+      if (pqrLine(line, v) != 1) continue;
+      if (m->nChar == cap) {
+        cap *= 2;
+        m->pos = realloc(m->pos, sizeof(double) * 3 * cap);
+        m->chr = realloc(m->chr, sizeof(double) * cap);
+      }
+      m->pos[3 * m->nChar] = v[0]; m->pos[3 * m->nChar + 1] = v[1]; m->pos[3 * m->nChar + 2] = v[2];
+      m->chr[m->nChar] = v[3];
+      m->nChar++;
+    }
+  }
+  free(st); free(buf);
   fclose(fp);
   if (m->nChar <= 0) { setErr("no ATOM/HETATM records in '%s'", path); fb_mesh_free(m); return NULL; }
 
@@ -123,12 +228,34 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
     setErr("malformed vertex header in '%s'", path); fclose(fp); fb_mesh_free(m); return NULL;
   }
   double *sp = NEW(double, 3 * (size_t)nspt), *sn = NEW(double, 3 * (size_t)nspt);
-  for (int i = 0; i < nspt; i++) {
-    double a[6];
-    int i1, i2, i3;
-    ierr = fscanf(fp, "%lf %lf %lf %lf %lf %lf %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &i1, &i2, &i3);
-    if (ierr < 6) { setErr("'%s' truncated at vertex %d", path, i + 1); fclose(fp); free(sp); free(sn); fb_mesh_free(m); return NULL; }
-    for (int k = 0; k < 3; k++) { sp[3 * i + k] = a[k]; sn[3 * i + k] = a[3 + k]; }
+  {
+    /* one vertex per line (6 doubles, 3 integers) read in parallel; any other
+     * layout goes through fscanf's token stream as before */
+    long at = ftell(fp);
+    size_t blen = 0;
+    char *buf = slurp(fp, &blen);
+    size_t *st = buf ? NEW(size_t, (size_t)nspt + 1) : NULL;
+    int par = buf && lineStarts(buf, blen, nspt, st), bad = 0;
+    if (par) {
+#pragma omp parallel for schedule(static) reduction(|:bad)
+      for (long i = 0; i < nspt; i++) {
+        double a[6];
+        long iv[3];
+        if (!lineTokens(buf, st[i], st[i + 1], 6, a, 3, iv)) { bad |= 1; continue; }
+        for (int k = 0; k < 3; k++) { sp[3 * i + k] = a[k]; sn[3 * i + k] = a[3 + k]; }
+      }
+    }
+    free(st); free(buf);
+    if (!par || bad) {
+      fseek(fp, at, SEEK_SET);
+      for (int i = 0; i < nspt; i++) {
+        double a[6];
+        int i1, i2, i3;
+        ierr = fscanf(fp, "%lf %lf %lf %lf %lf %lf %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &i1, &i2, &i3);
+        if (ierr < 6) { setErr("'%s' truncated at vertex %d", path, i + 1); fclose(fp); free(sp); free(sn); fb_mesh_free(m); return NULL; }
+        for (int k = 0; k < 3; k++) { sp[3 * i + k] = a[k]; sn[3 * i + k] = a[3 + k]; }
+      }
+    }
   }
   fclose(fp);
 
@@ -139,13 +266,35 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
     setErr("malformed face header in '%s'", path); fclose(fp); free(sp); free(sn); fb_mesh_free(m); return NULL;
   }
   int *nv = NEW(int, 3 * (size_t)nface);
-  for (int i = 0; i < nface; i++) {
-    int j1, j2, j3, i1, i2;
-    ierr = fscanf(fp, "%d %d %d %d %d", &j1, &j2, &j3, &i1, &i2);
-    if (ierr < 3 || j1 < 1 || j1 > nspt || j2 < 1 || j2 > nspt || j3 < 1 || j3 > nspt) {
-      setErr("'%s' bad or truncated face %d", path, i + 1); fclose(fp); free(sp); free(sn); free(nv); fb_mesh_free(m); return NULL;
+  {
+    /* one face per line (5 integers, vertex numbers in range) in parallel; any
+     * other layout, or a bad face, goes through fscanf as before */
+    long at = ftell(fp);
+    size_t blen = 0;
+    char *buf = slurp(fp, &blen);
+    size_t *st = buf ? NEW(size_t, (size_t)nface + 1) : NULL;
+    int par = buf && lineStarts(buf, blen, nface, st), bad = 0;
+    if (par) {
+#pragma omp parallel for schedule(static) reduction(|:bad)
+      for (long i = 0; i < nface; i++) {
+        long iv[5];
+        if (!lineTokens(buf, st[i], st[i + 1], 0, NULL, 5, iv) || iv[0] < 1 || iv[0] > nspt || iv[1] < 1 ||
+            iv[1] > nspt || iv[2] < 1 || iv[2] > nspt) { bad |= 1; continue; }
+        nv[3 * i] = (int)iv[0]; nv[3 * i + 1] = (int)iv[1]; nv[3 * i + 2] = (int)iv[2];
+      }
     }
-    nv[3 * i] = j1; nv[3 * i + 1] = j2; nv[3 * i + 2] = j3;
+    free(st); free(buf);
+    if (!par || bad) {
+      fseek(fp, at, SEEK_SET);
+      for (int i = 0; i < nface; i++) {
+        int j1, j2, j3, i1, i2;
+        ierr = fscanf(fp, "%d %d %d %d %d", &j1, &j2, &j3, &i1, &i2);
+        if (ierr < 3 || j1 < 1 || j1 > nspt || j2 < 1 || j2 > nspt || j3 < 1 || j3 > nspt) {
+          setErr("'%s' bad or truncated face %d", path, i + 1); fclose(fp); free(sp); free(sn); free(nv); fb_mesh_free(m); return NULL;
+        }
+        nv[3 * i] = j1; nv[3 * i + 1] = j2; nv[3 * i + 2] = j3;
+      }
+    }
   }
   fclose(fp);
   m->nVert = nspt;
@@ -158,7 +307,13 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
   m->nrm = NEW(double, 3 * (size_t)nface);
   m->area = NEW(double, nface);
   m->face = NEW(int, nface);
-  for (int i = 0; i < nface; i++) {
+  /* each face's verdict depends on the input alone (its area and the 10
+   * faces before it in the file, kept or not), so the faces are judged in
+   * parallel, then numbered in file order and written in parallel */
+  double *far = NEW(double, nface);
+  int *slot = NEW(int, (size_t)nface + 1);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < nface; i++) {
     double face[3][3], xx[3] = {0, 0, 0};
     for (int j = 0; j < 3; j++)
       for (int k = 0; k < 3; k++) {
@@ -175,8 +330,18 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
       for (int k = 0; k < 3; k++) d += (xx[k] - yy[k]) * (xx[k] - yy[k]);
       if (d < 1.0e-5) alert = 1;
     }
-    if (!(ar >= 1.0e-5 && alert == 0)) continue;
-    int p = m->n++;
+    far[i] = ar;
+    slot[i] = ar >= 1.0e-5 && alert == 0;
+  }
+  for (int i = 0; i < nface; i++) {   /* kept faces numbered in file order */
+    int keep = slot[i];
+    slot[i] = keep ? m->n++ : -1;
+  }
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < nface; i++) {
+    if (slot[i] < 0) continue;
+    int p = slot[i];
+    double ar = far[i];
     double vn[3][3];
     for (int j = 0; j < 3; j++)
       for (int k = 0; k < 3; k++) {
@@ -203,8 +368,9 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
       nr[0] = 0.0; nr[1] = 0.0; nr[2] = 1.0;
     }
     m->area[p] = ar;
-    m->face[p] = i;
+    m->face[p] = (int)i;
   }
+  free(far); free(slot);
   free(sp); free(sn); free(nv);
   if (m->n <= 0) { setErr("no panels survived the face filter"); fb_mesh_free(m); return NULL; }
   return m;
@@ -325,10 +491,25 @@ static int nearCubes(const cube_t *a, const cube_t *b, double eta0) {
 
 /* src/gkSetup.c:363-437 (getNbrs): nbrs[0] = self, [1,nNbrs) near,
  * [nNbrs,n2Nbrs) interaction list */
+/* the cubes of level lev in list order; *cnt of them */
+static int *levelCubes(const otree_t *T, int lev, int *cnt) {
+  int n = 0;
+  for (int c = T->first[lev]; c >= 0; c = T->c[c].next) n++;
+  int *a = NEW(int, n + 1);
+  n = 0;
+  for (int c = T->first[lev]; c >= 0; c = T->c[c].next) a[n++] = c;
+  *cnt = n;
+  return a;
+}
+
+/* each cube's lists depend only on its parent's (the level above), so the
+ * cubes of a level are done in parallel */
 static void neighbours(otree_t *T, int height, int depth, double eta0) {
   int nTop = 0;
-  for (int c = T->first[height]; c >= 0; c = T->c[c].next) nTop++;
-  for (int c = T->first[height]; c >= 0; c = T->c[c].next) {
+  int *top = levelCubes(T, height, &nTop);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int ci = 0; ci < nTop; ci++) {
+    int c = top[ci];
     cube_t *C = &T->c[c];
     int nN = 0;
     for (int b = T->first[height]; b >= 0; b = T->c[b].next) if (nearCubes(C, &T->c[b], eta0)) nN++;
@@ -342,8 +523,13 @@ static void neighbours(otree_t *T, int height, int depth, double eta0) {
     }
     C->nNbrs = i1; C->n2Nbrs = i2;
   }
-  for (int lev = height + 1; lev <= depth; lev++)
-    for (int c = T->first[lev]; c >= 0; c = T->c[c].next) {
+  free(top);
+  for (int lev = height + 1; lev <= depth; lev++) {
+    int nl = 0;
+    int *cubes = levelCubes(T, lev, &nl);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int ci = 0; ci < nl; ci++) {
+      int c = cubes[ci];
       cube_t *C = &T->c[c];
       cube_t *P = &T->c[C->parent];
       int nN = 0, n2 = 0;
@@ -368,6 +554,8 @@ static void neighbours(otree_t *T, int height, int depth, double eta0) {
       }
       C->nNbrs = i1; C->n2Nbrs = i2;
     }
+    free(cubes);
+  }
 }
 
 static void otreeFree(otree_t *T) {
@@ -468,6 +656,7 @@ fb_tree *fb_tree_build(const fb_mesh *m, int depth, int height, int order, int o
   /* panels in idx order */
   t->vtx = NEW(double, 9 * (size_t)n); t->nrm = NEW(double, 3 * (size_t)n);
   t->area = NEW(double, n); t->face = NEW(int, n);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) {
     int p = t->perm[i];
     memcpy(&t->vtx[9 * i], &m->vtx[9 * p], 9 * sizeof(double));
