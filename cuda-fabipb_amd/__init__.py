@@ -193,6 +193,13 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def _ptr(fn, h, name, typ):
+    """An array of a host object by pointer (no copy) and its length."""
+    cnt, isd = C.c_int(0), C.c_int(0)
+    p = fn(h, name.encode(), C.byref(cnt), C.byref(isd))
+    return C.cast(p, typ), cnt.value
+
+
 def _arr(fn, h, name):
     cnt, isd = C.c_int(0), C.c_int(0)
     p = fn(h, name.encode(), C.byref(cnt), C.byref(isd))
@@ -234,13 +241,14 @@ class Tree:
         L = host_lib()
         if device:
             H = hip_lib()
-            vtx, nrm = _arr(L.fb_mesh_array, mesh.h, "vtx"), _arr(L.fb_mesh_array, mesh.h, "nrm")
-            area, face = _arr(L.fb_mesh_array, mesh.h, "area"), _arr(L.fb_mesh_array, mesh.h, "face")
-            n = len(area)
+            vtx, _ = _ptr(L.fb_mesh_array, mesh.h, "vtx", _dp)
+            nrm, _ = _ptr(L.fb_mesh_array, mesh.h, "nrm", _dp)
+            area, n = _ptr(L.fb_mesh_array, mesh.h, "area", _dp)
+            face, _ = _ptr(L.fb_mesh_array, mesh.h, "face", _ip)
             perm, faceR, cnt = np.empty(n, np.int32), np.empty(n, np.int32), np.empty(n, np.int32)
             vtxR, nrmR, areaR = np.empty(9 * n), np.empty(3 * n), np.empty(n)
             key, box, nleaf = np.empty(n, np.uint32), np.empty(6 * n), C.c_int(0)
-            if not H.fbh_tree_panels(n, _d(vtx), _d(nrm), _d(area), _i(face), depth, _i(perm), _d(vtxR), _d(nrmR),
+            if not H.fbh_tree_panels(n, vtx, nrm, area, face, depth, _i(perm), _d(vtxR), _d(nrmR),
                                      _d(areaR), _i(faceR), C.byref(nleaf), key.ctypes.data_as(_vp), _i(cnt),
                                      _d(box)):
                 raise FabipbError(H.fbh_last_error().decode())

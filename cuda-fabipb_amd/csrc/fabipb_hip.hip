@@ -2824,8 +2824,11 @@ __global__ __launch_bounds__(256) void k_ct_poly(int c0, const double *__restric
 }
 
 #ifndef EN_CHUNK
-#define EN_CHUNK 32   // doubles of walk coefficients per scalar-load chunk (0: no chunking)
+#define EN_CHUNK -1   // doubles of walk coefficients per scalar-load chunk (0: no chunking; -1: per order)
 #endif
+// per order: the largest chunk that leaves the scalar registers unspilled
+// (counted in the compiled code: 32 at orders 6 and 7, 8 elsewhere)
+__host__ __device__ constexpr int enChunk(int p) { return EN_CHUNK >= 0 ? EN_CHUNK : (p == 6 || p == 7) ? 32 : 8; }
 // MODE 0: the energy's kernels G0 - Gk and eps Gk - G0 (src/treecode.c:581-615);
 // MODE 1: the right-hand side's Coulomb kernel 1/r (:283-379), both halves;
 // MODE 3: d/dkappa of MODE 0's kernels; MODE 4: MODE 0 into (y0, y1) and
@@ -2868,7 +2871,7 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
   // nested Horner in x (outer), y, z (inner); a coefficient of r^g is the
   // j-sum over the radial values of degree |g| + j.  The coefficients are
   // wave-uniform and come in through scalar loads in chunks of (a, b) rows of
-  // at least EN_CHUNK doubles: after a chunk an empty asm takes the table
+  // at least enChunk(P) doubles: after a chunk an empty asm takes the table
   // pointer and the running sums, so the next chunk's loads cannot be issued
   // before this chunk is consumed (hoisted all at once, they overflow the
   // scalar registers and are spilled).
@@ -2903,7 +2906,7 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
       }
 #pragma unroll
       for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
-      if (EN_CHUNK > 0 && since >= EN_CHUNK) {   // folds at compile time
+      if (enChunk(P) > 0 && since >= enChunk(P)) {   // folds at compile time
         if constexpr (NS == 1)
           asm volatile("" : "+s"(co), "+v"(qa[0]));
         else
@@ -2940,7 +2943,7 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
       }
 #pragma unroll
       for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
-      if (EN_CHUNK > 0 && since >= EN_CHUNK) {   // folds at compile time
+      if (enChunk(P) > 0 && since >= enChunk(P)) {   // folds at compile time
         if constexpr (NS == 1)
           asm volatile("" : "+s"(co), "+v"(qa[0]));
         else
@@ -2976,8 +2979,12 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
 #ifndef EN_TARGET_WAVES
 #define EN_TARGET_WAVES 8192   // slices are added until the walk has this many waves
 #endif
+// waves per SIMD the walk's registers must allow: 4 for the energy's modes (a
+// few spilled registers; 2.5 against 2.8 ms with three waves at 317k panels),
+// the compiler's choice for the right-hand side (five waves) and MODE 4
+__host__ __device__ constexpr int enMinWaves(int mode) { return mode == 0 || mode == 2 || mode == 3 ? 4 : 1; }
 template <int P, int MODE>
-__global__ __launch_bounds__(64 * EN_WPB) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
+__global__ __launch_bounds__(64 * EN_WPB, enMinWaves(MODE)) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
                                                      const int *__restrict__ level, const double *__restrict__ cx,
                                                      const double *__restrict__ eRad,
                                                      const int *__restrict__ kidStart, const int *__restrict__ nKids,
