@@ -153,8 +153,8 @@ def hip_lib():
                                        _dp, _dp, _ip, _dp, _ip]
         L.fbh_gradient_tree_dev.restype = C.c_int
         L.fbh_gradient_tree_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _ip,
-                                            _ip, _ip, C.c_double, C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int,
-                                            _dp, _dp, _ip, _dp, _ip]
+                                            _ip, _ip, C.c_double, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double,
+                                            C.c_int, _dp, _dp, _ip, _dp, _ip]
         L.fbh_tree_adjoint_dev.restype = C.c_int
         L.fbh_tree_adjoint_dev.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _ip,
                                            _ip, _ip, C.c_double, C.c_int, _vp, _dp]
@@ -486,10 +486,11 @@ class Operator:
         return dq, dk.value, dict(iters=it.value, resid=res.value, info=info.value)
 
     def gradient_tree_dev(self, ct, pos, chr_, sgm, lam, theta=0.8, rhs_tree=False, restart=30, tol=1e-6,
-                          maxit=100):
+                          maxit=100, lam_from_rhs=False):
         """Exact gradient of the raw panel-tree energy (fbh_gradient_tree_dev): (dE/dq [nChar],
         dE/dkappa, adjoint solve info); rhs_tree: the solve's right-hand side was the
-        charge-tree one; lam in/out."""
+        charge-tree one; lam: the adjoint solution out, its initial guess in (or the adjoint's
+        right-hand side with lam_from_rhs)."""
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         chr_ = np.ascontiguousarray(chr_, dtype=np.float64)
         k = {q: np.ascontiguousarray(v) for q, v in ct.items() if isinstance(v, np.ndarray)}
@@ -499,8 +500,8 @@ class Operator:
         self._ck(hip_lib().fbh_gradient_tree_dev(
             self.h, len(chr_), _d(pos), _d(chr_), len(k["level"]), _i(k["level"]), _d(k["x"]), _d(k["eRad"]),
             _i(k["kidStart"]), _i(k["nKids"]), _i(k["chgStart"]), _i(k["nChgs"]), _i(k["chgIdx"]), _i(k["order"]),
-            theta, int(bool(rhs_tree)), C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()), restart, tol, maxit,
-            _d(dq), C.byref(dk), C.byref(it), C.byref(res), C.byref(info)))
+            theta, int(bool(rhs_tree)), C.c_void_p(sgm.data_ptr()), C.c_void_p(lam.data_ptr()), int(bool(lam_from_rhs)),
+            restart, tol, maxit, _d(dq), C.byref(dk), C.byref(it), C.byref(res), C.byref(info)))
         return dq, dk.value, dict(iters=it.value, resid=res.value, info=info.value)
 
     def tree_adjoint_dev(self, ct, pos, chr_, w, kind=0, theta=0.8):
@@ -677,7 +678,7 @@ class Problem:
         if self.energy_kind == "tree":
             dq, dk, info = op.gradient_tree_dev(self.ct, self.mesh.pos, q, u, lam, theta=self.theta,
                                                 rhs_tree=self.rhs_kind == "tree", restart=self.restart,
-                                                tol=self.tol, maxit=self.maxit)
+                                                tol=self.tol, maxit=self.maxit, lam_from_rhs=True)
         else:
             dq, dk, info = op.gradient_dev(self.mesh.pos, q, u, lam, restart=self.restart, tol=self.tol,
                                            maxit=self.maxit)

@@ -46,6 +46,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -2291,6 +2292,9 @@ __global__ __launch_bounds__(256) void k_near_dk(const int2 *__restrict__ tiles,
                                                  int n, double kappa, double eps, double t) {
   __shared__ double4 sP[256], sN[256];
   __shared__ int sY[256];
+  __shared__ double T[256];   // 2^(j/256) for expNeg
+  T[threadIdx.x] = gExp2Tab[threadIdx.x];
+  const double ieps = 1.0 / eps;
   const int2 tl = tiles[blockIdx.x];
   const int D = tl.x, r0 = tl.y;
   const int rows = min(NEAR_ROWS, leafCount[D] - r0);
@@ -2322,11 +2326,20 @@ __global__ __launch_bounds__(256) void k_near_dk(const int2 *__restrict__ tiles,
       for (int j = lane; j < cn; j += 64) {
         if (sY[j] == xi) continue;
         const double4 P = sP[j], N = sN[j];
-        const double rr[3] = {px[0] - P.x, px[1] - P.y, px[2] - P.z}, ny[3] = {N.x, N.y, N.z};
-        double f[4];
-        ker4dk(kappa, eps, rr, X.nr, ny, f);
-        s0 += f[0] * N.w + f[1] * P.w;
-        s1 += f[2] * N.w + f[3] * P.w;
+        // d ker4 / d kappa (ker4dk) with 1/r from rsq64 and e^{-kappa r} from expNeg
+        const double rx = px[0] - P.x, ry = px[1] - P.y, rz = px[2] - P.z;
+        const double r2 = rx * rx + ry * ry + rz * rz, ri = rsq64(r2), rn = r2 * ri, ri2 = ri * ri;
+        const double ek = expNeg(-kappa * rn, T);
+        const double ip0 = X.nr[0] * N.x + X.nr[1] * N.y + X.nr[2] * N.z;
+        const double ipX = X.nr[0] * rx + X.nr[1] * ry + X.nr[2] * rz;
+        const double ipY = N.x * rx + N.y * ry + N.z * rz;
+        const double G0 = ri * kFourPiI, dG0dy = ipY * G0 * ri2, dG0dx = ipX * G0 * ri2;
+        const double ddG0 = (ip0 * G0 - dG0dy * ipX * 3.0) * ri2;
+        const double dcoef = -kappa * r2 * ek;
+        const double f0 = ek * kFourPiI, f1 = eps * dcoef * dG0dy, f2 = dcoef * dG0dx * ieps;
+        const double f3 = dcoef * ddG0 - kappa * ek * (2.0 - kappa * rn) * ipX * dG0dy;
+        s0 += f0 * N.w + f1 * P.w;
+        s1 += f2 * N.w + f3 * P.w;
       }
       a0[r] += s0;
       a1[r] += s1;
@@ -3492,14 +3505,17 @@ __global__ void k_fill_int(int *__restrict__ y, int n, int v) {
 // Lam_A += the sums of cluster A's accepting events, in event order (one block
 // per cluster; ns strided partial sums of each multi-index, then added in
 // stride order)
-__global__ __launch_bounds__(256) void k_adj_red_acc(const int *__restrict__ s0, const int *__restrict__ s1,
-                                                     const int *__restrict__ idx, const double *__restrict__ vals,
-                                                     int p, int nmMax, double *__restrict__ lam) {
-  __shared__ double part[256];
+#define ADJ_RED_THREADS 1024
+__global__ __launch_bounds__(ADJ_RED_THREADS) void k_adj_red_acc(const int *__restrict__ s0,
+                                                                 const int *__restrict__ s1,
+                                                                 const int *__restrict__ idx,
+                                                                 const double *__restrict__ vals, int p, int nmMax,
+                                                                 double *__restrict__ lam) {
+  __shared__ double part[ADJ_RED_THREADS];
   const int nm = NMOM(p);
   const int A = blockIdx.x, i0 = s0[A], i1 = s1[A];
   if (i1 <= i0) return;
-  const int ns = 256 / nm, t = threadIdx.x, a = t % nm, st = t / nm;
+  const int ns = ADJ_RED_THREADS / nm, t = threadIdx.x, a = t % nm, st = t / nm;
   double v = 0.0;
   if (st < ns)
     for (int i = i0 + st; i < i1; i += ns) v += vals[(long long)idx[i] * nm + a];
@@ -3696,6 +3712,7 @@ struct fbh_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
   bool ownStream = false;
+  hipStream_t auxStream = nullptr;   // the tree gradient's second stream (created on first use)
   double kappa, eps1, eps2, eps;
   double rhsFac = 0.0;   // right-hand side factor; 0: 1/(4 pi eps1), src/fabipb.c:755
   int qorder;
@@ -3949,6 +3966,10 @@ extern "C" int fbh_available(void) {
 extern "C" void fbh_destroy(fbh_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->auxStream) {
+    hipStreamSynchronize(c->auxStream);
+    hipStreamDestroy(c->auxStream);
+  }
   for (auto &b : c->bufs) hipFree(b.p);
   if (c->V) hipFree(c->V);
   if (c->w) hipFree(c->w);
@@ -5865,10 +5886,13 @@ static int kappaOperatorTerms(fbh_ctx *c, const double *sgm_dev, const double *l
           hipLaunchKernelGGL(k_spec_vals<true>, dim3(n), dim3(64), 0, c->stream, c->pnl, c->specOff, c->specCol,
                              (float4 *)nullptr, c->specDk, c->kappa, c->eps, c->Q, c->qorder);
       }
-      if (ok)
+      if (ok) {
+        // expNeg's table, also when no tile of the operator is computed on the fly
+        hipLaunchKernelGGL(k_exp2_tab, dim3(1), dim3(256), 0, c->stream);
         hipLaunchKernelGGL(k_near_dk, dim3(c->nTiles), dim3(256), 0, c->stream, c->tiles, c->leafStart,
                            c->leafCount, c->srcOff, c->ncol, c->srcList, c->pnl, sgm_dev, lam_dev, c->specOff,
                            c->specCol, c->specDk, part, n, c->kappa, c->eps, c->Q.t[0]);
+      }
     } else {
       hipLaunchKernelGGL(k_near_dk_rules, dim3(n), dim3(256), 0, c->stream, c->pnl, c->leafOf, c->srcOff, c->ncol,
                          c->srcList, sgm_dev, lam_dev, part, n, c->kappa, c->eps, c->Q, c->qorder);
@@ -6230,7 +6254,7 @@ struct GradBufs {
 // A count pass sizes each block's events; the blocks then run in batches of at
 // most ADJ_BATCH_BYTES of events: emit, sort by cluster, add per cluster.
 static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const double *w, double *lamBuf,
-                   double *phi) {
+                   double *phi, hipStream_t st) {
   if (!s.uni) { setErr("tree gradient: the accepted levels must share one expansion order"); return 0; }
   // slices: each wave's walk is split over S waves (one per share of the
   // clusters at level height), enough for ~16k waves in flight
@@ -6243,10 +6267,10 @@ static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const
   ev.cnt = B.get<long long>(3 * (size_t)blocks + 3);
   ev.err = B.get<int>(1);
   if (!ev.cnt || !ev.err) { setErr("tree gradient: allocation failed"); return 0; }
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, ev.err, 1, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, ev.err, 1, 0);
 #define ADJ_WALK(PP, KK, EE, G, G0)                                                                                \
   case PP:                                                                                                         \
-    hipLaunchKernelGGL((k_adj_walk<PP, KK, EE>), dim3(G), dim3(64), 0, c->stream, c->pnl, c->n, w, s.dLevel, s.dX,  \
+    hipLaunchKernelGGL((k_adj_walk<PP, KK, EE>), dim3(G), dim3(64), 0, st, c->pnl, c->n, w, s.dLevel, s.dX,  \
                        s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.c0, c->depth, c->height, theta, c->kappa, \
                        c->eps, fac, c->Q, c->qorder, nH, S, G0, ev);                                                \
     break;
@@ -6263,8 +6287,8 @@ static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const
   }
   HCK(hipGetLastError());
   std::vector<long long> cnt(3 * (size_t)blocks), base(3 * (size_t)blocks + 3, 0);
-  HCK(hipMemcpyAsync(cnt.data(), ev.cnt, cnt.size() * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-  HCK(hipStreamSynchronize(c->stream));
+  HCK(hipMemcpyAsync(cnt.data(), ev.cnt, cnt.size() * sizeof(long long), hipMemcpyDeviceToHost, st));
+  HCK(hipStreamSynchronize(st));
   for (int g = 0; g < blocks; g++)
     for (int k = 0; k < 3; k++) base[3 * (g + 1) + k] = base[3 * g + k] + cnt[3 * g + k];
   // batches of whole blocks: accepting events nm values and four ints, direct events three ints,
@@ -6300,8 +6324,8 @@ static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const
   int bits = 1;
   while ((1ll << bits) < nc) bits++;
   size_t tmpA = 0, tmpD = 0;
-  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, ev.accKey, keyS, ev.accIdx, idxS, (int)mA, 0, bits, c->stream));
-  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpD, ev.dirKey, keyS, ev.dirIdx, idxS, (int)mD, 0, bits, c->stream));
+  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, ev.accKey, keyS, ev.accIdx, idxS, (int)mA, 0, bits, st));
+  HCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpD, ev.dirKey, keyS, ev.dirIdx, idxS, (int)mD, 0, bits, st));
   size_t tmpBytes = std::max(tmpA, tmpD);
   void *tmp = B.get<unsigned char>(tmpBytes);
   if (!dBase || !ev.accKey || !ev.accIdx || !ev.accVals || !ev.dirKey || !ev.dirIdx || !ev.dirOff || !ev.dirVals ||
@@ -6309,7 +6333,7 @@ static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const
     setErr("tree gradient: event buffers (%lld accepting, %lld direct) allocation failed", mA, mD);
     return 0;
   }
-  HCK(hipMemcpyAsync(dBase, base.data(), base.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HCK(hipMemcpyAsync(dBase, base.data(), base.size() * sizeof(long long), hipMemcpyHostToDevice, st));
   ev.base = dBase;
   const int depth = c->depth;
   for (size_t i = 0; i + 1 < cut.size(); i++) {
@@ -6326,20 +6350,20 @@ static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const
     HCK(hipGetLastError());
     if (nA > 0) {
       size_t tb = tmpBytes;
-      HCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ev.accKey, keyS, ev.accIdx, idxS, nA, 0, bits, c->stream));
-      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s0, nc, 0);
-      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s1, nc, 0);
-      hipLaunchKernelGGL(k_seg_bounds, dim3((nA + 255) / 256), dim3(256), 0, c->stream, keyS, nA, s0, s1);
-      hipLaunchKernelGGL(k_adj_red_acc, dim3(nc), dim3(256), 0, c->stream, s0, s1, idxS, ev.accVals, s.P, c->nmMax,
-                         lamBuf);
+      HCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ev.accKey, keyS, ev.accIdx, idxS, nA, 0, bits, st));
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, st, s0, nc, 0);
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, st, s1, nc, 0);
+      hipLaunchKernelGGL(k_seg_bounds, dim3((nA + 255) / 256), dim3(256), 0, st, keyS, nA, s0, s1);
+      hipLaunchKernelGGL(k_adj_red_acc, dim3(nc), dim3(ADJ_RED_THREADS), 0, st, s0, s1, idxS, ev.accVals, s.P,
+                         c->nmMax, lamBuf);
     }
     if (nD > 0 && s.lvCount[depth] > 0) {
       size_t tb = tmpBytes;
-      HCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ev.dirKey, keyS, ev.dirIdx, idxS, nD, 0, bits, c->stream));
-      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s0, nc, 0);
-      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, c->stream, s1, nc, 0);
-      hipLaunchKernelGGL(k_seg_bounds, dim3((nD + 255) / 256), dim3(256), 0, c->stream, keyS, nD, s0, s1);
-      hipLaunchKernelGGL(k_adj_red_dir, dim3(s.lvCount[depth]), dim3(64), 0, c->stream, s.lvStart[depth], s0, s1,
+      HCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ev.dirKey, keyS, ev.dirIdx, idxS, nD, 0, bits, st));
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, st, s0, nc, 0);
+      hipLaunchKernelGGL(k_fill_int, dim3((nc + 255) / 256), dim3(256), 0, st, s1, nc, 0);
+      hipLaunchKernelGGL(k_seg_bounds, dim3((nD + 255) / 256), dim3(256), 0, st, keyS, nD, s0, s1);
+      hipLaunchKernelGGL(k_adj_red_dir, dim3(s.lvCount[depth]), dim3(64), 0, st, s.lvStart[depth], s0, s1,
                          idxS, ev.dirOff, ev.dirVals, s.dCS, s.dNC, s.dCI, phi);
     }
     HCK(hipGetLastError());
@@ -6347,8 +6371,8 @@ static int adjWalk(fbh_ctx *c, const ChargeSet &s, int kind, double theta, const
 #undef ADJ_SWITCH
 #undef ADJ_WALK
   int err = 0;
-  HCK(hipMemcpyAsync(&err, ev.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HCK(hipStreamSynchronize(c->stream));   // the buffers are freed on return
+  HCK(hipMemcpyAsync(&err, ev.err, sizeof(int), hipMemcpyDeviceToHost, st));
+  HCK(hipStreamSynchronize(st));   // the buffers are freed on return
   if (err) { setErr("tree gradient: the transposed walk's passes disagree or its stack overflowed"); return 0; }
   return 1;
 }
@@ -6378,7 +6402,7 @@ extern "C" int fbh_tree_adjoint_dev(fbh_ctx *c, int nChar, const double *pos, co
   if (!lamBuf || !phi || !dq) { setErr("fbh_tree_adjoint_dev: allocation failed"); return 0; }
   hipLaunchKernelGGL(k_fill, dim3((nc * c->nmMax + 255) / 256), dim3(256), 0, c->stream, lamBuf, nc * c->nmMax, 0.0);
   hipLaunchKernelGGL(k_fill, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, phi, nChar, 0.0);
-  if (!adjWalk(c, s, kind, theta, w_dev, lamBuf, phi) || !adjEval(c, s, lamBuf, phi, dq)) return 0;
+  if (!adjWalk(c, s, kind, theta, w_dev, lamBuf, phi, c->stream) || !adjEval(c, s, lamBuf, phi, dq)) return 0;
   HCK(hipMemcpyAsync(dq_host, dq, (size_t)nChar * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
   return 1;
@@ -6395,9 +6419,9 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
                                      const int *ctLevel, const double *ctX, const double *ctERad,
                                      const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
                                      const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
-                                     int rhsTree, const double *sgm_dev, double *lam_dev, int restrt, double tol,
-                                     int maxit, double *dEdq, double *dEdkappa, int *iters, double *resid,
-                                     int *info) {
+                                     int rhsTree, const double *sgm_dev, double *lam_dev, int lamFromRhs,
+                                     int restrt, double tol, int maxit, double *dEdq, double *dEdkappa, int *iters,
+                                     double *resid, int *info) {
   if (!c || !sgm_dev || !lam_dev || !dEdq || !dEdkappa || !iters || !resid || !info || restrt < 1 || maxit < 1 ||
       !(theta > 0.0)) {
     setErr("fbh_gradient_tree_dev: bad arguments");
@@ -6417,14 +6441,29 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   // the adjoint: A^T lam = dE/dsgm (the walk's mode 2), with dE/dkappa at
   // fixed sgm (mode 3) from the same walk
   double eK = 0.0;
-  if (!walkCharges(c, s, 4, theta, sgm_dev, g, &eK) ||
-      !gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info))
-    return 0;
-  // dE/dq
-  hipLaunchKernelGGL(k_fill, dim3((nc * c->nmMax + 255) / 256), dim3(256), 0, c->stream, lamBuf, nc * c->nmMax, 0.0);
-  hipLaunchKernelGGL(k_fill, dim3((nChar + 255) / 256), dim3(256), 0, c->stream, phi, nChar, 0.0);
-  if (!adjWalk(c, s, 0, theta, sgm_dev, lamBuf, phi)) return 0;
-  if (rhsTree && !adjWalk(c, s, 1, theta, lam_dev, lamBuf, phi)) return 0;
+  if (!walkCharges(c, s, 4, theta, sgm_dev, g, &eK)) return 0;   // ends with the stream synchronized
+  // dE/dq at fixed sgm (the transposed energy walk) does not need lam: a second
+  // host thread runs it on a second stream while this one drives the adjoint GMRES
+  if (!c->auxStream) HCK(hipStreamCreate(&c->auxStream));
+  int okAux = 1;
+  std::thread aux([&] {
+    if (hipSetDevice(c->dev) != hipSuccess) { okAux = 0; return; }
+    hipStream_t st = c->auxStream;
+    hipLaunchKernelGGL(k_fill, dim3((nc * c->nmMax + 255) / 256), dim3(256), 0, st, lamBuf, nc * c->nmMax, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3((nChar + 255) / 256), dim3(256), 0, st, phi, nChar, 0.0);
+    okAux = adjWalk(c, s, 0, theta, sgm_dev, lamBuf, phi, st);   // ends with st synchronized
+  });
+  int ok = 1;
+  if (lamFromRhs && hipMemcpyAsync(lam_dev, g, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                   c->stream) != hipSuccess) {
+    setErr("fbh_gradient_tree_dev: initial guess copy failed");
+    ok = 0;
+  }
+  ok = ok && gmresCore(c, OPT64, g, lam_dev, restrt, tol, maxit, iters, resid, nullptr, info);
+  aux.join();
+  if (!ok || !okAux) return 0;
+  // dE/dq: the right-hand side's part needs lam
+  if (rhsTree && !adjWalk(c, s, 1, theta, lam_dev, lamBuf, phi, c->stream)) return 0;
   if (!adjEval(c, s, lamBuf, phi, dq)) return 0;
   if (!rhsTree) {   // R^T lam by the direct sums
     if (c->qorder == 1)

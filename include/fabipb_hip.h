@@ -299,13 +299,17 @@ int fbh_gradient_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *c
  * dE/dq comes from the transposed walk (fbh_tree_adjoint_dev of u, plus R^T lam
  * by direct sums or by the transposed right-hand-side walk of lam); dE/dkappa
  * from the walk's kappa derivative at fixed u plus lam^T (dK/dkappa) u.  Needs
- * one expansion order over the levels the walk can accept. */
+ * one expansion order over the levels the walk can accept.  lam_dev: the
+ * adjoint solution out; its initial guess is lam_dev itself, or with
+ * lamFromRhs 1 the adjoint's right-hand side (as the plain solve starts from
+ * x0 = b). */
 int fbh_gradient_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *chr,
                           int nc, const int *ctLevel, const double *ctX, const double *ctERad,
                           const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
                           const int *ctNChgs, const int *ctChgIdx, const int *ctOrder, double theta,
-                          int rhsTree, const double *sgm_dev, double *lam_dev, int restart, double tol,
-                          int maxit, double *dEdq, double *dEdkappa, int *iters, double *resid, int *info);
+                          int rhsTree, const double *sgm_dev, double *lam_dev, int lamFromRhs, int restart,
+                          double tol, int maxit, double *dEdq, double *dEdkappa, int *iters, double *resid,
+                          int *info);
 /* The transposed walk alone: dq_host[j] = d/dq_j at fixed densities of the
  * panel-tree energy of w_dev (kind 0, w = sgm) or of lam . (charge-tree
  * right-hand side) (kind 1, w = lam).  Exact for the walk's own truncation:
