@@ -425,7 +425,7 @@ def capsid(fa, torch, tmp):
         lam_t = torch.zeros_like(xs)   # each call solves the adjoint from zero
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        dqt, dkt, tinfo = op.gradient_tree_dev(S.ct, mesh.pos, q, xs, lam_t, theta=0.8, rhs_tree=True, lam_from_rhs=True,
+        dqt, dkt, tinfo = op.gradient_tree_dev(S.ct, mesh.pos, q, xs, lam_t, theta=0.8, rhs_tree=True,
                                                restart=RESTART, tol=TOL, maxit=200)
         torch.cuda.synchronize()
         t_gtree = time.perf_counter() - t0

@@ -678,7 +678,7 @@ class Problem:
         if self.energy_kind == "tree":
             dq, dk, info = op.gradient_tree_dev(self.ct, self.mesh.pos, q, u, lam, theta=self.theta,
                                                 rhs_tree=self.rhs_kind == "tree", restart=self.restart,
-                                                tol=self.tol, maxit=self.maxit, lam_from_rhs=True)
+                                                tol=self.tol, maxit=self.maxit)
         else:
             dq, dk, info = op.gradient_dev(self.mesh.pos, q, u, lam, restart=self.restart, tol=self.tol,
                                            maxit=self.maxit)
