@@ -3963,6 +3963,22 @@ extern "C" int fbh_available(void) {
   return 1;
 }
 
+// A stream on a hardware queue of its own.  HIP hands out a small pool of
+// hardware queues to the process's streams, and streams that land on one queue
+// run one after the other: the nearfield on the side stream would then wait
+// for the whole far chain (or the reverse) instead of sharing the CUs with it.
+// A stream created with a CU mask gets a queue outside the pool; the mask here
+// holds every CU.
+static hipError_t streamOwnQueue(hipStream_t *s) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+    return hipStreamCreate(s);
+  std::vector<uint32_t> mask((ncu + 31) / 32, 0xffffffffu);
+  if (ncu % 32) mask.back() = (1u << (ncu % 32)) - 1u;
+  return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
+}
+
 extern "C" void fbh_destroy(fbh_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
@@ -4405,7 +4421,7 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
     c->ownStream = true;
   }
 #define FAIL() do { fbh_destroy(c); return nullptr; } while (0)
-  if (hipStreamCreate(&c->side) != hipSuccess ||
+  if (streamOwnQueue(&c->side) != hipSuccess ||
       hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess) {
     setErr("stream/event creation failed");
@@ -6444,7 +6460,7 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   if (!walkCharges(c, s, 4, theta, sgm_dev, g, &eK)) return 0;   // ends with the stream synchronized
   // dE/dq at fixed sgm (the transposed energy walk) does not need lam: a second
   // host thread runs it on a second stream while this one drives the adjoint GMRES
-  if (!c->auxStream) HCK(hipStreamCreate(&c->auxStream));
+  if (!c->auxStream) HCK(streamOwnQueue(&c->auxStream));
   int okAux = 1;
   std::thread aux([&] {
     if (hipSetDevice(c->dev) != hipSuccess) { okAux = 0; return; }
