@@ -600,10 +600,10 @@ class Operator:
         return {"fp64": (int(cnt[0]), float(ms[0])), "fp32": (int(cnt[1]), float(ms[1]))}
 
     def profile(self, x, y, iters=10, precision="fp64"):
-        ms = np.zeros(8)
+        ms = np.zeros(9)
         f = hip_lib().fbh_profile32 if precision == "fp32" else hip_lib().fbh_profile
         self._ck(f(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), iters, _d(ms)))
-        return dict(zip(["near", "q2m", "m2m", "m2l", "l2l", "l2p", "combine", "total"], ms))
+        return dict(zip(["near", "q2m", "m2m", "m2l", "l2l", "l2p", "combine", "total", "far_chain"], ms))
 
 
 class Problem:

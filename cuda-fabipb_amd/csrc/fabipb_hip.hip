@@ -1958,6 +1958,222 @@ __global__ __launch_bounds__(256) void k_l2p(const R *__restrict__ Q0, const R *
   }
 }
 
+// L2L down one leaf's own ancestry and L2P, in one launch (src/expan.c:403-452,
+// convL2L :295-326; src/fmm.c:815-828): one block per leaf.  The block starts
+// from the M2L locals of its ancestor at the top FMM level and translates them
+// down to the leaf one level at a time, adding each cube's own M2L locals on
+// the way; siblings repeat the steps above their parent (arithmetic the
+// level-by-level launches share) in exchange for no launch per level and no
+// wait between levels.  A step L'[a] = sum_b t^b/b! L[a+b] (|a+b| <= p_in) is
+// done one axis at a time, z, then y, then x, each pass
+//   M[g] = sum_{k <= p_in - |g|} t_d^k/k! L[g + k e_d],
+// which gives the same sum (the three bounds add up to |a| + |b| <= p_in) in
+// at most p_in + 1 terms per slot instead of up to nMom(p_in); only the
+// rounding differs from the level-by-level L2L.  The global locals are only
+// read; the intermediate locals stay in LDS.  Then the leaf's panels as k_l2p.
+#define L2LP_MAXUP 16
+struct LevOrd {   // expansion order of the levels height .. depth
+  int p[L2LP_MAXUP + 1];
+};
+__device__ __forceinline__ int idx3(int i1, int i2, int i3) {   // graded: degree, then i1, then i2
+  const int d = i1 + i2 + i3;
+  return NMOM(d - 1) + i1 * (d + 1) - i1 * (i1 - 1) / 2 + i2;
+}
+template <typename R, int V>
+__global__ __launch_bounds__(256) void k_l2lp(const R *__restrict__ Q0, const R *__restrict__ Q1,
+                                              const int *__restrict__ leafStart, const int *__restrict__ leafCount,
+                                              const int *__restrict__ parent, LevOrd O,
+                                              const double *__restrict__ cubeX, const R *__restrict__ lec,
+                                              R *__restrict__ far, int n, int nm, int nmMax, int nUp, DevTabs T) {
+  typedef R VT __attribute__((ext_vector_type(V)));
+  typedef typename Vec4<R>::type R4;
+  R *sm = reinterpret_cast<R *>(dynLds);
+  __shared__ int chain[L2LP_MAXUP + 1];   // chain[nUp] = the leaf, chain[s - 1] = parent of chain[s]
+  __shared__ R co[3][MAXP + 1];           // t_d^k / k! of the current step
+  const int L = blockIdx.x, t = threadIdx.x;
+  // locals [slot][l1..l4]: two working buffers, then the M2L locals of chain[1..nUp]
+  R4 *cur = reinterpret_cast<R4 *>(sm), *nxt = cur + nmMax, *own = nxt + nmMax;
+  if (t == 0) {
+    int cb = L;
+    for (int s = nUp; s >= 0; s--) {
+      chain[s] = cb;
+      if (s > 0) cb = parent[cb];
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s <= nUp; s++) {   // every M2L local the block needs, loads all in flight at once
+    const R *l = lec + (long long)chain[s] * 4 * nmMax;
+    R4 *dst = s == 0 ? cur : own + (s - 1) * nmMax;
+    for (int a = t; a < nmMax; a += blockDim.x) {
+      R4 v;
+      v.x = l[a];
+      v.y = l[nmMax + a];
+      v.z = l[2 * nmMax + a];
+      v.w = l[3 * nmMax + a];
+      dst[a] = v;
+    }
+  }
+  for (int s = 1; s <= nUp; s++) {
+    const int kid = chain[s], par = chain[s - 1];
+    const int pin = O.p[s - 1], nIn = NMOM(pin), nOut = NMOM(O.p[s]);
+    if (t < 3) {
+      const double tr = cubeX[3 * kid + t] - cubeX[3 * par + t];
+      double f = 1.0;
+      for (int k = 0; k <= pin; k++) {
+        co[t][k] = (R)f;
+        f = f * tr / (k + 1);
+      }
+    }
+    __syncthreads();
+    for (int d = 2; d >= 0; d--) {   // z, y, x
+      const R *cd = co[d];
+      const R4 *lk = own + (s - 1) * nmMax;
+      const int nO = d == 0 ? nOut : nIn;
+      for (int a = t; a < nO; a += blockDim.x) {
+        const int4 m = T.mi[a];
+        const int i1 = m.x & 0xff, i2 = (m.x >> 8) & 0xff, i3 = (m.x >> 16) & 0xff, kmax = pin - deg_of(m);
+        R4 u;
+        u.x = u.y = u.z = u.w = 0;
+        for (int k = 0; k <= kmax; k++) {
+          const R4 v = cur[idx3(i1 + (d == 0 ? k : 0), i2 + (d == 1 ? k : 0), i3 + (d == 2 ? k : 0))];
+          const R c = cd[k];
+          u.x += c * v.x;
+          u.y += c * v.y;
+          u.z += c * v.z;
+          u.w += c * v.w;
+        }
+        if (d == 0) {   // the kid's own M2L locals
+          const R4 w = lk[a];
+          u.x += w.x;
+          u.y += w.y;
+          u.z += w.z;
+          u.w += w.w;
+        }
+        nxt[a] = u;
+      }
+      __syncthreads();
+      R4 *sw = cur;
+      cur = nxt;
+      nxt = sw;
+    }
+  }
+  __syncthreads();
+  // L2P from the leaf's locals (k_l2p's arithmetic)
+  const int G = nm / V, rows = max(1, 256 / G), g = t % G, r = t / G, s0 = leafStart[L], c = leafCount[L];
+  VT l12, l34;
+  for (int v = 0; v < V; v++) {
+    const R4 w = cur[g * V + v];
+    l12[v] = w.x + w.y;
+    l34[v] = w.z + w.w;
+  }
+  __syncthreads();   // the partial sums below overlay the locals
+  R *P0 = sm, *P1 = sm + L2P_CH * G;
+  for (int j0 = 0; j0 < c; j0 += L2P_CH) {
+    const int cn = min(L2P_CH, c - j0);
+    if (r < rows) {
+#pragma unroll 2
+      for (int jj = r; jj < cn; jj += rows) {
+        const long long col = (long long)(s0 + j0 + jj) * nm + g * V;
+        const VT q0 = *reinterpret_cast<const VT *>(Q0 + col) * l12;
+        const VT q1 = *reinterpret_cast<const VT *>(Q1 + col) * l34;
+        R t0 = q0[0], t1 = q1[0];
+        for (int v = 1; v < V; v++) {
+          t0 += q0[v];
+          t1 += q1[v];
+        }
+        P0[jj * G + g] = t0;
+        P1[jj * G + g] = t1;
+      }
+    }
+    __syncthreads();
+    for (int jj = t; jj < cn; jj += blockDim.x) {
+      R t0 = 0, t1 = 0;
+      for (int k = 0; k < G; k++) {
+        t0 += P0[jj * G + k];
+        t1 += P1[jj * G + k];
+      }
+      far[s0 + j0 + jj] = t0;
+      far[s0 + j0 + jj + n] = t1;
+    }
+    __syncthreads();
+  }
+}
+
+// M2M of one level (src/expan.c:342-382, convM2M :223-247): one block per
+// parent, one wave per child.  A translation M'[a] = sum_{b <= a} t^b/b! M[a-b]
+// runs one axis at a time (z, y, x; at most p + 1 terms per slot instead of up
+// to nMom(p) for the full convolution of k_m2m, so the block's chain of
+// dependent LDS reads is short); only the rounding differs from k_m2m.  The
+// children's contributions are added in child order.
+template <typename R>
+__global__ __launch_bounds__(512) void k_m2m_ax(int levStart, int pOut, int nIn, const int *__restrict__ kidStart,
+                                                const int *__restrict__ nKids, const double *__restrict__ cubeX,
+                                                R *__restrict__ mom, int nmMax, DevTabs T) {
+  typedef typename Vec2<R>::type R2;
+  R *sm = reinterpret_cast<R *>(dynLds);
+  const int par = levStart + blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63, nw = blockDim.x >> 6;
+  const int nk = nKids[par], k0 = kidStart[par], nOut = NMOM(pOut);
+  R2 *kidT = reinterpret_cast<R2 *>(sm);   // [8][nmMax] translated children
+  R2 *wa = kidT + 8 * nmMax + 2 * w * nmMax, *wb = wa + nmMax;
+  R *co = reinterpret_cast<R *>(kidT + 8 * nmMax + 2 * nw * nmMax) + w * 3 * (MAXP + 1);
+  for (int k = w; k < nk; k += nw) {
+    const int kid = k0 + k;
+    const R *mk = mom + (long long)kid * 2 * nmMax;
+    for (int a = lane; a < nOut; a += 64) {   // children of a lower order: zeros above it
+      R2 v;
+      const int ai = a < nIn ? a : 0;
+      const R x0 = mk[ai], x1 = mk[nmMax + ai];
+      v.x = a < nIn ? x0 : (R)0;
+      v.y = a < nIn ? x1 : (R)0;
+      wa[a] = v;
+    }
+    if (lane < 3) {
+      const double tr = cubeX[3 * kid + lane] - cubeX[3 * par + lane];
+      double f = 1.0;
+      for (int j = 0; j <= pOut; j++) {
+        co[lane * (MAXP + 1) + j] = (R)f;
+        f = f * tr / (j + 1);
+      }
+    }
+    waveSync();
+    R2 *src = wa, *dst = wb;
+    for (int d = 2; d >= 0; d--) {   // z, y, x
+      R2 *out = d == 0 ? kidT + k * nmMax : dst;
+      const R *cd = co + d * (MAXP + 1);
+      for (int a = lane; a < nOut; a += 64) {
+        const int4 m = T.mi[a];
+        const int i1 = m.x & 0xff, i2 = (m.x >> 8) & 0xff, i3 = (m.x >> 16) & 0xff;
+        const int kmax = d == 0 ? i1 : d == 1 ? i2 : i3;
+        R2 u;
+        u.x = u.y = 0;
+        for (int j = 0; j <= kmax; j++) {
+          const R2 v = src[idx3(i1 - (d == 0 ? j : 0), i2 - (d == 1 ? j : 0), i3 - (d == 2 ? j : 0))];
+          const R c = cd[j];
+          u.x += c * v.x;
+          u.y += c * v.y;
+        }
+        out[a] = u;
+      }
+      waveSync();
+      R2 *sw = src;
+      src = dst;
+      dst = sw;
+    }
+  }
+  __syncthreads();
+  R *mp = mom + (long long)par * 2 * nmMax;
+  for (int a = t; a < nOut; a += blockDim.x) {
+    R s1 = 0, s2 = 0;
+    for (int k = 0; k < nk; k++) {
+      const R2 v = kidT[k * nmMax + a];
+      s1 += v.x;
+      s2 += v.y;
+    }
+    mp[a] = s1;
+    mp[nmMax + a] = s2;
+  }
+}
+
 // y = alpha*(D x - far - near) + beta*y (src/fabipb.c:1271-1279)
 template <typename R>
 __global__ void k_combine(const double *__restrict__ area, const double *__restrict__ x, const R *__restrict__ far,
@@ -4737,12 +4953,58 @@ static void l2pPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
   launchL2P<R>(c, B.Q0, B.Q1, B.lec, B.far, st);
 }
 
-// far field into B.far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P
+// L2L (every level) and L2P in one launch; the locals in B.lec stay the M2L ones.
+// False (nothing launched) when the block's tables do not fit in LDS.
 template <typename R>
-static void farPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st) {
+static size_t l2lpLds(const fbh_ctx *c) {
+  const int V = vecWidth<R>(c->nmLeaf), G = c->nmLeaf / V;
+  return std::max(4 * (size_t)(c->depth - c->height + 2) * c->nmMax * sizeof(R),
+                  2 * (size_t)L2P_CH * G * sizeof(R));
+}
+template <typename R>
+static bool l2lpFits(const fbh_ctx *c) {
+  return l2lpLds<R>(c) <= (size_t)c->ldsMax && c->depth - c->height <= L2LP_MAXUP;
+}
+template <typename R>
+static bool l2lpPass(fbh_ctx *c, const OpBufs<R> &B, hipStream_t st) {
+  if (!l2lpFits<R>(c)) return false;
+  const int V = vecWidth<R>(c->nmLeaf), nmx = c->nmMax, nUp = c->depth - c->height;
+  const size_t lds = l2lpLds<R>(c);
+  LevOrd O{};
+  for (int s = 0; s <= nUp; s++) O.p[s] = c->ordMom[c->height + s];
+  if (V == 4)
+    hipLaunchKernelGGL((k_l2lp<R, 4>), dim3(c->nLeaf), dim3(256), lds, st, B.Q0, B.Q1, c->leafStart, c->leafCount,
+                       c->parent, O, c->cubeX, B.lec, B.far, c->n, c->nmLeaf, nmx, nUp, c->T);
+  else if (V == 2)
+    hipLaunchKernelGGL((k_l2lp<R, 2>), dim3(c->nLeaf), dim3(256), lds, st, B.Q0, B.Q1, c->leafStart, c->leafCount,
+                       c->parent, O, c->cubeX, B.lec, B.far, c->n, c->nmLeaf, nmx, nUp, c->T);
+  else
+    hipLaunchKernelGGL((k_l2lp<R, 1>), dim3(c->nLeaf), dim3(256), lds, st, B.Q0, B.Q1, c->leafStart, c->leafCount,
+                       c->parent, O, c->cubeX, B.lec, B.far, c->n, c->nmLeaf, nmx, nUp, c->T);
+  return true;
+}
+
+// M2M of one level by axis-wise translation (k_m2m_ax); false when its LDS does not fit
+template <typename R>
+static bool m2mAxPass(fbh_ctx *c, const OpBufs<R> &B, int lev, hipStream_t st) {
+  const size_t lds = (24 * (size_t)c->nmMax * 2 + 8 * 3 * (MAXP + 1)) * sizeof(R);
+  if (lds > (size_t)c->ldsMax) return false;
+  if (c->levCount[lev] > 0)
+    hipLaunchKernelGGL(k_m2m_ax<R>, dim3(c->levCount[lev]), dim3(512), lds, st, c->levStart[lev], c->ordMom[lev],
+                       NMOM(c->ordMom[lev + 1]), c->kidStart, c->nKids, c->cubeX, B.mom, c->nmMax, c->T);
+  return true;
+}
+
+// far field into B.far: Q2M, M2M (finest parent level up), M2L, L2L (down), L2P.
+// staged: L2L level by level into B.lec and a separate L2P (the stage ABI reads
+// the locals of every cube afterwards); else L2L and L2P in one launch.
+template <typename R>
+static void farPass(fbh_ctx *c, const OpBufs<R> &B, const double *x, hipStream_t st, bool staged = false) {
   q2mPass(c, B, x, st);
-  for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, st);
+  for (int lev = c->depth - 1; lev >= c->height; lev--)
+    if (staged || !m2mAxPass(c, B, lev, st)) m2mPass(c, B, lev, st);
   m2lPass(c, B, st);
+  if (!staged && l2lpPass(c, B, st)) return;
   for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, B, lev, st);
   l2pPass(c, B, st);
 }
@@ -5012,7 +5274,7 @@ static int stageApply(fbh_ctx *c, double alpha, const double *sgm, double beta, 
   size_t b = 2 * (size_t)c->n * sizeof(double);
   HCK(hipMemcpyAsync(c->xs, sgm, b, hipMemcpyHostToDevice, c->stream));
   HCK(hipMemcpyAsync(c->ys, pot, b, hipMemcpyHostToDevice, c->stream));
-  if (useFar) farPass(c, bufs64(c), c->xs, c->stream);
+  if (useFar) farPass(c, bufs64(c), c->xs, c->stream, true);
   if (useNear) nearPass(c, bufs64(c), c->xs, c->stream);
   int n2 = 2 * c->n;
   hipLaunchKernelGGL(k_accum, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->far, c->nearv, c->ys, n2, alpha,
@@ -6501,7 +6763,9 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   return 1;
 }
 // per-stage device times of one operator application, stages run back to back
-// on one stream (near, Q2M, M2M, M2L, L2L, L2P, combine, total)
+// on one stream (near, Q2M, M2M, M2L, L2L, L2P, combine, total); where L2L
+// and L2P are one launch, its time is the L2P entry and L2L is zero.  Then the
+// far chain alone with no events between its launches (ms[8]).
 template <typename R>
 static int profileT(fbh_ctx *c, const OpBufs<R> &B, const double *x, double *y, int iters, double *ms) {
   hipEvent_t ev[9];
@@ -6515,13 +6779,19 @@ static int profileT(fbh_ctx *c, const OpBufs<R> &B, const double *x, double *y, 
     hipEventRecord(ev[1], c->stream);
     q2mPass(c, B, x, c->stream);
     hipEventRecord(ev[2], c->stream);
-    for (int lev = c->depth - 1; lev >= c->height; lev--) m2mPass(c, B, lev, c->stream);
+    for (int lev = c->depth - 1; lev >= c->height; lev--)
+      if (!m2mAxPass(c, B, lev, c->stream)) m2mPass(c, B, lev, c->stream);
     hipEventRecord(ev[3], c->stream);
     m2lPass(c, B, c->stream);
     hipEventRecord(ev[4], c->stream);
-    for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, B, lev, c->stream);
-    hipEventRecord(ev[5], c->stream);
-    l2pPass(c, B, c->stream);
+    if (l2lpFits<R>(c)) {   // one launch: all of it counts as L2P
+      hipEventRecord(ev[5], c->stream);
+      l2lpPass(c, B, c->stream);
+    } else {
+      for (int lev = c->height; lev < c->depth; lev++) l2lPass(c, B, lev, c->stream);
+      hipEventRecord(ev[5], c->stream);
+      l2pPass(c, B, c->stream);
+    }
     hipEventRecord(ev[6], c->stream);
     hipLaunchKernelGGL(k_combine<R>, dim3((n2 + 255) / 256), dim3(256), 0, c->stream, c->area, x, B.far, B.nearv, y,
                        c->n, 1.0, 0.0, s1, s2, 1, 1);
@@ -6537,6 +6807,18 @@ static int profileT(fbh_ctx *c, const OpBufs<R> &B, const double *x, double *y, 
     ms[7] += t;
   }
   for (int i = 0; i < 8; i++) ms[i] /= iters;
+  // the far chain alone, back to back: events only around the whole chain
+  ms[8] = 0.0;
+  for (int it = 0; it < iters; it++) {
+    hipEventRecord(ev[0], c->stream);
+    farPass(c, B, x, c->stream);
+    hipEventRecord(ev[1], c->stream);
+    HCK(hipEventSynchronize(ev[1]));
+    float t = 0.f;
+    hipEventElapsedTime(&t, ev[0], ev[1]);
+    ms[8] += t;
+  }
+  ms[8] /= iters;
   for (int i = 0; i < 9; i++) hipEventDestroy(ev[i]);
   return 1;
 }

@@ -324,7 +324,9 @@ int fbh_tree_adjoint_dev(fbh_ctx *ctx, int nChar, const double *pos, const doubl
 
 /* per-stage device timing over `iters` matvecs (HIP events on the context stream,
  * stages back to back): ms[0..7] = near, q2m, m2m, m2l, l2l, l2p, combine, whole
- * matvec */
+ * matvec (where L2L runs inside the L2P launch, l2l is the empty gap between two
+ * events); ms[8] = the far chain (Q2M .. L2P) alone with events only around it.
+ * ms holds 9 doubles. */
 int fbh_profile(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
 /* the same for the FP32 operator (after fbh_enable_fp32) */
 int fbh_profile32(fbh_ctx *ctx, const double *x_dev, double *y_dev, int iters, double *ms);
