@@ -1733,11 +1733,12 @@ __global__ __launch_bounds__(256) void k_m2l_grid(const int *__restrict__ m2lOff
                                                   DevTabs T, const unsigned *__restrict__ gridTerms,
                                                   const int *__restrict__ gridOff, const int *__restrict__ redStart,
                                                   const unsigned short *__restrict__ redCell, int slotZ, int nccP,
-                                                  int nggP) {
+                                                  int nggP, const int *__restrict__ cubeOrder) {
   typedef typename Vec2<R>::type R2;
   constexpr int NR = K / L;
   constexpr int NQ = K <= 8 ? 1 : K <= 32 ? 2 : 3;   // moments per lane: nMom(5) = 56, nMom(7) = 120, nMom(8) = 165
-  const int c = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
+  // cubes with the longest interaction lists first, so that none starts last
+  const int c = cubeOrder[blockIdx.x], tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
   const int p = ordM2L[cubeLevel[c]], nm = NMOM(p);
   R2 *__restrict__ cc = reinterpret_cast<R2 *>(dynLds) + w * (nccP + nggP);   // c1 = sgn M_pot, c2 = sgn M_dpdn
   R2 *__restrict__ gg = cc + nccP;                                            // a0 = D G0, ak = D Gk
@@ -3974,6 +3975,8 @@ struct fbh_ctx {
   // M2L term grid (k_m2l_grid): K steps x 64 lanes per order, runs of L steps
   int gridK = 0, gridL = 0, gridSlotZ = 0, gridNcc = 0, gridNgg = 0;
   unsigned *gridTerms = nullptr;
+  int *m2lOrder = nullptr;     // cubes by interaction-list length, longest first (k_m2l_grid)
+  std::vector<int> hM2lOff;    // host copy of m2lOff
   int *gridOff = nullptr, *gridRedStart = nullptr;
   unsigned short *gridRedCell = nullptr;
   size_t gridLds = 0, gridLds32 = 0;
@@ -4540,6 +4543,12 @@ static bool buildM2LGrid(fbh_ctx *c) {
     return std::max((size_t)nw * (ncc + ngg) * 2 * elem, (size_t)(K / L) * 64 * 4 * elem);
   };
   if (lds(sizeof(double)) > (size_t)c->ldsMax) return true;
+  std::vector<int> order(c->nCube);   // cubes by interaction-list length, longest first
+  for (int i = 0; i < c->nCube; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c->hM2lOff[a + 1] - c->hM2lOff[a] > c->hM2lOff[b + 1] - c->hM2lOff[b];
+  });
+  if (!(c->m2lOrder = dupload(c, order.data(), order.size()))) return false;
   if (!(c->gridTerms = dupload(c, allTerms.data(), allTerms.size())) ||
       !(c->gridOff = dupload(c, off.data(), off.size())) ||
       !(c->gridRedStart = dupload(c, allStart.data(), allStart.size())) ||
@@ -4630,6 +4639,7 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
   c->hNKids.assign(nKids, nKids + nCube);
   c->hLeafStart.assign(leafStart, leafStart + nLeaf);
   c->hLeafCount.assign(leafCount, leafCount + nLeaf);
+  c->hM2lOff.assign(m2lOff, m2lOff + nCube + 1);
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {
@@ -4855,7 +4865,8 @@ template <typename R, int K, int L, bool TR = false, bool DK = false>
 static void m2lGridLaunch(fbh_ctx *c, const R *G, const R *mom, R *lec, size_t lds, hipStream_t st) {
   hipLaunchKernelGGL((k_m2l_grid<R, K, L, TR, DK>), dim3(c->nCube), dim3(256), lds, st, c->m2lOff, c->m2lSrc,
                      c->cubeLevelD, c->ordM2LD, c->ordMomD, G, mom, lec, c->nmMax, (R)c->eps, c->T, c->gridTerms,
-                     c->gridOff, c->gridRedStart, c->gridRedCell, c->gridSlotZ, c->gridNcc, c->gridNgg);
+                     c->gridOff, c->gridRedStart, c->gridRedCell, c->gridSlotZ, c->gridNcc, c->gridNgg,
+                     c->m2lOrder);
 }
 
 // k_m2l_grid at the shape the plan was built for; false when there is no grid
