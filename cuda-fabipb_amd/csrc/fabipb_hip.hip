@@ -1014,21 +1014,29 @@ __global__ __launch_bounds__(64) void k_spec_vals(const Pnl *__restrict__ pnl, c
 
 #define FLY_CHUNK 512
 #define FLY_RB 4   // rows per pass over the staged sources
+// A quadrature point in the frame of a destination leaf: (hi - O) + lo with O
+// the hi point of the leaf's first panel.  hi - O is exact (Sterbenz) for the
+// points of a leaf and its neighbours away from the coordinate planes and at
+// worst rounded at their small magnitude, so a difference of two local points
+// carries an absolute error of a few 1e-7 of the neighbourhood's extent.
+__device__ __forceinline__ float4 flyLocal(const float4 &hi, const float4 &lo, const float4 &O) {
+  return make_float4((hi.x - O.x) + lo.x, (hi.y - O.y) + lo.y, (hi.z - O.z) + lo.z, 0.f);
+}
 // One FP32 pair (kernelKER4 at the one-point rule, src/kernel.c:109-147, with
 // G0 = 1/(4 pi r) replaced by 1/r; the row's total is scaled by 1/(4 pi) once):
-// r from the hi + lo point pairs, 1/r from the hardware reciprocal square root
-// (r2 is never denormal: the closest distinct quadrature points are 0.1 A apart,
-// the panel itself has r2 set to 1), e^{-kappa r} from the hardware 2^x with
-// kappa log2(e) folded into one constant nk = -kappa / ln 2.  1 - e and
-// (1 + kappa r) e - 1 are formed directly: their absolute error (~1e-7) stays
-// inside the FP32 operator's error budget, and the pairs closest together (a
-// shared vertex or edge, or the panel itself) carry stored corrections taken
-// against this same FP32 formula.  48 VALU + 2 transcendental instructions.
+// r from the two points in their destination leaf's frame (flyLocal), 1/r from
+// the hardware reciprocal square root (r2 is never denormal: the closest
+// distinct quadrature points are 0.1 A apart, the panel itself has r2 set to
+// 1), e^{-kappa r} from the hardware 2^x with kappa log2(e) folded into one
+// constant nk = -kappa / ln 2.  1 - e and (1 + kappa r) e - 1 are formed
+// directly: their absolute error (~1e-7) stays inside the FP32 operator's error
+// budget, and the pairs closest together (a shared vertex or edge, or the
+// panel itself) carry stored corrections taken against this same FP32 formula
+// on the same local points.  42 VALU + 2 transcendental instructions.
 // Returns the four kernels y0..y3 (times 4 pi, unweighted) of the pair.
-__device__ __forceinline__ float4 fly32Ker(const float4 &XP, const float4 &XN, const float4 &XL, const float4 &P,
-                                           const float4 &N, const float4 &Lo, bool skip, float kappa, float nk,
-                                           float kap2, float eps, float ieps) {
-  const float rx = (XP.x - P.x) + (XL.x - Lo.x), ry = (XP.y - P.y) + (XL.y - Lo.y), rz = (XP.z - P.z) + (XL.z - Lo.z);
+__device__ __forceinline__ float4 fly32Ker(const float4 &XQ, const float4 &XN, const float4 &Q, const float4 &N,
+                                           bool skip, float kappa, float nk, float kap2, float eps, float ieps) {
+  const float rx = XQ.x - Q.x, ry = XQ.y - Q.y, rz = XQ.z - Q.z;
   const float r2 = skip ? 1.f : fmaf(rx, rx, fmaf(ry, ry, rz * rz));
   const float ri = __builtin_amdgcn_rsqf(r2), rr = r2 * ri;
   const float e = __builtin_amdgcn_exp2f(rr * nk), t = kappa * rr;
@@ -1046,18 +1054,17 @@ __device__ __forceinline__ float4 fly32Ker(const float4 &XP, const float4 &XN, c
 // column per lane (FP32).  CHK: the columns may hold the row's own panel.
 template <bool CHK>
 __device__ __forceinline__ void fly32Cols(int h, int j0, int j1, int lane, int w, int rows, const float4 *sP,
-                                          const float4 *sN, const float4 *sL, const float4 (&XP)[4],
-                                          const float4 (&XN)[4], const float4 (&XL)[4], const int (&xi)[4],
-                                          float kappa, float nk, float kap2, float eps, float ieps, float (&a0)[4],
-                                          float (&a1)[4]) {
+                                          const float4 *sN, const int *sY, const float4 (&XQ)[4],
+                                          const float4 (&XN)[4], const int (&xi)[4], float kappa, float nk,
+                                          float kap2, float eps, float ieps, float (&a0)[4], float (&a1)[4]) {
   for (int j = j0 + lane; j < j1; j += 64) {
-    const float4 Lo = sL[j], P = sP[j], N = sN[j];
-    const int y = __float_as_int(Lo.w);
+    const float4 P = sP[j], N = sN[j];
+    const int y = CHK ? sY[j] : -1;
 #pragma unroll
     for (int r = h; r < h + FLY_RB; r++) {
       if (w + 4 * r >= rows) continue;
       const bool skip = CHK && y == xi[r];   // the panel itself: weights zeroed, r2 replaced by 1
-      const float4 k = fly32Ker(XP[r], XN[r], XL[r], P, N, Lo, skip, kappa, nk, kap2, eps, ieps);
+      const float4 k = fly32Ker(XQ[r], XN[r], P, N, skip, kappa, nk, kap2, eps, ieps);
       const float pw = skip ? 0.f : P.w, nw = skip ? 0.f : N.w;
       a0[r] = fmaf(k.x, nw, fmaf(k.y, pw, a0[r]));
       a1[r] = fmaf(k.z, nw, fmaf(k.w, pw, a1[r]));
@@ -1072,12 +1079,15 @@ __device__ __forceinline__ void fly32Cols(int h, int j0, int j1, int lane, int w
 // difference.  One block per row.
 __global__ __launch_bounds__(64) void k_spec_vals32(const Pnl *__restrict__ pnl, const float4 *__restrict__ PA,
                                                     const float4 *__restrict__ PN, const float4 *__restrict__ PL,
+                                                    const int *__restrict__ leafOf,
+                                                    const int *__restrict__ leafStart,
                                                     const long long *__restrict__ specOff,
                                                     const int *__restrict__ specCol, float4 *__restrict__ v32,
                                                     double kappa, double eps, QTab Q, int q) {
   const int x = blockIdx.x;
   const Pnl X = pnl[x];
-  const float4 XP = PA[x], XN = PN[x], XL = PL[x];
+  const float4 O = PA[leafStart[leafOf[x]]];   // the frame of x's leaf, as its tiles use it
+  const float4 XP = PA[x], XN = PN[x], XQ = flyLocal(XP, PL[x], O);
   const float kf = (float)kappa, ef = (float)eps, nk = -kf * (float)M_LOG2E, kap2 = kf * kf, ieps = 1.f / ef;
   const double sx = (double)(XP.w * (float)kFourPiI);
   for (long long e = specOff[x] + threadIdx.x; e < specOff[x + 1]; e += 64) {
@@ -1086,7 +1096,7 @@ __global__ __launch_bounds__(64) void k_spec_vals32(const Pnl *__restrict__ pnl,
     panelIA0(X, pnl[y], x == y, kappa, eps, Q, q, k, false);
     if (x != y) {
       const float4 P = PA[y];
-      const float4 kk = fly32Ker(XP, XN, XL, P, PN[y], PL[y], false, kf, nk, kap2, ef, ieps);
+      const float4 kk = fly32Ker(XQ, XN, flyLocal(P, PL[y], O), PN[y], false, kf, nk, kap2, ef, ieps);
       const double s = sx * (double)P.w;
       f[0] = kk.x * s;
       f[1] = kk.y * s;
@@ -1097,7 +1107,7 @@ __global__ __launch_bounds__(64) void k_spec_vals32(const Pnl *__restrict__ pnl,
   }
 }
 
-// LDS: (p hi, A sigma), (n, A sigma_n), (p lo, source panel index), FLY_CHUNK each
+// LDS: (p in the destination leaf's frame, A sigma), (n, A sigma_n), source panel index; FLY_CHUNK each
 __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ tiles,
                                               const int *__restrict__ leafStart, const int *__restrict__ leafCount,
                                               const int *__restrict__ srcOff, const int *__restrict__ ncolArr,
@@ -1106,7 +1116,7 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
                                               const double *__restrict__ x, const long long *__restrict__ specOff,
                                               const int *__restrict__ specCol, const float4 *__restrict__ specVal,
                                               float *__restrict__ out, int n, float kappa, float eps, float4 *sP,
-                                              float4 *sN, float4 *sL, const int *__restrict__ ownOff) {
+                                              float4 *sN, int *sY, const int *__restrict__ ownOff) {
   const int2 tl = tiles[t];
   const int D = tl.x, r0 = tl.y;
   const int oc = ownOff[D], nOwn = oc >= 0 ? leafCount[D] : 0;   // D's own columns: [oc, oc + nOwn)
@@ -1115,15 +1125,18 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
   const int *src = srcList + srcOff[D];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float k4pi = (float)kFourPiI, kap2 = kappa * kappa, ieps = 1.f / eps, nk = -kappa * (float)M_LOG2E;
-  float4 XP[4], XN[4], XL[4];
+  const float4 O = PA[leafStart[D]];   // the leaf's frame
+  float4 XQ[4], XN[4];
+  float XA[4];
   int xi[4];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int row = w + 4 * r;
     xi[r] = leafStart[D] + r0 + min(row, rows - 1);
-    XP[r] = PA[xi[r]];
+    const float4 p = PA[xi[r]];
+    XA[r] = p.w;
+    XQ[r] = flyLocal(p, PL[xi[r]], O);
     XN[r] = PN[xi[r]];
-    XL[r] = PL[xi[r]];
   }
   float a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
   for (int c0 = 0; c0 < ncol; c0 += FLY_CHUNK) {
@@ -1131,10 +1144,10 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
     __syncthreads();
     for (int j = threadIdx.x; j < cn; j += 256) {
       const int y = src[c0 + j];
-      const float4 p = PA[y], nn = PN[y], l = PL[y];
-      sP[j] = make_float4(p.x, p.y, p.z, p.w * (float)x[y]);
+      const float4 p = PA[y], nn = PN[y], q = flyLocal(p, PL[y], O);
+      sP[j] = make_float4(q.x, q.y, q.z, p.w * (float)x[y]);
       sN[j] = make_float4(nn.x, nn.y, nn.z, p.w * (float)x[y + n]);
-      sL[j] = make_float4(l.x, l.y, l.z, __int_as_float(y));
+      sY[j] = y;
     }
     __syncthreads();
     // each staged source is read once for FLY_RB rows at a time
@@ -1144,15 +1157,15 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
       // D's own columns, padded to whole passes of 64, are checked for the panel itself
       const int lo = min(max(oc - c0, 0), cn);
       const int hi = min(lo + ((min(max(oc + nOwn - c0, 0), cn) - lo + 63) & ~63), cn);
-      fly32Cols<false>(h, 0, lo, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, nk, kap2, eps, ieps, a0, a1);
-      fly32Cols<true>(h, lo, hi, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, nk, kap2, eps, ieps, a0, a1);
-      fly32Cols<false>(h, hi, cn, lane, w, rows, sP, sN, sL, XP, XN, XL, xi, kappa, nk, kap2, eps, ieps, a0, a1);
+      fly32Cols<false>(h, 0, lo, lane, w, rows, sP, sN, sY, XQ, XN, xi, kappa, nk, kap2, eps, ieps, a0, a1);
+      fly32Cols<true>(h, lo, hi, lane, w, rows, sP, sN, sY, XQ, XN, xi, kappa, nk, kap2, eps, ieps, a0, a1);
+      fly32Cols<false>(h, hi, cn, lane, w, rows, sP, sN, sY, XQ, XN, xi, kappa, nk, kap2, eps, ieps, a0, a1);
     }
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int row = w + 4 * r;
-    const float A = XP[r].w * k4pi;
+    const float A = XA[r] * k4pi;
     float t0 = A * a0[r], t1 = A * a1[r];
     if (row < rows) {   // the stored corrections of this row
       for (long long e = specOff[xi[r]] + lane; e < specOff[xi[r] + 1]; e += 64) {
@@ -1172,7 +1185,7 @@ __device__ __forceinline__ void nearFly32Tile(int t, const int2 *__restrict__ ti
   }
 }
 
-#define MIX32_LDS (3 * FLY_CHUNK * 16)
+#define MIX32_LDS (2 * FLY_CHUNK * 16 + FLY_CHUNK * 4)
 static_assert(MIX32_LDS >= 2 * NEAR_CHUNK * 4, "the stream tiles' LDS must fit the union");
 __global__ __launch_bounds__(256) void k_near_mix32(const int2 *__restrict__ tiles, const int *__restrict__ leafStart,
                                                     const int *__restrict__ leafCount,
@@ -1196,7 +1209,7 @@ __global__ __launch_bounds__(256) void k_near_mix32(const int2 *__restrict__ til
   } else {
     float4 *sP = reinterpret_cast<float4 *>(lds);
     nearFly32Tile(t, tiles, leafStart, leafCount, srcOff, ncolArr, srcList, PA, PN, PL, x, specOff, specCol, specVal,
-                  out, n, kappa, eps, sP, sP + FLY_CHUNK, sP + 2 * FLY_CHUNK, ownOff);
+                  out, n, kappa, eps, sP, sP + FLY_CHUNK, reinterpret_cast<int *>(sP + 2 * FLY_CHUNK), ownOff);
   }
 }
 
@@ -5247,7 +5260,8 @@ extern "C" int fbh_enable_fp32(fbh_ctx *c) {
     hipLaunchKernelGGL(k_fly_panels, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->pnl, (int)n, c->Q.t[0],
                        c->flyPA, c->flyPN, c->flyPL);
     hipLaunchKernelGGL(k_spec_vals32, dim3(n), dim3(64), 0, c->stream, c->pnl, c->flyPA, c->flyPN, c->flyPL,
-                       c->specOff, c->specCol, c->specVal, c->kappa, c->eps, c->Q, c->qorder);
+                       c->leafOf, c->leafStart, c->specOff, c->specCol, c->specVal, c->kappa, c->eps, c->Q,
+                       c->qorder);
   }
   HCK(hipMemsetAsync(c->momf, 0, 2 * ne * sizeof(float), c->stream));
   HCK(hipMemsetAsync(c->lecf, 0, 4 * ne * sizeof(float), c->stream));
