@@ -2295,23 +2295,35 @@ __global__ __launch_bounds__(256) void k_rhs_q1(const Pnl *__restrict__ pnl, int
   }
 }
 
-// Precision form: the direct right-hand side at q = 1 with r = x - x_j formed
-// in FP64 (so r keeps FP32 relative accuracy whatever the coordinates' size)
-// and the kernel in FP32: 1/r from the hardware reciprocal square root (r2 is
+// Precision form: the direct right-hand side at q = 1 with the panel's point
+// and the charges moved in FP64 into the frame of the block's first panel and
+// rounded to FP32 there (r = x - x_j then carries an absolute error of a few
+// 1e-7 of the charges' distance from that panel, well inside the FP32
+// operator's error budget) and the kernel in FP32: 1/r from the hardware
+// reciprocal square root (r2 is
 // not denormal: the charges sit at least 1.5 A inside the surface, and 0
 // gives inf as the FP64 kernel does), sums in FP32 over each staged block of
 // 256 charges and in FP64 across blocks.
 __global__ __launch_bounds__(256) void k_rhs32_q1(const Pnl *__restrict__ pnl, int n, int nChar,
                                                   const double *__restrict__ pos, const double *__restrict__ chr,
                                                   double *__restrict__ b, double fac, double t, double ww) {
-  __shared__ double4 sp[256];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double xq[3] = {0, 0, 0}, W = 0.0;
-  float nr[3] = {0, 0, 0};
+  // FP32 coordinates in the frame of the block's first panel (its 256 panels are
+  // consecutive in leaf order, so close together): the charges are moved into it
+  // once per block as they are staged, and a pair's separation is then three
+  // FP32 subtractions
+  __shared__ float4 sp[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, i0 = blockIdx.x * blockDim.x;
+  double O[3];
+  {
+    const Pnl &P0 = pnl[i0];
+    for (int k = 0; k < 3; k++) O[k] = P0.v[k] + t * (P0.a[6 + k] + t * P0.a[k]);
+  }
+  float xl[3] = {0, 0, 0}, nr[3] = {0, 0, 0};
+  double W = 0.0;
   if (i < n) {
     const Pnl &P = pnl[i];
     for (int k = 0; k < 3; k++) {
-      xq[k] = P.v[k] + t * (P.a[6 + k] + t * P.a[k]);
+      xl[k] = (float)(P.v[k] + t * (P.a[6 + k] + t * P.a[k]) - O[k]);
       nr[k] = (float)P.nr[k];
     }
     W = ww * 2.0 * P.area * fac;
@@ -2320,18 +2332,20 @@ __global__ __launch_bounds__(256) void k_rhs32_q1(const Pnl *__restrict__ pnl, i
   for (int j0 = 0; j0 < nChar; j0 += 256) {
     __syncthreads();
     const int j = j0 + threadIdx.x;
-    if (j < nChar) sp[threadIdx.x] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], chr[j]);
+    if (j < nChar)
+      sp[threadIdx.x] = make_float4((float)(pos[3 * j] - O[0]), (float)(pos[3 * j + 1] - O[1]),
+                                    (float)(pos[3 * j + 2] - O[2]), (float)chr[j]);
     __syncthreads();
     const int cnt = min(256, nChar - j0);
     float s0 = 0.f, s1 = 0.f;
 #pragma unroll 4
     for (int jj = 0; jj < cnt; jj++) {
-      const double4 c = sp[jj];
-      const float rx = (float)(xq[0] - c.x), ry = (float)(xq[1] - c.y), rz = (float)(xq[2] - c.z);
+      const float4 c = sp[jj];
+      const float rx = xl[0] - c.x, ry = xl[1] - c.y, rz = xl[2] - c.z;
       const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz)), ri = __builtin_amdgcn_rsqf(r2);
-      const float ip = nr[0] * rx + nr[1] * ry + nr[2] * rz, q = (float)c.w;
-      s0 = fmaf(q, ri, s0);
-      s1 = fmaf(-q * ip, ri * ri * ri, s1);
+      const float ip = nr[0] * rx + nr[1] * ry + nr[2] * rz;
+      s0 = fmaf(c.w, ri, s0);
+      s1 = fmaf(-c.w * ip, ri * ri * ri, s1);
     }
     b0 += s0;
     b1 += s1;
