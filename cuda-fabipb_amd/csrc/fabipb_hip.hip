@@ -2300,10 +2300,9 @@ __global__ __launch_bounds__(256) void k_rhs_q1(const Pnl *__restrict__ pnl, int
 // rounded to FP32 there (r = x - x_j then carries an absolute error of a few
 // 1e-7 of the charges' distance from that panel, well inside the FP32
 // operator's error budget) and the kernel in FP32: 1/r from the hardware
-// reciprocal square root (r2 is
-// not denormal: the charges sit at least 1.5 A inside the surface, and 0
-// gives inf as the FP64 kernel does), sums in FP32 over each staged block of
-// 256 charges and in FP64 across blocks.
+// reciprocal square root (r2 is not denormal: the charges sit at least 1.5 A
+// inside the surface, and 0 gives inf as the FP64 kernel does), sums in FP32
+// over each staged block of 256 charges and in FP64 across blocks.
 __global__ __launch_bounds__(256) void k_rhs32_q1(const Pnl *__restrict__ pnl, int n, int nChar,
                                                   const double *__restrict__ pos, const double *__restrict__ chr,
                                                   double *__restrict__ b, double fac, double t, double ww) {
@@ -3942,6 +3941,9 @@ struct ChargeSet {
   size_t arenaBytes = 0;
   std::vector<unsigned char> stage;
   bool ready = false;
+  // the cluster moments (and walk coefficients) of the staged charges are built:
+  // they depend on the charges alone, so later walks of the same set reuse them
+  mutable bool momReady = false;
   int nChar = 0, nc = 0, c0 = -1, P = 0, S = 1, SZ = 0, nClus = 0, nDesc = 0;
   bool uni = false;
   std::vector<int> lvStart, lvCount, ord;
@@ -3956,7 +3958,7 @@ struct fbh_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
   bool ownStream = false;
-  hipStream_t auxStream = nullptr;   // the tree gradient's second stream (created on first use)
+  hipStream_t auxStream = nullptr;   // the tree gradient's second stream (shared per device, on first use)
   double kappa, eps1, eps2, eps;
   double rhsFac = 0.0;   // right-hand side factor; 0: 1/(4 pi eps1), src/fabipb.c:755
   int qorder;
@@ -4209,29 +4211,38 @@ extern "C" int fbh_available(void) {
   return 1;
 }
 
-// A stream on a hardware queue of its own.  HIP hands out a small pool of
+// Streams on hardware queues of their own.  HIP hands out a small pool of
 // hardware queues to the process's streams, and streams that land on one queue
 // run one after the other: the nearfield on the side stream would then wait
 // for the whole far chain (or the reverse) instead of sharing the CUs with it.
-// A stream created with a CU mask gets a queue outside the pool; the mask here
-// holds every CU.
-static hipError_t streamOwnQueue(hipStream_t *s) {
+// A stream created with a CU mask (here every CU) gets a queue outside the
+// pool.  Two such streams per device -- the nearfield's side stream and the
+// tree gradient's second stream -- are shared by every context of the process
+// and live until it ends: queues of their own per context would add up, with
+// many contexts alive, until the hardware has to time-slice them.
+#define FBH_MAX_DEV 64
+static hipStream_t sharedQueueStream(int which) {
+  static std::mutex mu;
+  static hipStream_t made[FBH_MAX_DEV][2] = {};
   int dev = 0, ncu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
-    return hipStreamCreate(s);
-  std::vector<uint32_t> mask((ncu + 31) / 32, 0xffffffffu);
-  if (ncu % 32) mask.back() = (1u << (ncu % 32)) - 1u;
-  return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= FBH_MAX_DEV) return nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  if (!made[dev][which]) {
+    hipStream_t st = nullptr;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+      return nullptr;
+    std::vector<uint32_t> mask((ncu + 31) / 32, 0xffffffffu);
+    if (ncu % 32) mask.back() = (1u << (ncu % 32)) - 1u;
+    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) != hipSuccess) return nullptr;
+    made[dev][which] = st;
+  }
+  return made[dev][which];
 }
 
 extern "C" void fbh_destroy(fbh_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->auxStream) {
-    hipStreamSynchronize(c->auxStream);
-    hipStreamDestroy(c->auxStream);
-  }
+  if (c->auxStream) hipStreamSynchronize(c->auxStream);   // shared: not destroyed
   for (auto &b : c->bufs) hipFree(b.p);
   if (c->V) hipFree(c->V);
   if (c->w) hipFree(c->w);
@@ -4253,10 +4264,7 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
                   (void *)c->dTileCoef})
     if (q) hipFree(q);
   if (c->dx) hipFree(c->dx);
-  if (c->side) {
-    hipStreamSynchronize(c->side);
-    hipStreamDestroy(c->side);
-  }
+  if (c->side) hipStreamSynchronize(c->side);   // shared: not destroyed
   for (hipEvent_t e : c->ntEv) hipEventDestroy(e);
   if (c->evFork) hipEventDestroy(c->evFork);
   if (c->evJoin) hipEventDestroy(c->evJoin);
@@ -4674,7 +4682,7 @@ extern "C" fbh_ctx *fbh_create(double kappa, double eps1, double eps2, int qorde
     c->ownStream = true;
   }
 #define FAIL() do { fbh_destroy(c); return nullptr; } while (0)
-  if (streamOwnQueue(&c->side) != hipSuccess ||
+  if (!(c->side = sharedQueueStream(0)) ||
       hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess) {
     setErr("stream/event creation failed");
@@ -6265,6 +6273,7 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
                         const int *ctNKids, const int *ctChgStart, const int *ctNChgs, const int *ctChgIdx,
                         const int *ctOrder) {
   s.ready = false;
+  s.momReady = false;
   if (!c || nChar <= 0 || nc <= 0 || !pos || !chr || !ctLevel || !ctX || !ctERad || !ctKidStart || !ctNKids ||
       !ctChgStart || !ctNChgs || !ctChgIdx || !ctOrder) {
     setErr("charge set: bad arguments");
@@ -6417,20 +6426,27 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
   }
   const int depth = c->depth, nmMax = c->nmMax, P = s.P, S = s.S;
   const std::vector<int> &lvStart = s.lvStart, &lvCount = s.lvCount, &ord = s.ord;
-  // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416)
-  if (lvCount[depth] > 0)
-    hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
-                       lvStart[depth], s.dCS, s.dNC, s.dCI, s.dX, s.dPos, s.dChr, ord[depth], s.dMom, nmMax, c->T);
-  for (int l = depth - 1; l >= c->height; l--) {
-    if (lvCount[l] <= 0) continue;
-    int nOut = NMOM(ord[l]);
-    const size_t lds = 2 * nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut] - c->hM2mStart[0]) * sizeof(int);
-    hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l],
-                       s.dKid, s.dNK, s.dX, s.dOrd, l, s.dMom, nmMax, c->T);
+  // charge moments: leaves, then upward to `height` (src/chargeTree.c:397-416),
+  // and the walk coefficients, once per staged set
+  if (!s.momReady) {
+    if (lvCount[depth] > 0)
+      hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
+                         lvStart[depth], s.dCS, s.dNC, s.dCI, s.dX, s.dPos, s.dChr, ord[depth], s.dMom, nmMax, c->T);
+    for (int l = depth - 1; l >= c->height; l--) {
+      if (lvCount[l] <= 0) continue;
+      int nOut = NMOM(ord[l]);
+      const size_t lds =
+          2 * nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut] - c->hM2mStart[0]) * sizeof(int);
+      hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l],
+                         s.dKid, s.dNK, s.dX, s.dOrd, l, s.dMom, nmMax, c->T);
+    }
+    if (s.uni)
+      hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, c->stream, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc,
+                         s.SZ, s.dPoly);
+    if (hipGetLastError() != hipSuccess) { setErr("charge moment kernels failed"); return 0; }
+    s.momReady = true;
   }
   if (s.uni) {
-    hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, c->stream, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc, s.SZ,
-                       s.dPoly);
     const int blocks = ((c->n + 64 * EN_WPB - 1) / (64 * EN_WPB)) * S;
 #define EN_WAVE(PP, MM)                                                                                           \
   case PP:                                                                                                        \
@@ -6761,7 +6777,10 @@ extern "C" int fbh_gradient_tree_dev(fbh_ctx *c, int nChar, const double *pos, c
   if (!walkCharges(c, s, 4, theta, sgm_dev, g, &eK)) return 0;   // ends with the stream synchronized
   // dE/dq at fixed sgm (the transposed energy walk) does not need lam: a second
   // host thread runs it on a second stream while this one drives the adjoint GMRES
-  if (!c->auxStream) HCK(streamOwnQueue(&c->auxStream));
+  if (!c->auxStream && !(c->auxStream = sharedQueueStream(1))) {
+    setErr("fbh_gradient_tree_dev: no second stream");
+    return 0;
+  }
   int okAux = 1;
   std::thread aux([&] {
     if (hipSetDevice(c->dev) != hipSuccess) { okAux = 0; return; }
