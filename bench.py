@@ -205,13 +205,13 @@ class Solver:
         self.x.copy_(self.b)   # x0 = b, the reference's default
         return self.op.gmres_dev(self.b, self.x, restart=RESTART, tol=tol, maxit=maxit)
 
-    def energy(self):
-        return self.op.energy_res_dev(self.x)
+    def energy(self, precision="fp64"):   # the precision form's walk has its far polynomials in FP32
+        return self.op.energy_res_dev(self.x, precision=precision)
 
     def solve(self, precision, tol=TOL, maxit=None):
         self.rhs(precision)
         g = self.gmres(precision, tol, maxit)
-        return g, self.energy()
+        return g, self.energy(precision)
 
     def staged(self, precision):
         """One solve with a synchronisation between the stages, for the breakdown."""
@@ -224,7 +224,7 @@ class Solver:
         g = self.gmres(precision)
         sync()
         t2 = pc()
-        e = self.energy()
+        e = self.energy(precision)
         sync()
         t3 = pc()
         r = {"solve_ms": (t3 - t0) * 1e3, "rhs_ms": (t1 - t0) * 1e3, "gmres_ms": (t2 - t1) * 1e3,

@@ -142,6 +142,8 @@ def hip_lib():
         L.fbh_rhs_tree_res_dev.argtypes = [_vp, C.c_double, _vp]
         L.fbh_energy_res_dev.restype = C.c_int
         L.fbh_energy_res_dev.argtypes = [_vp, C.c_double, _vp, _dp]
+        L.fbh_energy32_res_dev.restype = C.c_int
+        L.fbh_energy32_res_dev.argtypes = [_vp, C.c_double, _vp, _dp]
         L.fbh_matvec_t_dev.restype = C.c_int
         L.fbh_matvec_t_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp]
         L.fbh_gmres_t_dev.restype = C.c_int
@@ -582,10 +584,12 @@ class Operator:
         """Charge-tree right-hand side of the resident charges into b."""
         self._ck(hip_lib().fbh_rhs_tree_res_dev(self.h, theta, C.c_void_p(b.data_ptr())))
 
-    def energy_res_dev(self, sigma, theta=0.8):
-        """Raw panel-tree energy of the resident charges for the density sigma."""
+    def energy_res_dev(self, sigma, theta=0.8, precision="fp64"):
+        """Raw panel-tree energy of the resident charges for the density sigma
+        (precision="fp32": the walk's far polynomials in FP32, the precision form's)."""
         e = C.c_double(0.0)
-        self._ck(hip_lib().fbh_energy_res_dev(self.h, theta, C.c_void_p(sigma.data_ptr()), C.byref(e)))
+        f = hip_lib().fbh_energy32_res_dev if precision == "fp32" else hip_lib().fbh_energy_res_dev
+        self._ck(f(self.h, theta, C.c_void_p(sigma.data_ptr()), C.byref(e)))
         return e.value
 
     def near_timing(self, on):
@@ -758,7 +762,7 @@ def solve(base, depth, order=6, sep=0.8, kappa=0.1257, eps1=1.0, eps2=80.0, qord
         g = op.gmres_dev(b, x, restart=restart, tol=tol, maxit=maxit)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
-    raw = op.energy_res_dev(x)
+    raw = op.energy_res_dev(x, precision=precision)
     torch.cuda.synchronize()
     t5 = time.perf_counter()
     return dict(energy=raw * ENERGY_SCALE, energy_raw=raw, iters=g["iters"], resid=g["resid"],

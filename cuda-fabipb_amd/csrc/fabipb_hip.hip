@@ -3089,12 +3089,16 @@ __host__ __device__ constexpr int enChunk(int p) { return EN_CHUNK >= 0 ? EN_CHU
 // MODE 1: the right-hand side's Coulomb kernel 1/r (:283-379), both halves;
 // MODE 3: d/dkappa of MODE 0's kernels; MODE 4: MODE 0 into (y0, y1) and
 // MODE 3 into (z0, z1) from one pass over the coefficients
-template <int P, int MODE>
-__device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx, double ry, double rz, double nx,
-                                      double ny, double nz, double kappa, double eps, double &y0, double &y1,
+// R: the type of the coefficients and of the polynomial's arithmetic (float for
+// the precision form's energy: half the scalar-load bytes per cluster; the
+// radial values are formed in FP64 and rounded, the sums return to FP64)
+template <int P, int MODE, typename R = double>
+__device__ __forceinline__ void enFar(const R *__restrict__ co0, double rxd, double ryd, double rzd, double nxd,
+                                      double nyd, double nzd, double kappa, double eps, double &y0, double &y1,
                                       double &z0, double &z1) {
   constexpr int NS = MODE == 4 ? 2 : 1;   // radial sequence pairs evaluated together
-  const double rho = sqrt(rx * rx + ry * ry + rz * rz);
+  const double rho = sqrt(rxd * rxd + ryd * ryd + rzd * rzd);
+  const R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd, nx = (R)nxd, ny = (R)nyd, nz = (R)nzd;
   double h[NS][P + 2], u[NS][P + 2];
   if constexpr (MODE == 1) {
     const double nr2 = -1.0 / (rho * rho);
@@ -3131,30 +3135,39 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
   // pointer and the running sums, so the next chunk's loads cannot be issued
   // before this chunk is consumed (hoisted all at once, they overflow the
   // scalar registers and are spilled).
-  using CD = const __attribute__((address_space(4))) double;
+  using CD = const __attribute__((address_space(4))) R;
   CD *co = (CD *)co0;
-  int o = 0, since = 0;   // since: doubles read since the last chunk boundary
-  double s0[NS], s1[NS];
+  constexpr int CHUNK = enChunk(P) * (int)(sizeof(double) / sizeof(R));   // the same bytes per chunk
+  R hr[NS][P + 2], ur[NS][P + 2];
 #pragma unroll
-  for (int t = 0; t < NS; t++) s0[t] = s1[t] = 0.0;
+  for (int t = 0; t < NS; t++)
+#pragma unroll
+    for (int k = 0; k <= P + 1; k++) {
+      hr[t][k] = (R)h[t][k];
+      ur[t][k] = (R)u[t][k];
+    }
+  int o = 0, since = 0;   // since: coefficients read since the last chunk boundary
+  R s0[NS], s1[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) s0[t] = s1[t] = 0;
 #pragma unroll
   for (int a = P; a >= 0; a--) {
-    double qa[NS];
+    R qa[NS];
 #pragma unroll
-    for (int t = 0; t < NS; t++) qa[t] = 0.0;
+    for (int t = 0; t < NS; t++) qa[t] = 0;
 #pragma unroll
     for (int b = P - a; b >= 0; b--) {
-      double rb[NS];
+      R rb[NS];
 #pragma unroll
-      for (int t = 0; t < NS; t++) rb[t] = 0.0;
+      for (int t = 0; t < NS; t++) rb[t] = 0;
 #pragma unroll
       for (int c = P - a - b; c >= 0; c--) {
         const int d = a + b + c;
 #pragma unroll
         for (int t = 0; t < NS; t++) {
-          double cw = 0.0;
+          R cw = 0;
 #pragma unroll
-          for (int j = 0; j <= (P - d) / 2; j++) cw += h[t][d + j] * co[o + j];
+          for (int j = 0; j <= (P - d) / 2; j++) cw += hr[t][d + j] * co[o + j];
           rb[t] = rb[t] * rz + cw;
         }
         o += (P - d) / 2 + 1;
@@ -3162,7 +3175,7 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
       }
 #pragma unroll
       for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
-      if (enChunk(P) > 0 && since >= enChunk(P)) {   // folds at compile time
+      if (CHUNK > 0 && since >= CHUNK) {   // folds at compile time
         if constexpr (NS == 1)
           asm volatile("" : "+s"(co), "+v"(qa[0]));
         else
@@ -3175,23 +3188,23 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
   }
 #pragma unroll
   for (int a = P + 1; a >= 0; a--) {
-    double qa[NS];
+    R qa[NS];
 #pragma unroll
-    for (int t = 0; t < NS; t++) qa[t] = 0.0;
+    for (int t = 0; t < NS; t++) qa[t] = 0;
 #pragma unroll
     for (int b = P + 1 - a; b >= 0; b--) {
-      double rb[NS];
+      R rb[NS];
 #pragma unroll
-      for (int t = 0; t < NS; t++) rb[t] = 0.0;
+      for (int t = 0; t < NS; t++) rb[t] = 0;
 #pragma unroll
       for (int c = P + 1 - a - b; c >= 0; c--) {
         const int d = a + b + c;
 #pragma unroll
         for (int t = 0; t < NS; t++) {
-          double cv = 0.0;
+          R cv = 0;
 #pragma unroll
           for (int j = 0; j <= (P + 1 - d) / 2; j++)
-            cv += u[t][d + j] * (nx * co[o + 3 * j] + ny * co[o + 3 * j + 1] + nz * co[o + 3 * j + 2]);
+            cv += ur[t][d + j] * (nx * co[o + 3 * j] + ny * co[o + 3 * j + 1] + nz * co[o + 3 * j + 2]);
           rb[t] = rb[t] * rz + cv;
         }
         o += 3 * ((P + 1 - d) / 2 + 1);
@@ -3199,7 +3212,7 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
       }
 #pragma unroll
       for (int t = 0; t < NS; t++) qa[t] = qa[t] * ry + rb[t];
-      if (enChunk(P) > 0 && since >= enChunk(P)) {   // folds at compile time
+      if (CHUNK > 0 && since >= CHUNK) {   // folds at compile time
         if constexpr (NS == 1)
           asm volatile("" : "+s"(co), "+v"(qa[0]));
         else
@@ -3210,11 +3223,11 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
 #pragma unroll
     for (int t = 0; t < NS; t++) s1[t] = s1[t] * rx + qa[t];
   }
-  y0 += s0[0];
-  y1 += s1[0];
+  y0 += (double)s0[0];
+  y1 += (double)s1[0];
   if constexpr (NS == 2) {
-    z0 += s0[1];
-    z1 += s1[1];
+    z0 += (double)s0[1];
+    z1 += (double)s1[1];
   }
 }
 
@@ -3239,14 +3252,14 @@ __device__ __forceinline__ void enFar(const double *__restrict__ co0, double rx,
 // few spilled registers; 2.5 against 2.8 ms with three waves at 317k panels),
 // the compiler's choice for the right-hand side (five waves) and MODE 4
 __host__ __device__ constexpr int enMinWaves(int mode) { return mode == 0 || mode == 2 || mode == 3 ? 4 : 1; }
-template <int P, int MODE>
+template <int P, int MODE, typename R = double>
 __global__ __launch_bounds__(64 * EN_WPB, enMinWaves(MODE)) void k_energy_wave(const Pnl *__restrict__ pnl, int n, const double *__restrict__ sgm,
                                                      const int *__restrict__ level, const double *__restrict__ cx,
                                                      const double *__restrict__ eRad,
                                                      const int *__restrict__ kidStart, const int *__restrict__ nKids,
                                                      const int *__restrict__ chgStart, const int *__restrict__ nChgs,
                                                      const int *__restrict__ chgIdx, const double *__restrict__ pos,
-                                                     const double *__restrict__ chr, const double *__restrict__ poly,
+                                                     const double *__restrict__ chr, const R *__restrict__ poly,
                                                      int c0, int depth, int height, double theta, double kappa,
                                                      double eps, QTab Q, int q, int nH, int S,
                                                      double *__restrict__ part, double *__restrict__ part2) {
@@ -3303,7 +3316,7 @@ __global__ __launch_bounds__(64 * EN_WPB, enMinWaves(MODE)) void k_energy_wave(c
         const bool dir = on && !acc && lev == depth;
         const unsigned long long down = __ballot(on && !acc && lev < depth);
         if (acc)
-          enFar<P, MODE == 2 ? 0 : MODE>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps,
+          enFar<P, MODE == 2 ? 0 : MODE, R>(poly + (long long)(cb - c0) * SZ, rx, ry, rz, nr[0], nr[1], nr[2], kappa, eps,
                                          y0, y1, z0, z1);
         if (__ballot(dir) != 0ull) {
           const int s = chgStart[cb], nq = nChgs[cb];
@@ -3951,6 +3964,7 @@ struct ChargeSet {
       *dOrd = nullptr, *dPar = nullptr;
   double *dX = nullptr, *dR = nullptr, *dPos = nullptr, *dChr = nullptr, *dMom = nullptr, *dPart = nullptr,
          *dOut = nullptr, *dRed = nullptr, *dPoly = nullptr;
+  float *dPoly32 = nullptr;   // the walk coefficients rounded to FP32 (the precision form's energy)
   const int4 *dDesc = nullptr;
 };
 
@@ -6361,7 +6375,8 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
   const size_t iMom = add(nullptr, (size_t)nc * nmMax * sizeof(double)),
                iPart = add(nullptr, (size_t)S * 3 * c->n * sizeof(double)), iOut = add(nullptr, sizeof(double)),
                iRed = add(nullptr, RED_BLOCKS * sizeof(double)),
-               iPoly = add(nullptr, (size_t)nClus * SZ * sizeof(double));
+               iPoly = add(nullptr, (size_t)nClus * SZ * sizeof(double)),
+               iPoly32 = add(nullptr, (size_t)nClus * SZ * sizeof(float));
   if (s.arenaBytes < top) {
     if (s.arena) {
       HCK(hipStreamSynchronize(c->stream));   // a walk on the old arena may still run
@@ -6394,6 +6409,7 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
   s.dOut = (double *)(A + seg[iOut].off);
   s.dRed = (double *)(A + seg[iRed].off);
   s.dPoly = (double *)(A + seg[iPoly].off);
+  s.dPoly32 = (float *)(A + seg[iPoly32].off);
   s.dDesc = (const int4 *)(A + seg[iDesc].off);
   s.nChar = nChar;
   s.nc = nc;
@@ -6417,7 +6433,7 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
 // dE/dsgm into b_dev, 3 the energy's kappa derivative at fixed sgm into
 // *energy_raw
 static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, const double *sgm_dev, double *b_dev,
-                       double *energy_raw) {
+                       double *energy_raw, bool f32 = false) {
   if (!s.ready) { setErr("fbh_energy/fbh_rhs_tree: no charge set staged"); return 0; }
   const bool scalar = mode == 0 || mode == 3 || mode == 4;   // 4: the weights into b_dev as well
   if ((scalar && (!energy_raw || !sgm_dev)) || (!(mode == 0 || mode == 3) && !b_dev)) {
@@ -6440,9 +6456,14 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
       hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l],
                          s.dKid, s.dNK, s.dX, s.dOrd, l, s.dMom, nmMax, c->T);
     }
-    if (s.uni)
+    if (s.uni) {
       hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, c->stream, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc,
                          s.SZ, s.dPoly);
+      const long long nP = (long long)s.nClus * s.SZ;
+      if (nP > 0)
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nP + 255) / 256)), dim3(256), 0, c->stream, s.dPoly, s.dPoly32,
+                           nP);
+    }
     if (hipGetLastError() != hipSuccess) { setErr("charge moment kernels failed"); return 0; }
     s.momReady = true;
   }
@@ -6456,7 +6477,17 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
                        depth, c->height, theta, c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder,       \
                        lvCount[c->height], S, (MM && S == 1) ? b_dev : s.dPart, s.dPart + 2 * (size_t)S * c->n);  \
     break;
-    if (mode == 0) {
+    if (mode == 0 && f32) {
+#define EN_WAVE32(PP)                                                                                             \
+  case PP:                                                                                                        \
+    hipLaunchKernelGGL((k_energy_wave<PP, 0, float>), dim3(blocks), dim3(64 * EN_WPB), 0, c->stream, c->pnl,      \
+                       c->n, sgm_dev, s.dLevel, s.dX, s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr,    \
+                       s.dPoly32, s.c0, depth, c->height, theta, c->kappa, c->eps, c->Q, c->qorder,              \
+                       lvCount[c->height], S, s.dPart, s.dPart + 2 * (size_t)S * c->n);                           \
+    break;
+      switch (P) { EN_WAVE32(1) EN_WAVE32(2) EN_WAVE32(3) EN_WAVE32(4) EN_WAVE32(5) EN_WAVE32(6) EN_WAVE32(7) EN_WAVE32(8) EN_WAVE32(9) }
+#undef EN_WAVE32
+    } else if (mode == 0) {
       switch (P) { EN_WAVE(1, 0) EN_WAVE(2, 0) EN_WAVE(3, 0) EN_WAVE(4, 0) EN_WAVE(5, 0) EN_WAVE(6, 0) EN_WAVE(7, 0) EN_WAVE(8, 0) EN_WAVE(9, 0) }
     } else if (mode == 3) {
       switch (P) { EN_WAVE(1, 3) EN_WAVE(2, 3) EN_WAVE(3, 3) EN_WAVE(4, 3) EN_WAVE(5, 3) EN_WAVE(6, 3) EN_WAVE(7, 3) EN_WAVE(8, 3) EN_WAVE(9, 3) }
@@ -6558,6 +6589,12 @@ extern "C" int fbh_rhs_tree_res_dev(fbh_ctx *c, double theta, double *b_dev) {
 extern "C" int fbh_energy_res_dev(fbh_ctx *c, double theta, const double *sgm_dev, double *energy_raw) {
   if (!c || !sgm_dev || !energy_raw) { setErr("fbh_energy_res_dev: bad arguments"); return 0; }
   return walkCharges(c, c->csRes, 0, theta, sgm_dev, nullptr, energy_raw);
+}
+
+// the same walk with the far polynomials in FP32 (precision form)
+extern "C" int fbh_energy32_res_dev(fbh_ctx *c, double theta, const double *sgm_dev, double *energy_raw) {
+  if (!c || !sgm_dev || !energy_raw) { setErr("fbh_energy32_res_dev: bad arguments"); return 0; }
+  return walkCharges(c, c->csRes, 0, theta, sgm_dev, nullptr, energy_raw, true);
 }
 
 // ------------------------------------------ exact charge gradients of the tree forms

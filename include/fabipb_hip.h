@@ -259,12 +259,19 @@ int fbh_set_charges(fbh_ctx *ctx, int nChar, const double *pos, const double *ch
                     const int *ctKidStart, const int *ctNKids, const int *ctChgStart,
                     const int *ctNChgs, const int *ctChgIdx, const int *ctOrder);
 int fbh_rhs_res_dev(fbh_ctx *ctx, double *b_dev);
-/* the precision form's direct right-hand side of the resident set (q = 1): r in
- * FP64, the kernel in FP32, sums in FP32 per 256 charges and FP64 across them;
+/* the precision form's direct right-hand side of the resident set (q = 1): the
+ * points moved in FP64 into the frame of each 256-panel block and rounded to
+ * FP32, the kernel in FP32, sums in FP32 per 256 charges and FP64 across them;
  * other quadrature orders take the FP64 kernel */
 int fbh_rhs32_res_dev(fbh_ctx *ctx, double *b_dev);
 int fbh_rhs_tree_res_dev(fbh_ctx *ctx, double theta, double *b_dev);
 int fbh_energy_res_dev(fbh_ctx *ctx, double theta, const double *sgm_dev, double *energy_raw);
+/* the precision form's panel-tree energy of the resident set: the same walk with
+ * the clusters' far polynomials rounded to FP32 and evaluated in FP32 (radial
+ * values formed in FP64, each cluster's sums returned to FP64), the direct
+ * leaves and every sum above them in FP64; no counterpart in the reference,
+ * whose energy is FP64 only (src/treecode.c:617-695) */
+int fbh_energy32_res_dev(fbh_ctx *ctx, double theta, const double *sgm_dev, double *energy_raw);
 
 /* Differentiation form (no counterpart in the reference, which has no
  * adjoint; it differentiates what MtVmain, setupRHS and the panel-tree energy

@@ -65,6 +65,14 @@ def test_precision_form_317k(fa, meshdir):
           f"resid {g32['resid']:.2e} energy rel {abs(e32 - ref) / abs(ref):.2e}", flush=True)
     assert g32["info"] == 0 and g32["resid"] <= 1e-6
     assert abs(e32 - ref) <= 1e-6 * abs(ref)
+    # the precision form's energy walk (far polynomials in FP32) against the FP64
+    # walk on the same density, and the reference's energy
+    x32, _, _ = _solve(fa, torch, op, mesh, ct, tree.n, "fp32")
+    op.set_charges(ct, mesh.pos, mesh.chr)
+    w64, w32 = op.energy_res_dev(x32), op.energy_res_dev(x32, precision="fp32")
+    print(f"[{name}] energy walk with FP32 far polynomials: rel {abs(w32 - w64) / abs(w64):.2e} from the FP64 walk, "
+          f"{abs(w32 - ref) / abs(ref):.2e} from the reference", flush=True)
+    assert abs(w32 - w64) <= 3e-7 * abs(w64) and abs(w32 - ref) <= 1e-6 * abs(ref)
 
 
 _cfg4 = {}
@@ -205,6 +213,10 @@ def test_capsid_shell_properties(fa, tmp_path):
         print(f"[capsid] {prec}: its {g['iters']} energy {res[prec][1]!r}", flush=True)
     e64, e32 = res["fp64"][1], res["fp32"][1]
     assert abs(e32 - e64) <= 1e-6 * abs(e64)
+    op.set_charges(ct, mesh.pos, mesh.chr)   # the precision form's walk (order 8, FP32 far polynomials)
+    w64, w32 = op.energy_res_dev(res["fp32"][0]), op.energy_res_dev(res["fp32"][0], precision="fp32")
+    print(f"[capsid] energy walk with FP32 far polynomials: rel {abs(w32 - w64) / abs(w64):.2e}", flush=True)
+    assert abs(w32 - w64) <= 3e-7 * abs(w64)
 
     # the transpose identity and linearity of the whole operator
     gen = torch.Generator(device="cpu").manual_seed(11)
