@@ -248,9 +248,11 @@ int fbh_rhs_tree_dev(fbh_ctx *ctx, int nChar, const double *pos, const double *c
 /*
  * Resident charge set: the charges and their charge tree (the arrays of
  * fbh_energy_dev) validated and copied to the device once, kept by the context
- * until the next fbh_set_charges or fbh_destroy.  The three calls after it are
+ * until the next fbh_set_charges or fbh_destroy.  The calls after it are
  * fbh_rhs_dev, fbh_rhs_tree_dev and fbh_energy_dev on that set with no host
- * traffic: in the reference the charges live in host memory for the whole
+ * traffic (and the precision form's FP32 variants); the set's cluster moments
+ * and walk coefficients are built on its first walk and reused until it is
+ * staged again: in the reference the charges live in host memory for the whole
  * solve (setupRHS / setupRHSTree, src/fabipb.c:633-800, and
  * applyPanelChargeTreeEnergy, src/treecode.c:927-1017, read them in place).
  */
