@@ -210,6 +210,10 @@ class Solver:
 
     def solve(self, precision, tol=TOL, maxit=None):
         self.rhs(precision)
+        if precision == "fp32":   # the energy walk runs beside the refinement's last FP64 residual
+            self.x.zero_()
+            g = self.op.gmres_refined_energy_dev(self.b, self.x, restart=RESTART, tol=tol, maxit=maxit or self.maxit)
+            return g, g["energy_raw"]
         g = self.gmres(precision, tol, maxit)
         return g, self.energy(precision)
 

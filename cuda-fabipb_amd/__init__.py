@@ -120,6 +120,9 @@ def hip_lib():
         L.fbh_gmres_refined_dev.restype = C.c_int
         L.fbh_gmres_refined_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                             _ip, _dp, _dp, _ip, _ip, _ip]
+        L.fbh_gmres_refined_energy_dev.restype = C.c_int
+        L.fbh_gmres_refined_energy_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                                   C.c_double, _ip, _dp, _dp, _ip, _ip, _ip, _dp]
         L.fbh_rhs.restype = C.c_int
         L.fbh_rhs.argtypes = [_vp, C.c_int, _dp, _dp, _dp]
         L.fbh_rhs_dev.restype = C.c_int
@@ -451,6 +454,18 @@ class Operator:
             C.byref(it), C.byref(res), _d(hist), C.byref(info), C.byref(outer), C.byref(mv64)))
         return dict(iters=it.value, resid=res.value, info=info.value, outer=outer.value, mv64=mv64.value,
                     hist=hist[:outer.value])
+
+    def gmres_refined_energy_dev(self, b, x, restart=30, tol=1e-6, maxit=100, max_outer=10, theta=0.8):
+        """gmres_refined_dev, then the precision form's energy of the resident charges
+        for the solution, the walk overlapped with the last FP64 residual product."""
+        it, res, info, outer, mv64 = C.c_int(0), C.c_double(0.0), C.c_int(0), C.c_int(0), C.c_int(0)
+        e = C.c_double(0.0)
+        hist = np.zeros(max_outer)
+        self._ck(hip_lib().fbh_gmres_refined_energy_dev(
+            self.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), restart, tol, maxit, max_outer, theta,
+            C.byref(it), C.byref(res), _d(hist), C.byref(info), C.byref(outer), C.byref(mv64), C.byref(e)))
+        return dict(iters=it.value, resid=res.value, info=info.value, outer=outer.value, mv64=mv64.value,
+                    hist=hist[:outer.value], energy_raw=e.value)
 
     def matvec_t_dev(self, x, y, alpha=1.0, beta=0.0):
         """y = alpha A^T x + beta y (the adjoint operator)."""

@@ -4067,6 +4067,7 @@ struct fbh_ctx {
   // nearfield on a second stream, overlapped with the far-field chain
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
+  hipEvent_t evX = nullptr;   // the refined solve's x, ready for the energy walk beside its residual
   // nearfield launch timing (fbh_near_timing): an event pair around every
   // nearfield launch on the side stream, read back after the timed region
   bool ntOn = false;
@@ -4282,6 +4283,7 @@ extern "C" void fbh_destroy(fbh_ctx *c) {
   for (hipEvent_t e : c->ntEv) hipEventDestroy(e);
   if (c->evFork) hipEventDestroy(c->evFork);
   if (c->evJoin) hipEventDestroy(c->evJoin);
+  if (c->evX) hipEventDestroy(c->evX);
   if (c->ownStream && c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -5931,12 +5933,26 @@ __global__ void k_axpy(double *__restrict__ y, const double *__restrict__ x, int
 // of the test on that update is confirmed with b - A64 x before returning.
 // iters counts inner iterations (FP32 products), outer the corrections, mv64
 // the FP64 products.
-extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit,
-                                     int maxOuter, int *iters, double *resid, double *hist, int *info, int *outer,
-                                     int *mv64) {
+static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, const double *sgm_dev, double *b_dev,
+                       double *energy_raw, bool f32, hipStream_t st);
+static hipStream_t sharedQueueStream(int which);
+
+// energyRaw: also the precision form's energy of the resident charges for the
+// solution (theta the walk's MAC): each correction's walk runs on a second
+// stream, from a second host thread, beside the FP64 residual product that
+// decides whether it is the last; the walk of a correction that is not the
+// last is discarded
+static int refinedSolve(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit, int maxOuter,
+                        int *iters, double *resid, double *hist, int *info, int *outer, int *mv64, double theta,
+                        double *energyRaw) {
   if (!c || !b || !x || restrt < 1 || maxit < 1 || maxOuter < 1) {
     setErr("fbh_gmres_refined_dev: bad arguments");
     return 0;
+  }
+  if (energyRaw) {
+    if (!c->csRes.ready) { setErr("fbh_gmres_refined_energy_dev: no resident charge set (fbh_set_charges)"); return 0; }
+    if (!c->auxStream && !(c->auxStream = sharedQueueStream(1))) { setErr("no second stream"); return 0; }
+    if (!c->evX) HCK(hipEventCreateWithFlags(&c->evX, hipEventDisableTiming));
   }
   if (!c->fp32 && !fbh_enable_fp32(c)) return 0;
   if (!ensureGmres(c, restrt)) return 0;
@@ -5991,6 +6007,34 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
     *iters += it;
     ++*outer;
     hipLaunchKernelGGL(k_axpy, dim3(g), dim3(256), 0, c->stream, x, c->dx, n2);
+    std::thread walk;
+    int okWalk = 1;
+    double eWalk = 0.0;
+    struct Join {
+      std::thread &t;
+      ~Join() {
+        if (t.joinable()) t.join();
+      }
+    } joinWalk{walk};
+    if (energyRaw) {   // the walk of this x beside its FP64 residual
+      HCK(hipEventRecord(c->evX, c->stream));
+      walk = std::thread([&] {
+        if (hipSetDevice(c->dev) != hipSuccess || hipStreamWaitEvent(c->auxStream, c->evX, 0) != hipSuccess) {
+          okWalk = 0;
+          return;
+        }
+        okWalk = walkCharges(c, c->csRes, 0, theta, x, nullptr, &eWalk, true, c->auxStream);
+      });
+    }
+    // x is final for this correction when the loop leaves: the walk's energy is x's
+    auto leave = [&](int rc) {
+      if (walk.joinable()) walk.join();
+      if (rc && energyRaw) {
+        if (!okWalk) return 0;
+        *energyRaw = eWalk;
+      }
+      return rc;
+    };
     double rn;
     bool updated = false;
     if (*mv64 > 0 && rr <= 100.0 * tol * braw) {
@@ -6009,18 +6053,33 @@ extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int
       return 0;
     }
     *resid = rn / bnrm2;
-    if (rn != rn) { if (hist) hist[nh++] = *resid; *info = 2; return 1; }
+    if (rn != rn) { if (hist) hist[nh++] = *resid; *info = 2; return leave(1); }
     if (updated && *resid <= tol) {   // the FP64 residual decides
       if (!residual64(&rn)) return 0;
       *resid = rn / bnrm2;
     }
     if (hist) hist[nh++] = *resid;
-    if (rn != rn) { *info = 2; return 1; }
-    if (*resid <= tol) return 1;
-    if (*iters >= maxit || *outer >= maxOuter) { *info = 1; return 1; }
+    if (rn != rn) { *info = 2; return leave(1); }
+    if (*resid <= tol) return leave(1);
+    if (*iters >= maxit || *outer >= maxOuter) { *info = 1; return leave(1); }
   }
   *resid = 0.0;
+  if (energyRaw) return walkCharges(c, c->csRes, 0, theta, x, nullptr, energyRaw, true, nullptr);
   return 1;
+}
+
+extern "C" int fbh_gmres_refined_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol, int maxit,
+                                     int maxOuter, int *iters, double *resid, double *hist, int *info, int *outer,
+                                     int *mv64) {
+  return refinedSolve(c, b, x, restrt, tol, maxit, maxOuter, iters, resid, hist, info, outer, mv64, 0.0, nullptr);
+}
+
+extern "C" int fbh_gmres_refined_energy_dev(fbh_ctx *c, const double *b, double *x, int restrt, double tol,
+                                            int maxit, int maxOuter, double theta, int *iters, double *resid,
+                                            double *hist, int *info, int *outer, int *mv64, double *energy_raw) {
+  if (!energy_raw) { setErr("fbh_gmres_refined_energy_dev: bad arguments"); return 0; }
+  return refinedSolve(c, b, x, restrt, tol, maxit, maxOuter, iters, resid, hist, info, outer, mv64, theta,
+                      energy_raw);
 }
 
 // the factor every right-hand side (direct, tree, and the gradient's R^T) is scaled by
@@ -6432,8 +6491,10 @@ static int stageCharges(fbh_ctx *c, ChargeSet &s, int nChar, const double *pos, 
 // right-hand side (src/fabipb.c:633-742) into b_dev, 2 the energy's weights
 // dE/dsgm into b_dev, 3 the energy's kappa derivative at fixed sgm into
 // *energy_raw
+// st: the stream to walk on (the context's stream when null)
 static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, const double *sgm_dev, double *b_dev,
-                       double *energy_raw, bool f32 = false) {
+                       double *energy_raw, bool f32 = false, hipStream_t st = nullptr) {
+  const hipStream_t strm = st ? st : c->stream;
   if (!s.ready) { setErr("fbh_energy/fbh_rhs_tree: no charge set staged"); return 0; }
   const bool scalar = mode == 0 || mode == 3 || mode == 4;   // 4: the weights into b_dev as well
   if ((scalar && (!energy_raw || !sgm_dev)) || (!(mode == 0 || mode == 3) && !b_dev)) {
@@ -6446,22 +6507,22 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
   // and the walk coefficients, once per staged set
   if (!s.momReady) {
     if (lvCount[depth] > 0)
-      hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, c->stream,
+      hipLaunchKernelGGL(k_ct_leaf, dim3(lvCount[depth]), dim3(blk64(NMOM(ord[depth]), 512)), 0, strm,
                          lvStart[depth], s.dCS, s.dNC, s.dCI, s.dX, s.dPos, s.dChr, ord[depth], s.dMom, nmMax, c->T);
     for (int l = depth - 1; l >= c->height; l--) {
       if (lvCount[l] <= 0) continue;
       int nOut = NMOM(ord[l]);
       const size_t lds =
           2 * nOut * sizeof(double) + (size_t)(nOut + 1 + c->hM2mStart[nOut] - c->hM2mStart[0]) * sizeof(int);
-      hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, c->stream, lvStart[l],
+      hipLaunchKernelGGL(k_ct_m2m, dim3(lvCount[l]), dim3(blk64((nOut + 3) / 4, 256)), lds, strm, lvStart[l],
                          s.dKid, s.dNK, s.dX, s.dOrd, l, s.dMom, nmMax, c->T);
     }
     if (s.uni) {
-      hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, c->stream, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc,
+      hipLaunchKernelGGL(k_ct_poly, dim3(s.nClus), dim3(256), 0, strm, s.c0, s.dMom, nmMax, s.dDesc, s.nDesc,
                          s.SZ, s.dPoly);
       const long long nP = (long long)s.nClus * s.SZ;
       if (nP > 0)
-        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nP + 255) / 256)), dim3(256), 0, c->stream, s.dPoly, s.dPoly32,
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nP + 255) / 256)), dim3(256), 0, strm, s.dPoly, s.dPoly32,
                            nP);
     }
     if (hipGetLastError() != hipSuccess) { setErr("charge moment kernels failed"); return 0; }
@@ -6471,7 +6532,7 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
     const int blocks = ((c->n + 64 * EN_WPB - 1) / (64 * EN_WPB)) * S;
 #define EN_WAVE(PP, MM)                                                                                           \
   case PP:                                                                                                        \
-    hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(64 * EN_WPB), 0, c->stream, c->pnl, c->n,       \
+    hipLaunchKernelGGL((k_energy_wave<PP, MM>), dim3(blocks), dim3(64 * EN_WPB), 0, strm, c->pnl, c->n,       \
                        sgm_dev,                                                                                   \
                        s.dLevel, s.dX, s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr, s.dPoly, s.c0,     \
                        depth, c->height, theta, c->kappa, MM == 1 ? rhsFactor(c) : c->eps, c->Q, c->qorder,       \
@@ -6480,7 +6541,7 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
     if (mode == 0 && f32) {
 #define EN_WAVE32(PP)                                                                                             \
   case PP:                                                                                                        \
-    hipLaunchKernelGGL((k_energy_wave<PP, 0, float>), dim3(blocks), dim3(64 * EN_WPB), 0, c->stream, c->pnl,      \
+    hipLaunchKernelGGL((k_energy_wave<PP, 0, float>), dim3(blocks), dim3(64 * EN_WPB), 0, strm, c->pnl,      \
                        c->n, sgm_dev, s.dLevel, s.dX, s.dR, s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr,    \
                        s.dPoly32, s.c0, depth, c->height, theta, c->kappa, c->eps, c->Q, c->qorder,              \
                        lvCount[c->height], S, s.dPart, s.dPart + 2 * (size_t)S * c->n);                           \
@@ -6500,7 +6561,7 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
     }
 #undef EN_WAVE
     if ((mode == 1 || mode == 2 || mode == 4) && S > 1)
-      hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, c->stream, s.dPart, S,
+      hipLaunchKernelGGL(k_sum_slices, dim3((2 * c->n + 255) / 256), dim3(256), 0, strm, s.dPart, S,
                          2 * (long long)c->n, b_dev);
   } else if (mode != 0) {
     setErr("fbh_rhs_tree / tree gradient: the accepted levels must share one expansion order");
@@ -6508,21 +6569,21 @@ static int walkCharges(fbh_ctx *c, const ChargeSet &s, int mode, double theta, c
   } else {
     int P1 = ord[depth + 1];
     size_t lds = (2 * (size_t)derivScratch(P1) + 2 * (size_t)NMOM(P1)) * sizeof(double);
-    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, c->stream, c->pnl, c->n, sgm_dev, s.dLevel, s.dX, s.dR,
+    hipLaunchKernelGGL(k_energy, dim3(c->n), dim3(64), lds, strm, c->pnl, c->n, sgm_dev, s.dLevel, s.dX, s.dR,
                        s.dKid, s.dNK, s.dCS, s.dNC, s.dCI, s.dPos, s.dChr, s.dOrd, s.dMom, nmMax, depth, c->height,
                        theta, c->kappa, c->eps, c->Q, c->qorder, c->T, s.dPart);
   }
   hipError_t e = hipSuccess;
   if (mode == 0 || mode == 3 || mode == 4) {
-    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_sum_part, dim3(RED_BLOCKS), dim3(256), 0, strm,
                        mode == 4 ? s.dPart + 2 * (size_t)S * c->n : s.dPart, S * c->n, s.dRed);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, s.dRed, RED_BLOCKS, s.dOut);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, strm, s.dRed, RED_BLOCKS, s.dOut);
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, s.dOut, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(energy_raw, s.dOut, sizeof(double), hipMemcpyDeviceToHost, strm);
   } else {
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(strm);
   if (e != hipSuccess) { setErr("energy kernels failed: %s", hipGetErrorString(e)); return 0; }
   return 1;
 }

@@ -197,6 +197,14 @@ int fbh_matvec32_dev(fbh_ctx *ctx, double alpha, const double *x, double beta, d
 int fbh_gmres_refined_dev(fbh_ctx *ctx, const double *b, double *x, int restart, double tol,
                           int maxit, int maxOuter, int *iters, double *resid, double *hist,
                           int *info, int *outer, int *mv64);
+/* fbh_gmres_refined_dev followed by fbh_energy32_res_dev on the solution (the
+ * resident charge set, theta the walk's MAC), with each correction's energy
+ * walk run on a second stream beside the FP64 residual product that decides
+ * whether the correction is the last: the energy of the returned x.  No
+ * counterpart in the reference (its solve and energy are FP64 and in turn). */
+int fbh_gmres_refined_energy_dev(fbh_ctx *ctx, const double *b, double *x, int restart, double tol,
+                                 int maxit, int maxOuter, double theta, int *iters, double *resid,
+                                 double *hist, int *info, int *outer, int *mv64, double *energy_raw);
 
 /* direct right-hand side on host arrays: b[2n] = fac * sum_q ... (setupRHS direct loop,
  * src/fabipb.c:789-798; gpuSetupRHS, gpu_backend.h:21), fac = 1/(4 pi eps1) unless

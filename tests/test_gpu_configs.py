@@ -73,6 +73,13 @@ def test_precision_form_317k(fa, meshdir):
     print(f"[{name}] energy walk with FP32 far polynomials: rel {abs(w32 - w64) / abs(w64):.2e} from the FP64 walk, "
           f"{abs(w32 - ref) / abs(ref):.2e} from the reference", flush=True)
     assert abs(w32 - w64) <= 3e-7 * abs(w64) and abs(w32 - ref) <= 1e-6 * abs(ref)
+    # the refined solve with the walk beside its last FP64 residual: the same x and energy
+    b = torch.empty(2 * tree.n, dtype=torch.float64, device="cuda")
+    op.rhs_dev(mesh.pos, mesh.chr, b)
+    xf = torch.zeros_like(b)
+    gf = op.gmres_refined_energy_dev(b, xf, restart=FL["a"], tol=1e-6, maxit=100)
+    assert gf["info"] == 0 and gf["outer"] == g32["outer"] and gf["iters"] == g32["iters"]
+    assert torch.equal(xf, x32) and gf["energy_raw"] == w32
 
 
 _cfg4 = {}
