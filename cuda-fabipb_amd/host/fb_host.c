@@ -307,8 +307,8 @@ fb_mesh *fb_mesh_load(const char *base, int useGeom) {
   m->nrm = NEW(double, 3 * (size_t)nface);
   m->area = NEW(double, nface);
   m->face = NEW(int, nface);
-  /* each face's verdict depends on the input alone (its area and the 10
-   * faces before it in the file, kept or not), so the faces are judged in
+  /* each face's outcome depends on the input alone (its area and the 10
+   * faces before it in the file, kept or not), so the faces are decided in
    * parallel, then numbered in file order and written in parallel */
   double *far = NEW(double, nface);
   int *slot = NEW(int, (size_t)nface + 1);
