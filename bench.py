@@ -347,8 +347,10 @@ def headline(fa, torch, args, tmp):
             "fp64_launches": nt32["fp64"][0],
             "fp64_kernel_ms": nt32["fp64"][1] / nt32["fp64"][0] if nt32["fp64"][0] else None,
             "fp64_plain_kernel_ms": nt64["fp64"][1] / nt64["fp64"][0] if nt64["fp64"][0] else None,
-            "note": "one FP32 operation per lane per cycle without packed FP32 (disabled in this build): "
-                    "the kernel's attainable VALU rate is half the published FP32 peak"}
+            "kernel_ms_alone": mv["stages32_ms"]["near"],
+            "frac_alone": flops / FP32_PEAK / (mv["stages32_ms"]["near"] * 1e-3),
+            "note": "kernel_ms: the launches inside the timed solves, beside the far chain on the other "
+                    "stream; kernel_ms_alone: the same launch with the GPU to itself (stage profile)"}
     cfg = {"workload": "precision form: FP32-refined PB solve against FP64 plain, star-shaped globule R=40 A "
                        "level-7 icosphere with faces under 0.05 A^2 merged (317,112 panels), 8,000 charges, "
                        "eps 1/80, kappa 0.1257, Taylor order 6, MAC 0.8, depth 5, q=1, GMRES tol 1e-6 restart 30, "
