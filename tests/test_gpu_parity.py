@@ -259,7 +259,33 @@ def test_resident_charges(fa, meshdir, name):
             op.rhs_tree_dev(ct, mesh.pos, q, b1)
             op.rhs_tree_res_dev(b2)
             assert torch.equal(b1, b2)
-            assert op.energy_dev(ct, mesh.pos, q, s) == op.energy_res_dev(s)
+            e64 = op.energy_dev(ct, mesh.pos, q, s)
+            assert e64 == op.energy_res_dev(s)
+            # the precision form's walk (far polynomials in FP32): repeatable, near the FP64 walk
+            e32 = op.energy_res_dev(s, precision="fp32")
+            assert e32 == op.energy_res_dev(s, precision="fp32")
+            assert abs(e32 - e64) <= 1e-5 * abs(e64)
+
+
+@pytest.mark.parametrize("name", ["sph3_ball", "glob4"])
+def test_refined_energy_overlap(fa, meshdir, name):
+    """The refined solve with the energy walk beside its last FP64 residual
+    (gmres_refined_energy_dev) returns the x and the energy of the two calls in turn."""
+    import torch
+    mesh, tree, op = case_op(fa, meshdir, name)
+    ct = tree.charge_tree()
+    op.enable_fp32()
+    op.set_charges(ct, mesh.pos, mesh.chr)
+    b = torch.empty(2 * tree.n, dtype=torch.float64, device="cuda")
+    op.rhs32_res_dev(b)
+    xa, xb = torch.zeros_like(b), torch.zeros_like(b)
+    ga = op.gmres_refined_dev(b, xa, restart=10, tol=1e-6, maxit=100)
+    ea = op.energy_res_dev(xa, precision="fp32")
+    for _ in range(2):
+        xb.zero_()
+        gb = op.gmres_refined_energy_dev(b, xb, restart=10, tol=1e-6, maxit=100)
+        assert gb["info"] == ga["info"] == 0 and gb["iters"] == ga["iters"] and gb["outer"] == ga["outer"]
+        assert torch.equal(xa, xb) and gb["energy_raw"] == ea
 
 
 @pytest.mark.parametrize("name", [c for c in cli_cases() if c != "cfg2_globule"])
