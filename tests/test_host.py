@@ -46,6 +46,38 @@ def test_mesh_and_tree_match_reference(fa, meshdir, name):
 
 
 @pytest.mark.parametrize("name", harness_cases())
+def test_tree_layout_of_the_fmm_passes(fa, meshdir, name):
+    """The cube numbering the device passes index by: the leaves are cubes
+    0 .. nLeaf-1, all at the deepest level; every cube of the FMM levels has at
+    most eight kids, numbered contiguously on the next level, each with it as
+    parent, the kid ranges of a level tiling the next one; each leaf reaches the
+    top FMM level in depth - height parent steps (the fused L2L + L2P launch
+    walks exactly that chain, the M2M launches take one wave per kid)."""
+    mesh = fa.Mesh(str(meshdir / name))
+    t0 = CASES["cases"][name]["t"]
+    for depth in sorted({max(FL["H"], t0 - 1), t0, t0 + 1}):
+        tree = fa.Tree(mesh, depth, order=FL["p"], height=FL["H"], sep=FL["S"])
+        a, D, H = tree.a, tree.depth, tree.height
+        lev, ls, lc = a["cubeLevel"], a["levStart"], a["levCount"]
+        ks, nk, par = a["kidStart"], a["nKids"], a["parent"]
+        assert ls[D] == 0 and lc[D] == tree.nleaf
+        assert np.all(lev[: tree.nleaf] == D)
+        for l in range(H, D + 1):
+            assert np.all(lev[ls[l]: ls[l] + lc[l]] == l)
+        for l in range(H, D):
+            cubes = np.arange(ls[l], ls[l] + lc[l])
+            assert np.all((nk[cubes] >= 1) & (nk[cubes] <= 8))
+            assert ks[cubes[0]] == ls[l + 1]
+            assert np.array_equal(ks[cubes[1:]], ks[cubes[:-1]] + nk[cubes[:-1]])
+            assert ks[cubes[-1]] + nk[cubes[-1]] == ls[l + 1] + lc[l + 1]
+            assert np.array_equal(par[ls[l + 1]: ls[l + 1] + lc[l + 1]], np.repeat(cubes, nk[cubes]))
+        c = np.arange(tree.nleaf)
+        for _ in range(D - H):
+            c = par[c]
+        assert np.all(lev[c] == H)
+
+
+@pytest.mark.parametrize("name", harness_cases())
 def test_charge_tree_matches_reference(fa, meshdir, name):
     F = fixture(name)
     mesh = fa.Mesh(str(meshdir / name))
