@@ -3849,10 +3849,10 @@ __global__ __launch_bounds__(256) void k_sum_part(const double *__restrict__ a, 
 // ------------------------------------------------- reductions and GMRES
 
 // The reductions' grid is one block per CU, so a thread's grid-stride loop
-// takes about ten trips at 300k panels; RED_U trips' loads are issued before
-// their arithmetic (one memory round trip per RED_U trips instead of one per
-// trip), and each thread still adds its terms in index order.
-#define RED_U 8
+// takes about ten trips at 300k panels; the loads of RED_U trips (those inside
+// n) are issued before their arithmetic, one memory round trip per RED_U trips
+// instead of one per trip, and each thread still adds its terms in index order.
+#define RED_U 16
 
 // block partial of sum a[i]*b[i] over a fixed grid (deterministic)
 __global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ a, const double *__restrict__ b, int n,
@@ -3860,18 +3860,18 @@ __global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ a, 
   __shared__ double s[256];
   const int st = gridDim.x * 256;
   double v = 0.0;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  for (; i + (RED_U - 1) * st < n; i += RED_U * st) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_U * st) {
     double av[RED_U], bv[RED_U];
 #pragma unroll
     for (int u = 0; u < RED_U; u++) {
-      av[u] = a[i + u * st];
-      bv[u] = b[i + u * st];
+      const bool in = i + u * st < n;
+      av[u] = in ? a[i + u * st] : 0.0;
+      bv[u] = in ? b[i + u * st] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < RED_U; u++) v += av[u] * bv[u];
+    for (int u = 0; u < RED_U; u++)
+      if (i + u * st < n) v += av[u] * bv[u];
   }
-  for (; i < n; i += st) v += a[i] * b[i];
   s[threadIdx.x] = v;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -3908,26 +3908,22 @@ __global__ __launch_bounds__(256) void k_mgs_dot(double *__restrict__ w, const d
   const bool self = vnext == w;
   const int st = gridDim.x * 256;
   double acc = 0.0;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  for (; i + (RED_U - 1) * st < n; i += RED_U * st) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_U * st) {
     double wv[RED_U], vv[RED_U], nv[RED_U];
 #pragma unroll
     for (int u = 0; u < RED_U; u++) {   // every load of the batch before its stores
-      wv[u] = w[i + u * st];
-      vv[u] = v[i + u * st];
-      nv[u] = self ? 0.0 : vnext[i + u * st];
+      const bool in = i + u * st < n;
+      wv[u] = in ? w[i + u * st] : 0.0;
+      vv[u] = in ? v[i + u * st] : 0.0;
+      nv[u] = in && !self ? vnext[i + u * st] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < RED_U; u++) {
-      const double wi = wv[u] + -hk * vv[u];
-      w[i + u * st] = wi;
-      acc += wi * (self ? wi : nv[u]);
-    }
-  }
-  for (; i < n; i += st) {
-    const double wi = w[i] + -hk * v[i];
-    w[i] = wi;
-    acc += wi * (self ? wi : vnext[i]);
+    for (int u = 0; u < RED_U; u++)
+      if (i + u * st < n) {
+        const double wi = wv[u] + -hk * vv[u];
+        w[i + u * st] = wi;
+        acc += wi * (self ? wi : nv[u]);
+      }
   }
   s[threadIdx.x] = acc;
   __syncthreads();
@@ -3945,15 +3941,14 @@ __global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ w,
   if (blockIdx.x == 0 && threadIdx.x == 0) *h = hh;
   double sc = hh > 0.0 ? 1.0 / hh : 1.0;
   const int st = gridDim.x * 256;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  for (; i + (RED_U - 1) * st < n; i += RED_U * st) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_U * st) {
     double wv[RED_U];
 #pragma unroll
-    for (int u = 0; u < RED_U; u++) wv[u] = w[i + u * st];
+    for (int u = 0; u < RED_U; u++) wv[u] = i + u * st < n ? w[i + u * st] : 0.0;
 #pragma unroll
-    for (int u = 0; u < RED_U; u++) v[i + u * st] = hh > 0.0 ? wv[u] * sc : wv[u];
+    for (int u = 0; u < RED_U; u++)
+      if (i + u * st < n) v[i + u * st] = hh > 0.0 ? wv[u] * sc : wv[u];
   }
-  for (; i < n; i += st) v[i] = hh > 0.0 ? w[i] * sc : w[i];
 }
 
 __global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, int np, double *__restrict__ out) {
