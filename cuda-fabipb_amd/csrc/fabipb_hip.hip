@@ -3894,6 +3894,45 @@ __device__ double reduceParts(const double *part, int np) {   // same order in e
   return v;
 }
 
+// The diagonal preconditioner (k_psolve, in place) fused with the first MGS
+// dot product of an Arnoldi step: w[i] = w[i] / (s area), then the block
+// partials of <w, v> over the reductions' grid.  The same expressions and
+// reduction as k_psolve followed by k_dot_part, so the same bits in one pass.
+__global__ __launch_bounds__(256) void k_psolve_dot(double *__restrict__ w, const double *__restrict__ v,
+                                                    const double *__restrict__ area, int np, double s1, double s2,
+                                                    double *__restrict__ part) {
+  __shared__ double s[256];
+  const int n = 2 * np, st = gridDim.x * 256;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += RED_U * st) {
+    double wv[RED_U], vv[RED_U], av[RED_U];
+#pragma unroll
+    for (int u = 0; u < RED_U; u++) {
+      const int j = i + u * st;
+      const bool in = j < n;
+      wv[u] = in ? w[j] : 0.0;
+      vv[u] = in ? v[j] : 0.0;
+      av[u] = in ? area[j < np ? j : j - np] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < RED_U; u++) {
+      const int j = i + u * st;
+      if (j < n) {
+        const double wp = wv[u] / ((j < np ? s1 : s2) * av[u]);
+        w[j] = wp;
+        acc += wp * vv[u];
+      }
+    }
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
 // MGS step (src/gmres.c:606-609) fused with the next step's dot product:
 // h = <w, v> from the partials of the previous launch, w -= h v, then the block
 // partials of <w, vnext> (vnext = w for the closing norm).  Element order, FMA
@@ -5900,10 +5939,16 @@ static int gmresCore(fbh_ctx *c, int kind, const double *b, double *x, int restr
       ++*iters;
       double *vi = c->V + (size_t)(i - 1) * n2, *vn = c->V + (size_t)i * n2;
       if (!applyOp(c, kind, 1.0, vi, 0.0, c->w)) return 0;
-      if (!precApply(c, kind, c->w, c->w)) return 0;
-      // modified Gram-Schmidt: <w, v0>, then per k the update fused with the next dot product
+      // modified Gram-Schmidt: <w, v0> (with the diagonal preconditioner applied
+      // in the same pass), then per k the update fused with the next dot product
       double *pa = c->part, *pb = c->part + RED_BLOCKS;
-      hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, c->V, n2, pa);
+      if (c->precMode == 3) {   // the diagonal, as fbh_psolve_diag_dev
+        hipLaunchKernelGGL(k_psolve_dot, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, c->V, c->area, c->n,
+                           (1.0 + c->eps) / 2.0, (1.0 + 1.0 / c->eps) / 2.0, pa);
+      } else {
+        if (!precApply(c, kind, c->w, c->w)) return 0;
+        hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, c->V, n2, pa);
+      }
       for (int k = 0; k < i; k++) {
         const double *vk = c->V + (size_t)k * n2, *vnext = k + 1 < i ? vk + n2 : c->w;
         hipLaunchKernelGGL(k_mgs_dot, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->w, vk, vnext, n2, pa,
